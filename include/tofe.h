@@ -1,0 +1,210 @@
+/*
+ * tofe.h -- C ABI of the MI355X-native TopOpt3D finite-element analysis path.
+ *
+ * One context owns a structured trilinear-hexahedral mesh, its density filter,
+ * its geometric-multigrid hierarchy and all device memory for one GPU.  The
+ * entry points replace the TopOpt member functions and PETSc handles that the
+ * reference's design loop, functions and eigensolvers call; each declaration
+ * names the reference interface it stands in for.
+ *
+ * Conventions (shared with the reference):
+ *   - nodes and elements are numbered x-fastest, then y, then z
+ *     (RecMesh.cpp:486-512, 565-597); DOF = 3*node + component
+ *     (FEAnalysis.cpp:241, Compliance.cpp:40);
+ *   - element-local node order (0,0,0),(1,0,0),(1,1,0),(0,1,0), then +z
+ *     (RecMesh.cpp:486-512).
+ * Errors: every function returns 0 on success and a negative TOFE_ERR_* code on
+ * failure; tofe_last_error() gives the message (the CHKERRQ/SETERRQ convention
+ * of the reference, with no exceptions crossing the ABI).  A solve that does
+ * not converge is not an error: it is reported through `reason`, which uses
+ * KSPConvergedReason values (FEAnalysis.cpp:677-684).
+ * Host-pointer entry points copy in and out; *_dev entry points take device
+ * pointers that must stay valid for the call.  One context per host thread;
+ * each context runs on its own HIP stream and every call returns with its
+ * stream synchronised.
+ */
+#ifndef TOFE_H
+#define TOFE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tofe_ctx tofe_ctx;
+
+enum {
+  TOFE_OK = 0,
+  TOFE_ERR_ARG = -1,       /* invalid argument */
+  TOFE_ERR_HIP = -2,       /* HIP runtime or device failure */
+  TOFE_ERR_STATE = -3,     /* call out of order (e.g. solve before assemble) */
+  TOFE_ERR_NUMERIC = -4,   /* factorisation failed (coarse operator not SPD) */
+  TOFE_ERR_NOMEM = -5
+};
+
+/* KSPConvergedReason values reported by tofe_solve (PETSc semantics). */
+enum {
+  TOFE_CONVERGED_RTOL = 2,
+  TOFE_CONVERGED_ATOL = 3,
+  TOFE_DIVERGED_ITS = -3,
+  TOFE_DIVERGED_DTOL = -4,
+  TOFE_DIVERGED_INDEFINITE_PC = -8,
+  TOFE_DIVERGED_NANORINF = -9,
+  TOFE_DIVERGED_INDEFINITE_MAT = -10
+};
+
+enum { TOFE_SMOOTHER_CHEBYSHEV = 0, TOFE_SMOOTHER_JACOBI = 1 };
+enum { TOFE_PREC_FP64 = 0, TOFE_PREC_MIXED = 1 };
+
+/* Fields readable with tofe_get_field (the TopOpt Vecs of the same names,
+ * TopOpt.h:180-184). */
+enum {
+  TOFE_FIELD_X = 0,      /* design variables x            (nelem) */
+  TOFE_FIELD_RHO = 1,    /* filtered densities rho = P x  (nelem) */
+  TOFE_FIELD_E = 2,      /* element stiffness scale E     (nelem) */
+  TOFE_FIELD_DEDZ = 3,   /* dE/dz                         (nelem) */
+  TOFE_FIELD_ES = 4,     /* Es = z^p                      (nelem) */
+  TOFE_FIELD_DESDZ = 5,  /* dEs/dz                        (nelem) */
+  TOFE_FIELD_V = 6,      /* volume interpolation V = rho  (nelem) */
+  TOFE_FIELD_DIAG = 7    /* Jacobi diagonal of fine K     (ndof)  */
+};
+
+typedef struct tofe_desc {
+  int nx, ny, nz;          /* elements per direction (Nel, RecMesh.cpp:426) */
+  double hx, hy, hz;       /* element edge lengths (dx, RecMesh.cpp:528-533) */
+  double E0, nu;           /* material (TopOpt::E0, Nu0) */
+  double Emin;             /* minimum stiffness eps (Material_Interpolation.cpp:28: 1e-10) */
+  double penal;            /* SIMP exponent p */
+  double rmin;             /* density filter radius (absolute length) */
+  int mg_levels;           /* levels incl. the fine one; 0 = floor(log2(min nel)) (RecMesh.cpp:552) */
+  int smoother;            /* TOFE_SMOOTHER_* */
+  int nu_pre, nu_post;     /* smoothing steps (V(nu_pre, nu_post)); 0 -> 2 */
+  double cheb_lo, cheb_hi; /* Chebyshev interval as fractions of lambda_max(D^-1 A); 0 -> 0.1, 1.1 */
+  int eig_its;             /* power-iteration steps for lambda_max; 0 -> 10 */
+  int precision;           /* TOFE_PREC_*: FP64, or FP32 V-cycle inside FP64 flexible CG */
+  const uint8_t *fixed_dof;/* 3*nnode flags, 1 = Dirichlet DOF (fixedDof, FEAnalysis.cpp:236-246); may be NULL */
+} tofe_desc;
+
+typedef struct tofe_stats {
+  int levels;              /* multigrid levels used */
+  int64_t coarse_size;     /* DOFs of the coarsest (directly solved) level */
+  int its;                 /* CG iterations of the last solve */
+  int reason;              /* KSPConvergedReason of the last solve */
+  double rnorm;            /* final unpreconditioned residual 2-norm */
+  double bnorm;            /* ||F|| of the last solve */
+  double setup_seconds;    /* last tofe_assemble (Galerkin, diagonals, eigen bounds, coarse factor) */
+  double solve_seconds;    /* last tofe_solve wall time */
+} tofe_stats;
+
+/* Create the context: mesh numbering, KE, density filter, transfer operators and
+ * device buffers.  Replaces TopOpt::CreateMesh's filter/P setup (RecMesh.cpp:426-728,
+ * Filter.cpp:21-150, Interpolation.cpp:31-355) and TopOpt::FEInitialize
+ * (FEAnalysis.cpp:215-424). */
+int tofe_create(const tofe_desc *desc, tofe_ctx **ctx);
+
+/* Release all device memory (TopOpt::Clear, TopOpt.cpp:59-89). */
+int tofe_destroy(tofe_ctx *ctx);
+
+/* Message of the last failing call on this context (PETSc error text). */
+const char *tofe_last_error(const tofe_ctx *ctx);
+
+/* Mesh and hierarchy sizes (TopOpt::nElem, nNode, PR.size()+1). */
+int tofe_sizes(const tofe_ctx *ctx, int64_t *nelem, int64_t *nnode, int64_t *ndof,
+               int *levels);
+
+/* Element stiffness matrix actually used, 24x24 row-major (TopOpt::ke[0] =
+ * LocalK(0), FEAnalysis.cpp:340-341, 869-892). */
+int tofe_get_ke(const tofe_ctx *ctx, double *ke576);
+
+/* rho = P x; SIMP: E, dEdz, Es, dEsdz, V (TopOpt::MatIntFnc,
+ * Material_Interpolation.cpp:20-140, plain density filter y = 1). */
+int tofe_set_design(tofe_ctx *ctx, const double *x);
+int tofe_set_design_dev(tofe_ctx *ctx, const double *d_x);
+
+/* Override the element stiffness scale E directly (dEdz is left as is). */
+int tofe_set_E(tofe_ctx *ctx, const double *E);
+
+/* Copy out one of the TOFE_FIELD_* vectors. */
+int tofe_get_field(const tofe_ctx *ctx, int field, double *out);
+
+/* Rebuild the solver for the current E: fine Jacobi diagonal, Galerkin coarse
+ * operators P^T A P, Chebyshev eigenvalue bounds and the coarsest factorisation.
+ * Replaces TopOpt::FEAssemble (FEAnalysis.cpp:432-490) plus PCSetUp of PCMG
+ * (FEAnalysis.cpp:541, 555-573). */
+int tofe_assemble(tofe_ctx *ctx);
+
+/* Solve K U = F for nrhs right-hand sides stored one after another (ndof each),
+ * warm-started from U, with CG + V-cycle to ||r|| <= rtol*||F||.  its/reason
+ * receive the last right-hand side's iteration count and KSPConvergedReason
+ * (or may be NULL).  Replaces TopOpt::FESolve's KSPSolve(KUF, F, U)
+ * (FEAnalysis.cpp:507-732) and the adjoint KSPSolve of Buckling.cpp:199. */
+int tofe_solve(tofe_ctx *ctx, const double *F, double *U, int nrhs, double rtol,
+               int maxit, int *its, int *reason);
+int tofe_solve_dev(tofe_ctx *ctx, const double *d_F, double *d_U, int nrhs, double rtol,
+                   int maxit, int *its, int *reason);
+
+/* Y = K X with the Dirichlet rows/columns of MatZeroRowsColumns(K, fixed, 1.0);
+ * MatMult(K) as used by PRINVIT/LOPGMRES (PRINVIT.cpp:353-354). */
+int tofe_apply_K(tofe_ctx *ctx, const double *X, double *Y, int nrhs);
+int tofe_apply_K_dev(tofe_ctx *ctx, const double *d_X, double *d_Y, int nrhs);
+
+/* Z = M^-1 R, one multigrid V-cycle; PCApply of the KUF preconditioner
+ * (LOPGMRES.cpp:125). */
+int tofe_pc_apply(tofe_ctx *ctx, const double *R, double *Z, int nrhs);
+int tofe_pc_apply_dev(tofe_ctx *ctx, const double *d_R, double *d_Z, int nrhs);
+
+/* c = U.F and dc/dx = P^T(-dEdz .* u_e^T KE u_e): Compliance::Function
+ * (Compliance.cpp:15-57) including TopOpt::Chain_Filter
+ * (Material_Interpolation.cpp:150-187).  dcdx may be NULL. */
+int tofe_compliance(tofe_ctx *ctx, const double *U, const double *F, double *c,
+                    double *dcdx);
+int tofe_compliance_dev(tofe_ctx *ctx, const double *d_U, const double *d_F, double *c,
+                        double *d_dcdx);
+
+/* Element sensitivities g_e = -dEdz_e u_e^T KE u_e before the filter chain rule
+ * (Compliance.cpp:37-46). */
+int tofe_sensitivity(tofe_ctx *ctx, const double *U, double *g);
+
+/* rho = P x and gx = P^T g with the density filter (MatMult / MatMultTranspose of
+ * TopOpt::P, Material_Interpolation.cpp:35, 159-162). */
+int tofe_filter(tofe_ctx *ctx, const double *x, double *rho);
+int tofe_filter_T(tofe_ctx *ctx, const double *g, double *gx);
+
+/* The density filter as CSR computed on the device: rows in element order,
+ * columns ascending, row-scaled weights (TopOpt::P after Assemble_Filter,
+ * Filter.cpp:96-150).  rowptr has nelem+1 entries; with col == NULL only rowptr
+ * is filled (rowptr[nelem] = nnz). */
+int tofe_filter_pattern(tofe_ctx *ctx, int64_t *rowptr, int32_t *col, double *w);
+
+/* Switch the solve precision (TOFE_PREC_*); takes effect at the next assemble. */
+int tofe_set_precision(tofe_ctx *ctx, int precision);
+
+/* Statistics of the last assemble/solve: the quantities of FESolve's verbose
+ * timing line (FEAnalysis.cpp:687-711). */
+int tofe_get_stats(const tofe_ctx *ctx, tofe_stats *stats);
+
+/* Galerkin operator of level l (1 <= l < levels) as a dense 27-point block stencil:
+ * out[(o*9 + a*3 + b) * nnode_l + n] is the (a,b) entry of the 3x3 block coupling
+ * node n with its neighbour at offset o = (dx+1) + 3(dy+1) + 9(dz+1).  Level 0 is
+ * the matrix-free fine operator (returns the assembled stencil of K).
+ * Used to check the coarse operators against P^T A P. */
+int tofe_get_level_stencil(tofe_ctx *ctx, int level, double *out, int64_t *nnode_l);
+
+/* lambda_max(D^-1 A) estimate of level l used for the Chebyshev bounds. */
+int tofe_get_level_lambda(const tofe_ctx *ctx, int level, double *lambda);
+
+/* Average duration (ms) of nrep back-to-back launches of one level operator kernel,
+ * timed with HIP events on the context's stream: level 0 is the matrix-free fine
+ * operator, level >= 1 the Galerkin stencil; precision TOFE_PREC_FP64 or
+ * TOFE_PREC_MIXED (FP32 kernel); epilogue 0 = y = K u, 1 = Chebyshev smoothing step.
+ * No reference counterpart: it instruments the kernels that FESolve's timing line
+ * (FEAnalysis.cpp:687-711) only sees in aggregate.  Requires tofe_assemble (FP32
+ * kernels need an assemble in mixed precision). */
+int tofe_time_operator(tofe_ctx *ctx, int level, int precision, int epilogue, int nrep,
+                       double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOFE_H */

@@ -1,0 +1,189 @@
+"""Parity of the HIP path with the CPU oracle, through the C ABI, on seeded inputs.
+
+Bars: bit-exact for the filter pattern/weights, rho = P x, P^T g, the SIMP fields
+(p = 3) and element sensitivities for a given U; <= 1e-14 relative for the matrix-free
+operator and the Galerkin coarse operators; solves driven to 1e-12 agree to 1e-10
+relative in U, compliance and sensitivities (FP64 tolerance of the north star).
+"""
+import numpy as np
+import pytest
+
+from topopt3d_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((8, 4, 4), 2), ((7, 5, 3), 2), ((9, 6, 5), 3), ((16, 8, 8), 3), ((48, 24, 24), 3)]
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def setup_pair(oracle, tofe, nel, L, seed=7, R=1.5, p=3.0, fixed=True):
+    pb = configs.cantilever(nel, seed=seed, lo=0.05, hi=1.0, rmin=R, penal=p, mg_levels=L)
+    print(f"case nel={nel} L={L} seed={seed} R={R} p={p}", flush=True)
+    ctx = tofe.Context(nel, rmin=R, penal=p, mg_levels=L,
+                       fixed_dof=pb.fixed_dof if fixed else None)
+    ctx.set_design(pb.x)
+    Pf = oracle.filter_csr(*nel, R)
+    rho = Pf.matvec(pb.x)
+    E, dEdz, Es, dEsdz = oracle.simp(rho, p, 1e-10)
+    K = oracle.assemble_K(*nel, oracle.ke(), E, pb.fixed_dof if fixed else None)
+    return pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K
+
+
+@pytest.mark.parametrize("nel,L", CASES)
+def test_filter_and_simp_bit_exact(oracle, tofe, nel, L):
+    for R, p in [(1.5, 3.0), (2.0, 3.0), (2.4, 3.5)]:
+        pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, L, R=R, p=p)
+        rp, col, w = ctx.filter_pattern()
+        rpo, colo, wo = Pf.arrays()
+        assert np.array_equal(rp, rpo) and np.array_equal(col, colo)
+        assert np.array_equal(w, wo)
+        assert np.array_equal(ctx.field(tofe.FIELD_RHO), rho)
+        assert np.array_equal(ctx.filter(pb.x), rho)
+        g = np.random.default_rng(3).standard_normal(pb.nelem)
+        assert np.array_equal(ctx.filter_T(g), Pf.rmatvec(g))
+        if p == 3.0:
+            assert np.array_equal(ctx.field(tofe.FIELD_E), E)
+            assert np.array_equal(ctx.field(tofe.FIELD_DEDZ), dEdz)
+            assert np.array_equal(ctx.field(tofe.FIELD_ES), Es)
+            assert np.array_equal(ctx.field(tofe.FIELD_DESDZ), dEsdz)
+        else:  # general exponent: device pow() against glibc pow()
+            assert rel(ctx.field(tofe.FIELD_E), E) <= 1e-15
+            assert rel(ctx.field(tofe.FIELD_DEDZ), dEdz) <= 1e-15
+        ctx.close()
+
+
+@pytest.mark.parametrize("nel,L", CASES)
+def test_operator_matches_assembled_K(oracle, tofe, nel, L):
+    for fixed in (True, False):
+        pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, L, fixed=fixed)
+        rng = np.random.default_rng(11)
+        X = rng.standard_normal((2, pb.ndof))
+        Y = ctx.apply_K(X.ravel()).reshape(2, -1)
+        for k in range(2):
+            assert rel(Y[k], K.matvec(X[k])) <= 1e-14
+        # symmetry u.Kv = v.Ku
+        assert X[0] @ Y[1] == pytest.approx(X[1] @ Y[0], rel=1e-12)
+        ctx.close()
+
+
+@pytest.mark.parametrize("nel,L", CASES)
+def test_galerkin_levels_match_ptap(oracle, tofe, nel, L):
+    pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, L)
+    ctx.assemble()
+    mg = oracle.MG(K, tuple(n + 1 for n in nel), L)
+    # level 0 stencil = assembled K, entry by entry (same summation order, no FMA)
+    for l in range(0, L):
+        S = ctx.level_stencil(l)
+        A = (K if l == 0 else mg.A(l)).to_scipy().tocoo()
+        d = mg.dims(l)
+        n = A.row // 3
+        m = A.col // 3
+        dn = m - n
+        di = dn % d[0]
+        # offset of m relative to n as (dx, dy, dz)
+        ni, nj, nk = n % d[0], (n // d[0]) % d[1], n // (d[0] * d[1])
+        mi, mj, mk = m % d[0], (m // d[0]) % d[1], m // (d[0] * d[1])
+        o = (mi - ni + 1) + 3 * (mj - nj + 1) + 9 * (mk - nk + 1)
+        vals = S[o, A.row % 3, A.col % 3, n]
+        if l == 0:
+            assert np.array_equal(vals, A.data)
+        else:
+            assert np.abs(vals - A.data).max() <= 1e-14 * np.abs(A.data).max()
+        assert np.abs(S).sum() == pytest.approx(np.abs(A.data).sum(), rel=1e-13)
+        if l < L - 1:
+            assert ctx.level_lambda(l) == pytest.approx(mg.lam(l), rel=1e-10)
+    ctx.close()
+
+
+@pytest.mark.parametrize("nel,L", CASES)
+def test_solve_compliance_sensitivity(oracle, tofe, nel, L):
+    pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, L)
+    ctx.assemble()
+    U, its, reason = ctx.solve(pb.F, rtol=1e-12, maxit=500)
+    mg = oracle.MG(K, tuple(n + 1 for n in nel), L)
+    Uo, itso, reasono, _ = mg.solve(pb.F, rtol=1e-12, maxit=500)
+    assert reason == 2 and reasono == 2
+    assert abs(its - itso) <= 1
+    assert rel(U, Uo) <= 1e-10
+    r = pb.F - K.matvec(U)
+    assert np.linalg.norm(r) <= 1e-12 * np.linalg.norm(pb.F) * 1.01
+    c, g = ctx.compliance(U, pb.F)
+    co = Uo @ pb.F
+    assert c == pytest.approx(co, rel=1e-10)
+    go = Pf.rmatvec(oracle.sensitivity(*nel, oracle.ke(), Uo, dEdz))
+    assert rel(g, go) <= 1e-10
+    # sensitivities from identical U are bit-exact
+    assert np.array_equal(ctx.sensitivity(Uo), oracle.sensitivity(*nel, oracle.ke(), Uo, dEdz))
+    st = ctx.stats()
+    assert st["its"] == its and st["reason"] == 2 and st["levels"] == L
+    ctx.close()
+
+
+def test_reason_codes_and_warm_start(oracle, tofe):
+    nel = (16, 8, 8)
+    pb, ctx, *_ = setup_pair(oracle, tofe, nel, 3)
+    ctx.assemble()
+    U, its, reason = ctx.solve(pb.F, rtol=1e-12, maxit=2)
+    assert reason == tofe.DIVERGED_ITS and its == 2
+    U, its, reason = ctx.solve(pb.F, rtol=1e-8)
+    assert reason == tofe.CONVERGED_RTOL
+    U2, its2, reason2 = ctx.solve(pb.F, U, rtol=1e-8)
+    assert its2 == 0 and reason2 == tofe.CONVERGED_RTOL and np.array_equal(U, U2)
+    # rtol = 0 falls back to the absolute tolerance 1e-50 and runs into maxit
+    U3, its3, reason3 = ctx.solve(pb.F, rtol=0.0, maxit=5)
+    assert reason3 == tofe.DIVERGED_ITS
+    # zero right-hand side with zero guess converges at iteration 0
+    U4, its4, reason4 = ctx.solve(np.zeros(pb.ndof), rtol=1e-8)
+    assert its4 == 0 and reason4 in (2, 3) and not U4.any()
+    ctx.close()
+
+
+def test_state_errors(tofe):
+    nel = (8, 4, 4)
+    pb = configs.cantilever(nel)
+    ctx = tofe.Context(nel, mg_levels=2, fixed_dof=pb.fixed_dof)
+    with pytest.raises(tofe.TofeError) as e:
+        ctx.solve(pb.F)
+    assert e.value.code == tofe.TOFE_ERR_STATE
+    ctx.set_design(pb.x)
+    with pytest.raises(tofe.TofeError):
+        ctx.solve(pb.F)            # not assembled for this design
+    ctx.assemble()
+    ctx.solve(pb.F)
+    ctx.set_precision(tofe.PREC_MIXED)
+    with pytest.raises(tofe.TofeError):
+        ctx.solve(pb.F)            # precision changed: needs a new assemble
+    with pytest.raises(tofe.TofeError):
+        tofe.Context((0, 4, 4))
+    ctx.close()
+
+
+def test_multiple_rhs(oracle, tofe):
+    nel = (16, 8, 8)
+    pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, 3)
+    ctx.assemble()
+    rng = np.random.default_rng(5)
+    Fs = rng.standard_normal((3, pb.ndof))
+    Fs[:, pb.fixed_dof.astype(bool)] = 0
+    U, its, reason = ctx.solve(Fs.ravel(), rtol=1e-12)
+    U = U.reshape(3, -1)
+    for k in range(3):
+        Uk, _, rk = ctx.solve(Fs[k], rtol=1e-12)
+        assert rk == 2
+        assert np.array_equal(U[k], Uk)
+    ctx.close()
+
+
+def test_pc_apply_is_symmetric(oracle, tofe):
+    nel = (16, 8, 8)
+    pb, ctx, *_ = setup_pair(oracle, tofe, nel, 3)
+    ctx.assemble()
+    rng = np.random.default_rng(2)
+    a, b = rng.standard_normal((2, pb.ndof))
+    Ma, Mb = ctx.pc_apply(a), ctx.pc_apply(b)
+    assert b @ Ma == pytest.approx(a @ Mb, rel=1e-10)
+    assert a @ Ma > 0
+    ctx.close()

@@ -1,0 +1,307 @@
+"""ctypes binding of libtofe.so, the C ABI declared in include/tofe.h.
+
+The library must have been built in-tree (``python topopt3d_amd/build.py`` or
+``__graft_entry__.build()``).  There is no CPU fallback: loading fails loudly when the
+HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libtofe.so")
+
+TOFE_OK = 0
+TOFE_ERR_ARG, TOFE_ERR_HIP, TOFE_ERR_STATE, TOFE_ERR_NUMERIC, TOFE_ERR_NOMEM = -1, -2, -3, -4, -5
+
+CONVERGED_RTOL = 2
+CONVERGED_ATOL = 3
+DIVERGED_ITS = -3
+DIVERGED_DTOL = -4
+DIVERGED_INDEFINITE_PC = -8
+DIVERGED_NANORINF = -9
+DIVERGED_INDEFINITE_MAT = -10
+
+SMOOTHER_CHEBYSHEV, SMOOTHER_JACOBI = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+FIELD_X, FIELD_RHO, FIELD_E, FIELD_DEDZ, FIELD_ES, FIELD_DESDZ, FIELD_V, FIELD_DIAG = range(8)
+
+# Every entry point include/tofe.h declares.
+EXPORTS = (
+    "tofe_create", "tofe_destroy", "tofe_last_error", "tofe_sizes", "tofe_get_ke",
+    "tofe_set_design", "tofe_set_design_dev", "tofe_set_E", "tofe_get_field", "tofe_assemble",
+    "tofe_solve", "tofe_solve_dev", "tofe_apply_K", "tofe_apply_K_dev", "tofe_pc_apply",
+    "tofe_pc_apply_dev", "tofe_compliance", "tofe_compliance_dev", "tofe_sensitivity",
+    "tofe_filter", "tofe_filter_T", "tofe_filter_pattern", "tofe_set_precision",
+    "tofe_get_stats", "tofe_get_level_stencil", "tofe_get_level_lambda", "tofe_time_operator",
+)
+
+
+class TofeDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
+        ("E0", C.c_double), ("nu", C.c_double), ("Emin", C.c_double), ("penal", C.c_double),
+        ("rmin", C.c_double),
+        ("mg_levels", C.c_int), ("smoother", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
+        ("cheb_lo", C.c_double), ("cheb_hi", C.c_double),
+        ("eig_its", C.c_int), ("precision", C.c_int),
+        ("fixed_dof", C.c_void_p),
+    ]
+
+
+class TofeStats(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int), ("coarse_size", C.c_int64), ("its", C.c_int), ("reason", C.c_int),
+        ("rnorm", C.c_double), ("bnorm", C.c_double),
+        ("setup_seconds", C.c_double), ("solve_seconds", C.c_double),
+    ]
+
+
+class TofeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tofe error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libtofe.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python topopt3d_amd/build.py or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, i64p, i32p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int32)
+    ip = C.POINTER(C.c_int)
+    sig = {
+        "tofe_create": ([C.POINTER(TofeDesc), C.POINTER(vp)], C.c_int),
+        "tofe_destroy": ([vp], C.c_int),
+        "tofe_last_error": ([vp], C.c_char_p),
+        "tofe_sizes": ([vp, i64p, i64p, i64p, ip], C.c_int),
+        "tofe_get_ke": ([vp, dp], C.c_int),
+        "tofe_set_design": ([vp, dp], C.c_int),
+        "tofe_set_design_dev": ([vp, vp], C.c_int),
+        "tofe_set_E": ([vp, dp], C.c_int),
+        "tofe_get_field": ([vp, C.c_int, dp], C.c_int),
+        "tofe_assemble": ([vp], C.c_int),
+        "tofe_solve": ([vp, dp, dp, C.c_int, C.c_double, C.c_int, ip, ip], C.c_int),
+        "tofe_solve_dev": ([vp, vp, vp, C.c_int, C.c_double, C.c_int, ip, ip], C.c_int),
+        "tofe_apply_K": ([vp, dp, dp, C.c_int], C.c_int),
+        "tofe_apply_K_dev": ([vp, vp, vp, C.c_int], C.c_int),
+        "tofe_pc_apply": ([vp, dp, dp, C.c_int], C.c_int),
+        "tofe_pc_apply_dev": ([vp, vp, vp, C.c_int], C.c_int),
+        "tofe_compliance": ([vp, dp, dp, dp, dp], C.c_int),
+        "tofe_compliance_dev": ([vp, vp, vp, dp, vp], C.c_int),
+        "tofe_sensitivity": ([vp, dp, dp], C.c_int),
+        "tofe_filter": ([vp, dp, dp], C.c_int),
+        "tofe_filter_T": ([vp, dp, dp], C.c_int),
+        "tofe_filter_pattern": ([vp, i64p, i32p, dp], C.c_int),
+        "tofe_set_precision": ([vp, C.c_int], C.c_int),
+        "tofe_get_stats": ([vp, C.POINTER(TofeStats)], C.c_int),
+        "tofe_get_level_stencil": ([vp, C.c_int, dp, i64p], C.c_int),
+        "tofe_get_level_lambda": ([vp, C.c_int, dp], C.c_int),
+        "tofe_time_operator": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One tofe_ctx: a structured Q1 mesh, its filter and multigrid hierarchy on one GPU."""
+
+    def __init__(self, nel, h=(1.0, 1.0, 1.0), E0=1.0, nu=0.3, Emin=1e-10, penal=3.0,
+                 rmin=1.5, mg_levels=0, smoother=SMOOTHER_CHEBYSHEV, nu_pre=2, nu_post=2,
+                 cheb_lo=0.1, cheb_hi=1.1, eig_its=10, precision=PREC_FP64, fixed_dof=None):
+        L = lib()
+        self.nel = tuple(int(v) for v in nel)
+        d = TofeDesc()
+        d.nx, d.ny, d.nz = self.nel
+        d.hx, d.hy, d.hz = (float(v) for v in h)
+        d.E0, d.nu, d.Emin, d.penal, d.rmin = E0, nu, Emin, penal, rmin
+        d.mg_levels, d.smoother, d.nu_pre, d.nu_post = mg_levels, smoother, nu_pre, nu_post
+        d.cheb_lo, d.cheb_hi, d.eig_its, d.precision = cheb_lo, cheb_hi, eig_its, precision
+        self._fixed = None
+        if fixed_dof is not None:
+            self._fixed = np.ascontiguousarray(fixed_dof, dtype=np.uint8)
+            d.fixed_dof = self._fixed.ctypes.data
+        self.h = C.c_void_p()
+        rc = L.tofe_create(C.byref(d), C.byref(self.h))
+        if rc != TOFE_OK:
+            raise TofeError(rc, "tofe_create failed")
+        ne, nn, nd, lv = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
+        L.tofe_sizes(self.h, C.byref(ne), C.byref(nn), C.byref(nd), C.byref(lv))
+        self.nelem, self.nnode, self.ndof, self.levels = ne.value, nn.value, nd.value, lv.value
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().tofe_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != TOFE_OK:
+            msg = lib().tofe_last_error(self.h)
+            raise TofeError(rc, msg.decode() if msg else "")
+
+    # -- state --------------------------------------------------------------
+    def ke(self):
+        KE = np.empty(576)
+        self._check(lib().tofe_get_ke(self.h, _dp(KE)))
+        return KE.reshape(24, 24)
+
+    def set_design(self, x):
+        x = _f64(x)
+        assert x.size == self.nelem
+        self._check(lib().tofe_set_design(self.h, _dp(x)))
+
+    def set_design_dev(self, ptr):
+        self._check(lib().tofe_set_design_dev(self.h, C.c_void_p(ptr)))
+
+    def set_E(self, E):
+        E = _f64(E)
+        assert E.size == self.nelem
+        self._check(lib().tofe_set_E(self.h, _dp(E)))
+
+    def field(self, which):
+        n = self.ndof if which == FIELD_DIAG else self.nelem
+        out = np.empty(n)
+        self._check(lib().tofe_get_field(self.h, which, _dp(out)))
+        return out
+
+    def assemble(self):
+        self._check(lib().tofe_assemble(self.h))
+
+    def set_precision(self, p):
+        self._check(lib().tofe_set_precision(self.h, p))
+
+    # -- solves and operators -------------------------------------------------
+    def solve(self, F, U=None, rtol=1e-8, maxit=1000):
+        F = _f64(F)
+        nrhs = F.size // self.ndof
+        U = np.zeros_like(F) if U is None else np.array(U, dtype=np.float64, copy=True)
+        its, reason = C.c_int(), C.c_int()
+        self._check(lib().tofe_solve(self.h, _dp(F), _dp(U), nrhs, rtol, maxit,
+                                     C.byref(its), C.byref(reason)))
+        return U, its.value, reason.value
+
+    def solve_dev(self, F_ptr, U_ptr, nrhs=1, rtol=1e-8, maxit=1000):
+        its, reason = C.c_int(), C.c_int()
+        self._check(lib().tofe_solve_dev(self.h, C.c_void_p(F_ptr), C.c_void_p(U_ptr), nrhs,
+                                         rtol, maxit, C.byref(its), C.byref(reason)))
+        return its.value, reason.value
+
+    def apply_K(self, X):
+        X = _f64(X)
+        Y = np.empty_like(X)
+        self._check(lib().tofe_apply_K(self.h, _dp(X), _dp(Y), X.size // self.ndof))
+        return Y
+
+    def apply_K_dev(self, X_ptr, Y_ptr, nrhs=1):
+        self._check(lib().tofe_apply_K_dev(self.h, C.c_void_p(X_ptr), C.c_void_p(Y_ptr), nrhs))
+
+    def pc_apply(self, R):
+        R = _f64(R)
+        Z = np.empty_like(R)
+        self._check(lib().tofe_pc_apply(self.h, _dp(R), _dp(Z), R.size // self.ndof))
+        return Z
+
+    def pc_apply_dev(self, R_ptr, Z_ptr, nrhs=1):
+        self._check(lib().tofe_pc_apply_dev(self.h, C.c_void_p(R_ptr), C.c_void_p(Z_ptr), nrhs))
+
+    def compliance(self, U, F, gradient=True):
+        U, F = _f64(U), _f64(F)
+        c = C.c_double()
+        g = np.empty(self.nelem) if gradient else None
+        self._check(lib().tofe_compliance(self.h, _dp(U), _dp(F), C.byref(c), _dp(g)))
+        return (c.value, g) if gradient else c.value
+
+    def compliance_dev(self, U_ptr, F_ptr, g_ptr=None):
+        c = C.c_double()
+        self._check(lib().tofe_compliance_dev(self.h, C.c_void_p(U_ptr), C.c_void_p(F_ptr),
+                                              C.byref(c), C.c_void_p(g_ptr) if g_ptr else None))
+        return c.value
+
+    def sensitivity(self, U):
+        U = _f64(U)
+        g = np.empty(self.nelem)
+        self._check(lib().tofe_sensitivity(self.h, _dp(U), _dp(g)))
+        return g
+
+    def filter(self, x):
+        x = _f64(x)
+        rho = np.empty_like(x)
+        self._check(lib().tofe_filter(self.h, _dp(x), _dp(rho)))
+        return rho
+
+    def filter_T(self, g):
+        g = _f64(g)
+        gx = np.empty_like(g)
+        self._check(lib().tofe_filter_T(self.h, _dp(g), _dp(gx)))
+        return gx
+
+    def filter_pattern(self):
+        rp = np.empty(self.nelem + 1, np.int64)
+        self._check(lib().tofe_filter_pattern(self.h, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              None, None))
+        nnz = int(rp[-1])
+        col = np.empty(nnz, np.int32)
+        w = np.empty(nnz)
+        self._check(lib().tofe_filter_pattern(self.h, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              col.ctypes.data_as(C.POINTER(C.c_int32)), _dp(w)))
+        return rp, col, w
+
+    def stats(self):
+        st = TofeStats()
+        self._check(lib().tofe_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in TofeStats._fields_}
+
+    def level_stencil(self, level):
+        nn = C.c_int64()
+        self._check(lib().tofe_get_level_stencil(self.h, level, None, C.byref(nn)))
+        out = np.empty(243 * nn.value)
+        self._check(lib().tofe_get_level_stencil(self.h, level, _dp(out), C.byref(nn)))
+        return out.reshape(27, 3, 3, nn.value)
+
+    def time_operator(self, level=0, precision=PREC_FP64, epilogue=0, nrep=20):
+        """Average ms per launch of one level operator kernel (HIP events)."""
+        ms = C.c_double()
+        self._check(lib().tofe_time_operator(self.h, level, precision, epilogue, nrep, C.byref(ms)))
+        return ms.value
+
+    def level_lambda(self, level):
+        lam = C.c_double()
+        self._check(lib().tofe_get_level_lambda(self.h, level, C.byref(lam)))
+        return lam.value

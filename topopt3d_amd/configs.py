@@ -1,0 +1,108 @@
+"""Synthetic workloads: structured cantilever / column meshes with their boundary
+conditions and seeded densities.
+
+Randomness is SplitMix64 with seed = configuration index and u = (r >> 11) * 2^-53, so
+every component of the build (tests, bench, oracle) draws the same numbers.
+
+Mesh conventions follow the reference: nodes and elements x-fastest, then y, then z
+(RecMesh.cpp:486-512, 565-597); DOF = 3*node + component (FEAnalysis.cpp:241).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """The first n outputs of SplitMix64 started from state `seed`."""
+    with np.errstate(over="ignore"):
+        state = np.uint64(seed) + _GOLDEN * np.arange(1, n + 1, dtype=np.uint64)
+        z = state
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+    """n i.i.d. uniform values in [lo, hi): u = (r >> 11) * 2^-53."""
+    u = (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return lo + (hi - lo) * u
+
+
+def node_index(nel, i, j, k):
+    nx, ny, _ = nel
+    return i + (nx + 1) * (j + (ny + 1) * k)
+
+
+def face_fixed_dof(nel, axis: int, value: int) -> np.ndarray:
+    """Per-DOF Dirichlet flags clamping every node with coordinate index `value` along
+    `axis` in all three components."""
+    nx, ny, nz = nel
+    N = (nx + 1, ny + 1, nz + 1)
+    idx = np.indices((N[2], N[1], N[0]))  # k, j, i
+    coord = (idx[2], idx[1], idx[0])[axis]
+    nodes = np.flatnonzero((coord == value).reshape(-1))
+    fixed = np.zeros(3 * N[0] * N[1] * N[2], np.uint8)
+    for c in range(3):
+        fixed[3 * nodes + c] = 1
+    return fixed
+
+
+@dataclass
+class Problem:
+    name: str
+    nel: tuple
+    fixed_dof: np.ndarray
+    F: np.ndarray
+    x: np.ndarray
+    rmin: float = 1.5
+    penal: float = 3.0
+    Emin: float = 1e-10
+    mg_levels: int = 0
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def ndof(self):
+        nx, ny, nz = self.nel
+        return 3 * (nx + 1) * (ny + 1) * (nz + 1)
+
+    @property
+    def nelem(self):
+        nx, ny, nz = self.nel
+        return nx * ny * nz
+
+
+def cantilever(nel, x=None, rmin=1.5, penal=3.0, Emin=1e-10, mg_levels=0, seed=None,
+               lo=0.01, hi=1.0, name="cantilever") -> Problem:
+    """x = 0 face fully clamped; unit downward (-z) point load at the bottom-centre node
+    (nx, ny/2, 0) of the x = max face."""
+    nx, ny, nz = nel
+    fixed = face_fixed_dof(nel, 0, 0)
+    F = np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    F[3 * node_index(nel, nx, ny // 2, 0) + 2] = -1.0
+    ne = nx * ny * nz
+    if x is None:
+        x = uniform(seed, ne, lo, hi) if seed is not None else np.full(ne, 0.3)
+    elif np.isscalar(x):
+        x = np.full(ne, float(x))
+    return Problem(name, tuple(nel), fixed, F, np.asarray(x, np.float64), rmin, penal, Emin,
+                   mg_levels)
+
+
+def config1(Emin=1e-10) -> Problem:
+    """48x24x24 cantilever, uniform density 0.3, filter R = 1.5, 3-level GMG."""
+    return cantilever((48, 24, 24), x=0.3, rmin=1.5, penal=3.0, Emin=Emin, mg_levels=3,
+                      name="cantilever-48x24x24")
+
+
+def config2(Emin=1e-9) -> Problem:
+    """192x96x96 cantilever (5,447,811 DOF), densities U[0.01,1] seed 2, R = 1.5, p = 3,
+    5-level V(2,2) Chebyshev GMG.  The densities are filtered inside tofe_set_design."""
+    return cantilever((192, 96, 96), seed=2, lo=0.01, hi=1.0, rmin=1.5, penal=3.0, Emin=Emin,
+                      mg_levels=5, name="cantilever-192x96x96")

@@ -1,0 +1,142 @@
+// Host-side set-up computations that run once per context: the element stiffness
+// matrix and the density-filter neighbourhood template.
+#include "host_setup.hpp"
+
+#include <cmath>
+#include <algorithm>
+#include <cstring>
+
+namespace tofe {
+
+// Element-local node order (RecMesh.cpp:486-512).
+static const int kLN[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                              {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE[576])
+{
+  // Isotropic constitutive matrix, Voigt [xx,yy,zz,xy,yz,xz] (RecMesh.cpp:453-462).
+  const double c = E0 / ((1 + nu) * (1 - 2 * nu));
+  const double G = E0 / (2 * (1 + nu));
+  const double t1 = (1 - nu) * c, t2 = G, t3 = nu * c;
+  double D[6][6] = {{t1, t3, t3, 0, 0, 0}, {t3, t1, t3, 0, 0, 0}, {t3, t3, t1, 0, 0, 0},
+                    {0, 0, 0, t2, 0, 0},   {0, 0, 0, 0, t2, 0},   {0, 0, 0, 0, 0, t2}};
+  const double g = 1 / std::sqrt(3.0);
+  const double h[3] = {hx, hy, hz};
+  std::memset(KE, 0, 576 * sizeof(double));
+  // 2x2x2 Gauss points at +-1/sqrt(3), unit weights, in local-node sign order
+  // (GaussPoints FEAnalysis.cpp:900-926; LocalK 869-892).
+  for (int q = 0; q < 8; q++) {
+    const double xi = (2 * kLN[q][0] - 1) * g, eta = (2 * kLN[q][1] - 1) * g,
+                 zeta = (2 * kLN[q][2] - 1) * g;
+    double dNdxi[3][8];  // trilinear shape-function derivatives (dN, 936-990)
+    for (int n = 0; n < 8; n++) {
+      const double sx = 2 * kLN[n][0] - 1, sy = 2 * kLN[n][1] - 1, sz = 2 * kLN[n][2] - 1;
+      dNdxi[0][n] = sx / 8 * (1 + sy * eta) * (1 + sz * zeta);
+      dNdxi[1][n] = sy / 8 * (1 + sx * xi) * (1 + sz * zeta);
+      dNdxi[2][n] = sz / 8 * (1 + sx * xi) * (1 + sy * eta);
+    }
+    // The element is an axis-aligned box, so J = diag(h/2).
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        for (int n = 0; n < 8; n++) J[a][b] += dNdxi[a][n] * (h[b] * kLN[n][b]);
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    double inv[3][3];
+    inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+    inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+    inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    double dNdx[3][8];
+    for (int a = 0; a < 3; a++)
+      for (int n = 0; n < 8; n++) {
+        double s = 0;
+        for (int b = 0; b < 3; b++) s += inv[a][b] * dNdxi[b][n];
+        dNdx[a][n] = s;
+      }
+    // Strain-displacement matrix B (AssignB, 1024-1046)
+    double B[6][24];
+    std::memset(B, 0, sizeof(B));
+    for (int n = 0; n < 8; n++) {
+      B[0][3 * n + 0] = dNdx[0][n];
+      B[1][3 * n + 1] = dNdx[1][n];
+      B[2][3 * n + 2] = dNdx[2][n];
+      B[3][3 * n + 0] = dNdx[1][n];
+      B[3][3 * n + 1] = dNdx[0][n];
+      B[4][3 * n + 1] = dNdx[2][n];
+      B[4][3 * n + 2] = dNdx[1][n];
+      B[5][3 * n + 0] = dNdx[2][n];
+      B[5][3 * n + 2] = dNdx[0][n];
+    }
+    double DB[6][24];
+    for (int a = 0; a < 6; a++)
+      for (int cc = 0; cc < 24; cc++) {
+        double s = 0;
+        for (int b = 0; b < 6; b++) s += D[a][b] * B[b][cc];
+        DB[a][cc] = s;
+      }
+    for (int r = 0; r < 24; r++)
+      for (int cc = 0; cc < 24; cc++) {
+        double s = 0;
+        for (int a = 0; a < 6; a++) s += B[a][r] * DB[a][cc];
+        KE[r * 24 + cc] += s * det;
+      }
+  }
+}
+
+int filter_template(double dx, double dy, double dz, double R, std::vector<int> &off,
+                    std::vector<double> &w)
+{
+  // N = (short)(R/dx) per direction (Filter.cpp:32-35)
+  const short N[3] = {(short)(R / dx), (short)(R / dy), (short)(R / dz)};
+  off.clear();
+  w.clear();
+  for (int k = -N[2]; k < N[2] + 1; k++)
+    for (int j = -N[1]; j < N[1] + 1; j++)
+      for (int i = -N[0]; i < N[0] + 1; i++) {
+        // dist = sqrt(pow(i dx,2) + pow(j dx,2) + pow(k dx,2)) (Filter.cpp:47), pow(v,2)
+        // being the correctly rounded v*v; strict dist < R (48); weight 1 - dist/R (80)
+        const double vi = i * dx, vj = j * dy, vk = k * dz;
+        const double dist = std::sqrt(vi * vi + vj * vj + vk * vk);
+        if (dist < R) {
+          off.push_back(i);
+          off.push_back(j);
+          off.push_back(k);
+          w.push_back(1 - dist / R);
+        }
+      }
+  return (int)w.size();
+}
+
+double tridiag_max_eig(int k, const double *a, const double *b)
+{
+  double lo = a[0], hi = a[0];
+  for (int j = 0; j < k; j++) {
+    const double r = (j > 0 ? std::fabs(b[j - 1]) : 0) + (j + 1 < k ? std::fabs(b[j]) : 0);
+    lo = std::min(lo, a[j] - r);
+    hi = std::max(hi, a[j] + r);
+  }
+  for (int it = 0; it < 200; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    int cnt = 0;  // eigenvalues below mid
+    double q = a[0] - mid;
+    if (q < 0) cnt++;
+    for (int j = 1; j < k; j++) {
+      if (q == 0) q = 1e-300;
+      q = a[j] - mid - b[j - 1] * b[j - 1] / q;
+      if (q < 0) cnt++;
+    }
+    if (cnt == k) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+}  // namespace tofe

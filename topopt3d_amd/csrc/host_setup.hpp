@@ -1,0 +1,20 @@
+#pragma once
+
+#include <vector>
+
+namespace tofe {
+
+// 24x24 Q1 hexahedron stiffness, row-major, DOF 3*localnode + component
+// (TopOpt::LocalK, FEAnalysis.cpp:869-1050).
+void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE[576]);
+
+// Density-filter neighbourhood: offsets (i,j,k) walked k, j, i ascending with
+// dist < R, and weights 1 - dist/R (TopOpt::RecFilter, Filter.cpp:21-94).
+int filter_template(double dx, double dy, double dz, double R, std::vector<int> &off,
+                    std::vector<double> &w);
+
+// Largest eigenvalue of a symmetric tridiagonal matrix (diagonal a, off-diagonal b) by
+// Sturm-sequence bisection.
+double tridiag_max_eig(int k, const double *a, const double *b);
+
+}  // namespace tofe

@@ -1,0 +1,84 @@
+// Device helpers shared by the kernel translation units.
+#pragma once
+
+#include "tofe_internal.hpp"
+
+namespace tofe {
+
+// Element-local node order (0,0,0),(1,0,0),(1,1,0),(0,1,0), +z layer (RecMesh.cpp:486-512).
+__host__ __device__ constexpr int lidx(int lx, int ly, int lz)
+{
+  return lz * 4 + (ly ? (lx ? 2 : 3) : (lx ? 1 : 0));
+}
+__host__ __device__ constexpr int lnx(int l) { return (l == 1 || l == 2 || l == 5 || l == 6) ? 1 : 0; }
+__host__ __device__ constexpr int lny(int l) { return (l == 2 || l == 3 || l == 6 || l == 7) ? 1 : 0; }
+__host__ __device__ constexpr int lnz(int l) { return l >= 4 ? 1 : 0; }
+
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X; give each XCD a
+// contiguous range of logical tiles so neighbouring tiles share its L2.
+__device__ inline int xcd_remap(int bid, int nb)
+{
+  const int xcd = bid & 7, local = bid >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Deterministic block sum; the result is valid in thread 0.  smem holds NT/64 doubles.
+template <int NT>
+__device__ inline double block_sum(double v, double *smem)
+{
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) smem[w] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) t += smem[i];
+  }
+  return t;
+}
+
+// Row-scaled 1-D trilinear transfer weight between fine node i and coarse node J of a
+// grid with Nf fine nodes (Interpolation.cpp:135-163, 331-347).  The 3-D weight is the
+// product of three of these; all values are exact dyadic numbers.
+__host__ __device__ inline double w1d(int i, int J, int Nf)
+{
+  const int t = i - 2 * J;
+  if (t < -1 || t > 1) return 0.0;
+  if (t == 0) return 1.0;
+  if ((Nf & 1) == 0 && i == Nf - 1) return 1.0;  // last fine node of an even grid: one parent
+  return 0.5;
+}
+
+template <typename T, int EPI>
+__device__ inline void apply_epilogue(const EpiArgs<T> &ep, int64_t dof, T Au, T uval,
+                                      double &dsum, double &dsum2)
+{
+  if constexpr (EPI == EPI_APPLY) {
+    ep.y[dof] = Au;
+  } else if constexpr (EPI == EPI_APPLY_DOT) {
+    ep.y[dof] = Au;
+    dsum += (double)uval * (double)Au;
+  } else if constexpr (EPI == EPI_APPLY_DOT2) {
+    ep.y[dof] = Au;
+    dsum += (double)uval * (double)Au;
+    dsum2 += (double)uval * (double)ep.b[dof];
+  } else if constexpr (EPI == EPI_RESID) {
+    ep.y[dof] = ep.b[dof] - Au;
+  } else if constexpr (EPI == EPI_CHEB) {
+    const T r = ep.dinv[dof] * (ep.b[dof] - Au);
+    const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * ep.d[dof] + ep.c2 * r;
+    ep.d[dof] = dn;
+    ep.xout[dof] = uval + dn;
+  }
+}
+
+}  // namespace tofe

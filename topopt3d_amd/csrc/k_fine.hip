@@ -1,0 +1,351 @@
+// Matrix-free SIMP stiffness operator on the structured Q1 hexahedral mesh.
+//
+// y = K(E) u with K = sum_e E_e KE (FEAnalysis.cpp:448-464) and the Dirichlet rows and
+// columns of MatZeroRowsColumns(K, fixed, 1.0) (FEAnalysis.cpp:473): fixed entries of u
+// are zeroed on load and fixed rows return u itself.
+//
+// Node-centric, deterministic, no atomics: a 256-thread block owns a 32x8 column tile of
+// nodes and marches up a chunk of z.  Each step stages one node plane of u (with a
+// one-node halo) and one element layer of E in LDS; every thread keeps the 3x3 node
+// neighbourhood of the two planes around the current element layer in registers and
+// adds the layer's four elements to its lower node (which is then complete and written
+// through the epilogue) and to its upper node.  Per node: 8 elements x 3 rows x 24 =
+// 576 FMAs.  KE is indexed with compile-time constants only, so it is read through
+// scalar loads, uniform across the wavefront.
+#include "k_common.hpp"
+
+namespace tofe {
+namespace {
+
+constexpr int TX = 32, TY = 8, NT = TX * TY;
+constexpr int UW = TX + 2, UH = TY + 2, UPL = UW * UH * 3;
+constexpr int EW = TX + 1, EH = TY + 1, EPL = EW * EH;
+constexpr int ZLEN = 8;
+
+typedef double v8d __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef float v8f __attribute__((ext_vector_type(8)));
+
+// One 24-entry KE row, read with scalar (SMEM) loads into SGPRs.  The loads are issued
+// explicitly so the compiler can neither hoist all 576 KE values out of the z loop nor
+// batch them up front; at most one row (48 SGPRs in FP64) is live at a time and every
+// FMA takes its KE operand straight from an SGPR.
+template <typename T, int OFF>
+__device__ inline void ke_row(const T *base, T (&k)[24])
+{
+  if constexpr (sizeof(T) == 8) {
+    v8d a, b, c;
+    asm volatile(
+        "s_load_dwordx16 %0, %3, %4\n\t"
+        "s_load_dwordx16 %1, %3, %5\n\t"
+        "s_load_dwordx16 %2, %3, %6\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&s"(a), "=&s"(b), "=&s"(c)
+        : "s"(base), "i"(OFF * 8), "i"(OFF * 8 + 64), "i"(OFF * 8 + 128));
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      k[q] = a[q];
+      k[8 + q] = b[q];
+      k[16 + q] = c[q];
+    }
+  } else {
+    v16f a;
+    v8f b;
+    asm volatile(
+        "s_load_dwordx16 %0, %2, %3\n\t"
+        "s_load_dwordx8 %1, %2, %4\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&s"(a), "=&s"(b)
+        : "s"(base), "i"(OFF * 4), "i"(OFF * 4 + 64));
+#pragma unroll
+    for (int q = 0; q < 16; ++q) k[q] = a[q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) k[16 + q] = b[q];
+  }
+}
+
+// Rows 3*L+r of KE applied to the element vector ue, for compile-time L and r.
+template <typename T, int L, int R>
+__device__ inline T ke_row_dot(const T *base, const T (&ue)[24])
+{
+  T k[24];
+  ke_row<T, (3 * L + R) * 24>(base, k);
+  T t = 0;
+#pragma unroll
+  for (int q = 0; q < 24; ++q) t += k[q] * ue[q];
+  // keep this row's FMAs ahead of the next row's loads
+  asm volatile("" : "+v"(t));
+  __builtin_amdgcn_sched_barrier(0);
+  return t;
+}
+
+template <typename T, int A, int B>
+__device__ inline void element_rows(const T *base, const T (&ue)[24], T Ee, T (&alo)[3], T (&ahi)[3])
+{
+  constexpr int Llo = lidx(1 - A, 1 - B, 0), Lhi = lidx(1 - A, 1 - B, 1);
+  alo[0] += Ee * ke_row_dot<T, Llo, 0>(base, ue);
+  alo[1] += Ee * ke_row_dot<T, Llo, 1>(base, ue);
+  alo[2] += Ee * ke_row_dot<T, Llo, 2>(base, ue);
+  ahi[0] += Ee * ke_row_dot<T, Lhi, 0>(base, ue);
+  ahi[1] += Ee * ke_row_dot<T, Lhi, 1>(base, ue);
+  ahi[2] += Ee * ke_row_dot<T, Lhi, 2>(base, ue);
+}
+
+template <typename T>
+struct FineGrid {
+  int tiles_x, tiles_y, zchunks, nblocks;
+};
+
+template <typename T>
+FineGrid<T> fine_grid(const FineOp<T> &op)
+{
+  FineGrid<T> g;
+  g.tiles_x = (op.nx + 1 + TX - 1) / TX;
+  g.tiles_y = (op.ny + 1 + TY - 1) / TY;
+  g.zchunks = (op.nz + 1 + ZLEN - 1) / ZLEN;
+  g.nblocks = g.tiles_x * g.tiles_y * g.zchunks;
+  return g;
+}
+
+template <typename T, int EPI>
+__global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restrict__ u,
+                                                EpiArgs<T> ep, int tiles_x, int tiles_y,
+                                                int nblocks)
+{
+  __shared__ T su[2][UPL];
+  __shared__ T sE[2][EPL];
+  __shared__ double sred[NT / 64];
+
+  const int nx = op.nx, ny = op.ny, nz = op.nz;
+  const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
+  const int bid = xcd_remap(blockIdx.x, nblocks);
+  const int txy = tiles_x * tiles_y;
+  const int bz = bid / txy;
+  const int rem = bid - bz * txy;
+  const int by = rem / tiles_x, bx = rem - by * tiles_x;
+  const int x0 = bx * TX, y0 = by * TY;
+  const int k0 = bz * ZLEN, k1 = min(k0 + ZLEN, Nz);
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int i = x0 + tx, j = y0 + ty;
+  const bool own = (i < Nx) && (j < Ny);
+  const T *KEbase = op.KE;
+  const T *__restrict__ E = op.E;
+  const uint8_t *__restrict__ fix = op.fix;
+
+  auto stage_u = [&](int buf, int gz) {
+    const bool zin = gz >= 0 && gz < Nz;
+    for (int v = threadIdx.x; v < UPL; v += NT) {
+      const int nl = v / 3, c = v - 3 * nl;
+      const int ly = nl / UW, lx = nl - ly * UW;
+      const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+      T val = 0;
+      if (zin && gx >= 0 && gx < Nx && gy >= 0 && gy < Ny) {
+        const int64_t n = gx + (int64_t)Nx * (gy + (int64_t)Ny * gz);
+        val = u[3 * n + c];
+        if (fix && ((fix[n] >> c) & 1)) val = 0;
+      }
+      su[buf][v] = val;
+    }
+  };
+  auto stage_E = [&](int buf, int ez) {
+    const bool zin = ez >= 0 && ez < nz;
+    for (int v = threadIdx.x; v < EPL; v += NT) {
+      const int ly = v / EW, lx = v - ly * EW;
+      const int ex = x0 - 1 + lx, ey = y0 - 1 + ly;
+      T val = 0;
+      if (zin && ex >= 0 && ex < nx && ey >= 0 && ey < ny)
+        val = E[ex + (int64_t)nx * (ey + (int64_t)ny * ez)];
+      sE[buf][v] = val;
+    }
+  };
+
+  T lo[9][3], hi[9][3];
+  T alo[3], ahi[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) ahi[c] = 0;
+  double dsum = 0, dsum2 = 0;
+
+  stage_u(0, k0 - 1);
+  __syncthreads();
+#pragma unroll
+  for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) hi[dy * 3 + dx][c] = su[0][((ty + dy) * UW + tx + dx) * 3 + c];
+
+  int buf = 0;
+  for (int ez = k0 - 1; ez < k1; ++ez) {
+    buf ^= 1;
+    stage_u(buf, ez + 1);
+    stage_E(buf, ez);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lo[q][c] = hi[q][c];
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          hi[dy * 3 + dx][c] = su[buf][((ty + dy) * UW + tx + dx) * 3 + c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      alo[c] = ahi[c];
+      ahi[c] = 0;
+    }
+    auto gather = [&](int a, int b, T(&ue)[24]) {
+#pragma unroll
+      for (int l = 0; l < 8; ++l)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          ue[3 * l + c] = lnz(l) ? hi[(a + lnx(l)) + 3 * (b + lny(l))][c]
+                                 : lo[(a + lnx(l)) + 3 * (b + lny(l))][c];
+    };
+    {
+      T ue[24];
+      gather(0, 0, ue);
+      element_rows<T, 0, 0>(KEbase, ue, sE[buf][(ty + 0) * EW + tx + 0], alo, ahi);
+      gather(1, 0, ue);
+      element_rows<T, 1, 0>(KEbase, ue, sE[buf][(ty + 0) * EW + tx + 1], alo, ahi);
+      gather(0, 1, ue);
+      element_rows<T, 0, 1>(KEbase, ue, sE[buf][(ty + 1) * EW + tx + 0], alo, ahi);
+      gather(1, 1, ue);
+      element_rows<T, 1, 1>(KEbase, ue, sE[buf][(ty + 1) * EW + tx + 1], alo, ahi);
+    }
+    if (own && ez >= k0) {
+      const int64_t n = i + (int64_t)Nx * (j + (int64_t)Ny * ez);
+      const uint8_t fm = fix ? fix[n] : 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int64_t dof = 3 * n + c;
+        const T uval = u[dof];
+        const T Au = ((fm >> c) & 1) ? uval : alo[c];
+        apply_epilogue<T, EPI>(ep, dof, Au, uval, dsum, dsum2);
+      }
+    }
+  }
+  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2) {
+    __syncthreads();
+    const double t = block_sum<NT>(dsum, sred);
+    if (threadIdx.x == 0) ep.partial[blockIdx.x] = t;
+    if constexpr (EPI == EPI_APPLY_DOT2) {
+      __syncthreads();
+      const double t2 = block_sum<NT>(dsum2, sred);
+      if (threadIdx.x == 0) ep.partial2[blockIdx.x] = t2;
+    }
+  }
+}
+
+// Assembled fine stencil: for node n and neighbour m = n + off(o), block(a,b) =
+// sum over shared elements (ascending element order) of E_e * KE, each product
+// rounded before it is added exactly as FEAssemble's ke = E*ke[0] then ADD_VALUES
+// (FEAnalysis.cpp:448-464); then the MatZeroRowsColumns rows/columns (473) and the
+// diag <= 0 -> 1 rule (476-484).  Used to form the level-1 Galerkin operator and the
+// fine Jacobi diagonal.
+__global__ __launch_bounds__(256) void k_fine_stencil(FineOp<double> op, double *S, int64_t nn)
+{
+#pragma clang fp contract(off)
+  __shared__ double sKE[576];
+  for (int v = threadIdx.x; v < 576; v += 256) sKE[v] = op.KE[v];
+  __syncthreads();
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= 27 * nn) return;
+  const int o = (int)(t / nn);
+  const int64_t n = t - (int64_t)o * nn;
+  const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+  const int Nx = op.nx + 1, Ny = op.ny + 1, Nz = op.nz + 1;
+  const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+  const int mi = i + dx, mj = j + dy, mk = k + dz;
+  double blk[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) blk[q] = 0.0;
+  const bool inside = mi >= 0 && mi < Nx && mj >= 0 && mj < Ny && mk >= 0 && mk < Nz;
+  if (inside) {
+    for (int ez = max(k, mk) - 1; ez <= min(k, mk); ++ez) {
+      if (ez < 0 || ez >= op.nz) continue;
+      for (int ey = max(j, mj) - 1; ey <= min(j, mj); ++ey) {
+        if (ey < 0 || ey >= op.ny) continue;
+        for (int ex = max(i, mi) - 1; ex <= min(i, mi); ++ex) {
+          if (ex < 0 || ex >= op.nx) continue;
+          const double Ee = op.E[ex + (int64_t)op.nx * (ey + (int64_t)op.ny * ez)];
+          const int ln = lidx(i - ex, j - ey, k - ez), lm = lidx(mi - ex, mj - ey, mk - ez);
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+              blk[a * 3 + b] = blk[a * 3 + b] + Ee * sKE[(3 * ln + a) * 24 + 3 * lm + b];
+        }
+      }
+    }
+    if (op.fix) {
+      const int64_t m = mi + (int64_t)Nx * (mj + (int64_t)Ny * mk);
+      const uint8_t fn = op.fix[n], fmm = op.fix[m];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+          if (((fn >> a) & 1) || ((fmm >> b) & 1)) blk[a * 3 + b] = (o == 13 && a == b) ? 1.0 : 0.0;
+    }
+    if (o == 13) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        if (!(blk[a * 4] > 0)) blk[a * 4] = 1.0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) S[(int64_t)(o * 9 + q) * nn + n] = blk[q];
+}
+
+}  // namespace
+
+template <typename T>
+int fine_op_blocks(const FineOp<T> &op)
+{
+  return fine_grid(op).nblocks;
+}
+
+template <typename T>
+void fine_op(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep)
+{
+  const FineGrid<T> g = fine_grid(op);
+  dim3 grid(g.nblocks), block(NT);
+  switch (epi) {
+    case EPI_APPLY:
+      k_fine_op<T, EPI_APPLY><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
+      break;
+    case EPI_APPLY_DOT:
+      k_fine_op<T, EPI_APPLY_DOT><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
+      break;
+    case EPI_APPLY_DOT2:
+      k_fine_op<T, EPI_APPLY_DOT2><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
+      break;
+    case EPI_RESID:
+      k_fine_op<T, EPI_RESID><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
+      break;
+    case EPI_CHEB:
+      k_fine_op<T, EPI_CHEB><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
+      break;
+  }
+  TOFE_LAUNCH_CHECK();
+}
+
+void fine_stencil_build(hipStream_t s, const FineOp<double> &op, double *S)
+{
+  const int64_t nn = (int64_t)(op.nx + 1) * (op.ny + 1) * (op.nz + 1);
+  const int64_t nthreads = 27 * nn;
+  dim3 grid((unsigned)((nthreads + 255) / 256)), block(256);
+  k_fine_stencil<<<grid, block, 0, s>>>(op, S, nn);
+  TOFE_LAUNCH_CHECK();
+}
+
+template int fine_op_blocks<double>(const FineOp<double> &);
+template int fine_op_blocks<float>(const FineOp<float> &);
+template void fine_op<double>(hipStream_t, const FineOp<double> &, const double *, Epi,
+                              const EpiArgs<double> &);
+template void fine_op<float>(hipStream_t, const FineOp<float> &, const float *, Epi,
+                             const EpiArgs<float> &);
+
+}  // namespace tofe

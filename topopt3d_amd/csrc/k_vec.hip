@@ -1,0 +1,302 @@
+// Vector kernels of the multigrid-preconditioned CG solve: deterministic two-stage
+// reductions, CG updates driven by device-side scalars (no host round trip for alpha
+// and beta), the zero-guess Chebyshev step, trilinear restriction/prolongation
+// (Interpolation.cpp:123-169 row-scaled, applied matrix-free) and the Jacobi-CG steps
+// whose coefficients estimate lambda_max(D^-1 A) for the Chebyshev bounds.
+#include "k_common.hpp"
+
+namespace tofe {
+namespace {
+
+constexpr int NT = 256;
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_dot_partial(const T *__restrict__ a,
+                                                    const T *__restrict__ b, int64_t n,
+                                                    double *partial)
+{
+  __shared__ double sred[NT / 64];
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+    s += (double)a[i] * (double)b[i];
+  const double t = block_sum<NT>(s, sred);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(NT) void k_reduce_final(const double *__restrict__ partial,
+                                                     int nparts, double *out, int post,
+                                                     double *scal)
+{
+  __shared__ double sred[NT / 64];
+  double s = 0;
+  for (int i = threadIdx.x; i < nparts; i += NT) s += partial[i];
+  const double v = block_sum<NT>(s, sred);
+  if (threadIdx.x == 0) {
+    if (out) *out = v;
+    switch (post) {
+      case POST_ALPHA_CG: scal[SL_ALPHA] = scal[SL_RZ] / v; break;
+      case POST_BETA_CG:
+        scal[SL_BETA] = v / scal[SL_RZ];
+        scal[SL_RZ] = v;
+        break;
+      case POST_ALPHA_FCG: scal[SL_ALPHA] = scal[SL_PR] / v; break;
+      case POST_BETA_FCG: scal[SL_BETA] = -v / scal[SL_PQ]; break;
+      default: break;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_cheb_zero(const T *__restrict__ b, const T *__restrict__ dinv, T c2, T *d,
+                            T *x, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const T v = c2 * (dinv[i] * b[i]);
+    d[i] = v;
+    x[i] = v;
+  }
+}
+
+// b_c(I) = sum over fine nodes i within one fine step of 2I of w(i,I) r_f(i)  (P^T r)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int Nfx, int Nfy,
+                                                 int Nfz, T *__restrict__ bc, int Ncx, int Ncy,
+                                                 int Ncz)
+{
+  const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
+  const int64_t I = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (I >= ncn) return;
+  const int Ix = (int)(I % Ncx), Iy = (int)((I / Ncx) % Ncy), Iz = (int)(I / ((int64_t)Ncx * Ncy));
+  double acc[3] = {0, 0, 0};
+  for (int iz = 2 * Iz - 1; iz <= 2 * Iz + 1; ++iz) {
+    if (iz < 0 || iz >= Nfz) continue;
+    const double wz = w1d(iz, Iz, Nfz);
+    for (int iy = 2 * Iy - 1; iy <= 2 * Iy + 1; ++iy) {
+      if (iy < 0 || iy >= Nfy) continue;
+      const double wy = w1d(iy, Iy, Nfy);
+      for (int ix = 2 * Ix - 1; ix <= 2 * Ix + 1; ++ix) {
+        if (ix < 0 || ix >= Nfx) continue;
+        const double w = w1d(ix, Ix, Nfx) * wy * wz;
+        const int64_t f = ix + (int64_t)Nfx * (iy + (int64_t)Nfy * iz);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) acc[a] += w * (double)rf[3 * f + a];
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) bc[3 * I + a] = (T)acc[a];
+}
+
+// x_f(i) += sum over coarse parents J of w(i,J) x_c(J)  (P x_c)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_prolong_add(const T *__restrict__ xc, int Ncx, int Ncy,
+                                                    int Ncz, T *__restrict__ xf, int Nfx, int Nfy,
+                                                    int Nfz)
+{
+  const int64_t nfn = (int64_t)Nfx * Nfy * Nfz;
+  const int64_t f = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (f >= nfn) return;
+  const int ix = (int)(f % Nfx), iy = (int)((f / Nfx) % Nfy), iz = (int)(f / ((int64_t)Nfx * Nfy));
+  double acc[3] = {0, 0, 0};
+  for (int Jz = (iz - 1) >> 1; Jz <= (iz + 1) >> 1; ++Jz) {
+    if (Jz < 0 || Jz >= Ncz) continue;
+    const double wz = w1d(iz, Jz, Nfz);
+    if (wz == 0.0) continue;
+    for (int Jy = (iy - 1) >> 1; Jy <= (iy + 1) >> 1; ++Jy) {
+      if (Jy < 0 || Jy >= Ncy) continue;
+      const double wy = w1d(iy, Jy, Nfy);
+      if (wy == 0.0) continue;
+      for (int Jx = (ix - 1) >> 1; Jx <= (ix + 1) >> 1; ++Jx) {
+        if (Jx < 0 || Jx >= Ncx) continue;
+        const double wx = w1d(ix, Jx, Nfx);
+        if (wx == 0.0) continue;
+        const double w = wx * wy * wz;
+        const int64_t c = Jx + (int64_t)Ncx * (Jy + (int64_t)Ncy * Jz);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) acc[a] += w * (double)xc[3 * c + a];
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) xf[3 * f + a] = (T)((double)xf[3 * f + a] + acc[a]);
+}
+
+template <typename Ta, typename Tb>
+__global__ void k_convert(const Ta *__restrict__ a, Tb *__restrict__ b, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+    b[i] = (Tb)a[i];
+}
+
+__global__ void k_fill(double *a, double v, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+    a[i] = v;
+}
+
+// x += alpha p; r -= alpha q; partial ||r||^2
+__global__ __launch_bounds__(NT) void k_cg_xr(double *__restrict__ x, double *__restrict__ r,
+                                              const double *__restrict__ p,
+                                              const double *__restrict__ q, int64_t n,
+                                              const double *__restrict__ scal, int alpha_slot,
+                                              double *partial)
+{
+  __shared__ double sred[NT / 64];
+  const double alpha = scal[alpha_slot];
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  const double t = block_sum<NT>(s, sred);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// p = z + beta p
+__global__ void k_cg_p(double *__restrict__ p, const double *__restrict__ z, int64_t n,
+                       const double *__restrict__ scal, int beta_slot)
+{
+  const double beta = scal[beta_slot];
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+    p[i] = z[i] + beta * p[i];
+}
+
+// v_i = 0.5 + frac(i * 2654435761 / 2^32): deterministic start vector of the estimate
+__global__ void k_power_start(double *v, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const uint32_t h = (uint32_t)((uint64_t)i * 2654435761ull);
+    v[i] = 0.5 + (double)h / 4294967296.0;
+  }
+}
+
+// z = dinv r, partial r.z (Jacobi-preconditioned CG of the eigenvalue estimate)
+__global__ __launch_bounds__(NT) void k_jacobi_dot(const double *__restrict__ r,
+                                                   const double *__restrict__ dinv, double *z,
+                                                   int64_t n, double *partial)
+{
+  __shared__ double sred[NT / 64];
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const double zi = dinv[i] * r[i];
+    z[i] = zi;
+    s += r[i] * zi;
+  }
+  const double t = block_sum<NT>(s, sred);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void k_record(const double *scal, int slot, double *dst)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = scal[slot];
+}
+
+inline unsigned stride_blocks(int64_t n)
+{
+  const int64_t b = (n + NT - 1) / NT;
+  return (unsigned)(b < kRedBlocks ? (b > 0 ? b : 1) : kRedBlocks);
+}
+
+}  // namespace
+
+template <typename T>
+void dot_partial(hipStream_t s, const T *a, const T *b, int64_t n, double *partial)
+{
+  k_dot_partial<T><<<dim3(kRedBlocks), dim3(NT), 0, s>>>(a, b, n, partial);
+  TOFE_LAUNCH_CHECK();
+}
+
+void reduce_final(hipStream_t s, const double *partial, int nparts, double *out, int post,
+                  double *scal)
+{
+  k_reduce_final<<<dim3(1), dim3(NT), 0, s>>>(partial, nparts, out, post, scal);
+  TOFE_LAUNCH_CHECK();
+}
+
+template <typename T>
+void cheb_zero(hipStream_t s, const T *b, const T *dinv, T c2, T *d, T *x, int64_t n)
+{
+  k_cheb_zero<T><<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(b, dinv, c2, d, x, n);
+  TOFE_LAUNCH_CHECK();
+}
+
+template <typename T>
+void restrict_op(hipStream_t s, const T *rf, int Nfx, int Nfy, int Nfz, T *bc, int Ncx, int Ncy,
+                 int Ncz)
+{
+  const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
+  k_restrict<T><<<dim3((unsigned)((ncn + NT - 1) / NT)), dim3(NT), 0, s>>>(rf, Nfx, Nfy, Nfz, bc,
+                                                                           Ncx, Ncy, Ncz);
+  TOFE_LAUNCH_CHECK();
+}
+
+template <typename T>
+void prolong_add(hipStream_t s, const T *xc, int Ncx, int Ncy, int Ncz, T *xf, int Nfx, int Nfy,
+                 int Nfz)
+{
+  const int64_t nfn = (int64_t)Nfx * Nfy * Nfz;
+  k_prolong_add<T><<<dim3((unsigned)((nfn + NT - 1) / NT)), dim3(NT), 0, s>>>(xc, Ncx, Ncy, Ncz,
+                                                                              xf, Nfx, Nfy, Nfz);
+  TOFE_LAUNCH_CHECK();
+}
+
+template <typename Ta, typename Tb>
+void convert(hipStream_t s, const Ta *a, Tb *b, int64_t n)
+{
+  k_convert<Ta, Tb><<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(a, b, n);
+  TOFE_LAUNCH_CHECK();
+}
+
+void fill_d(hipStream_t s, double *a, double v, int64_t n)
+{
+  k_fill<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(a, v, n);
+  TOFE_LAUNCH_CHECK();
+}
+
+void cg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
+                  int64_t n, const double *scal, int alpha_slot, double *partial)
+{
+  k_cg_xr<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(x, r, p, q, n, scal, alpha_slot, partial);
+  TOFE_LAUNCH_CHECK();
+}
+
+void cg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *scal,
+                 int beta_slot)
+{
+  k_cg_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, n, scal, beta_slot);
+  TOFE_LAUNCH_CHECK();
+}
+
+void power_start(hipStream_t s, double *v, int64_t n)
+{
+  k_power_start<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(v, n);
+  TOFE_LAUNCH_CHECK();
+}
+
+void jacobi_dot(hipStream_t s, const double *r, const double *dinv, double *z, int64_t n,
+                double *partial)
+{
+  k_jacobi_dot<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, dinv, z, n, partial);
+  TOFE_LAUNCH_CHECK();
+}
+
+void record_scalar(hipStream_t s, const double *scal, int slot, double *dst)
+{
+  k_record<<<dim3(1), dim3(64), 0, s>>>(scal, slot, dst);
+  TOFE_LAUNCH_CHECK();
+}
+
+#define TOFE_INST_T(T)                                                                      \
+  template void dot_partial<T>(hipStream_t, const T *, const T *, int64_t, double *);       \
+  template void cheb_zero<T>(hipStream_t, const T *, const T *, T, T *, T *, int64_t);      \
+  template void restrict_op<T>(hipStream_t, const T *, int, int, int, T *, int, int, int);  \
+  template void prolong_add<T>(hipStream_t, const T *, int, int, int, T *, int, int, int);  \
+
+TOFE_INST_T(double)
+TOFE_INST_T(float)
+template void convert<double, float>(hipStream_t, const double *, float *, int64_t);
+template void convert<float, double>(hipStream_t, const float *, double *, int64_t);
+
+}  // namespace tofe

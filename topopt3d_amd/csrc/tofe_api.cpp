@@ -1,0 +1,1064 @@
+// C ABI of libtofe: context life cycle, design update, solver set-up and the
+// multigrid-preconditioned CG solve on one GPU.
+//
+// FP64 path: CG (KSPCG semantics, FEAnalysis.cpp:387-395) preconditioned by a Galerkin
+// V-cycle with Chebyshev-Jacobi smoothing and an FP64 dense coarse solve.
+// Mixed path: the same V-cycle in FP32 (FP32 operator, stencils, vectors) inside an
+// FP64 flexible CG (truncated FCG(1)); the outer residual, the dot products and the
+// coarse solve stay FP64.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "host_setup.hpp"
+#include "tofe_internal.hpp"
+
+using namespace tofe;
+
+namespace {
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void alloc(size_t count)
+  {
+    if (count == n && p) return;
+    release();
+    if (count) {
+      TOFE_HIP(hipMalloc(&p, count * sizeof(T)));
+      n = count;
+    }
+  }
+  void zero(hipStream_t s)
+  {
+    if (p) TOFE_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct Grid {
+  int Nx = 0, Ny = 0, Nz = 0;
+  int64_t nn() const { return (int64_t)Nx * Ny * Nz; }
+  int64_t ndof() const { return 3 * nn(); }
+};
+
+template <typename T>
+struct Level {
+  Grid g;
+  DBuf<T> S;     // 27-point block stencil (levels >= 1)
+  DBuf<T> dinv;  // inverse Jacobi diagonal
+  DBuf<T> b, x, x2, r, d;
+};
+
+constexpr int kMaxEig = 64;
+
+double now_s()
+{
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct tofe_ctx {
+  tofe_desc desc{};
+  int device = 0;
+  int L = 1;
+  std::vector<Grid> grids;
+  int64_t ne = 0, nn = 0, ndof = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle rb = nullptr;
+  double KE[576];
+  std::vector<int> tpl_off;
+  std::vector<double> tpl_w;
+
+  DBuf<double> dKE;
+  DBuf<float> dKE32;
+  DBuf<uint8_t> dfix;
+  bool has_fix = false;
+  DBuf<int> dtpl_off;
+  DBuf<double> dtpl_w, filt_inv;
+  DBuf<double> x, rho, E, dEdz, Es, dEsdz;
+  DBuf<float> E32;
+  bool design_set = false;
+
+  std::vector<std::unique_ptr<Level<double>>> lv64;
+  std::vector<std::unique_ptr<Level<float>>> lv32;
+  DBuf<double> fineS, diag0;
+  DBuf<double> Ainv;
+  DBuf<int> dinfo;
+  int64_t nc = 0;
+  DBuf<double> partial, partial2, scal, eig_ab;
+  int npart = 0;
+  std::vector<double> lam;
+  std::vector<double> c1, c2;  // Chebyshev coefficients, per level * kmax
+
+  DBuf<double> vr, vz, vp, vq, vF, vU;
+  DBuf<float> vr32, vz32;
+  DBuf<double> ebuf, ebuf2;
+
+  bool assembled = false;
+  int precision = TOFE_PREC_FP64;
+  int built_precision = -1;
+  tofe_stats stats{};
+  std::string err;
+
+  int kmax() const { return std::max(std::max(desc.nu_pre, desc.nu_post), 1); }
+};
+
+namespace {
+
+FilterTemplate filter_tpl(const tofe_ctx *c)
+{
+  FilterTemplate t;
+  t.off = c->dtpl_off.p;
+  t.w = c->dtpl_w.p;
+  t.ntpl = (int)c->tpl_w.size();
+  t.nx = c->desc.nx;
+  t.ny = c->desc.ny;
+  t.nz = c->desc.nz;
+  return t;
+}
+
+template <typename T>
+std::vector<std::unique_ptr<Level<T>>> &levels(tofe_ctx *c);
+template <>
+std::vector<std::unique_ptr<Level<double>>> &levels<double>(tofe_ctx *c) { return c->lv64; }
+template <>
+std::vector<std::unique_ptr<Level<float>>> &levels<float>(tofe_ctx *c) { return c->lv32; }
+
+template <typename T>
+FineOp<T> fine_of(tofe_ctx *c);
+template <>
+FineOp<double> fine_of<double>(tofe_ctx *c)
+{
+  FineOp<double> f;
+  f.E = c->E.p;
+  f.fix = c->has_fix ? c->dfix.p : nullptr;
+  f.KE = c->dKE.p;
+  f.nx = c->desc.nx; f.ny = c->desc.ny; f.nz = c->desc.nz;
+  return f;
+}
+template <>
+FineOp<float> fine_of<float>(tofe_ctx *c)
+{
+  FineOp<float> f;
+  f.E = c->E32.p;
+  f.fix = c->has_fix ? c->dfix.p : nullptr;
+  f.KE = c->dKE32.p;
+  f.nx = c->desc.nx; f.ny = c->desc.ny; f.nz = c->desc.nz;
+  return f;
+}
+
+template <typename T>
+void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
+{
+  if (l == 0) {
+    fine_op<T>(c->stream, fine_of<T>(c), u, epi, ep);
+  } else {
+    StencilOp<T> op;
+    auto &lv = *levels<T>(c)[l];
+    op.S = lv.S.p;
+    op.Nx = lv.g.Nx; op.Ny = lv.g.Ny; op.Nz = lv.g.Nz;
+    stencil_op<T>(c->stream, op, u, epi, ep);
+  }
+}
+
+template <typename T>
+int level_blocks(tofe_ctx *c, int l)
+{
+  if (l == 0) return fine_op_blocks<T>(fine_of<T>(c));
+  StencilOp<T> op;
+  op.Nx = c->grids[l].Nx; op.Ny = c->grids[l].Ny; op.Nz = c->grids[l].Nz;
+  return stencil_op_blocks<T>(op);
+}
+
+void alloc_level(tofe_ctx *c, Level<double> &lv, int l)
+{
+  lv.g = c->grids[l];
+  const int64_t nd = lv.g.ndof();
+  if (l > 0) lv.S.alloc((size_t)243 * lv.g.nn());
+  lv.dinv.alloc(nd);
+  if (l > 0) lv.b.alloc(nd);
+  lv.x.alloc(nd);
+  lv.x2.alloc(nd);
+  lv.r.alloc(nd);
+  lv.d.alloc(nd);
+  lv.d.zero(c->stream);
+}
+
+void alloc_level32(tofe_ctx *c, Level<float> &lv, int l)
+{
+  lv.g = c->grids[l];
+  const int64_t nd = lv.g.ndof();
+  if (l > 0) lv.S.alloc((size_t)243 * lv.g.nn());
+  lv.dinv.alloc(nd);
+  lv.b.alloc(nd);
+  lv.x.alloc(nd);
+  lv.x2.alloc(nd);
+  lv.r.alloc(nd);
+  lv.d.alloc(nd);
+  lv.d.zero(c->stream);
+}
+
+// Multigrid V-cycle: x0 = M^-1 b0 (PCMG V-cycle, Chebyshev-Jacobi smoothers,
+// Galerkin coarse operators, direct coarse solve).
+template <typename T>
+void vcycle(tofe_ctx *c, const T *b0, T *x0)
+{
+  auto &LV = levels<T>(c);
+  const int L = c->L;
+  const hipStream_t s = c->stream;
+  if (L == 1) {
+    dense_gemv<T, T>(s, c->Ainv.p, c->nc, b0, x0);
+    return;
+  }
+  const int nu1 = c->desc.nu_pre, nu2 = c->desc.nu_post, km = c->kmax();
+  std::vector<T *> cur(L, nullptr), oth(L, nullptr);
+  for (int l = 0; l < L - 1; ++l) {
+    Level<T> &lv = *LV[l];
+    const T *b = (l == 0) ? b0 : lv.b.p;
+    T *xa = (l == 0) ? x0 : lv.x.p, *xb = lv.x2.p;
+    const int flips = std::max(nu1 - 1, 0) + nu2;
+    T *xc = (flips & 1) ? xb : xa, *xo = (xc == xa) ? xb : xa;
+    const int64_t nd = lv.g.ndof();
+    const double *c1 = &c->c1[(size_t)l * km], *c2 = &c->c2[(size_t)l * km];
+    if (nu1 > 0) {
+      cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd);
+      for (int k = 1; k < nu1; ++k) {
+        EpiArgs<T> ep;
+        ep.b = b; ep.dinv = lv.dinv.p; ep.d = lv.d.p; ep.xout = xo;
+        ep.c1 = (T)c1[k]; ep.c2 = (T)c2[k];
+        level_apply<T>(c, l, xc, EPI_CHEB, ep);
+        std::swap(xc, xo);
+      }
+    } else {
+      TOFE_HIP(hipMemsetAsync(xc, 0, nd * sizeof(T), s));
+    }
+    EpiArgs<T> er;
+    er.y = lv.r.p; er.b = b;
+    level_apply<T>(c, l, xc, EPI_RESID, er);
+    const Grid &gf = lv.g, &gc = LV[l + 1]->g;
+    restrict_op<T>(s, lv.r.p, gf.Nx, gf.Ny, gf.Nz, LV[l + 1]->b.p, gc.Nx, gc.Ny, gc.Nz);
+    cur[l] = xc;
+    oth[l] = xo;
+  }
+  Level<T> &lc = *LV[L - 1];
+  dense_gemv<T, T>(s, c->Ainv.p, c->nc, lc.b.p, lc.x.p);
+  cur[L - 1] = lc.x.p;
+  for (int l = L - 2; l >= 0; --l) {
+    Level<T> &lv = *LV[l];
+    const T *b = (l == 0) ? b0 : lv.b.p;
+    T *xc = cur[l], *xo = oth[l];
+    const Grid &gf = lv.g, &gc = LV[l + 1]->g;
+    prolong_add<T>(s, cur[l + 1], gc.Nx, gc.Ny, gc.Nz, xc, gf.Nx, gf.Ny, gf.Nz);
+    const double *c1 = &c->c1[(size_t)l * km], *c2 = &c->c2[(size_t)l * km];
+    for (int k = 0; k < nu2; ++k) {
+      EpiArgs<T> ep;
+      ep.b = b; ep.dinv = lv.dinv.p; ep.d = lv.d.p; ep.xout = xo;
+      ep.c1 = (T)(k == 0 ? 0.0 : c1[k]); ep.c2 = (T)c2[k];
+      level_apply<T>(c, l, xc, EPI_CHEB, ep);
+      std::swap(xc, xo);
+    }
+    cur[l] = xc;
+  }
+  if (cur[0] != x0) TOFE_HIP(hipMemcpyAsync(x0, cur[0], c->ndof * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+
+void precondition(tofe_ctx *c, const double *r, double *z)
+{
+  if (c->built_precision == TOFE_PREC_MIXED) {
+    convert<double, float>(c->stream, r, c->vr32.p, c->ndof);
+    vcycle<float>(c, c->vr32.p, c->vz32.p);
+    convert<float, double>(c->stream, c->vz32.p, z, c->ndof);
+  } else {
+    vcycle<double>(c, r, z);
+  }
+}
+
+void dot_into(tofe_ctx *c, const double *a, const double *b, int64_t n, int slot, int post = POST_NONE)
+{
+  dot_partial<double>(c->stream, a, b, n, c->partial.p);
+  reduce_final(c->stream, c->partial.p, kRedBlocks, c->scal.p + slot, post, c->scal.p);
+}
+
+void read_scal(tofe_ctx *c, double *h)
+{
+  TOFE_HIP(hipMemcpyAsync(h, c->scal.p, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TOFE_HIP(hipStreamSynchronize(c->stream));
+}
+
+// CG / flexible CG on the fine level with the KSPConvergedDefault test
+// ||r|| <= max(rtol ||b||, 1e-50) (FEAnalysis.cpp:390-393).
+int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, int *its_out,
+              double *rn_out, double *bn_out)
+{
+  const hipStream_t s = c->stream;
+  const int64_t n = c->ndof;
+  double *r = c->vr.p, *z = c->vz.p, *p = c->vp.p, *q = c->vq.p;
+  const bool flex = c->built_precision == TOFE_PREC_MIXED;
+  const int nfb = level_blocks<double>(c, 0);
+  double h[SL_COUNT];
+
+  EpiArgs<double> er;
+  er.y = r; er.b = F;
+  level_apply<double>(c, 0, U, EPI_RESID, er);
+  dot_into(c, F, F, n, SL_BB);
+  dot_into(c, r, r, n, SL_RR);
+  read_scal(c, h);
+  const double bnorm = std::sqrt(h[SL_BB]);
+  double rn = std::sqrt(h[SL_RR]);
+  const double rnorm0 = bnorm > 0 ? bnorm : rn;
+  const double ttol = std::max(rtol * rnorm0, 1e-50);
+  const int conv = (rtol * rnorm0 > 1e-50) ? TOFE_CONVERGED_RTOL : TOFE_CONVERGED_ATOL;
+  *bn_out = bnorm;
+  int k = 0, reason = 0;
+  if (std::isnan(rn) || std::isinf(rn)) reason = TOFE_DIVERGED_NANORINF;
+  else if (rn <= ttol) reason = conv;
+  if (!reason) {
+    precondition(c, r, z);
+    dot_into(c, r, z, n, SL_RZ);
+    TOFE_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  while (!reason) {
+    if (k >= maxit) { reason = TOFE_DIVERGED_ITS; break; }
+    EpiArgs<double> ep;
+    ep.y = q; ep.partial = c->partial.p;
+    if (flex) {
+      ep.b = r;
+      ep.partial2 = c->partial2.p;
+      level_apply<double>(c, 0, p, EPI_APPLY_DOT2, ep);
+      reduce_final(s, c->partial2.p, nfb, c->scal.p + SL_PR, POST_NONE, c->scal.p);
+      reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_FCG, c->scal.p);
+    } else {
+      level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
+      reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p);
+    }
+    cg_update_xr(s, U, r, p, q, n, c->scal.p, SL_ALPHA, c->partial.p);
+    reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p);
+    read_scal(c, h);
+    ++k;
+    if (!flex && h[SL_RZ] < 0) { reason = TOFE_DIVERGED_INDEFINITE_PC; break; }
+    if (!(h[SL_PQ] > 0)) {
+      reason = std::isnan(h[SL_PQ]) ? TOFE_DIVERGED_NANORINF : TOFE_DIVERGED_INDEFINITE_MAT;
+      break;
+    }
+    rn = std::sqrt(h[SL_RR]);
+    if (std::isnan(rn) || std::isinf(rn)) { reason = TOFE_DIVERGED_NANORINF; break; }
+    if (rn <= ttol) { reason = conv; break; }
+    if (rn >= 1e5 * rnorm0) { reason = TOFE_DIVERGED_DTOL; break; }
+    precondition(c, r, z);
+    if (flex) dot_into(c, z, q, n, SL_ZQ, POST_BETA_FCG);
+    else dot_into(c, r, z, n, SL_TMP, POST_BETA_CG);
+    cg_update_p(s, p, z, n, c->scal.p, SL_BETA);
+  }
+  *its_out = k;
+  *rn_out = rn;
+  return reason;
+}
+
+// lambda_max(D^-1 A) of level l from eig_its steps of Jacobi-preconditioned CG: the CG
+// coefficients alpha_k, beta_k (recorded on the device, read back once per assemble)
+// form the Lanczos tridiagonal whose top eigenvalue is the estimate; PETSc's Chebyshev
+// smoother estimates the same quantity (KSPChebyshevEstEigSet 0, 0.1, 0, 1.1).
+void lanczos_coefficients(tofe_ctx *c, int l)
+{
+  const hipStream_t s = c->stream;
+  Level<double> &lv = *c->lv64[l];
+  const int64_t nd = lv.g.ndof();
+  double *r = c->vr.p, *z = c->vz.p, *p = c->vp.p, *q = c->vq.p, *x = c->vU.p;
+  const int nb = level_blocks<double>(c, l);
+  const int its = std::min(c->desc.eig_its, kMaxEig);
+  power_start(s, r, nd);
+  jacobi_dot(s, r, lv.dinv.p, z, nd, c->partial.p);
+  reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RZ, POST_NONE, c->scal.p);
+  TOFE_HIP(hipMemcpyAsync(p, z, nd * sizeof(double), hipMemcpyDeviceToDevice, s));
+  for (int k = 0; k < its; ++k) {
+    EpiArgs<double> ep;
+    ep.y = q;
+    ep.partial = c->partial.p;
+    level_apply<double>(c, l, p, EPI_APPLY_DOT, ep);
+    reduce_final(s, c->partial.p, nb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p);
+    record_scalar(s, c->scal.p, SL_ALPHA, c->eig_ab.p + (size_t)(2 * l) * kMaxEig + k);
+    cg_update_xr(s, x, r, p, q, nd, c->scal.p, SL_ALPHA, c->partial2.p);
+    jacobi_dot(s, r, lv.dinv.p, z, nd, c->partial.p);
+    reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_TMP, POST_BETA_CG, c->scal.p);
+    record_scalar(s, c->scal.p, SL_BETA, c->eig_ab.p + (size_t)(2 * l + 1) * kMaxEig + k);
+    cg_update_p(s, p, z, nd, c->scal.p, SL_BETA);
+  }
+}
+
+void lanczos_lambdas(tofe_ctx *c)
+{
+  const int L = c->L;
+  const int its = std::min(c->desc.eig_its, kMaxEig);
+  c->eig_ab.alloc((size_t)2 * L * kMaxEig);
+  for (int l = 0; l + 1 < L; ++l) lanczos_coefficients(c, l);
+  std::vector<double> ab((size_t)2 * L * kMaxEig);
+  TOFE_HIP(hipMemcpyAsync(ab.data(), c->eig_ab.p, ab.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  TOFE_HIP(hipStreamSynchronize(c->stream));
+  c->lam.assign(L, 0.0);
+  for (int l = 0; l + 1 < L; ++l) {
+    const double *alpha = &ab[(size_t)(2 * l) * kMaxEig], *beta = &ab[(size_t)(2 * l + 1) * kMaxEig];
+    std::vector<double> ta, tb;
+    for (int k = 0; k < its; ++k) {
+      if (!(alpha[k] > 0) || !(beta[k] >= 0) || !std::isfinite(alpha[k]) || !std::isfinite(beta[k])) break;
+      ta.push_back(1.0 / alpha[k] + (k > 0 ? beta[k - 1] / alpha[k - 1] : 0.0));
+      tb.push_back(std::sqrt(beta[k]) / alpha[k]);
+    }
+    c->lam[l] = ta.empty() ? 1.0 : tridiag_max_eig((int)ta.size(), ta.data(), tb.data());
+  }
+}
+
+void chebyshev_coefficients(tofe_ctx *c)
+{
+  const int km = c->kmax();
+  c->c1.assign((size_t)c->L * km, 0.0);
+  c->c2.assign((size_t)c->L * km, 0.0);
+  for (int l = 0; l + 1 < c->L; ++l) {
+    const double emax = c->desc.cheb_hi * c->lam[l], emin = c->desc.cheb_lo * c->lam[l];
+    const double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin);
+    const double sigma = theta / delta;
+    double rho_old = 1.0 / sigma;
+    for (int k = 0; k < km; ++k) {
+      double a = 0.0, b = 1.0 / theta;
+      if (c->desc.smoother == TOFE_SMOOTHER_CHEBYSHEV && k > 0) {
+        const double rho = 1.0 / (2 * sigma - rho_old);
+        a = rho * rho_old;
+        b = 2 * rho / delta;
+        rho_old = rho;
+      }
+      c->c1[(size_t)l * km + k] = a;
+      c->c2[(size_t)l * km + k] = b;
+    }
+  }
+}
+
+void assemble(tofe_ctx *c)
+{
+  const double t0 = now_s();
+  const hipStream_t s = c->stream;
+  const int L = c->L;
+  // fine level: assembled stencil of K (for the Galerkin product and the diagonal)
+  c->fineS.alloc((size_t)243 * c->nn);
+  c->diag0.alloc(c->ndof);
+  fine_stencil_build(s, fine_of<double>(c), c->fineS.p);
+  stencil_diag(s, c->fineS.p, c->nn, c->diag0.p, c->lv64[0]->dinv.p);
+  for (int l = 0; l + 1 < L; ++l) {
+    const double *Sf = (l == 0) ? c->fineS.p : c->lv64[l]->S.p;
+    const Grid &gf = c->grids[l], &gc = c->grids[l + 1];
+    galerkin(s, Sf, gf.Nx, gf.Ny, gf.Nz, c->lv64[l + 1]->S.p, gc.Nx, gc.Ny, gc.Nz);
+    stencil_diag(s, c->lv64[l + 1]->S.p, gc.nn(), nullptr, c->lv64[l + 1]->dinv.p);
+  }
+  // coarsest level: dense Cholesky factor and explicit inverse (FP64)
+  const Grid &gL = c->grids[L - 1];
+  const double *SL = (L == 1) ? c->fineS.p : c->lv64[L - 1]->S.p;
+  c->nc = gL.ndof();
+  c->Ainv.alloc((size_t)c->nc * c->nc);
+  c->dinfo.alloc(2);
+  stencil_to_dense(s, SL, gL.Nx, gL.Ny, gL.Nz, c->Ainv.p);
+  rocblas_status st = rocsolver_dpotrf(c->rb, rocblas_fill_lower, (rocblas_int)c->nc, c->Ainv.p,
+                                       (rocblas_int)c->nc, c->dinfo.p);
+  if (st != rocblas_status_success) throw Error(TOFE_ERR_HIP, "rocsolver_dpotrf failed");
+  int info[2] = {0, 0};
+  TOFE_HIP(hipMemcpyAsync(info, c->dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  TOFE_HIP(hipStreamSynchronize(s));
+  if (info[0] != 0)
+    throw Error(TOFE_ERR_NUMERIC, "coarse operator is not positive definite (potrf info " +
+                                      std::to_string(info[0]) + ")");
+  st = rocsolver_dpotri(c->rb, rocblas_fill_lower, (rocblas_int)c->nc, c->Ainv.p,
+                        (rocblas_int)c->nc, c->dinfo.p);
+  if (st != rocblas_status_success) throw Error(TOFE_ERR_HIP, "rocsolver_dpotri failed");
+  symmetrize_lower(s, c->Ainv.p, c->nc);
+  // Chebyshev bounds from lambda_max(D^-1 A) per smoothed level
+  lanczos_lambdas(c);
+  chebyshev_coefficients(c);
+  // FP32 copies for the mixed-precision V-cycle
+  if (c->precision == TOFE_PREC_MIXED) {
+    if (c->lv32.empty()) {
+      for (int l = 0; l < L; ++l) {
+        c->lv32.emplace_back(new Level<float>());
+        alloc_level32(c, *c->lv32.back(), l);
+      }
+      c->vr32.alloc(c->ndof);
+      c->vz32.alloc(c->ndof);
+    }
+    for (int l = 0; l < L; ++l) {
+      const int64_t nd = c->grids[l].ndof();
+      convert<double, float>(s, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, nd);
+      if (l > 0)
+        convert<double, float>(s, c->lv64[l]->S.p, c->lv32[l]->S.p, (int64_t)243 * c->grids[l].nn());
+    }
+    convert<double, float>(s, c->E.p, c->E32.p, c->ne);
+  }
+  TOFE_HIP(hipStreamSynchronize(s));
+  c->built_precision = c->precision;
+  c->assembled = true;
+  c->stats.setup_seconds = now_s() - t0;
+}
+
+void check_ctx(const tofe_ctx *c)
+{
+  if (!c) throw Error(TOFE_ERR_ARG, "null context");
+  TOFE_HIP(hipSetDevice(c->device));
+}
+
+template <typename F>
+int guarded(tofe_ctx *c, F &&f)
+{
+  if (!c) return TOFE_ERR_ARG;
+  try {
+    check_ctx(c);
+    f();
+    return TOFE_OK;
+  } catch (const Error &e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    c->err = "host allocation failed";
+    return TOFE_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return TOFE_ERR_HIP;
+  }
+}
+
+void upload_design(tofe_ctx *c, const double *dx)
+{
+  const hipStream_t s = c->stream;
+  const FilterTemplate t = filter_tpl(c);
+  if (dx != c->x.p)
+    TOFE_HIP(hipMemcpyAsync(c->x.p, dx, c->ne * sizeof(double), hipMemcpyDeviceToDevice, s));
+  filter_apply(s, t, c->filt_inv.p, c->x.p, c->rho.p);
+  simp(s, c->rho.p, c->ne, c->desc.penal, c->desc.Emin, c->E.p, c->dEdz.p, c->Es.p, c->dEsdz.p,
+       c->E32.p);
+  TOFE_HIP(hipStreamSynchronize(s));
+  c->design_set = true;
+  c->assembled = false;
+}
+
+void need_assembled(tofe_ctx *c)
+{
+  if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set (call tofe_set_design or tofe_set_E)");
+  if (!c->assembled || c->built_precision != c->precision)
+    throw Error(TOFE_ERR_STATE, "operator not assembled for the current design (call tofe_assemble)");
+}
+
+void solve_dev(tofe_ctx *c, const double *dF, double *dU, int nrhs, double rtol, int maxit,
+               int *its, int *reason)
+{
+  need_assembled(c);
+  if (nrhs < 1 || !dF || !dU) throw Error(TOFE_ERR_ARG, "bad solve arguments");
+  const double t0 = now_s();
+  int k = 0, rs = 0;
+  double rn = 0, bn = 0;
+  for (int i = 0; i < nrhs; ++i)
+    rs = solve_one(c, dF + (size_t)i * c->ndof, dU + (size_t)i * c->ndof, rtol, maxit, &k, &rn, &bn);
+  TOFE_HIP(hipStreamSynchronize(c->stream));
+  c->stats.solve_seconds = now_s() - t0;
+  c->stats.its = k;
+  c->stats.reason = rs;
+  c->stats.rnorm = rn;
+  c->stats.bnorm = bn;
+  if (its) *its = k;
+  if (reason) *reason = rs;
+}
+
+void h2d(tofe_ctx *c, double *d, const double *h, size_t n)
+{
+  TOFE_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+}
+void d2h(tofe_ctx *c, double *h, const double *d, size_t n)
+{
+  TOFE_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TOFE_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int tofe_create(const tofe_desc *desc, tofe_ctx **out)
+{
+  if (!desc || !out) return TOFE_ERR_ARG;
+  *out = nullptr;
+  std::unique_ptr<tofe_ctx> c(new tofe_ctx());
+  c->desc = *desc;
+  tofe_desc &d = c->desc;
+  if (d.nx < 1 || d.ny < 1 || d.nz < 1 || !(d.hx > 0) || !(d.hy > 0) || !(d.hz > 0) ||
+      !(d.E0 > 0) || !(d.nu > -1 && d.nu < 0.5) || d.Emin < 0 || !(d.rmin > 0)) {
+    return TOFE_ERR_ARG;
+  }
+  if (d.nu_pre <= 0 && d.nu_post <= 0) d.nu_pre = d.nu_post = 2;
+  if (d.nu_pre < 0) d.nu_pre = 0;
+  if (d.nu_post < 0) d.nu_post = 0;
+  if (!(d.cheb_hi > 0)) d.cheb_hi = 1.1;
+  if (!(d.cheb_lo > 0)) d.cheb_lo = 0.1;
+  if (d.eig_its <= 0) d.eig_its = 10;
+  if (d.penal == 0) d.penal = 3.0;
+  d.fixed_dof = nullptr;
+  c->precision = (desc->precision == TOFE_PREC_MIXED) ? TOFE_PREC_MIXED : TOFE_PREC_FP64;
+  try {
+    TOFE_HIP(hipGetDevice(&c->device));
+    // level count: floor(log2(min nel)) unless given (RecMesh.cpp:552-554), truncated
+    // once a coarse grid has <= 10 DOFs (Interpolation.cpp:70-71)
+    int L = d.mg_levels;
+    if (L <= 0) L = (int)std::floor(std::log2((double)std::min(d.nx, std::min(d.ny, d.nz))));
+    if (L < 1) L = 1;
+    if (L > 16) L = 16;
+    Grid g;
+    g.Nx = d.nx + 1; g.Ny = d.ny + 1; g.Nz = d.nz + 1;
+    c->grids.push_back(g);
+    for (int i = 0; i + 1 < L; ++i) {
+      Grid gc;
+      gc.Nx = (g.Nx + 1) / 2; gc.Ny = (g.Ny + 1) / 2; gc.Nz = (g.Nz + 1) / 2;
+      c->grids.push_back(gc);
+      g = gc;
+      if (3 * gc.nn() <= 10) break;
+    }
+    c->L = (int)c->grids.size();
+    d.mg_levels = c->L;
+    c->ne = (int64_t)d.nx * d.ny * d.nz;
+    c->nn = c->grids[0].nn();
+    c->ndof = 3 * c->nn;
+    if (c->grids[c->L - 1].ndof() > 20000)
+      throw Error(TOFE_ERR_ARG, "coarsest level has " + std::to_string(c->grids[c->L - 1].ndof()) +
+                                    " DOFs; use more multigrid levels");
+
+    TOFE_HIP(hipStreamCreate(&c->stream));
+    if (rocblas_create_handle(&c->rb) != rocblas_status_success)
+      throw Error(TOFE_ERR_HIP, "rocblas_create_handle failed");
+    rocblas_set_stream(c->rb, c->stream);
+
+    compute_ke(d.E0, d.nu, d.hx, d.hy, d.hz, c->KE);
+    float ke32[576];
+    for (int i = 0; i < 576; ++i) ke32[i] = (float)c->KE[i];
+    c->dKE.alloc(576);
+    c->dKE32.alloc(576);
+    h2d(c.get(), c->dKE.p, c->KE, 576);
+    TOFE_HIP(hipMemcpyAsync(c->dKE32.p, ke32, sizeof(ke32), hipMemcpyHostToDevice, c->stream));
+
+    if (filter_template(d.hx, d.hy, d.hz, d.rmin, c->tpl_off, c->tpl_w) < 1 || c->tpl_w.size() > 1024)
+      throw Error(TOFE_ERR_ARG, "filter radius gives an unsupported neighbourhood");
+    c->dtpl_off.alloc(c->tpl_off.size());
+    c->dtpl_w.alloc(c->tpl_w.size());
+    TOFE_HIP(hipMemcpyAsync(c->dtpl_off.p, c->tpl_off.data(), c->tpl_off.size() * sizeof(int),
+                            hipMemcpyHostToDevice, c->stream));
+    h2d(c.get(), c->dtpl_w.p, c->tpl_w.data(), c->tpl_w.size());
+    c->filt_inv.alloc(c->ne);
+    filter_rowsum_inv(c->stream, filter_tpl(c.get()), c->filt_inv.p);
+
+    c->dfix.alloc(c->nn);
+    if (desc->fixed_dof) {
+      std::vector<uint8_t> m(c->nn, 0);
+      for (int64_t n = 0; n < c->nn; ++n)
+        for (int a = 0; a < 3; ++a)
+          if (desc->fixed_dof[3 * n + a]) m[n] |= (uint8_t)(1u << a);
+      TOFE_HIP(hipMemcpyAsync(c->dfix.p, m.data(), c->nn, hipMemcpyHostToDevice, c->stream));
+      TOFE_HIP(hipStreamSynchronize(c->stream));
+      c->has_fix = true;
+    } else {
+      c->dfix.zero(c->stream);
+    }
+
+    for (auto *b : {&c->x, &c->rho, &c->E, &c->dEdz, &c->Es, &c->dEsdz, &c->ebuf, &c->ebuf2})
+      b->alloc(c->ne);
+    c->E32.alloc(c->ne);
+    for (int l = 0; l < c->L; ++l) {
+      c->lv64.emplace_back(new Level<double>());
+      alloc_level(c.get(), *c->lv64.back(), l);
+    }
+    for (auto *b : {&c->vr, &c->vz, &c->vp, &c->vq, &c->vF, &c->vU}) b->alloc(c->ndof);
+    int np = kRedBlocks;
+    for (int l = 0; l < c->L; ++l) {
+      np = std::max(np, level_blocks<double>(c.get(), l));
+      np = std::max(np, level_blocks<float>(c.get(), l));
+    }
+    c->npart = np;
+    c->partial.alloc(np);
+    c->partial2.alloc(np);
+    c->scal.alloc(SL_COUNT);
+    c->scal.zero(c->stream);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+    c->stats.levels = c->L;
+    c->stats.coarse_size = c->grids[c->L - 1].ndof();
+  } catch (const Error &e) {
+    return e.code;
+  } catch (...) {
+    return TOFE_ERR_HIP;
+  }
+  *out = c.release();
+  return TOFE_OK;
+}
+
+int tofe_destroy(tofe_ctx *c)
+{
+  if (!c) return TOFE_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->rb) rocblas_destroy_handle(c->rb);
+  hipStream_t s = c->stream;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+  return TOFE_OK;
+}
+
+const char *tofe_last_error(const tofe_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int tofe_sizes(const tofe_ctx *c, int64_t *nelem, int64_t *nnode, int64_t *ndof, int *levels)
+{
+  if (!c) return TOFE_ERR_ARG;
+  if (nelem) *nelem = c->ne;
+  if (nnode) *nnode = c->nn;
+  if (ndof) *ndof = c->ndof;
+  if (levels) *levels = c->L;
+  return TOFE_OK;
+}
+
+int tofe_get_ke(const tofe_ctx *c, double *ke)
+{
+  if (!c || !ke) return TOFE_ERR_ARG;
+  std::memcpy(ke, c->KE, sizeof(c->KE));
+  return TOFE_OK;
+}
+
+int tofe_set_design(tofe_ctx *c, const double *x)
+{
+  return guarded(c, [&] {
+    if (!x) throw Error(TOFE_ERR_ARG, "null design");
+    h2d(c, c->x.p, x, c->ne);
+    upload_design(c, c->x.p);
+  });
+}
+
+int tofe_set_design_dev(tofe_ctx *c, const double *dx)
+{
+  return guarded(c, [&] {
+    if (!dx) throw Error(TOFE_ERR_ARG, "null design");
+    upload_design(c, dx);
+  });
+}
+
+int tofe_set_E(tofe_ctx *c, const double *E)
+{
+  return guarded(c, [&] {
+    if (!E) throw Error(TOFE_ERR_ARG, "null E");
+    h2d(c, c->E.p, E, c->ne);
+    convert<double, float>(c->stream, c->E.p, c->E32.p, c->ne);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+    c->design_set = true;
+    c->assembled = false;
+  });
+}
+
+int tofe_get_field(const tofe_ctx *cc, int field, double *out)
+{
+  tofe_ctx *c = const_cast<tofe_ctx *>(cc);
+  return guarded(c, [&] {
+    if (!out) throw Error(TOFE_ERR_ARG, "null output");
+    const double *src = nullptr;
+    int64_t n = c->ne;
+    switch (field) {
+      case TOFE_FIELD_X: src = c->x.p; break;
+      case TOFE_FIELD_RHO: src = c->rho.p; break;
+      case TOFE_FIELD_V: src = c->rho.p; break;
+      case TOFE_FIELD_E: src = c->E.p; break;
+      case TOFE_FIELD_DEDZ: src = c->dEdz.p; break;
+      case TOFE_FIELD_ES: src = c->Es.p; break;
+      case TOFE_FIELD_DESDZ: src = c->dEsdz.p; break;
+      case TOFE_FIELD_DIAG:
+        if (!c->assembled) throw Error(TOFE_ERR_STATE, "diagonal needs tofe_assemble");
+        src = c->diag0.p;
+        n = c->ndof;
+        break;
+      default: throw Error(TOFE_ERR_ARG, "unknown field");
+    }
+    d2h(c, out, src, n);
+  });
+}
+
+int tofe_assemble(tofe_ctx *c)
+{
+  return guarded(c, [&] {
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    assemble(c);
+  });
+}
+
+int tofe_solve_dev(tofe_ctx *c, const double *dF, double *dU, int nrhs, double rtol, int maxit,
+                   int *its, int *reason)
+{
+  return guarded(c, [&] { solve_dev(c, dF, dU, nrhs, rtol, maxit, its, reason); });
+}
+
+int tofe_solve(tofe_ctx *c, const double *F, double *U, int nrhs, double rtol, int maxit,
+               int *its, int *reason)
+{
+  return guarded(c, [&] {
+    if (!F || !U || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad solve arguments");
+    int k = 0, rs = 0;
+    for (int i = 0; i < nrhs; ++i) {
+      h2d(c, c->vF.p, F + (size_t)i * c->ndof, c->ndof);
+      h2d(c, c->vU.p, U + (size_t)i * c->ndof, c->ndof);
+      solve_dev(c, c->vF.p, c->vU.p, 1, rtol, maxit, &k, &rs);
+      d2h(c, U + (size_t)i * c->ndof, c->vU.p, c->ndof);
+    }
+    if (its) *its = k;
+    if (reason) *reason = rs;
+  });
+}
+
+int tofe_apply_K_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
+{
+  return guarded(c, [&] {
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    if (!dX || !dY || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
+    for (int i = 0; i < nrhs; ++i) {
+      EpiArgs<double> ep;
+      ep.y = dY + (size_t)i * c->ndof;
+      level_apply<double>(c, 0, dX + (size_t)i * c->ndof, EPI_APPLY, ep);
+    }
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tofe_apply_K(tofe_ctx *c, const double *X, double *Y, int nrhs)
+{
+  return guarded(c, [&] {
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    if (!X || !Y || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
+    for (int i = 0; i < nrhs; ++i) {
+      h2d(c, c->vp.p, X + (size_t)i * c->ndof, c->ndof);
+      EpiArgs<double> ep;
+      ep.y = c->vq.p;
+      level_apply<double>(c, 0, c->vp.p, EPI_APPLY, ep);
+      d2h(c, Y + (size_t)i * c->ndof, c->vq.p, c->ndof);
+    }
+  });
+}
+
+int tofe_pc_apply_dev(tofe_ctx *c, const double *dR, double *dZ, int nrhs)
+{
+  return guarded(c, [&] {
+    need_assembled(c);
+    if (!dR || !dZ || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad pc arguments");
+    for (int i = 0; i < nrhs; ++i) precondition(c, dR + (size_t)i * c->ndof, dZ + (size_t)i * c->ndof);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tofe_pc_apply(tofe_ctx *c, const double *R, double *Z, int nrhs)
+{
+  return guarded(c, [&] {
+    need_assembled(c);
+    if (!R || !Z || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad pc arguments");
+    for (int i = 0; i < nrhs; ++i) {
+      h2d(c, c->vp.p, R + (size_t)i * c->ndof, c->ndof);
+      precondition(c, c->vp.p, c->vq.p);
+      d2h(c, Z + (size_t)i * c->ndof, c->vq.p, c->ndof);
+    }
+  });
+}
+
+int tofe_compliance_dev(tofe_ctx *c, const double *dU, const double *dF, double *cval,
+                        double *d_dcdx)
+{
+  return guarded(c, [&] {
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    if (!dU || !dF) throw Error(TOFE_ERR_ARG, "bad compliance arguments");
+    if (cval) {
+      dot_into(c, dU, dF, c->ndof, SL_TMP);
+      double h[SL_COUNT];
+      read_scal(c, h);
+      *cval = h[SL_TMP];
+    }
+    if (d_dcdx) {
+      sensitivity(c->stream, dU, c->dEdz.p, c->dKE.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ebuf.p);
+      filter_apply_T(c->stream, filter_tpl(c), c->filt_inv.p, c->ebuf.p, d_dcdx);
+    }
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tofe_compliance(tofe_ctx *c, const double *U, const double *F, double *cval, double *dcdx)
+{
+  return guarded(c, [&] {
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    if (!U || !F) throw Error(TOFE_ERR_ARG, "bad compliance arguments");
+    h2d(c, c->vp.p, U, c->ndof);
+    h2d(c, c->vq.p, F, c->ndof);
+    if (cval) {
+      dot_into(c, c->vp.p, c->vq.p, c->ndof, SL_TMP);
+      double h[SL_COUNT];
+      read_scal(c, h);
+      *cval = h[SL_TMP];
+    }
+    if (dcdx) {
+      sensitivity(c->stream, c->vp.p, c->dEdz.p, c->dKE.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ebuf.p);
+      filter_apply_T(c->stream, filter_tpl(c), c->filt_inv.p, c->ebuf.p, c->ebuf2.p);
+      d2h(c, dcdx, c->ebuf2.p, c->ne);
+    }
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tofe_sensitivity(tofe_ctx *c, const double *U, double *g)
+{
+  return guarded(c, [&] {
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    if (!U || !g) throw Error(TOFE_ERR_ARG, "bad arguments");
+    h2d(c, c->vp.p, U, c->ndof);
+    sensitivity(c->stream, c->vp.p, c->dEdz.p, c->dKE.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ebuf.p);
+    d2h(c, g, c->ebuf.p, c->ne);
+  });
+}
+
+int tofe_filter(tofe_ctx *c, const double *x, double *rho)
+{
+  return guarded(c, [&] {
+    if (!x || !rho) throw Error(TOFE_ERR_ARG, "bad arguments");
+    h2d(c, c->ebuf.p, x, c->ne);
+    filter_apply(c->stream, filter_tpl(c), c->filt_inv.p, c->ebuf.p, c->ebuf2.p);
+    d2h(c, rho, c->ebuf2.p, c->ne);
+  });
+}
+
+int tofe_filter_T(tofe_ctx *c, const double *g, double *gx)
+{
+  return guarded(c, [&] {
+    if (!g || !gx) throw Error(TOFE_ERR_ARG, "bad arguments");
+    h2d(c, c->ebuf.p, g, c->ne);
+    filter_apply_T(c->stream, filter_tpl(c), c->filt_inv.p, c->ebuf.p, c->ebuf2.p);
+    d2h(c, gx, c->ebuf2.p, c->ne);
+  });
+}
+
+int tofe_filter_pattern(tofe_ctx *c, int64_t *rowptr, int32_t *col, double *w)
+{
+  return guarded(c, [&] {
+    if (!rowptr) throw Error(TOFE_ERR_ARG, "null rowptr");
+    const int64_t ne = c->ne;
+    DBuf<int64_t> cnt;
+    cnt.alloc(ne + 1);
+    filter_count(c->stream, filter_tpl(c), cnt.p);
+    std::vector<int64_t> h(ne + 1);
+    TOFE_HIP(hipMemcpyAsync(h.data() + 1, cnt.p, ne * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+    h[0] = 0;
+    for (int64_t e = 0; e < ne; ++e) h[e + 1] += h[e];
+    std::memcpy(rowptr, h.data(), (ne + 1) * sizeof(int64_t));
+    if (col && w) {
+      const int64_t nnz = h[ne];
+      DBuf<int32_t> dcol;
+      DBuf<double> dw;
+      dcol.alloc(nnz);
+      dw.alloc(nnz);
+      TOFE_HIP(hipMemcpyAsync(cnt.p, h.data(), (ne + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+      filter_fill(c->stream, filter_tpl(c), c->filt_inv.p, cnt.p, dcol.p, dw.p);
+      TOFE_HIP(hipMemcpyAsync(col, dcol.p, nnz * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+      TOFE_HIP(hipMemcpyAsync(w, dw.p, nnz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      TOFE_HIP(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
+int tofe_set_precision(tofe_ctx *c, int precision)
+{
+  if (!c) return TOFE_ERR_ARG;
+  if (precision != TOFE_PREC_FP64 && precision != TOFE_PREC_MIXED) {
+    c->err = "unknown precision";
+    return TOFE_ERR_ARG;
+  }
+  c->precision = precision;
+  return TOFE_OK;
+}
+
+int tofe_get_stats(const tofe_ctx *c, tofe_stats *st)
+{
+  if (!c || !st) return TOFE_ERR_ARG;
+  *st = c->stats;
+  return TOFE_OK;
+}
+
+int tofe_get_level_stencil(tofe_ctx *c, int level, double *out, int64_t *nnode_l)
+{
+  return guarded(c, [&] {
+    if (level < 0 || level >= c->L) throw Error(TOFE_ERR_ARG, "bad level");
+    if (!c->assembled) throw Error(TOFE_ERR_STATE, "not assembled");
+    const int64_t nn = c->grids[level].nn();
+    if (nnode_l) *nnode_l = nn;
+    if (out) d2h(c, out, level == 0 ? c->fineS.p : c->lv64[level]->S.p, (size_t)243 * nn);
+  });
+}
+
+int tofe_time_operator(tofe_ctx *c, int level, int precision, int epilogue, int nrep, double *ms)
+{
+  return guarded(c, [&] {
+    if (level < 0 || level >= c->L || nrep < 1 || !ms) throw Error(TOFE_ERR_ARG, "bad arguments");
+    need_assembled(c);
+    const bool f32 = precision == TOFE_PREC_MIXED;
+    if (f32 && c->built_precision != TOFE_PREC_MIXED)
+      throw Error(TOFE_ERR_STATE, "FP32 kernels need an assemble in mixed precision");
+    const hipStream_t s = c->stream;
+    const int64_t nd = c->grids[level].ndof();
+    hipEvent_t e0, e1;
+    TOFE_HIP(hipEventCreate(&e0));
+    TOFE_HIP(hipEventCreate(&e1));
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      auto &lv = *levels<T>(c)[level];
+      Level<double> &l64 = *c->lv64[level];
+      power_start(s, l64.r.p, nd);
+      if constexpr (sizeof(T) == 4) convert<double, float>(s, l64.r.p, lv.x.p, nd);
+      else TOFE_HIP(hipMemcpyAsync(lv.x.p, l64.r.p, nd * sizeof(double), hipMemcpyDeviceToDevice, s));
+      TOFE_HIP(hipMemsetAsync(lv.d.p, 0, nd * sizeof(T), s));
+      EpiArgs<T> ep;
+      ep.y = lv.r.p;
+      ep.b = lv.x.p;
+      ep.dinv = lv.dinv.p;
+      ep.d = lv.d.p;
+      ep.xout = lv.x2.p;
+      ep.c1 = (T)0.5;
+      ep.c2 = (T)0.5;
+      const Epi epi = epilogue == 1 ? EPI_CHEB : EPI_APPLY;
+      level_apply<T>(c, level, lv.x.p, epi, ep);  // warm-up
+      TOFE_HIP(hipEventRecord(e0, s));
+      for (int k = 0; k < nrep; ++k) level_apply<T>(c, level, lv.x.p, epi, ep);
+      TOFE_HIP(hipEventRecord(e1, s));
+    };
+    if (f32) run(float{});
+    else run(double{});
+    TOFE_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    TOFE_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / nrep;
+  });
+}
+
+int tofe_get_level_lambda(const tofe_ctx *c, int level, double *lambda)
+{
+  if (!c || !lambda || level < 0 || level >= c->L) return TOFE_ERR_ARG;
+  if (!c->assembled) return TOFE_ERR_STATE;
+  *lambda = c->lam[level];
+  return TOFE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+// Internal declarations shared by the HIP translation units of libtofe.
+//
+// Data layout in HBM (one context, one GPU):
+//   vectors      3*nnode values, DOF = 3*node + component, nodes x-fastest;
+//   element data nelem values, elements x-fastest;
+//   fixed DOFs   one byte per node, bit c set when component c is a Dirichlet DOF;
+//   stencils     level >= 1 operators as 27-point 3x3-block stencils, structure of
+//                arrays: S[(o*9 + a*3 + b) * nnode + n] couples DOF (n,a) with DOF
+//                (n + off(o), b), o = (dx+1) + 3(dy+1) + 9(dz+1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/tofe.h"
+
+namespace tofe {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define TOFE_HIP(call)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw ::tofe::Error(TOFE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TOFE_LAUNCH_CHECK() TOFE_HIP(hipGetLastError())
+
+// Operator epilogues: what a K-apply kernel does with (Au)_i for DOF i.
+enum Epi : int {
+  EPI_APPLY = 0,      // y = Au
+  EPI_APPLY_DOT = 1,  // y = Au, partial += u.Au
+  EPI_RESID = 2,      // y = b - Au
+  EPI_CHEB = 3,       // r = dinv (b - Au); d = c1 d + c2 r; xout = u + d
+  EPI_APPLY_DOT2 = 4  // y = Au, partial += u.Au, partial2 += u.b
+};
+
+template <typename T>
+struct EpiArgs {
+  T *y = nullptr;
+  const T *b = nullptr;
+  const T *dinv = nullptr;
+  T *d = nullptr;
+  T *xout = nullptr;
+  double *partial = nullptr;
+  double *partial2 = nullptr;
+  T c1 = 0, c2 = 0;
+};
+
+// Matrix-free fine operator y = K(E) u with MatZeroRowsColumns Dirichlet rows/cols.
+template <typename T>
+struct FineOp {
+  const T *E = nullptr;        // nelem element stiffness scales
+  const uint8_t *fix = nullptr;// nnode fixed-DOF bitmasks (may be null)
+  const T *KE = nullptr;       // 576, row-major
+  int nx = 0, ny = 0, nz = 0;  // elements
+};
+
+// Stored 27-point block stencil operator of a coarse level.
+template <typename T>
+struct StencilOp {
+  const T *S = nullptr;        // 243 * nnode
+  int Nx = 0, Ny = 0, Nz = 0;  // nodes
+};
+
+constexpr int kRedBlocks = 1024;   // grid of the grid-stride reduction kernels
+
+// ---- fine operator (k_fine.hip) -------------------------------------------
+template <typename T>
+int fine_op_blocks(const FineOp<T> &op);
+template <typename T>
+void fine_op(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
+void fine_stencil_build(hipStream_t s, const FineOp<double> &op, double *S);
+
+// ---- coarse operators and Galerkin products (k_coarse.hip) ---------------
+template <typename T>
+int stencil_op_blocks(const StencilOp<T> &op);
+template <typename T>
+void stencil_op(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
+void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double *Sc, int Ncx,
+              int Ncy, int Ncz);
+void stencil_diag(hipStream_t s, const double *S, int64_t nn, double *diag, double *dinv);
+void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A);
+void symmetrize_lower(hipStream_t s, double *A, int64_t n);
+template <typename Tin, typename Tout>
+void dense_gemv(hipStream_t s, const double *A, int64_t n, const Tin *x, Tout *y);
+
+// ---- vector kernels (k_vec.hip) -------------------------------------------
+template <typename T>
+void dot_partial(hipStream_t s, const T *a, const T *b, int64_t n, double *partial);
+void reduce_final(hipStream_t s, const double *partial, int nparts, double *out, int post,
+                  double *scal);
+template <typename T>
+void cheb_zero(hipStream_t s, const T *b, const T *dinv, T c2, T *d, T *x, int64_t n);
+template <typename T>
+void restrict_op(hipStream_t s, const T *rf, int Nfx, int Nfy, int Nfz, T *bc, int Ncx, int Ncy,
+                 int Ncz);
+template <typename T>
+void prolong_add(hipStream_t s, const T *xc, int Ncx, int Ncy, int Ncz, T *xf, int Nfx, int Nfy,
+                 int Nfz);
+template <typename Ta, typename Tb>
+void convert(hipStream_t s, const Ta *a, Tb *b, int64_t n);
+void fill_d(hipStream_t s, double *a, double v, int64_t n);
+void cg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
+                  int64_t n, const double *scal, int alpha_slot, double *partial);
+void cg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *scal,
+                 int beta_slot);
+void power_start(hipStream_t s, double *v, int64_t n);
+void jacobi_dot(hipStream_t s, const double *r, const double *dinv, double *z, int64_t n,
+                double *partial);
+void record_scalar(hipStream_t s, const double *scal, int slot, double *dst);
+
+// device scalar slots (ctx->scal)
+enum Slot : int {
+  SL_RZ = 0, SL_PQ = 1, SL_RR = 2, SL_ALPHA = 3, SL_BETA = 4, SL_BB = 5, SL_PR = 6, SL_ZQ = 7,
+  SL_TMP = 11, SL_COUNT = 16
+};
+// reduce_final post-processing
+enum Post : int {
+  POST_NONE = 0,
+  POST_ALPHA_CG = 1,   // scal[ALPHA] = scal[RZ] / v
+  POST_BETA_CG = 2,    // scal[BETA] = v / scal[RZ]; scal[RZ] = v
+  POST_ALPHA_FCG = 3,  // scal[ALPHA] = scal[PR] / v   (v = p.Ap)
+  POST_BETA_FCG = 4    // scal[BETA] = -v / scal[PQ]  (v = z.Ap)
+};
+
+// ---- design-side kernels (k_design.hip) -----------------------------------
+struct FilterTemplate {
+  const int *off = nullptr;    // 3 per entry (di, dj, dk)
+  const double *w = nullptr;   // 1 - dist/R per entry
+  int ntpl = 0;
+  int nx = 0, ny = 0, nz = 0;
+};
+void filter_rowsum_inv(hipStream_t s, const FilterTemplate &t, double *inv);
+void filter_apply(hipStream_t s, const FilterTemplate &t, const double *inv, const double *x,
+                  double *rho);
+void filter_apply_T(hipStream_t s, const FilterTemplate &t, const double *inv, const double *g,
+                    double *gx);
+void filter_count(hipStream_t s, const FilterTemplate &t, int64_t *cnt);
+void filter_fill(hipStream_t s, const FilterTemplate &t, const double *inv, const int64_t *rowptr,
+                 int32_t *col, double *w);
+void simp(hipStream_t s, const double *rho, int64_t ne, double penal, double eps, double *E,
+          double *dEdz, double *Es, double *dEsdz, float *E32);
+void sensitivity(hipStream_t s, const double *U, const double *dEdz, const double *KE, int nx,
+                 int ny, int nz, double *g);
+
+}  // namespace tofe

@@ -1,0 +1,173 @@
+"""Host-side mirror of the reference's FE-analysis call surface over the C ABI.
+
+The method names, their order in the design loop and their meaning follow the
+reference's ``TopOpt`` class and ``Compliance`` function (TopOpt.h:282-292,
+Functions.h:65-77): CreateMesh -> Set_BC -> FEInitialize -> (MatIntFnc -> FEAssemble ->
+FESolve -> Compliance.Function -> Chain_Filter) per design iteration.  PETSc Vec/Mat/
+KSP handles are replaced by numpy arrays and one tofe context; every numerical step
+runs in the HIP library.
+"""
+from __future__ import annotations
+
+from enum import IntEnum
+
+import numpy as np
+
+from . import _lib
+from ._lib import Context
+
+
+class BCType(IntEnum):
+    SUPPORT = 0
+    LOAD = 1
+
+
+class TopOpt:
+    """Single-GPU stand-in for the reference TopOpt object's FE-analysis members."""
+
+    def __init__(self, E0=1.0, Nu0=0.3, penal=3.0, verbose=0):
+        self.E0 = E0
+        self.Nu0 = Nu0
+        self.penal = penal
+        self.verbose = verbose
+        self.eps = 1e-10  # Material_Interpolation.cpp:28
+        self.ctx = None
+        self.suppNode = np.zeros(0, np.int64)
+        self.supports = np.zeros((0, 3), bool)
+        self.loadNode = np.zeros(0, np.int64)
+        self.loads = np.zeros((0, 3))
+        self.mg_levels = 0
+        self.smoother = _lib.SMOOTHER_CHEBYSHEV
+        self.precision = _lib.PREC_FP64
+        self.rtol = 1e-8  # KSPSetTolerances(KUF, 1e-8, ...) FEAnalysis.cpp:390
+        self.maxit = 10000
+        self.KUF_reason = 0
+        self.its = 0
+
+    # -- mesh (RecMesh.cpp:426-597, single rank, no reordering) ----------------
+    def CreateMesh(self, dimensions, Nel, Rmin):
+        """dimensions = (x0, x1, y0, y1, z0, z1); Nel = elements per direction; Rmin is
+        the filter radius in units of length."""
+        self.dimensions = np.asarray(dimensions, np.float64)
+        self.Nel = tuple(int(n) for n in Nel)
+        nx, ny, nz = self.Nel
+        self.dx = tuple((self.dimensions[2 * i + 1] - self.dimensions[2 * i]) / self.Nel[i]
+                        for i in range(3))
+        self.nElem = nx * ny * nz
+        self.nNode = (nx + 1) * (ny + 1) * (nz + 1)
+        k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1),
+                              indexing="ij")
+        self.node = np.stack([self.dimensions[0] + self.dx[0] * i.reshape(-1),
+                              self.dimensions[2] + self.dx[1] * j.reshape(-1),
+                              self.dimensions[4] + self.dx[2] * k.reshape(-1)], axis=1)
+        ez, ey, ex = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        base = (ex + (nx + 1) * (ey + (ny + 1) * ez)).reshape(-1)
+        Nx, NxNy = nx + 1, (nx + 1) * (ny + 1)
+        self.element = np.stack([base, base + 1, base + 1 + Nx, base + Nx,
+                                 base + NxNy, base + 1 + NxNy, base + 1 + Nx + NxNy,
+                                 base + Nx + NxNy], axis=1)
+        self.Rmin = float(Rmin)
+
+    # -- boundary conditions (Inputs.cpp:105-223, SUPPORT and LOAD) -------------
+    def Set_BC(self, center, radius, limits, values, TYPE):
+        """Select nodes with sum(((x_i - c_i)/r_i)^2) <= 1 inside the box limits."""
+        center = np.asarray(center, np.float64)
+        radius = np.asarray(radius, np.float64)
+        limits = np.asarray(limits, np.float64).reshape(3, 2)
+        values = np.asarray(values, np.float64)
+        d = np.zeros(self.nNode)
+        valid = np.ones(self.nNode, bool)
+        for i in range(3):
+            d += ((self.node[:, i] - center[i]) / radius[i]) ** 2
+            valid &= (self.node[:, i] >= limits[i, 0]) & (self.node[:, i] <= limits[i, 1])
+        valid &= d <= 1
+        newNode = np.flatnonzero(valid)
+        if TYPE == BCType.SUPPORT:
+            self.supports = np.vstack([self.supports, np.tile(values.astype(bool), (newNode.size, 1))])
+            self.suppNode = np.concatenate([self.suppNode, newNode])
+        elif TYPE == BCType.LOAD:
+            self.loads = np.vstack([self.loads, np.tile(values, (newNode.size, 1))])
+            self.loadNode = np.concatenate([self.loadNode, newNode])
+        else:
+            raise ValueError("unsupported BC type")
+        return newNode
+
+    # -- FE set-up (FEAnalysis.cpp:215-424) -----------------------------------
+    def FEInitialize(self):
+        ndof = 3 * self.nNode
+        self.fixed_dof = np.zeros(ndof, np.uint8)
+        for n, s in zip(self.suppNode, self.supports):
+            for j in range(3):
+                if s[j]:
+                    self.fixed_dof[3 * n + j] = 1
+        self.fixedDof = np.flatnonzero(self.fixed_dof)
+        self.F = np.zeros(ndof)
+        for n, l in zip(self.loadNode, self.loads):
+            for j in range(3):
+                self.F[3 * n + j] += l[j]   # F[3 loadNode + j] += loads (349-356)
+        self.U = np.zeros(ndof)
+        if self.ctx is not None:
+            self.ctx.close()
+        self.ctx = Context(self.Nel, h=self.dx, E0=self.E0, nu=self.Nu0, Emin=self.eps,
+                           penal=self.penal, rmin=self.Rmin, mg_levels=self.mg_levels,
+                           smoother=self.smoother, precision=self.precision,
+                           fixed_dof=self.fixed_dof)
+        self.ke = [self.ctx.ke()]
+
+    # -- design iteration ---------------------------------------------------------
+    def MatIntFnc(self, design):
+        """rho = P x and the SIMP interpolation (Material_Interpolation.cpp:20-140)."""
+        self.ctx.set_design(design)
+        self.x = np.asarray(design, np.float64)
+        self.rho = self.ctx.field(_lib.FIELD_RHO)
+        self.E = self.ctx.field(_lib.FIELD_E)
+        self.dEdz = self.ctx.field(_lib.FIELD_DEDZ)
+        self.V = self.rho
+        self.dVdrho = np.ones_like(self.rho)
+
+    def FEAssemble(self):
+        self.ctx.assemble()
+
+    def FESolve(self):
+        """KSPSolve(KUF, F, U), warm-started from the previous U (FEAnalysis.cpp:389, 673)."""
+        self.U, self.its, self.KUF_reason = self.ctx.solve(self.F, self.U, rtol=self.rtol,
+                                                           maxit=self.maxit)
+        return self.KUF_reason
+
+    def Chain_Filter(self, dfdV=None, dfdE=None):
+        """dfdV <- P^T dfdV; dfdE <- P^T (dfdE .* y), y = 1 for the plain density filter
+        (Material_Interpolation.cpp:150-187)."""
+        outV = None if dfdV is None else self.ctx.filter_T(dfdV)
+        outE = None if dfdE is None else self.ctx.filter_T(dfdE)
+        return outV, outE
+
+
+class Compliance:
+    """Compliance objective c = U.F and its sensitivities (Compliance.cpp:15-57)."""
+
+    def __init__(self, calc_gradient=True):
+        self.calc_gradient = calc_gradient
+        self.values = np.zeros(1)
+        self.gradients = None
+
+    def Function(self, topOpt: TopOpt):
+        if self.calc_gradient:
+            c, g = topOpt.ctx.compliance(topOpt.U, topOpt.F, gradient=True)
+            self.gradients = g
+        else:
+            c = topOpt.ctx.compliance(topOpt.U, topOpt.F, gradient=False)
+        self.values[0] = c
+        return 0
+
+
+class Volume:
+    """Volume fraction sum(rho)/nElem with gradient P^T (1/nElem) (Volume.cpp:15-41)."""
+
+    def __init__(self):
+        self.values = np.zeros(1)
+        self.gradients = None
+
+    def Function(self, topOpt: TopOpt):
+        self.values[0] = topOpt.V.sum() / topOpt.nElem
+        self.gradients = topOpt.ctx.filter_T(np.full(topOpt.nElem, 1.0 / topOpt.nElem))
+        return 0
