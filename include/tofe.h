@@ -55,6 +55,8 @@ enum {
 };
 
 enum { TOFE_SMOOTHER_CHEBYSHEV = 0, TOFE_SMOOTHER_JACOBI = 1 };
+/* Material interpolation (TopOpt::interpolation, TopOpt.h:27; Inputs.cpp:339-358). */
+enum { TOFE_INTERP_SIMP = 0, TOFE_INTERP_SIMP_CUT = 1, TOFE_INTERP_SIMP_LOGISTIC = 2 };
 enum { TOFE_PREC_FP64 = 0, TOFE_PREC_MIXED = 1 };
 
 /* Fields readable with tofe_get_field (the TopOpt Vecs of the same names,
@@ -67,7 +69,10 @@ enum {
   TOFE_FIELD_ES = 4,     /* Es = z^p                      (nelem) */
   TOFE_FIELD_DESDZ = 5,  /* dEs/dz                        (nelem) */
   TOFE_FIELD_V = 6,      /* volume interpolation V = rho  (nelem) */
-  TOFE_FIELD_DIAG = 7    /* Jacobi diagonal of fine K     (ndof)  */
+  TOFE_FIELD_DIAG = 7,   /* Jacobi diagonal of fine K     (ndof)  */
+  TOFE_FIELD_Y = 8,      /* max-length-scale factor y     (nelem) */
+  TOFE_FIELD_RHOQ = 9,   /* rhoq = (q/vdMin)(1-rho)^(q-1) (nelem) */
+  TOFE_FIELD_REDGE = 10  /* REdge = max row count of R - row count (nelem) */
 };
 
 typedef struct tofe_desc {
@@ -84,6 +89,12 @@ typedef struct tofe_desc {
   int eig_its;             /* power-iteration steps for lambda_max; 0 -> 10 */
   int precision;           /* TOFE_PREC_*: FP64, or FP32 V-cycle inside FP64 flexible CG */
   const uint8_t *fixed_dof;/* 3*nnode flags, 1 = Dirichlet DOF (fixedDof, FEAnalysis.cpp:236-246); may be NULL */
+  /* maximum length scale and interpolation (Material_Interpolation.cpp:37-116) */
+  double rmax;             /* radius of the unit-weight filter R (RecMesh.cpp:544); used when vd_min > 0 */
+  double vd_min;           /* Void_Minimum; <= 0: no maximum length scale, y = 1 */
+  double vd_penal;         /* void penalty q (vdPenal); 0 -> 1 */
+  int interpolation;       /* TOFE_INTERP_* */
+  double interp_param[2];  /* SIMP_CUT threshold; SIMP_LOGISTIC (a, b) */
 } tofe_desc;
 
 typedef struct tofe_stats {
@@ -122,6 +133,15 @@ int tofe_get_ke(const tofe_ctx *ctx, double *ke576);
 int tofe_set_design(tofe_ctx *ctx, const double *x);
 int tofe_set_design_dev(tofe_ctx *ctx, const double *d_x);
 
+/* Penalty continuation: the SIMP exponent and the void penalty of the next design
+ * iteration (Main.cpp:131-133 sets topOpt->penal and vdPenal per penalty step). */
+int tofe_set_penalty(tofe_ctx *ctx, double penal, double vd_penal);
+
+/* Filter chain rule in place on host arrays (either may be NULL):
+ * dfdV <- P^T dfdV, dfdE <- P^T[dfdE .* y - rhoq .* (y < 1) .* R^T(dfdE .* rho)]
+ * (TopOpt::Chain_Filter, Material_Interpolation.cpp:150-187). */
+int tofe_chain_filter(tofe_ctx *ctx, double *dfdV, double *dfdE);
+
 /* Override the element stiffness scale E directly (dEdz is left as is). */
 int tofe_set_E(tofe_ctx *ctx, const double *E);
 
@@ -154,7 +174,7 @@ int tofe_apply_K_dev(tofe_ctx *ctx, const double *d_X, double *d_Y, int nrhs);
 int tofe_pc_apply(tofe_ctx *ctx, const double *R, double *Z, int nrhs);
 int tofe_pc_apply_dev(tofe_ctx *ctx, const double *d_R, double *d_Z, int nrhs);
 
-/* c = U.F and dc/dx = P^T(-dEdz .* u_e^T KE u_e): Compliance::Function
+/* c = U.F and dc/dx = Chain_Filter(-dEdz .* u_e^T KE u_e): Compliance::Function
  * (Compliance.cpp:15-57) including TopOpt::Chain_Filter
  * (Material_Interpolation.cpp:150-187).  dcdx may be NULL. */
 int tofe_compliance(tofe_ctx *ctx, const double *U, const double *F, double *c,

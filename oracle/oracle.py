@@ -48,6 +48,11 @@ def lib():
     L.orc_filter_build.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [C.c_int, vp, vp, vp]
     L.orc_filter_csr.restype = vp
     L.orc_filter_csr.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [C.c_int]
+    L.orc_rfilter_csr.restype = vp
+    L.orc_rfilter_csr.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [vp]
+    L.orc_matint.argtypes = [vp, vp, vp, _f64p, C.c_double, C.c_double, C.c_double, C.c_int,
+                             _f64p, C.c_double] + [_f64p] * 7
+    L.orc_chain_filter.argtypes = [vp, vp, _f64p, _f64p, _f64p, vp, vp]
     L.orc_vecpow1.restype = C.c_double
     L.orc_vecpow1.argtypes = [C.c_double, C.c_double]
     L.orc_simp.argtypes = [C.c_int64, _f64p, C.c_double, C.c_double, vp, vp, vp, vp]
@@ -170,6 +175,36 @@ def ke(E0=1.0, nu=0.3, h=(1.0, 1.0, 1.0)):
 
 def filter_csr(nx, ny, nz, R, h=(1.0, 1.0, 1.0), scale=True):
     return CSR(lib().orc_filter_csr(nx, ny, nz, h[0], h[1], h[2], R, 1 if scale else 0))
+
+
+def rfilter_csr(nx, ny, nz, Rmax, h=(1.0, 1.0, 1.0)):
+    """Unit-weight max-length-scale filter R and REdge = max rowsum - rowsum."""
+    REdge = np.empty(nx * ny * nz)
+    A = CSR(lib().orc_rfilter_csr(nx, ny, nz, h[0], h[1], h[2], Rmax, REdge.ctypes.data_as(C.c_void_p)))
+    return A, REdge
+
+
+def matint(P, R, REdge, x, penal=3.0, vdPenal=1.0, vdMin=-1.0, interp=0, param=(0.0, 0.0),
+           eps=1e-10):
+    """Full MatIntFnc: returns dict rho, y, rhoq, E, dEdz, Es, dEsdz."""
+    ne = np.asarray(x).size
+    out = {k: np.empty(ne) for k in ("rho", "y", "rhoq", "E", "dEdz", "Es", "dEsdz")}
+    Rh = None if R is None else R.h
+    RE = None if REdge is None else np.ascontiguousarray(REdge, np.float64).ctypes.data_as(C.c_void_p)
+    lib().orc_matint(P.h, Rh, RE, np.ascontiguousarray(x, np.float64), penal, vdPenal, vdMin, interp,
+                     np.ascontiguousarray(param, np.float64), eps, out["rho"], out["y"], out["rhoq"],
+                     out["E"], out["dEdz"], out["Es"], out["dEsdz"])
+    return out
+
+
+def chain_filter(P, R, rho, y, rhoq, dfdV=None, dfdE=None):
+    """Full Chain_Filter on copies of dfdV / dfdE."""
+    V = None if dfdV is None else np.array(dfdV, np.float64, copy=True)
+    Ee = None if dfdE is None else np.array(dfdE, np.float64, copy=True)
+    lib().orc_chain_filter(P.h, None if R is None else R.h, np.ascontiguousarray(rho, np.float64),
+                           np.ascontiguousarray(y, np.float64), np.ascontiguousarray(rhoq, np.float64),
+                           _ptr(V), _ptr(Ee))
+    return V, Ee
 
 
 def vecpow(x, p):

@@ -282,6 +282,112 @@ void orc_simp(int64_t ne, const double *rho, double penal, double eps,
   }
 }
 
+/* Maximum-length-scale filter R: the same neighbourhood walk with radius Rmax and unit
+ * weights (RecFilter(..., nonzeros = 1), RecMesh.cpp:544; assembled unscaled, 706), and
+ * REdge = max_i rowsum_i - rowsum_i (RecMesh.cpp:710-718). */
+orc_csr *orc_rfilter_csr(int nx, int ny, int nz, double dx, double dy, double dz, double Rmax,
+                         double *REdge)
+{
+  orc_csr *A = orc_filter_csr(nx, ny, nz, dx, dy, dz, Rmax, 0);
+  if (!A) return NULL;
+  for (int64_t p = 0; p < A->nnz; p++) A->val[p] = 1.0;
+  if (REdge) {
+    double mx = 0;
+    for (int64_t i = 0; i < A->n; i++) {
+      double s = 0.0;
+      for (int64_t p = A->rowptr[i]; p < A->rowptr[i + 1]; p++) s += A->val[p];
+      REdge[i] = s;
+      if (s > mx) mx = s;
+    }
+    for (int64_t i = 0; i < A->n; i++) REdge[i] = -1 * REdge[i] + mx; /* VecAYPX(REdge,-1,max) */
+  }
+  return A;
+}
+
+/* TopOpt::MatIntFnc (Material_Interpolation.cpp:20-140), all branches:
+ *   rho = P x; with vdMin > 0: rho = min(rho,1), y = min(1, (REdge + R (1-rho)^q) / vdMin),
+ *   otherwise y = 1; rhoq = (q/vdMin) (1-rho)^(q-1); z = rho y; Es = z^p, dEsdz = p z^(p-1),
+ *   E = eps + (1-eps) Es, dEdz = (1-eps) dEsdz; then SIMP_CUT (interp 1) zeroes Es, dEsdz
+ *   where z <= c, SIMP_LOGISTIC (interp 2) divides by 1 + exp(a (b - z)).
+ * R/REdge may be NULL when vdMin <= 0. */
+void orc_matint(const orc_csr *P, const orc_csr *R, const double *REdge, const double *x,
+                double penal, double vdPenal, double vdMin, int interp, const double *param,
+                double eps, double *rho, double *y, double *rhoq, double *E, double *dEdz,
+                double *Es, double *dEsdz)
+{
+  int64_t ne = P->n;
+  orc_spmv(P, x, rho);
+  if (vdMin > 0) {
+    double *zt = (double *)malloc(ne * sizeof(double));
+    for (int64_t i = 0; i < ne; i++) {
+      if (rho[i] > 1.0) rho[i] = 1.0;                       /* VecPointwiseMin(rho,rho,1) */
+      zt[i] = orc_vecpow1(1.0 + -1 * rho[i], vdPenal);       /* z = (1-rho)^q */
+    }
+    double inv = 1 / vdMin;
+    for (int64_t i = 0; i < ne; i++) {
+      double s = REdge[i];                                   /* MatMultAdd(R, z, REdge, y) */
+      for (int64_t p = R->rowptr[i]; p < R->rowptr[i + 1]; p++) s += R->val[p] * zt[R->col[p]];
+      s = s * inv;                                           /* VecScale(y, 1/vdMin) */
+      y[i] = s < 1.0 ? s : 1.0;                              /* VecPointwiseMin(y, y, 1) */
+    }
+    free(zt);
+  } else {
+    for (int64_t i = 0; i < ne; i++) y[i] = 1.0;
+  }
+  double sc = vdPenal / vdMin;
+  for (int64_t i = 0; i < ne; i++) {
+    rhoq[i] = orc_vecpow1(1.0 + -1 * rho[i], vdPenal - 1) * sc;
+    double z = rho[i] * y[i];
+    double d = orc_vecpow1(z, penal - 1);
+    double es = d * z;
+    d = d * penal;
+    dEdz[i] = d * (1 - eps);
+    E[i] = eps + (1 - eps) * es;
+    if (interp == 1) {
+      double cut = (z > param[0]) ? 1.0 : 0.0;
+      es = es * cut;
+      d = d * cut;
+    } else if (interp == 2) {
+      double denom = 1 + exp(param[0] * (param[1] - z));
+      es = es / denom;
+      d = (d + (es * param[0]) * (1 - 1 / denom)) / denom;
+    }
+    Es[i] = es;
+    dEsdz[i] = d;
+  }
+}
+
+/* TopOpt::Chain_Filter (Material_Interpolation.cpp:150-187), in place:
+ *   dfdV <- P^T dfdV;
+ *   dfdE <- P^T [dfdE .* y - rhoq .* (y < 1) .* R^T (dfdE .* rho)]
+ * exactly as written there (the (y < 1) mask multiplies R^T(...) entrywise). */
+void orc_chain_filter(const orc_csr *P, const orc_csr *R, const double *rho, const double *y,
+                      const double *rhoq, double *dfdV, double *dfdE)
+{
+  int64_t ne = P->n;
+  double *t1 = (double *)malloc(ne * sizeof(double));
+  double *t2 = (double *)malloc(ne * sizeof(double));
+  if (dfdV) {
+    orc_spmv_t(P, dfdV, t1);
+    memcpy(dfdV, t1, ne * sizeof(double));
+  }
+  if (dfdE) {
+    for (int64_t i = 0; i < ne; i++) t2[i] = dfdE[i] * rho[i];
+    if (R) orc_spmv_t(R, t2, t1);
+    else memset(t1, 0, ne * sizeof(double));
+    for (int64_t i = 0; i < ne; i++) {
+      double mask = (y[i] < 1.0) ? 1.0 : 0.0;
+      double a = t1[i] * mask;
+      a = rhoq[i] * a;
+      double b = dfdE[i] * y[i];
+      t2[i] = b - a;                                   /* VecAYPX(temp1, -1, dfdE) */
+    }
+    orc_spmv_t(P, t2, dfdE);
+  }
+  free(t1);
+  free(t2);
+}
+
 /* ======================================================================
  * Sparse matrices
  * ==================================================================== */

@@ -50,10 +50,19 @@ int64_t orc_filter_build(int nx, int ny, int nz, double dx, double dy, double dz
 orc_csr *orc_filter_csr(int nx, int ny, int nz, double dx, double dy, double dz,
                         double R, int scale);
 
+orc_csr *orc_rfilter_csr(int nx, int ny, int nz, double dx, double dy, double dz, double Rmax,
+                         double *REdge);
+
 /* ---- SIMP ----------------------------------------------------------------- */
 double orc_vecpow1(double x, double p);
 void orc_simp(int64_t ne, const double *rho, double penal, double eps,
               double *E, double *dEdz, double *Es, double *dEsdz);
+void orc_matint(const orc_csr *P, const orc_csr *R, const double *REdge, const double *x,
+                double penal, double vdPenal, double vdMin, int interp, const double *param,
+                double eps, double *rho, double *y, double *rhoq, double *E, double *dEdz,
+                double *Es, double *dEsdz);
+void orc_chain_filter(const orc_csr *P, const orc_csr *R, const double *rho, const double *y,
+                      const double *rhoq, double *dfdV, double *dfdE);
 
 /* ---- sparse helpers ------------------------------------------------------- */
 void orc_csr_free(orc_csr *A);

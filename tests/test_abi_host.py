@@ -39,10 +39,28 @@ def test_library_exports_every_declared_symbol():
     _lib.lib()  # argtypes for every symbol bind
 
 
-def test_struct_layout():
-    assert C.sizeof(_lib.TofeDesc) == 4 * 3 + 4 + 8 * 8 + 4 * 4 + 8 * 2 + 4 * 2 + 8
-    assert _lib.TofeDesc.fixed_dof.offset == C.sizeof(_lib.TofeDesc) - 8
-    assert C.sizeof(_lib.TofeStats) == 4 + 4 + 8 + 4 + 4 + 8 * 4
+def _c_layout(struct, fields, tmp_path):
+    """sizeof and offsetof of a C struct from include/tofe.h, compiled with gcc."""
+    src = tmp_path / "layout.c"
+    body = "".join(f'printf("%zu\\n", offsetof({struct}, {f}));' for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "tofe.h"\n'
+                   f'int main(void){{printf("%zu\\n", sizeof({struct}));{body}return 0;}}\n')
+    exe = tmp_path / "layout"
+    import subprocess
+
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    return int(out[0]), [int(v) for v in out[1:]]
+
+
+@pytest.mark.parametrize("pystruct,cname", [(_lib.TofeDesc, "tofe_desc"),
+                                            (_lib.TofeStats, "tofe_stats")])
+def test_struct_layout_matches_header(pystruct, cname, tmp_path):
+    names = [f for f, _ in pystruct._fields_]
+    size, offs = _c_layout(cname, names, tmp_path)
+    assert C.sizeof(pystruct) == size
+    assert [getattr(pystruct, f).offset for f in names] == offs
 
 
 def test_splitmix64_known_answers():

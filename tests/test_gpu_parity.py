@@ -187,3 +187,24 @@ def test_pc_apply_is_symmetric(oracle, tofe):
     assert b @ Ma == pytest.approx(a @ Mb, rel=1e-10)
     assert a @ Ma > 0
     ctx.close()
+
+
+def test_config1_against_sparse_direct_fixture(tofe):
+    """48x24x24 cantilever (x = 0.3) on the GPU against the committed SciPy sparse-direct
+    fixture (tests/golden/make_cfg1_direct.py, independent of the oracle and the GPU)."""
+    import os
+
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg1_direct.npz"))
+    pb = configs.config1()
+    ctx = tofe.Context(pb.nel, mg_levels=3, fixed_dof=pb.fixed_dof)
+    ctx.set_design(pb.x)
+    ctx.assemble()
+    U, its, reason = ctx.solve(pb.F, rtol=1e-12)
+    assert reason == 2
+    assert np.abs(U[fx["dof_samples"]] - fx["U_samples"]).max() <= 1e-9 * np.abs(fx["U_samples"]).max()
+    assert np.linalg.norm(U) == pytest.approx(float(fx["U_norm"]), rel=1e-10)
+    c, g = ctx.compliance(U, pb.F)
+    assert c == pytest.approx(float(fx["compliance"]), rel=1e-10)
+    gs = ctx.sensitivity(U)
+    assert np.allclose(gs[fx["el_samples"]], fx["g_samples"], rtol=1e-9, atol=0)
+    ctx.close()

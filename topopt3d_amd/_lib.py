@@ -28,7 +28,9 @@ DIVERGED_INDEFINITE_MAT = -10
 SMOOTHER_CHEBYSHEV, SMOOTHER_JACOBI = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
-FIELD_X, FIELD_RHO, FIELD_E, FIELD_DEDZ, FIELD_ES, FIELD_DESDZ, FIELD_V, FIELD_DIAG = range(8)
+(FIELD_X, FIELD_RHO, FIELD_E, FIELD_DEDZ, FIELD_ES, FIELD_DESDZ, FIELD_V, FIELD_DIAG, FIELD_Y,
+ FIELD_RHOQ, FIELD_REDGE) = range(11)
+INTERP_SIMP, INTERP_SIMP_CUT, INTERP_SIMP_LOGISTIC = 0, 1, 2
 
 # Every entry point include/tofe.h declares.
 EXPORTS = (
@@ -38,6 +40,7 @@ EXPORTS = (
     "tofe_pc_apply_dev", "tofe_compliance", "tofe_compliance_dev", "tofe_sensitivity",
     "tofe_filter", "tofe_filter_T", "tofe_filter_pattern", "tofe_set_precision",
     "tofe_get_stats", "tofe_get_level_stencil", "tofe_get_level_lambda", "tofe_time_operator",
+    "tofe_set_penalty", "tofe_chain_filter",
 )
 
 
@@ -51,6 +54,8 @@ class TofeDesc(C.Structure):
         ("cheb_lo", C.c_double), ("cheb_hi", C.c_double),
         ("eig_its", C.c_int), ("precision", C.c_int),
         ("fixed_dof", C.c_void_p),
+        ("rmax", C.c_double), ("vd_min", C.c_double), ("vd_penal", C.c_double),
+        ("interpolation", C.c_int), ("interp_param", C.c_double * 2),
     ]
 
 
@@ -111,6 +116,8 @@ def lib():
         "tofe_get_level_stencil": ([vp, C.c_int, dp, i64p], C.c_int),
         "tofe_get_level_lambda": ([vp, C.c_int, dp], C.c_int),
         "tofe_time_operator": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, dp], C.c_int),
+        "tofe_set_penalty": ([vp, C.c_double, C.c_double], C.c_int),
+        "tofe_chain_filter": ([vp, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -133,7 +140,9 @@ class Context:
 
     def __init__(self, nel, h=(1.0, 1.0, 1.0), E0=1.0, nu=0.3, Emin=1e-10, penal=3.0,
                  rmin=1.5, mg_levels=0, smoother=SMOOTHER_CHEBYSHEV, nu_pre=2, nu_post=2,
-                 cheb_lo=0.1, cheb_hi=1.1, eig_its=10, precision=PREC_FP64, fixed_dof=None):
+                 cheb_lo=0.1, cheb_hi=1.1, eig_its=10, precision=PREC_FP64, fixed_dof=None,
+                 rmax=0.0, vd_min=-1.0, vd_penal=1.0, interpolation=INTERP_SIMP,
+                 interp_param=(0.0, 0.0)):
         L = lib()
         self.nel = tuple(int(v) for v in nel)
         d = TofeDesc()
@@ -142,6 +151,8 @@ class Context:
         d.E0, d.nu, d.Emin, d.penal, d.rmin = E0, nu, Emin, penal, rmin
         d.mg_levels, d.smoother, d.nu_pre, d.nu_post = mg_levels, smoother, nu_pre, nu_post
         d.cheb_lo, d.cheb_hi, d.eig_its, d.precision = cheb_lo, cheb_hi, eig_its, precision
+        d.rmax, d.vd_min, d.vd_penal, d.interpolation = rmax, vd_min, vd_penal, interpolation
+        d.interp_param[0], d.interp_param[1] = interp_param
         self._fixed = None
         if fixed_dof is not None:
             self._fixed = np.ascontiguousarray(fixed_dof, dtype=np.uint8)
@@ -271,6 +282,16 @@ class Context:
         gx = np.empty_like(g)
         self._check(lib().tofe_filter_T(self.h, _dp(g), _dp(gx)))
         return gx
+
+    def set_penalty(self, penal, vd_penal=1.0):
+        self._check(lib().tofe_set_penalty(self.h, penal, vd_penal))
+
+    def chain_filter(self, dfdV=None, dfdE=None):
+        """Chain_Filter on copies of dfdV / dfdE (either may be None); returns both."""
+        V = None if dfdV is None else _f64(dfdV).copy()
+        E = None if dfdE is None else _f64(dfdE).copy()
+        self._check(lib().tofe_chain_filter(self.h, _dp(V), _dp(E)))
+        return V, E
 
     def filter_pattern(self):
         rp = np.empty(self.nelem + 1, np.int64)

@@ -1,8 +1,8 @@
 """Build libtofe.so (HIP kernels + C ABI) in-tree for gfx950.
 
 Every translation unit under csrc/ is compiled with hipcc in parallel into
-topopt3d_amd/_build/, then linked into topopt3d_amd/libtofe.so against rocSOLVER
-and rocBLAS (used only for the coarsest-level dense Cholesky factor and inverse).
+topopt3d_amd/_build/, then linked into topopt3d_amd/libtofe.so against rocBLAS
+(dgemm/dsyrk/dtrtri inside the coarsest-level dense Cholesky factorisation and inverse).
 """
 from __future__ import annotations
 
@@ -55,14 +55,33 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + [
-        "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+        "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_examples()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+EXAMPLES = os.path.join(PKG, "examples")
+
+
+def build_examples() -> None:
+    """C++ host programs over the C ABI (include/tofe_topopt.hpp), linked to libtofe.so."""
+    cxx = shutil.which("g++") or "g++"
+    for src in sorted(os.listdir(EXAMPLES)):
+        if not src.endswith(".cpp"):
+            continue
+        out = os.path.join(EXAMPLES, src[:-4])
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(PKG, "..", "include"),
+               os.path.join(EXAMPLES, src), "-o", out, "-L" + PKG, "-ltofe",
+               "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -114,6 +114,37 @@ int filter_template(double dx, double dy, double dz, double R, std::vector<int> 
   return (int)w.size();
 }
 
+void galerkin_forms(const double KE[576], double Qforms[27 * 64], double KEC[27 * 576])
+{
+  // q[form][fine node l of the element edge][coarse node L of the window]
+  static const double q[3][2][2] = {{{1.0, 0.0}, {0.5, 0.5}},   // even element
+                                    {{0.5, 0.5}, {0.0, 1.0}},   // odd element
+                                    {{1.0, 0.0}, {1.0, 0.0}}};  // last element, single parent
+  for (int f = 0; f < 27; ++f) {
+    const int fx = f % 3, fy = (f / 3) % 3, fz = f / 9;
+    double *Q = Qforms + f * 64;
+    for (int l = 0; l < 8; ++l)
+      for (int L = 0; L < 8; ++L)
+        Q[l * 8 + L] = q[fx][kLN[l][0]][kLN[L][0]] * q[fy][kLN[l][1]][kLN[L][1]] *
+                       q[fz][kLN[l][2]][kLN[L][2]];
+    double X[576];
+    for (int r = 0; r < 24; ++r)
+      for (int c = 0; c < 24; ++c) {
+        const int L = c / 3, b = c % 3;
+        double s = 0;
+        for (int l = 0; l < 8; ++l) s += KE[r * 24 + 3 * l + b] * Q[l * 8 + L];
+        X[r * 24 + c] = s;
+      }
+    for (int r = 0; r < 24; ++r)
+      for (int c = 0; c < 24; ++c) {
+        const int L = r / 3, a = r % 3;
+        double s = 0;
+        for (int l = 0; l < 8; ++l) s += Q[l * 8 + L] * X[(3 * l + a) * 24 + c];
+        KEC[f * 576 + r * 24 + c] = s;
+      }
+  }
+}
+
 double tridiag_max_eig(int k, const double *a, const double *b)
 {
   double lo = a[0], hi = a[0];

@@ -57,18 +57,20 @@ __global__ __launch_bounds__(NT) void k_stencil_op(StencilOp<T> op, const T *__r
   }
 }
 
-// One thread per (coarse node I, coarse offset o >= 13): the block of P^T A P coupling
-// I with J = I + off(o), summed over fine pairs (i, i+d) with nonzero transfer weights;
-// the transposed block is written to (J, 26-o), so the operator is exactly symmetric.
+// One thread per (coarse node I, coarse offset o >= 13, block entry q = 3a + b): the
+// entry of P^T A P coupling I with J = I + off(o), summed over fine pairs (i, i+d) with
+// nonzero transfer weights; the transposed entry goes to (J, 26-o), so the operator is
+// exactly symmetric.  Nine threads per block keep the small coarse levels busy.
 __global__ __launch_bounds__(NT) void k_galerkin(const double *__restrict__ Sf, int Nfx, int Nfy,
                                                  int Nfz, double *__restrict__ Sc, int Ncx,
                                                  int Ncy, int Ncz)
 {
   const int64_t ncn = (int64_t)Ncx * Ncy * Ncz, nfn = (int64_t)Nfx * Nfy * Nfz;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
-  if (t >= 14 * ncn) return;
-  const int o = 13 + (int)(t / ncn);
-  const int64_t I = t - (int64_t)(o - 13) * ncn;
+  if (t >= 126 * ncn) return;
+  const int oq = (int)(t / ncn);
+  const int64_t I = t - (int64_t)oq * ncn;
+  const int o = 13 + oq / 9, q = oq % 9;
   const int Dx = o % 3 - 1, Dy = (o / 3) % 3 - 1, Dz = o / 9 - 1;
   const int Ix = (int)(I % Ncx), Iy = (int)((I / Ncx) % Ncy), Iz = (int)(I / ((int64_t)Ncx * Ncy));
   const int Jx = Ix + Dx, Jy = Iy + Dy, Jz = Iz + Dz;
@@ -98,27 +100,23 @@ __global__ __launch_bounds__(NT) void k_galerkin(const double *__restrict__ Sf, 
     }
     cnt[dim] = c;
   }
-  double acc[9];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+  const double *Sq = Sf + (int64_t)q * nfn;
+  double acc = 0.0;
   for (int cz = 0; cz < cnt[2]; ++cz)
-    for (int cy = 0; cy < cnt[1]; ++cy)
+    for (int cy = 0; cy < cnt[1]; ++cy) {
+      const double wyz = lw[1][cy] * lw[2][cz];
+      const int64_t rowyz = (int64_t)Nfx * (li[1][cy] + (int64_t)Nfy * li[2][cz]);
+      const int odyz = 3 * (ld[1][cy] + 1) + 9 * (ld[2][cz] + 1);
       for (int cx = 0; cx < cnt[0]; ++cx) {
-        const int64_t fi = li[0][cx] + (int64_t)Nfx * (li[1][cy] + (int64_t)Nfy * li[2][cz]);
-        const int od = (ld[0][cx] + 1) + 3 * (ld[1][cy] + 1) + 9 * (ld[2][cz] + 1);
-        const double w = lw[0][cx] * lw[1][cy] * lw[2][cz];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) acc[q] += w * Sf[(int64_t)(od * 9 + q) * nfn + fi];
+        const int od = (ld[0][cx] + 1) + odyz;
+        acc += lw[0][cx] * wyz * Sq[(int64_t)(od * 9) * nfn + li[0][cx] + rowyz];
       }
-#pragma unroll
-  for (int q = 0; q < 9; ++q) Sc[(int64_t)(o * 9 + q) * ncn + I] = acc[q];
+    }
+  Sc[(int64_t)(o * 9 + q) * ncn + I] = acc;
   if (o != 13) {
     const int64_t J = Jx + (int64_t)Ncx * (Jy + (int64_t)Ncy * Jz);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-        Sc[(int64_t)((26 - o) * 9 + b * 3 + a) * ncn + J] = acc[a * 3 + b];
+    const int a = q / 3, b = q % 3;
+    Sc[(int64_t)((26 - o) * 9 + b * 3 + a) * ncn + J] = acc;
   }
 }
 
@@ -176,7 +174,63 @@ __global__ __launch_bounds__(NT) void k_gemv(const double *__restrict__ A, int64
   if (lane == 0) y[row] = (Tout)s;
 }
 
+// Cholesky factor of one kb x kb diagonal block held in LDS (column-major A, leading
+// dimension lda, kb <= kCholNB): L_kk overwrites the lower triangle of the block and its
+// inverse goes to W (leading dimension kCholNB).  info[0] = global pivot index + 1 of the
+// first non-positive pivot (0 if none).
+__global__ __launch_bounds__(NT) void k_chol_block(double *A, int lda, int kb, int k0, double *W,
+                                                   int *info)
+{
+  extern __shared__ double sm[];  // a[i + j*kCholNB], then x[i + c*kCholNB]
+  double *a = sm;
+  const int tid = threadIdx.x;
+  for (int v = tid; v < kb * kb; v += NT) {
+    const int i = v % kb, j = v / kb;
+    a[i + j * kCholNB] = (i >= j) ? A[i + (int64_t)j * lda] : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    if (tid == 0) {
+      double d = a[j + j * kCholNB];
+      if (!(d > 0)) {
+        if (info[0] == 0) info[0] = k0 + j + 1;
+        d = 1.0;
+      }
+      a[j + j * kCholNB] = sqrt(d);
+    }
+    __syncthreads();
+    const double djj = a[j + j * kCholNB];
+    for (int i = j + 1 + tid; i < kb; i += NT) a[i + j * kCholNB] /= djj;
+    __syncthreads();
+    const int m = kb - j - 1;  // trailing lower triangle, (i, l) with j < l <= i
+    for (int v = tid; v < m * m; v += NT) {
+      const int i = j + 1 + v % m, l = j + 1 + v / m;
+      if (l <= i) a[i + l * kCholNB] -= a[i + j * kCholNB] * a[l + j * kCholNB];
+    }
+    __syncthreads();
+  }
+  for (int v = tid; v < kb * kb; v += NT) {
+    const int i = v % kb, j = v / kb;
+    if (i >= j) A[i + (int64_t)j * lda] = a[i + j * kCholNB];
+  }
+  // W = L^-1 by forward substitution, one column per thread
+  for (int c = tid; c < kb; c += NT) {
+    for (int i = 0; i < kb; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int q = c; q < i; ++q) s -= a[i + q * kCholNB] * W[q + c * kCholNB];
+      W[i + c * kCholNB] = (i >= c) ? s / a[i + i * kCholNB] : 0.0;
+    }
+  }
+}
+
 }  // namespace
+
+void chol_block(hipStream_t s, double *A, int lda, int kb, int k0, double *W, int *info)
+{
+  k_chol_block<<<dim3(1), dim3(NT), (size_t)kCholNB * kCholNB * sizeof(double), s>>>(A, lda, kb, k0,
+                                                                                     W, info);
+  TOFE_LAUNCH_CHECK();
+}
 
 template <typename T>
 int stencil_op_blocks(const StencilOp<T> &op)
@@ -205,7 +259,7 @@ void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double
 {
   const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
   TOFE_HIP(hipMemsetAsync(Sc, 0, sizeof(double) * 243 * ncn, s));
-  const int64_t nt = 14 * ncn;
+  const int64_t nt = 126 * ncn;
   k_galerkin<<<dim3((unsigned)((nt + NT - 1) / NT)), dim3(NT), 0, s>>>(Sf, Nfx, Nfy, Nfz, Sc, Ncx,
                                                                       Ncy, Ncz);
   TOFE_LAUNCH_CHECK();

@@ -1,5 +1,7 @@
-// Design-side kernels: density filter (Filter.cpp:21-150), SIMP interpolation
-// (Material_Interpolation.cpp:20-75) and compliance sensitivities (Compliance.cpp:37-46).
+// Design-side kernels: density and maximum-length-scale filters (Filter.cpp:21-150,
+// RecMesh.cpp:544,706-718), the material interpolation with its SIMP / SIMP_CUT /
+// SIMP_LOGISTIC branches and the filter chain rule (Material_Interpolation.cpp:20-187), and
+// compliance sensitivities (Compliance.cpp:37-46).
 //
 // These are HBM-bound element loops.  They reproduce the reference's floating-point
 // operation order exactly (file-wide fp contract(off): no fused multiply-add), so the filter
@@ -138,23 +140,112 @@ __device__ inline double vecpow1(double x, double p)
   return pow(x, p);
 }
 
-// Plain-density SIMP (y = 1): z = rho; dEsdz = z^(p-1); Es = dEsdz z; dEsdz *= p;
-// dEdz = (1-eps) dEsdz; E = eps + (1-eps) Es  (Material_Interpolation.cpp:61-75)
-__global__ void k_simp(const double *__restrict__ rho, int64_t ne, double penal, double eps,
-                       double *E, double *dEdz, double *Es, double *dEsdz, float *E32)
+// z = (1 - rho)^q after rho = min(rho, 1), in place on rho (Material_Interpolation.cpp:42-45)
+__global__ void k_void_pre(double *rho, int64_t ne, double q, double *zt)
 {
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < ne; i += (int64_t)gridDim.x * NT) {
-    const double z = (rho[i] * 1.0);
-    double d = vecpow1(z, penal - 1);
-    const double es = (d * z);
-    d = (d * penal);
-    const double e = eps + (1 - eps) * es;
-    Es[i] = es;
-    dEsdz[i] = d;
-    dEdz[i] = d * (1 - eps);
+    double r = rho[i];
+    if (r > 1.0) r = 1.0;
+    rho[i] = r;
+    zt[i] = vecpow1(1.0 + -1 * r, q);
+  }
+}
+
+// y = min(1, (REdge + R z) / vdMin), R with unit weights summed in ascending columns
+// after REdge (MatMultAdd, VecScale(1/vdMin), VecPointwiseMin; Material_Interpolation.cpp:46-52)
+__global__ __launch_bounds__(NT) void k_rfilter_madd(FilterTemplate t, const double *__restrict__ REdge,
+                                                     const double *__restrict__ zt, double inv_vdmin,
+                                                     double *y, int64_t ne)
+{
+  TOFE_STAGE_TEMPLATE();
+  const int64_t el = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (el >= ne) return;
+  const int ex = (int)(el % t.nx), ey = (int)((el / t.nx) % t.ny), ez = (int)(el / ((int64_t)t.nx * t.ny));
+  double s = REdge[el];
+  for (int q = 0; q < t.ntpl; ++q) {
+    int nb;
+    if (tpl_valid(t, ex, ey, ez, q, soff, nb)) s = s + 1.0 * zt[nb];
+  }
+  s = s * inv_vdmin;
+  y[el] = s < 1.0 ? s : 1.0;
+}
+
+// R^T g with unit weights: rows i = j - off_q in ascending order (template walked backwards)
+__global__ __launch_bounds__(NT) void k_rfilter_T(FilterTemplate t, const double *__restrict__ g,
+                                                  double *gx, int64_t ne)
+{
+  TOFE_STAGE_TEMPLATE();
+  const int64_t el = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (el >= ne) return;
+  const int ex = (int)(el % t.nx), ey = (int)((el / t.nx) % t.ny), ez = (int)(el / ((int64_t)t.nx * t.ny));
+  double s = 0.0;
+  for (int q = t.ntpl - 1; q >= 0; --q) {
+    const int i = ex - soff[3 * q], j = ey - soff[3 * q + 1], k = ez - soff[3 * q + 2];
+    if (i < 0 || i >= t.nx || j < 0 || j >= t.ny || k < 0 || k >= t.nz) continue;
+    s = s + 1.0 * g[i + (int64_t)t.nx * (j + (int64_t)t.ny * k)];
+  }
+  gx[el] = s;
+}
+
+// MatIntFnc after the filters (Material_Interpolation.cpp:54-116): rhoq = (q/vdMin)(1-rho)^(q-1);
+// z = rho y; dEsdz = z^(p-1); Es = dEsdz z; dEsdz *= p; dEdz = (1-eps) dEsdz;
+// E = eps + (1-eps) Es; then SIMP_CUT / SIMP_LOGISTIC modify Es and dEsdz only.
+// y == nullptr means the plain density filter (y = 1, written to yout).
+__global__ void k_simp(const double *__restrict__ rho, const double *y_in, int64_t ne,
+                       SimpParams sp, double *yout, double *rhoq, double *E, double *dEdz,
+                       double *Es, double *dEsdz, float *E32)
+{
+  const double sc = sp.vd_penal / sp.vd_min;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < ne; i += (int64_t)gridDim.x * NT) {
+    const double r = rho[i];
+    const double y = y_in ? y_in[i] : 1.0;
+    if (!y_in) yout[i] = 1.0;
+    rhoq[i] = vecpow1(1.0 + -1 * r, sp.vd_penal - 1) * sc;
+    const double z = r * y;
+    double d = vecpow1(z, sp.penal - 1);
+    double es = d * z;
+    d = d * sp.penal;
+    const double e = sp.eps + (1 - sp.eps) * es;
+    dEdz[i] = d * (1 - sp.eps);
     E[i] = e;
     if (E32) E32[i] = (float)e;
+    if (sp.interp == TOFE_INTERP_SIMP_CUT) {
+      const double cut = (z > sp.p0) ? 1.0 : 0.0;
+      es = es * cut;
+      d = d * cut;
+    } else if (sp.interp == TOFE_INTERP_SIMP_LOGISTIC) {
+      const double denom = 1 + exp(sp.p0 * (sp.p1 - z));
+      es = es / denom;
+      d = (d + (es * sp.p0) * (1 - 1 / denom)) / denom;
+    }
+    Es[i] = es;
+    dEsdz[i] = d;
   }
+}
+
+// Chain_Filter middle step (Material_Interpolation.cpp:168-179):
+// out = dfdE .* y - rhoq .* ((y < 1) .* RtT), RtT = R^T(dfdE .* rho) (or nullptr: no R term)
+__global__ void k_chain_mid(const double *__restrict__ dfdE, const double *__restrict__ y,
+                            const double *__restrict__ rhoq, const double *__restrict__ RtT,
+                            double *out, int64_t ne)
+{
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < ne; i += (int64_t)gridDim.x * NT) {
+    const double b = dfdE[i] * y[i];
+    if (RtT) {
+      const double mask = (y[i] < 1.0) ? 1.0 : 0.0;
+      double a = RtT[i] * mask;
+      a = rhoq[i] * a;
+      out[i] = b - a;
+    } else {
+      out[i] = b;
+    }
+  }
+}
+
+__global__ void k_mul(const double *__restrict__ a, const double *__restrict__ b, double *c, int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+    c[i] = a[i] * b[i];
 }
 
 // g_e = -dEdz_e * u_e^T (KE u_e), sequential sums without contraction (Compliance.cpp:37-43)
@@ -228,12 +319,51 @@ void filter_fill(hipStream_t s, const FilterTemplate &t, const double *inv, cons
   TOFE_LAUNCH_CHECK();
 }
 
-void simp(hipStream_t s, const double *rho, int64_t ne, double penal, double eps, double *E,
-          double *dEdz, double *Es, double *dEsdz, float *E32)
+static unsigned stride_blocks(int64_t n)
 {
-  unsigned nb = blocks_for(ne);
-  if (nb > 4096) nb = 4096;
-  k_simp<<<dim3(nb), dim3(NT), 0, s>>>(rho, ne, penal, eps, E, dEdz, Es, dEsdz, E32);
+  unsigned nb = blocks_for(n);
+  return nb > 4096 ? 4096 : (nb ? nb : 1);
+}
+
+void void_pre(hipStream_t s, double *rho, int64_t ne, double q, double *zt)
+{
+  k_void_pre<<<dim3(stride_blocks(ne)), dim3(NT), 0, s>>>(rho, ne, q, zt);
+  TOFE_LAUNCH_CHECK();
+}
+
+void rfilter_madd(hipStream_t s, const FilterTemplate &t, const double *REdge, const double *zt,
+                  double inv_vdmin, double *y)
+{
+  const int64_t ne = (int64_t)t.nx * t.ny * t.nz;
+  k_rfilter_madd<<<dim3(blocks_for(ne)), dim3(NT), 0, s>>>(t, REdge, zt, inv_vdmin, y, ne);
+  TOFE_LAUNCH_CHECK();
+}
+
+void rfilter_T(hipStream_t s, const FilterTemplate &t, const double *g, double *gx)
+{
+  const int64_t ne = (int64_t)t.nx * t.ny * t.nz;
+  k_rfilter_T<<<dim3(blocks_for(ne)), dim3(NT), 0, s>>>(t, g, gx, ne);
+  TOFE_LAUNCH_CHECK();
+}
+
+void simp(hipStream_t s, const double *rho, const double *y, int64_t ne, const SimpParams &sp,
+          double *yout, double *rhoq, double *E, double *dEdz, double *Es, double *dEsdz, float *E32)
+{
+  k_simp<<<dim3(stride_blocks(ne)), dim3(NT), 0, s>>>(rho, y, ne, sp, yout, rhoq, E, dEdz, Es, dEsdz,
+                                                     E32);
+  TOFE_LAUNCH_CHECK();
+}
+
+void chain_mid(hipStream_t s, const double *dfdE, const double *y, const double *rhoq,
+               const double *RtT, double *out, int64_t ne)
+{
+  k_chain_mid<<<dim3(stride_blocks(ne)), dim3(NT), 0, s>>>(dfdE, y, rhoq, RtT, out, ne);
+  TOFE_LAUNCH_CHECK();
+}
+
+void mul(hipStream_t s, const double *a, const double *b, double *c, int64_t n)
+{
+  k_mul<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(a, b, c, n);
   TOFE_LAUNCH_CHECK();
 }
 

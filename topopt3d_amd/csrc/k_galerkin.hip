@@ -1,0 +1,231 @@
+// Level-1 Galerkin operator P^T K P computed element by element from E and KE.
+//
+// Every fine element e interpolates from the 2x2x2 window of coarse nodes starting at
+// (ex/2, ey/2, ez/2); per direction the 2x2 interpolation block q takes one of three
+// forms (even element, odd element, last element of a grid with an even node count whose
+// last node has a single parent, Interpolation.cpp:331-347).  So
+//   P^T K P = sum_e E_e Q_e^T KE Q_e,   Q_e = q_x (x) q_y (x) q_z (x) I_3,
+// with 27 precomputed matrices KEC_f = Q_f^T KE Q_f.  Elements touching a Dirichlet DOF
+// use their own masked matrix E_e Q^T (I-M_e) KE (I-M_e) Q, and the unit diagonal of the
+// fixed rows (MatZeroRowsColumns, FEAnalysis.cpp:473) contributes P^T M P.  The fine
+// stencil is never formed.
+#include "k_common.hpp"
+
+namespace tofe {
+namespace {
+
+constexpr int NT = 256;
+
+// KCm[m] = E_e Q_f^T (I - M_e) KE (I - M_e) Q_f for masked element m (one block each).
+__global__ __launch_bounds__(NT) void k_masked_kc(const MaskedElems me, const double *__restrict__ E,
+                                                  const double *__restrict__ KE,
+                                                  const double *__restrict__ Qforms,
+                                                  double *__restrict__ KCm)
+{
+  __shared__ double sK[576], sX[576], sQ[64];
+  const int m = blockIdx.x;
+  const int64_t e = me.elem[m];
+  const int f = me.form[m];
+  const uint32_t mask = me.mask[m];
+  for (int v = threadIdx.x; v < 576; v += NT) {
+    const int r = v / 24, c = v - 24 * r;
+    sK[v] = (((mask >> r) & 1) || ((mask >> c) & 1)) ? 0.0 : KE[v];
+  }
+  for (int v = threadIdx.x; v < 64; v += NT) sQ[v] = Qforms[f * 64 + v];
+  __syncthreads();
+  // X = KEm Qd, Qd[(3l+a)][(3L+b)] = Q[l][L] delta_ab
+  for (int v = threadIdx.x; v < 576; v += NT) {
+    const int r = v / 24, c = v - 24 * r;
+    const int L = c / 3, b = c - 3 * L;
+    double s = 0;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) s += sK[r * 24 + 3 * l + b] * sQ[l * 8 + L];
+    sX[v] = s;
+  }
+  __syncthreads();
+  const double Ee = E[e];
+  for (int v = threadIdx.x; v < 576; v += NT) {
+    const int r = v / 24, c = v - 24 * r;
+    const int L = r / 3, a = r - 3 * L;
+    double s = 0;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) s += sQ[l * 8 + L] * sX[(3 * l + a) * 24 + c];
+    KCm[(int64_t)m * 576 + v] = Ee * s;
+  }
+}
+
+__device__ inline int elem_form(int ex, int nel, int Nf)
+{
+  if (ex & 1) return 1;
+  if ((Nf & 1) == 0 && ex == Nf - 2) return 2;
+  return 0;
+}
+
+// One thread per (coarse node I, coarse offset o >= 13); the transposed block goes to
+// (J, 26 - o) so the level-1 operator is exactly symmetric.
+__global__ __launch_bounds__(NT) void k_galerkin_fine(const double *__restrict__ E,
+                                                      const int *__restrict__ emap,
+                                                      const double *__restrict__ KEC,
+                                                      const double *__restrict__ KCm,
+                                                      const uint8_t *__restrict__ fix, int nx,
+                                                      int ny, int nz, double *__restrict__ Sc)
+{
+  const int Nf[3] = {nx + 1, ny + 1, nz + 1};
+  const int nel[3] = {nx, ny, nz};
+  const int Nc[3] = {(Nf[0] + 1) / 2, (Nf[1] + 1) / 2, (Nf[2] + 1) / 2};
+  const int64_t ncn = (int64_t)Nc[0] * Nc[1] * Nc[2];
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= 14 * ncn) return;
+  const int o = 13 + (int)(t / ncn);
+  const int64_t I = t - (int64_t)(o - 13) * ncn;
+  const int D[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+  const int Ic[3] = {(int)(I % Nc[0]), (int)((I / Nc[0]) % Nc[1]), (int)(I / ((int64_t)Nc[0] * Nc[1]))};
+  int Jc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    Jc[d] = Ic[d] + D[d];
+    if (Jc[d] < 0 || Jc[d] >= Nc[d]) return;
+  }
+  // per direction: fine elements whose coarse window holds both I and J
+  int ex_l[3][4], lI_l[3][4], lJ_l[3][4], f_l[3][4], cnt[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int c = 0;
+    const int lo = max(max(Ic[d], Jc[d]) - 1, 0), hi = min(min(Ic[d], Jc[d]), Nc[d] - 1);
+    for (int Jlo = lo; Jlo <= hi; ++Jlo)
+      for (int ex = 2 * Jlo; ex <= 2 * Jlo + 1; ++ex) {
+        if (ex >= nel[d]) continue;
+        ex_l[d][c] = ex;
+        lI_l[d][c] = Ic[d] - Jlo;
+        lJ_l[d][c] = Jc[d] - Jlo;
+        f_l[d][c] = elem_form(ex, nel[d], Nf[d]);
+        ++c;
+      }
+    cnt[d] = c;
+  }
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+  for (int cz = 0; cz < cnt[2]; ++cz)
+    for (int cy = 0; cy < cnt[1]; ++cy)
+      for (int cx = 0; cx < cnt[0]; ++cx) {
+        const int64_t e = ex_l[0][cx] + (int64_t)nx * (ex_l[1][cy] + (int64_t)ny * ex_l[2][cz]);
+        const int LI = lidx(lI_l[0][cx], lI_l[1][cy], lI_l[2][cz]);
+        const int LJ = lidx(lJ_l[0][cx], lJ_l[1][cy], lJ_l[2][cz]);
+        const int m = emap[e];
+        if (m < 0) {
+          const double Ee = E[e];
+          const double *K = KEC + (int64_t)(f_l[0][cx] + 3 * f_l[1][cy] + 9 * f_l[2][cz]) * 576;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += Ee * K[(3 * LI + a) * 24 + 3 * LJ + b];
+        } else {
+          const double *K = KCm + (int64_t)m * 576;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += K[(3 * LI + a) * 24 + 3 * LJ + b];
+        }
+      }
+  // P^T M P: unit diagonal of the fixed fine DOFs
+  if (fix) {
+    int il[3][3], cn[3];
+    double wl[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int c = 0;
+      for (int i = 2 * Ic[d] - 1; i <= 2 * Ic[d] + 1; ++i) {
+        if (i < 0 || i >= Nf[d]) continue;
+        const double w = w1d(i, Ic[d], Nf[d]) * w1d(i, Jc[d], Nf[d]);
+        if (w == 0.0) continue;
+        il[d][c] = i;
+        wl[d][c] = w;
+        ++c;
+      }
+      cn[d] = c;
+    }
+    for (int cz = 0; cz < cn[2]; ++cz)
+      for (int cy = 0; cy < cn[1]; ++cy)
+        for (int cx = 0; cx < cn[0]; ++cx) {
+          const int64_t n = il[0][cx] + (int64_t)Nf[0] * (il[1][cy] + (int64_t)Nf[1] * il[2][cz]);
+          const uint8_t fm = fix[n];
+          if (!fm) continue;
+          const double w = wl[0][cx] * wl[1][cy] * wl[2][cz];
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            if ((fm >> a) & 1) acc[a * 4] += w;
+        }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) Sc[(int64_t)(o * 9 + q) * ncn + I] = acc[q];
+  if (o != 13) {
+    const int64_t J = Jc[0] + (int64_t)Nc[0] * (Jc[1] + (int64_t)Nc[1] * Jc[2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Sc[(int64_t)((26 - o) * 9 + b * 3 + a) * ncn + J] = acc[a * 3 + b];
+  }
+}
+
+// Jacobi diagonal of the fine K with Dirichlet rows: sum over the node's elements in
+// ascending order of the rounded products E_e KE_ii (the assembly order of
+// FEAnalysis.cpp:448-464), 1 on fixed DOFs, and diag <= 0 -> 1 (476-484).
+__global__ void k_fine_diag(const double *__restrict__ E, const double *__restrict__ KE,
+                            const uint8_t *__restrict__ fix, int nx, int ny, int nz,
+                            double *diag, double *dinv)
+{
+#pragma clang fp contract(off)
+  const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
+  const int64_t nn = (int64_t)Nx * Ny * Nz;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= 3 * nn) return;
+  const int64_t n = t / 3;
+  const int a = (int)(t - 3 * n);
+  const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+  double s = 0.0;
+  for (int ez = k - 1; ez <= k; ++ez) {
+    if (ez < 0 || ez >= nz) continue;
+    for (int ey = j - 1; ey <= j; ++ey) {
+      if (ey < 0 || ey >= ny) continue;
+      for (int ex = i - 1; ex <= i; ++ex) {
+        if (ex < 0 || ex >= nx) continue;
+        const int l = lidx(i - ex, j - ey, k - ez);
+        s = s + E[ex + (int64_t)nx * (ey + (int64_t)ny * ez)] * KE[(3 * l + a) * 25];
+      }
+    }
+  }
+  if (fix && ((fix[n] >> a) & 1)) s = 1.0;
+  if (!(s > 0)) s = 1.0;
+  diag[t] = s;
+  dinv[t] = 1.0 / s;
+}
+
+}  // namespace
+
+void galerkin_fine(hipStream_t s, const double *E, const int *emap, const double *KEC,
+                   const MaskedElems &me, const double *KE, const double *Qforms, double *KCm,
+                   const uint8_t *fix, int nx, int ny, int nz, double *Sc)
+{
+  if (me.count > 0) {
+    k_masked_kc<<<dim3((unsigned)me.count), dim3(NT), 0, s>>>(me, E, KE, Qforms, KCm);
+    TOFE_LAUNCH_CHECK();
+  }
+  const int64_t ncn = (int64_t)((nx + 2) / 2) * ((ny + 2) / 2) * ((nz + 2) / 2);
+  TOFE_HIP(hipMemsetAsync(Sc, 0, sizeof(double) * 243 * ncn, s));
+  const int64_t nt = 14 * ncn;
+  k_galerkin_fine<<<dim3((unsigned)((nt + NT - 1) / NT)), dim3(NT), 0, s>>>(E, emap, KEC, KCm, fix,
+                                                                           nx, ny, nz, Sc);
+  TOFE_LAUNCH_CHECK();
+}
+
+void fine_diag(hipStream_t s, const double *E, const double *KE, const uint8_t *fix, int nx, int ny,
+               int nz, double *diag, double *dinv)
+{
+  const int64_t nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  k_fine_diag<<<dim3((unsigned)((3 * nn + NT - 1) / NT)), dim3(NT), 0, s>>>(E, KE, fix, nx, ny, nz,
+                                                                          diag, dinv);
+  TOFE_LAUNCH_CHECK();
+}
+
+}  // namespace tofe

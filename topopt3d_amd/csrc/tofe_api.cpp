@@ -7,11 +7,11 @@
 // FP64 flexible CG (truncated FCG(1)); the outer residual, the dot products and the
 // coarse solve stay FP64.
 #include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -96,12 +96,25 @@ struct tofe_ctx {
   DBuf<double> x, rho, E, dEdz, Es, dEsdz;
   DBuf<float> E32;
   bool design_set = false;
+  // maximum length scale: unit-weight filter R, REdge, y, rhoq (Material_Interpolation.cpp:37-60)
+  SimpParams sp;
+  bool has_vd = false;
+  std::vector<int> rtpl_off;
+  DBuf<int> drtpl_off;
+  DBuf<double> drtpl_w, REdge, y, rhoq, ebuf3;
 
   std::vector<std::unique_ptr<Level<double>>> lv64;
   std::vector<std::unique_ptr<Level<float>>> lv32;
   DBuf<double> fineS, diag0;
+  // level-1 Galerkin from E: interpolation forms, masked (Dirichlet) elements
+  DBuf<double> dQforms, dKEC, dKCm;
+  DBuf<int> demap, dme_form;
+  DBuf<int64_t> dme_elem;
+  DBuf<uint32_t> dme_mask;
+  int n_masked = 0;
   DBuf<double> Ainv;
   DBuf<int> dinfo;
+  DBuf<double> chol_w, chol_p, chol_x;
   int64_t nc = 0;
   DBuf<double> partial, partial2, scal, eig_ab;
   int npart = 0;
@@ -113,6 +126,7 @@ struct tofe_ctx {
   DBuf<double> ebuf, ebuf2;
 
   bool assembled = false;
+  bool fine_vector = false;  // TOFE_FINE_VECTOR=1: VALU fine operator instead of MFMA
   int precision = TOFE_PREC_FP64;
   int built_precision = -1;
   tofe_stats stats{};
@@ -122,6 +136,18 @@ struct tofe_ctx {
 };
 
 namespace {
+
+FilterTemplate rfilter_tpl(const tofe_ctx *c)
+{
+  FilterTemplate t;
+  t.off = c->drtpl_off.p;
+  t.w = c->drtpl_w.p;
+  t.ntpl = (int)(c->rtpl_off.size() / 3);
+  t.nx = c->desc.nx;
+  t.ny = c->desc.ny;
+  t.nz = c->desc.nz;
+  return t;
+}
 
 FilterTemplate filter_tpl(const tofe_ctx *c)
 {
@@ -169,7 +195,8 @@ template <typename T>
 void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
 {
   if (l == 0) {
-    fine_op<T>(c->stream, fine_of<T>(c), u, epi, ep);
+    if (c->fine_vector) fine_op<T>(c->stream, fine_of<T>(c), u, epi, ep);
+    else fine_mfma<T>(c->stream, fine_of<T>(c), u, epi, ep);
   } else {
     StencilOp<T> op;
     auto &lv = *levels<T>(c)[l];
@@ -182,7 +209,7 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
 template <typename T>
 int level_blocks(tofe_ctx *c, int l)
 {
-  if (l == 0) return fine_op_blocks<T>(fine_of<T>(c));
+  if (l == 0) return c->fine_vector ? fine_op_blocks<T>(fine_of<T>(c)) : fine_mfma_blocks<T>(fine_of<T>(c));
   StencilOp<T> op;
   op.Nx = c->grids[l].Nx; op.Ny = c->grids[l].Ny; op.Nz = c->grids[l].Nz;
   return stencil_op_blocks<T>(op);
@@ -450,20 +477,83 @@ void chebyshev_coefficients(tofe_ctx *c)
   }
 }
 
+void rb_check(rocblas_status st, const char *what)
+{
+  if (st != rocblas_status_success)
+    throw Error(TOFE_ERR_HIP, std::string(what) + " failed: " + rocblas_status_to_string(st));
+}
+
+// A (dense SPD, column-major, in c->Ainv) -> A^-1 in place: blocked right-looking Cholesky
+// (diagonal blocks factored and inverted in LDS, panels L21 = A21 L11^-T and the trailing
+// update A22 -= L21 L21^T with rocBLAS), then L^-1 (rocBLAS trtri) and A^-1 = L^-T L^-1
+// (rocBLAS syrk), symmetrised for the per-V-cycle GEMV.
+void coarse_inverse(tofe_ctx *c)
+{
+  const hipStream_t s = c->stream;
+  const int n = (int)c->nc;
+  double *A = c->Ainv.p;
+  c->chol_w.alloc((size_t)kCholNB * kCholNB);
+  c->chol_p.alloc((size_t)n * kCholNB);
+  c->chol_x.alloc((size_t)n * n);
+  TOFE_HIP(hipMemsetAsync(c->dinfo.p, 0, 2 * sizeof(int), s));
+  const double one = 1.0, zero = 0.0, mone = -1.0;
+  for (int k0 = 0; k0 < n; k0 += kCholNB) {
+    const int kb = std::min(kCholNB, n - k0);
+    double *Akk = A + k0 + (size_t)k0 * n;
+    chol_block(s, Akk, n, kb, k0, c->chol_w.p, c->dinfo.p);
+    const int m = n - k0 - kb;
+    if (m <= 0) break;
+    double *A21 = Akk + kb;
+    rb_check(rocblas_dgemm(c->rb, rocblas_operation_none, rocblas_operation_transpose, m, kb, kb, &one,
+                           A21, n, c->chol_w.p, kCholNB, &zero, c->chol_p.p, m),
+             "rocblas_dgemm");
+    TOFE_HIP(hipMemcpy2DAsync(A21, (size_t)n * sizeof(double), c->chol_p.p, (size_t)m * sizeof(double),
+                              (size_t)m * sizeof(double), kb, hipMemcpyDeviceToDevice, s));
+    rb_check(rocblas_dsyrk(c->rb, rocblas_fill_lower, rocblas_operation_none, m, kb, &mone, A21, n,
+                           &one, A21 + (size_t)kb * n, n),
+             "rocblas_dsyrk");
+  }
+  int info[2] = {0, 0};
+  TOFE_HIP(hipMemcpyAsync(info, c->dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  TOFE_HIP(hipStreamSynchronize(s));
+  if (info[0] != 0)
+    throw Error(TOFE_ERR_NUMERIC, "coarse operator is not positive definite (pivot " +
+                                      std::to_string(info[0]) + ")");
+  TOFE_HIP(hipMemsetAsync(c->chol_x.p, 0, (size_t)n * n * sizeof(double), s));
+  rb_check(rocblas_dtrtri(c->rb, rocblas_fill_lower, rocblas_diagonal_non_unit, n, A, n, c->chol_x.p, n),
+           "rocblas_dtrtri");
+  rb_check(rocblas_dsyrk(c->rb, rocblas_fill_lower, rocblas_operation_transpose, n, n, &one,
+                         c->chol_x.p, n, &zero, A, n),
+           "rocblas_dsyrk");
+  symmetrize_lower(s, A, n);
+}
+
 void assemble(tofe_ctx *c)
 {
   const double t0 = now_s();
   const hipStream_t s = c->stream;
   const int L = c->L;
-  // fine level: assembled stencil of K (for the Galerkin product and the diagonal)
-  c->fineS.alloc((size_t)243 * c->nn);
+  // fine level: Jacobi diagonal of K; level 1 = P^T K P element by element from E
   c->diag0.alloc(c->ndof);
-  fine_stencil_build(s, fine_of<double>(c), c->fineS.p);
-  stencil_diag(s, c->fineS.p, c->nn, c->diag0.p, c->lv64[0]->dinv.p);
-  for (int l = 0; l + 1 < L; ++l) {
-    const double *Sf = (l == 0) ? c->fineS.p : c->lv64[l]->S.p;
+  const uint8_t *fix = c->has_fix ? c->dfix.p : nullptr;
+  fine_diag(s, c->E.p, c->dKE.p, fix, c->desc.nx, c->desc.ny, c->desc.nz, c->diag0.p,
+            c->lv64[0]->dinv.p);
+  if (L > 1) {
+    MaskedElems me;
+    me.elem = c->dme_elem.p;
+    me.form = c->dme_form.p;
+    me.mask = c->dme_mask.p;
+    me.count = c->n_masked;
+    galerkin_fine(s, c->E.p, c->demap.p, c->dKEC.p, me, c->dKE.p, c->dQforms.p, c->dKCm.p, fix,
+                  c->desc.nx, c->desc.ny, c->desc.nz, c->lv64[1]->S.p);
+    stencil_diag(s, c->lv64[1]->S.p, c->grids[1].nn(), nullptr, c->lv64[1]->dinv.p);
+  } else {
+    c->fineS.alloc((size_t)243 * c->nn);
+    fine_stencil_build(s, fine_of<double>(c), c->fineS.p);
+  }
+  for (int l = 1; l + 1 < L; ++l) {
     const Grid &gf = c->grids[l], &gc = c->grids[l + 1];
-    galerkin(s, Sf, gf.Nx, gf.Ny, gf.Nz, c->lv64[l + 1]->S.p, gc.Nx, gc.Ny, gc.Nz);
+    galerkin(s, c->lv64[l]->S.p, gf.Nx, gf.Ny, gf.Nz, c->lv64[l + 1]->S.p, gc.Nx, gc.Ny, gc.Nz);
     stencil_diag(s, c->lv64[l + 1]->S.p, gc.nn(), nullptr, c->lv64[l + 1]->dinv.p);
   }
   // coarsest level: dense Cholesky factor and explicit inverse (FP64)
@@ -473,19 +563,7 @@ void assemble(tofe_ctx *c)
   c->Ainv.alloc((size_t)c->nc * c->nc);
   c->dinfo.alloc(2);
   stencil_to_dense(s, SL, gL.Nx, gL.Ny, gL.Nz, c->Ainv.p);
-  rocblas_status st = rocsolver_dpotrf(c->rb, rocblas_fill_lower, (rocblas_int)c->nc, c->Ainv.p,
-                                       (rocblas_int)c->nc, c->dinfo.p);
-  if (st != rocblas_status_success) throw Error(TOFE_ERR_HIP, "rocsolver_dpotrf failed");
-  int info[2] = {0, 0};
-  TOFE_HIP(hipMemcpyAsync(info, c->dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  TOFE_HIP(hipStreamSynchronize(s));
-  if (info[0] != 0)
-    throw Error(TOFE_ERR_NUMERIC, "coarse operator is not positive definite (potrf info " +
-                                      std::to_string(info[0]) + ")");
-  st = rocsolver_dpotri(c->rb, rocblas_fill_lower, (rocblas_int)c->nc, c->Ainv.p,
-                        (rocblas_int)c->nc, c->dinfo.p);
-  if (st != rocblas_status_success) throw Error(TOFE_ERR_HIP, "rocsolver_dpotri failed");
-  symmetrize_lower(s, c->Ainv.p, c->nc);
+  coarse_inverse(c);
   // Chebyshev bounds from lambda_max(D^-1 A) per smoothed level
   lanczos_lambdas(c);
   chebyshev_coefficients(c);
@@ -546,11 +624,31 @@ void upload_design(tofe_ctx *c, const double *dx)
   if (dx != c->x.p)
     TOFE_HIP(hipMemcpyAsync(c->x.p, dx, c->ne * sizeof(double), hipMemcpyDeviceToDevice, s));
   filter_apply(s, t, c->filt_inv.p, c->x.p, c->rho.p);
-  simp(s, c->rho.p, c->ne, c->desc.penal, c->desc.Emin, c->E.p, c->dEdz.p, c->Es.p, c->dEsdz.p,
-       c->E32.p);
+  if (c->has_vd) {
+    // rho <- min(rho, 1); y = min(1, (REdge + R (1-rho)^q) / vdMin)
+    void_pre(s, c->rho.p, c->ne, c->sp.vd_penal, c->ebuf3.p);
+    rfilter_madd(s, rfilter_tpl(c), c->REdge.p, c->ebuf3.p, 1 / c->sp.vd_min, c->y.p);
+  }
+  simp(s, c->rho.p, c->has_vd ? c->y.p : nullptr, c->ne, c->sp, c->y.p, c->rhoq.p, c->E.p,
+       c->dEdz.p, c->Es.p, c->dEsdz.p, c->E32.p);
   TOFE_HIP(hipStreamSynchronize(s));
   c->design_set = true;
   c->assembled = false;
+}
+
+// Chain_Filter for dfdE, device to device (Material_Interpolation.cpp:164-183);
+// g is overwritten when the maximum length scale is on.
+void chain_dev(tofe_ctx *c, double *g, double *out)
+{
+  const hipStream_t s = c->stream;
+  if (c->has_vd) {
+    mul(s, g, c->rho.p, c->ebuf3.p, c->ne);
+    rfilter_T(s, rfilter_tpl(c), c->ebuf3.p, c->ebuf2.p);
+    chain_mid(s, g, c->y.p, c->rhoq.p, c->ebuf2.p, c->ebuf3.p, c->ne);
+    filter_apply_T(s, filter_tpl(c), c->filt_inv.p, c->ebuf3.p, out);
+  } else {
+    filter_apply_T(s, filter_tpl(c), c->filt_inv.p, g, out);
+  }
 }
 
 void need_assembled(tofe_ctx *c)
@@ -612,10 +710,23 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
   if (!(d.cheb_lo > 0)) d.cheb_lo = 0.1;
   if (d.eig_its <= 0) d.eig_its = 10;
   if (d.penal == 0) d.penal = 3.0;
+  if (d.vd_penal == 0) d.vd_penal = 1.0;
+  if (d.interpolation < TOFE_INTERP_SIMP || d.interpolation > TOFE_INTERP_SIMP_LOGISTIC)
+    return TOFE_ERR_ARG;
+  if (d.vd_min > 0 && !(d.rmax > 0)) return TOFE_ERR_ARG;
+  c->sp.penal = d.penal;
+  c->sp.eps = d.Emin;
+  c->sp.vd_penal = d.vd_penal;
+  c->sp.vd_min = d.vd_min > 0 ? d.vd_min : -1.0;
+  c->sp.interp = d.interpolation;
+  c->sp.p0 = d.interp_param[0];
+  c->sp.p1 = d.interp_param[1];
+  c->has_vd = d.vd_min > 0;
   d.fixed_dof = nullptr;
   c->precision = (desc->precision == TOFE_PREC_MIXED) ? TOFE_PREC_MIXED : TOFE_PREC_FP64;
   try {
     TOFE_HIP(hipGetDevice(&c->device));
+    if (const char *fv = std::getenv("TOFE_FINE_VECTOR")) c->fine_vector = fv[0] == '1';
     // level count: floor(log2(min nel)) unless given (RecMesh.cpp:552-554), truncated
     // once a coarse grid has <= 10 DOFs (Interpolation.cpp:70-71)
     int L = d.mg_levels;
@@ -663,6 +774,35 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     h2d(c.get(), c->dtpl_w.p, c->tpl_w.data(), c->tpl_w.size());
     c->filt_inv.alloc(c->ne);
     filter_rowsum_inv(c->stream, filter_tpl(c.get()), c->filt_inv.p);
+    c->y.alloc(c->ne);
+    c->rhoq.alloc(c->ne);
+    if (c->has_vd) {
+      // R: the same neighbourhood rule with radius Rmax and unit weights (RecMesh.cpp:544,
+      // 706-718); REdge = max_i rowsum_i - rowsum (Material_Interpolation.cpp:37-41)
+      std::vector<double> rw;
+      if (filter_template(d.hx, d.hy, d.hz, d.rmax, c->rtpl_off, rw) < 1 || rw.size() > 1024)
+        throw Error(TOFE_ERR_ARG, "maximum length scale radius gives an unsupported neighbourhood");
+      std::fill(rw.begin(), rw.end(), 1.0);
+      c->drtpl_off.alloc(c->rtpl_off.size());
+      c->drtpl_w.alloc(rw.size());
+      TOFE_HIP(hipMemcpyAsync(c->drtpl_off.p, c->rtpl_off.data(), c->rtpl_off.size() * sizeof(int),
+                              hipMemcpyHostToDevice, c->stream));
+      h2d(c.get(), c->drtpl_w.p, rw.data(), rw.size());
+      DBuf<int64_t> cnt;
+      cnt.alloc(c->ne);
+      filter_count(c->stream, rfilter_tpl(c.get()), cnt.p);
+      std::vector<int64_t> hc(c->ne);
+      TOFE_HIP(hipMemcpyAsync(hc.data(), cnt.p, c->ne * sizeof(int64_t), hipMemcpyDeviceToHost,
+                              c->stream));
+      TOFE_HIP(hipStreamSynchronize(c->stream));
+      int64_t mx = 0;
+      for (int64_t v : hc) mx = std::max(mx, v);
+      std::vector<double> re(c->ne);
+      for (int64_t i = 0; i < c->ne; ++i) re[i] = -1 * (double)hc[i] + (double)mx;
+      c->REdge.alloc(c->ne);
+      h2d(c.get(), c->REdge.p, re.data(), c->ne);
+      c->ebuf3.alloc(c->ne);
+    }
 
     c->dfix.alloc(c->nn);
     if (desc->fixed_dof) {
@@ -677,6 +817,54 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
       c->dfix.zero(c->stream);
     }
 
+    {
+      // interpolation forms and the elements that touch a Dirichlet DOF
+      std::vector<double> Qf(27 * 64), KEC(27 * 576);
+      galerkin_forms(c->KE, Qf.data(), KEC.data());
+      c->dQforms.alloc(Qf.size());
+      c->dKEC.alloc(KEC.size());
+      h2d(c.get(), c->dQforms.p, Qf.data(), Qf.size());
+      h2d(c.get(), c->dKEC.p, KEC.data(), KEC.size());
+      std::vector<int> emap(c->ne, -1), mform;
+      std::vector<int64_t> melem;
+      std::vector<uint32_t> mmask;
+      if (desc->fixed_dof) {
+        const int nx = d.nx, ny = d.ny, nz = d.nz;
+        const int64_t Nx = nx + 1, Ny = ny + 1;
+        auto form = [](int ex, int nel) { return (ex & 1) ? 1 : (((nel + 1) % 2 == 0 && ex == nel - 1) ? 2 : 0); };
+        static const int LN[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                     {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+        for (int ez = 0; ez < nz; ++ez)
+          for (int ey = 0; ey < ny; ++ey)
+            for (int ex = 0; ex < nx; ++ex) {
+              uint32_t mask = 0;
+              for (int l = 0; l < 8; ++l) {
+                const int64_t n = (ex + LN[l][0]) + Nx * ((ey + LN[l][1]) + Ny * (ez + LN[l][2]));
+                for (int a = 0; a < 3; ++a)
+                  if (desc->fixed_dof[3 * n + a]) mask |= 1u << (3 * l + a);
+              }
+              if (!mask) continue;
+              const int64_t e = ex + (int64_t)nx * (ey + (int64_t)ny * ez);
+              emap[e] = (int)melem.size();
+              melem.push_back(e);
+              mform.push_back(form(ex, nx) + 3 * form(ey, ny) + 9 * form(ez, nz));
+              mmask.push_back(mask);
+            }
+      }
+      c->n_masked = (int)melem.size();
+      c->demap.alloc(c->ne);
+      TOFE_HIP(hipMemcpyAsync(c->demap.p, emap.data(), c->ne * sizeof(int), hipMemcpyHostToDevice, c->stream));
+      if (c->n_masked) {
+        c->dme_elem.alloc(melem.size());
+        c->dme_form.alloc(mform.size());
+        c->dme_mask.alloc(mmask.size());
+        c->dKCm.alloc((size_t)576 * melem.size());
+        TOFE_HIP(hipMemcpyAsync(c->dme_elem.p, melem.data(), melem.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+        TOFE_HIP(hipMemcpyAsync(c->dme_form.p, mform.data(), mform.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        TOFE_HIP(hipMemcpyAsync(c->dme_mask.p, mmask.data(), mmask.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+      }
+      TOFE_HIP(hipStreamSynchronize(c->stream));
+    }
     for (auto *b : {&c->x, &c->rho, &c->E, &c->dEdz, &c->Es, &c->dEsdz, &c->ebuf, &c->ebuf2})
       b->alloc(c->ne);
     c->E32.alloc(c->ne);
@@ -782,6 +970,12 @@ int tofe_get_field(const tofe_ctx *cc, int field, double *out)
       case TOFE_FIELD_DEDZ: src = c->dEdz.p; break;
       case TOFE_FIELD_ES: src = c->Es.p; break;
       case TOFE_FIELD_DESDZ: src = c->dEsdz.p; break;
+      case TOFE_FIELD_Y: src = c->y.p; break;
+      case TOFE_FIELD_RHOQ: src = c->rhoq.p; break;
+      case TOFE_FIELD_REDGE:
+        if (!c->has_vd) throw Error(TOFE_ERR_STATE, "no maximum length scale (vd_min <= 0)");
+        src = c->REdge.p;
+        break;
       case TOFE_FIELD_DIAG:
         if (!c->assembled) throw Error(TOFE_ERR_STATE, "diagonal needs tofe_assemble");
         src = c->diag0.p;
@@ -890,7 +1084,7 @@ int tofe_compliance_dev(tofe_ctx *c, const double *dU, const double *dF, double 
     }
     if (d_dcdx) {
       sensitivity(c->stream, dU, c->dEdz.p, c->dKE.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ebuf.p);
-      filter_apply_T(c->stream, filter_tpl(c), c->filt_inv.p, c->ebuf.p, d_dcdx);
+      chain_dev(c, c->ebuf.p, d_dcdx);
     }
     TOFE_HIP(hipStreamSynchronize(c->stream));
   });
@@ -911,8 +1105,8 @@ int tofe_compliance(tofe_ctx *c, const double *U, const double *F, double *cval,
     }
     if (dcdx) {
       sensitivity(c->stream, c->vp.p, c->dEdz.p, c->dKE.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ebuf.p);
-      filter_apply_T(c->stream, filter_tpl(c), c->filt_inv.p, c->ebuf.p, c->ebuf2.p);
-      d2h(c, dcdx, c->ebuf2.p, c->ne);
+      chain_dev(c, c->ebuf.p, c->vr.p);
+      d2h(c, dcdx, c->vr.p, c->ne);
     }
     TOFE_HIP(hipStreamSynchronize(c->stream));
   });
@@ -926,6 +1120,32 @@ int tofe_sensitivity(tofe_ctx *c, const double *U, double *g)
     h2d(c, c->vp.p, U, c->ndof);
     sensitivity(c->stream, c->vp.p, c->dEdz.p, c->dKE.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ebuf.p);
     d2h(c, g, c->ebuf.p, c->ne);
+  });
+}
+
+int tofe_set_penalty(tofe_ctx *c, double penal, double vd_penal)
+{
+  return guarded(c, [&] {
+    if (!(penal > 0) || !(vd_penal > 0)) throw Error(TOFE_ERR_ARG, "penalties must be positive");
+    c->desc.penal = c->sp.penal = penal;
+    c->desc.vd_penal = c->sp.vd_penal = vd_penal;
+  });
+}
+
+int tofe_chain_filter(tofe_ctx *c, double *dfdV, double *dfdE)
+{
+  return guarded(c, [&] {
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    if (dfdV) {
+      h2d(c, c->ebuf.p, dfdV, c->ne);
+      filter_apply_T(c->stream, filter_tpl(c), c->filt_inv.p, c->ebuf.p, c->vr.p);
+      d2h(c, dfdV, c->vr.p, c->ne);
+    }
+    if (dfdE) {
+      h2d(c, c->ebuf.p, dfdE, c->ne);
+      chain_dev(c, c->ebuf.p, c->vr.p);
+      d2h(c, dfdE, c->vr.p, c->ne);
+    }
   });
 }
 
@@ -1003,6 +1223,10 @@ int tofe_get_level_stencil(tofe_ctx *c, int level, double *out, int64_t *nnode_l
     if (!c->assembled) throw Error(TOFE_ERR_STATE, "not assembled");
     const int64_t nn = c->grids[level].nn();
     if (nnode_l) *nnode_l = nn;
+    if (level == 0 && out) {
+      c->fineS.alloc((size_t)243 * c->nn);
+      fine_stencil_build(c->stream, fine_of<double>(c), c->fineS.p);
+    }
     if (out) d2h(c, out, level == 0 ? c->fineS.p : c->lv64[level]->S.p, (size_t)243 * nn);
   });
 }

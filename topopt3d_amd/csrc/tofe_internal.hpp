@@ -78,6 +78,11 @@ int fine_op_blocks(const FineOp<T> &op);
 template <typename T>
 void fine_op(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
 void fine_stencil_build(hipStream_t s, const FineOp<double> &op, double *S);
+// ---- fine operator on the matrix cores (k_fine_mfma.hip) -------------------
+template <typename T>
+int fine_mfma_blocks(const FineOp<T> &op);
+template <typename T>
+void fine_mfma(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
 
 // ---- coarse operators and Galerkin products (k_coarse.hip) ---------------
 template <typename T>
@@ -89,8 +94,23 @@ void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double
 void stencil_diag(hipStream_t s, const double *S, int64_t nn, double *diag, double *dinv);
 void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A);
 void symmetrize_lower(hipStream_t s, double *A, int64_t n);
+constexpr int kCholNB = 64;  // diagonal block of the coarse Cholesky factorisation
+void chol_block(hipStream_t s, double *A, int lda, int kb, int k0, double *W, int *info);
 template <typename Tin, typename Tout>
 void dense_gemv(hipStream_t s, const double *A, int64_t n, const Tin *x, Tout *y);
+
+// ---- level-1 Galerkin operator from E and KE (k_galerkin.hip) -------------
+struct MaskedElems {
+  const int64_t *elem = nullptr;   // element index
+  const int *form = nullptr;       // interpolation form fx + 3 fy + 9 fz
+  const uint32_t *mask = nullptr;  // bit r set: local DOF r is a Dirichlet DOF
+  int count = 0;
+};
+void galerkin_fine(hipStream_t s, const double *E, const int *emap, const double *KEC,
+                   const MaskedElems &me, const double *KE, const double *Qforms, double *KCm,
+                   const uint8_t *fix, int nx, int ny, int nz, double *Sc);
+void fine_diag(hipStream_t s, const double *E, const double *KE, const uint8_t *fix, int nx, int ny,
+               int nz, double *diag, double *dinv);
 
 // ---- vector kernels (k_vec.hip) -------------------------------------------
 template <typename T>
@@ -146,8 +166,20 @@ void filter_apply_T(hipStream_t s, const FilterTemplate &t, const double *inv, c
 void filter_count(hipStream_t s, const FilterTemplate &t, int64_t *cnt);
 void filter_fill(hipStream_t s, const FilterTemplate &t, const double *inv, const int64_t *rowptr,
                  int32_t *col, double *w);
-void simp(hipStream_t s, const double *rho, int64_t ne, double penal, double eps, double *E,
-          double *dEdz, double *Es, double *dEsdz, float *E32);
+struct SimpParams {
+  double penal = 3, eps = 1e-10, vd_penal = 1, vd_min = -1;
+  int interp = TOFE_INTERP_SIMP;
+  double p0 = 0, p1 = 0;
+};
+void void_pre(hipStream_t s, double *rho, int64_t ne, double q, double *zt);
+void rfilter_madd(hipStream_t s, const FilterTemplate &t, const double *REdge, const double *zt,
+                  double inv_vdmin, double *y);
+void rfilter_T(hipStream_t s, const FilterTemplate &t, const double *g, double *gx);
+void simp(hipStream_t s, const double *rho, const double *y, int64_t ne, const SimpParams &sp,
+          double *yout, double *rhoq, double *E, double *dEdz, double *Es, double *dEsdz, float *E32);
+void chain_mid(hipStream_t s, const double *dfdE, const double *y, const double *rhoq,
+               const double *RtT, double *out, int64_t ne);
+void mul(hipStream_t s, const double *a, const double *b, double *c, int64_t n);
 void sensitivity(hipStream_t s, const double *U, const double *dEdz, const double *KE, int nx,
                  int ny, int nz, double *g);
 
