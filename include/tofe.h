@@ -72,7 +72,8 @@ enum {
   TOFE_FIELD_DIAG = 7,   /* Jacobi diagonal of fine K     (ndof)  */
   TOFE_FIELD_Y = 8,      /* max-length-scale factor y     (nelem) */
   TOFE_FIELD_RHOQ = 9,   /* rhoq = (q/vdMin)(1-rho)^(q-1) (nelem) */
-  TOFE_FIELD_REDGE = 10  /* REdge = max row count of R - row count (nelem) */
+  TOFE_FIELD_REDGE = 10, /* REdge = max row count of R - row count (nelem) */
+  TOFE_FIELD_MDIAG = 11  /* lumped mass diagonal           (ndof)  */
 };
 
 typedef struct tofe_desc {
@@ -168,6 +169,23 @@ int tofe_solve_dev(tofe_ctx *ctx, const double *d_F, double *d_U, int nrhs, doub
  * MatMult(K) as used by PRINVIT/LOPGMRES (PRINVIT.cpp:353-354). */
 int tofe_apply_K(tofe_ctx *ctx, const double *X, double *Y, int nrhs);
 int tofe_apply_K_dev(tofe_ctx *ctx, const double *d_X, double *d_Y, int nrhs);
+
+/* Lumped mass matrix of the current design (Frequency::DiagMassFnc, Dynamic.cpp:181-236):
+ * M_dd = sum over the node's elements of V_e * (elemSize * density / 24), plus mlump_d
+ * (the MASS boundary condition's MLump, FEAnalysis.cpp:374-384; may be NULL), with the
+ * Dirichlet rows and columns of MatZeroRowsColumns(M, fixed, 1e-8) (Dynamic.cpp:44-45). */
+int tofe_assemble_mass(tofe_ctx *ctx, double density, const double *mlump);
+
+/* Y = M X for nrhs vectors: MatMult(M) of the block eigensolvers (LOPGMRES/PRINVIT
+ * operators set at Dynamic.cpp:67). */
+int tofe_apply_M(tofe_ctx *ctx, const double *X, double *Y, int nrhs);
+int tofe_apply_M_dev(tofe_ctx *ctx, const double *d_X, double *d_Y, int nrhs);
+
+/* Element quadratic forms of nvec vectors phi (ndof each): kq[j*nelem+e] = phi_e^T KE phi_e
+ * and, if mq is not NULL, mq[j*nelem+e] = sum of phi^2 over the element's 24 DOFs -- the
+ * per-element terms of the eigenvalue sensitivities (Dynamic.cpp:113-154: dKdy = dEdz KE,
+ * dMdy = dVdrho m0 I). */
+int tofe_element_quadratic(tofe_ctx *ctx, const double *phi, int nvec, double *kq, double *mq);
 
 /* Z = M^-1 R, one multigrid V-cycle; PCApply of the KUF preconditioner
  * (LOPGMRES.cpp:125). */

@@ -34,6 +34,10 @@ class TopOpt {
  public:
   // material and solver parameters (TopOpt.h: E0, Nu0, penal; KSP rtol 1e-8 FEAnalysis.cpp:390)
   double E0 = 1.0, Nu0 = 0.3, penal = 3.0, eps = 1e-10;
+  // maximum length scale and interpolation (TopOpt.h:167-178): vdMin <= 0 disables it
+  double vdPenal = 1.0, vdMin = -1.0;
+  int interpolation = TOFE_INTERP_SIMP;
+  double interp_param[2] = {0, 0};
   int mg_levels = 0, smoother = TOFE_SMOOTHER_CHEBYSHEV, precision = TOFE_PREC_FP64;
   double rtol = 1e-8;
   int maxit = 10000;
@@ -42,7 +46,7 @@ class TopOpt {
   int Nel[3] = {0, 0, 0};
   double dimensions[6] = {0, 0, 0, 0, 0, 0};
   double dx[3] = {0, 0, 0};
-  double Rmin = 0;
+  double Rmin = 0, Rmax = 0;
   int64_t nElem = 0, nNode = 0;
 
   // boundary conditions (suppNode/supports, loadNode/loads)
@@ -52,7 +56,7 @@ class TopOpt {
 
   // FE state
   std::vector<uint8_t> fixed_dof;
-  std::vector<double> F, U, x, rho, E, dEdz, V;
+  std::vector<double> F, U, x, rho, y, rhoq, E, dEdz, Es, dEsdz, V;
   int KUF_reason = 0, its = 0;
 
   TopOpt() = default;
@@ -67,8 +71,9 @@ class TopOpt {
     return 0;
   }
 
-  int CreateMesh(const double dims[6], const int nel[3], double rmin)
+  int CreateMesh(const double dims[6], const int nel[3], double rmin, double rmax = 0)
   {
+    Rmax = rmax;
     for (int i = 0; i < 6; ++i) dimensions[i] = dims[i];
     for (int i = 0; i < 3; ++i) {
       Nel[i] = nel[i];
@@ -132,21 +137,26 @@ class TopOpt {
     d.nu_pre = 2; d.nu_post = 2;
     d.precision = precision;
     d.fixed_dof = fixed_dof.data();
+    d.rmax = Rmax; d.vd_min = vdMin; d.vd_penal = vdPenal;
+    d.interpolation = interpolation;
+    d.interp_param[0] = interp_param[0]; d.interp_param[1] = interp_param[1];
     return tofe_create(&d, &ctx_);
   }
 
-  // rho = P x and the SIMP interpolation (Material_Interpolation.cpp:20-140)
+  // rho = P x, y, and the SIMP / SIMP_CUT / SIMP_LOGISTIC interpolation with the current
+  // penal and vdPenal (Material_Interpolation.cpp:20-140; Main.cpp:131-133)
   int MatIntFnc(const std::vector<double> &design)
   {
     x = design;
-    int ierr = tofe_set_design(ctx_, x.data());
-    if (ierr) return ierr;
-    rho.resize(nElem);
-    E.resize(nElem);
-    dEdz.resize(nElem);
-    if ((ierr = tofe_get_field(ctx_, TOFE_FIELD_RHO, rho.data()))) return ierr;
-    if ((ierr = tofe_get_field(ctx_, TOFE_FIELD_E, E.data()))) return ierr;
-    if ((ierr = tofe_get_field(ctx_, TOFE_FIELD_DEDZ, dEdz.data()))) return ierr;
+    int ierr = tofe_set_penalty(ctx_, penal, vdPenal);
+    if (ierr || (ierr = tofe_set_design(ctx_, x.data()))) return ierr;
+    const struct { int f; std::vector<double> *v; } out[] = {
+        {TOFE_FIELD_RHO, &rho}, {TOFE_FIELD_Y, &y}, {TOFE_FIELD_RHOQ, &rhoq}, {TOFE_FIELD_E, &E},
+        {TOFE_FIELD_DEDZ, &dEdz}, {TOFE_FIELD_ES, &Es}, {TOFE_FIELD_DESDZ, &dEsdz}};
+    for (const auto &o : out) {
+      o.v->resize(nElem);
+      if ((ierr = tofe_get_field(ctx_, o.f, o.v->data()))) return ierr;
+    }
     V = rho;
     return 0;
   }
@@ -156,20 +166,11 @@ class TopOpt {
   // KSPSolve(KUF, F, U) warm-started from U (FEAnalysis.cpp:389, 673)
   int FESolve() { return tofe_solve(ctx_, F.data(), U.data(), 1, rtol, maxit, &its, &KUF_reason); }
 
-  // dfdV <- P^T dfdV, dfdE <- P^T dfdE (plain density filter; Material_Interpolation.cpp:150-187)
+  // dfdV <- P^T dfdV, dfdE <- P^T[dfdE .* y - rhoq .* (y < 1) .* R^T(dfdE .* rho)]
+  // in place (Material_Interpolation.cpp:150-187)
   int Chain_Filter(std::vector<double> *dfdV, std::vector<double> *dfdE)
   {
-    std::vector<double> tmp(nElem);
-    int ierr = 0;
-    if (dfdV) {
-      if ((ierr = tofe_filter_T(ctx_, dfdV->data(), tmp.data()))) return ierr;
-      *dfdV = tmp;
-    }
-    if (dfdE) {
-      if ((ierr = tofe_filter_T(ctx_, dfdE->data(), tmp.data()))) return ierr;
-      *dfdE = tmp;
-    }
-    return 0;
+    return tofe_chain_filter(ctx_, dfdV ? dfdV->data() : nullptr, dfdE ? dfdE->data() : nullptr);
   }
 
   std::string LastError() const { return ctx_ ? tofe_last_error(ctx_) : "no context"; }
@@ -179,7 +180,7 @@ class TopOpt {
   tofe_ctx *ctx_ = nullptr;
 };
 
-// c = U.F and dc/dx = P^T(-dEdz u_e^T KE u_e) (Compliance.cpp:15-57)
+// c = U.F and dc/dx = Chain_Filter(-dEdz u_e^T KE u_e) (Compliance.cpp:15-57)
 class Compliance {
  public:
   bool calc_gradient = true;

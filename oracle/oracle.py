@@ -93,6 +93,7 @@ def lib():
     L.orc_mg_setup_seconds.argtypes = [vp]
     L.orc_dot.restype = C.c_double
     L.orc_dot.argtypes = [C.c_int64, _f64p, _f64p]
+    L.orc_mass_diag.argtypes = [C.c_int] * 3 + [_f64p, C.c_double, vp, vp, _f64p]
     L.orc_sensitivity.argtypes = [C.c_int] * 3 + [_f64p, _f64p, _f64p, _f64p]
     L.orc_power_start.argtypes = [C.c_int64, _f64p]
     L.orc_num_threads.restype = C.c_int
@@ -240,6 +241,21 @@ def sensitivity(nx, ny, nz, KE, U, dEdz):
                           np.ascontiguousarray(U, np.float64),
                           np.ascontiguousarray(dEdz, np.float64), g)
     return g
+
+
+def mass_diag(nx, ny, nz, V, density, h=(1.0, 1.0, 1.0), mlump=None, fixed_dof=None):
+    """Lumped mass diagonal (Dynamic.cpp:181-236, 44-45)."""
+    elem = 1.0
+    for v in h:
+        elem *= v
+    m0 = 1.0 / 2.0 ** 3 / 3 * elem * density
+    M = np.empty(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    ml = None if mlump is None else np.ascontiguousarray(mlump, np.float64)
+    fx = None if fixed_dof is None else np.ascontiguousarray(fixed_dof, np.uint8)
+    lib().orc_mass_diag(nx, ny, nz, np.ascontiguousarray(V, np.float64), m0,
+                        None if ml is None else ml.ctypes.data_as(C.c_void_p),
+                        None if fx is None else fx.ctypes.data_as(C.c_void_p), M)
+    return M
 
 
 def power_start(n):

@@ -1029,6 +1029,29 @@ int orc_pcg(orc_mg *mg, const double *b, double *x, double rtol, int maxit, int 
  * ==================================================================== */
 
 /* g_e = -dEdz_e * u_e^T (KE u_e) */
+void orc_mass_diag(int nx, int ny, int nz, const double *V, double m0, const double *mlump,
+                   const uint8_t *fixed, double *M)
+{
+  int64_t Nx = nx + 1, Ny = ny + 1;
+  int64_t ndof = 3 * Nx * Ny * (nz + 1);
+  for (int64_t i = 0; i < ndof; i++) M[i] = 0.0;
+  for (int ez = 0; ez < nz; ez++)
+    for (int ey = 0; ey < ny; ey++)
+      for (int ex = 0; ex < nx; ex++) {
+        int64_t el = ex + (int64_t)nx * (ey + (int64_t)ny * ez);
+        double v = V[el] * m0;                  /* nodeMat = p_V[el] * mMat block */
+        for (int l = 0; l < 8; l++) {
+          int64_t n = (ex + LN[l][0]) + Nx * ((ey + LN[l][1]) + Ny * (ez + LN[l][2]));
+          for (int c = 0; c < 3; c++) M[3 * n + c] += v;
+        }
+      }
+  if (mlump)
+    for (int64_t i = 0; i < ndof; i++) M[i] = M[i] + mlump[i];
+  if (fixed)
+    for (int64_t i = 0; i < ndof; i++)
+      if (fixed[i]) M[i] = 1e-8;
+}
+
 void orc_sensitivity(int nx, int ny, int nz, const double KE[576], const double *U,
                      const double *dEdz, double *g)
 {

@@ -99,6 +99,13 @@ double orc_mg_setup_seconds(const orc_mg *mg);
 
 /* ---- functions ------------------------------------------------------------ */
 double orc_dot(int64_t n, const double *a, const double *b);
+/* Lumped mass diagonal, Frequency::DiagMassFnc (Dynamic.cpp:181-236) and the Dirichlet
+ * treatment MatZeroRowsColumns(M, fixed, 1e-8) (Dynamic.cpp:44-45): element blocks
+ * V_e * m0 * I(24), m0 = 1/2^3/3 * elemSize * density, added element by element, then
+ * MLump (may be NULL), then 1e-8 on fixed DOFs (fixed may be NULL). */
+void orc_mass_diag(int nx, int ny, int nz, const double *V, double m0, const double *mlump,
+                   const uint8_t *fixed, double *M);
+
 void orc_sensitivity(int nx, int ny, int nz, const double KE[576], const double *U,
                      const double *dEdz, double *g);
 void orc_power_start(int64_t n, double *v);

@@ -88,3 +88,26 @@ def test_cpp_facade_design_loop(tofe, precision):
     comp.Function(t)
     assert c0 == pytest.approx(comp.values[0], rel=1e-7)
     t.ctx.close()
+
+
+def test_python_facade_max_length_scale(oracle, tofe):
+    """MatIntFnc/Chain_Filter with Rmax, vdMin and the void-penalty continuation."""
+    nel = (12, 10, 8)
+    t = TopOpt()
+    t.CreateMesh([0, 12, 0, 10, 0, 8], nel, 1.5, 3.0)
+    t.vdMin = 55.0
+    t.FEInitialize()
+    x = configs.uniform(5, t.nElem, 0.05, 1.0)
+    P = oracle.filter_csr(*nel, 1.5)
+    R, REdge = oracle.rfilter_csr(*nel, 3.0)
+    g = np.random.default_rng(4).standard_normal(t.nElem)
+    for q in (1.0, 2.0):
+        t.vdPenal = q
+        t.MatIntFnc(x)
+        o = oracle.matint(P, R, REdge, x, 3.0, q, 55.0)
+        for k in ("rho", "y", "rhoq", "E", "dEdz", "Es", "dEsdz"):
+            assert np.array_equal(getattr(t, k), o[k]), (q, k)
+        dV, dE = t.Chain_Filter(g, g)
+        oV, oE = oracle.chain_filter(P, R, o["rho"], o["y"], o["rhoq"], g, g)
+        assert np.array_equal(dV, oV) and np.array_equal(dE, oE)
+    t.ctx.close()

@@ -29,7 +29,7 @@ SMOOTHER_CHEBYSHEV, SMOOTHER_JACOBI = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
 (FIELD_X, FIELD_RHO, FIELD_E, FIELD_DEDZ, FIELD_ES, FIELD_DESDZ, FIELD_V, FIELD_DIAG, FIELD_Y,
- FIELD_RHOQ, FIELD_REDGE) = range(11)
+ FIELD_RHOQ, FIELD_REDGE, FIELD_MDIAG) = range(12)
 INTERP_SIMP, INTERP_SIMP_CUT, INTERP_SIMP_LOGISTIC = 0, 1, 2
 
 # Every entry point include/tofe.h declares.
@@ -40,7 +40,8 @@ EXPORTS = (
     "tofe_pc_apply_dev", "tofe_compliance", "tofe_compliance_dev", "tofe_sensitivity",
     "tofe_filter", "tofe_filter_T", "tofe_filter_pattern", "tofe_set_precision",
     "tofe_get_stats", "tofe_get_level_stencil", "tofe_get_level_lambda", "tofe_time_operator",
-    "tofe_set_penalty", "tofe_chain_filter",
+    "tofe_set_penalty", "tofe_chain_filter", "tofe_assemble_mass", "tofe_apply_M",
+    "tofe_apply_M_dev", "tofe_element_quadratic",
 )
 
 
@@ -118,6 +119,10 @@ def lib():
         "tofe_time_operator": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, dp], C.c_int),
         "tofe_set_penalty": ([vp, C.c_double, C.c_double], C.c_int),
         "tofe_chain_filter": ([vp, dp, dp], C.c_int),
+        "tofe_assemble_mass": ([vp, C.c_double, dp], C.c_int),
+        "tofe_apply_M": ([vp, dp, dp, C.c_int], C.c_int),
+        "tofe_apply_M_dev": ([vp, vp, vp, C.c_int], C.c_int),
+        "tofe_element_quadratic": ([vp, dp, C.c_int, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -207,7 +212,7 @@ class Context:
         self._check(lib().tofe_set_E(self.h, _dp(E)))
 
     def field(self, which):
-        n = self.ndof if which == FIELD_DIAG else self.nelem
+        n = self.ndof if which in (FIELD_DIAG, FIELD_MDIAG) else self.nelem
         out = np.empty(n)
         self._check(lib().tofe_get_field(self.h, which, _dp(out)))
         return out
@@ -282,6 +287,26 @@ class Context:
         gx = np.empty_like(g)
         self._check(lib().tofe_filter_T(self.h, _dp(g), _dp(gx)))
         return gx
+
+    def assemble_mass(self, density, mlump=None):
+        self._check(lib().tofe_assemble_mass(self.h, density,
+                                             None if mlump is None else _dp(_f64(mlump))))
+
+    def apply_M(self, X):
+        X = _f64(X)
+        nrhs = X.size // self.ndof
+        Y = np.empty_like(X)
+        self._check(lib().tofe_apply_M(self.h, _dp(X), _dp(Y), nrhs))
+        return Y
+
+    def element_quadratic(self, phi, mass=True):
+        """(kq, mq) of shape (nvec, nelem): phi_e^T KE phi_e and sum of phi_e^2."""
+        phi = _f64(phi)
+        nvec = phi.size // self.ndof
+        kq = np.empty((nvec, self.nelem))
+        mq = np.empty((nvec, self.nelem)) if mass else None
+        self._check(lib().tofe_element_quadratic(self.h, _dp(phi), nvec, _dp(kq), _dp(mq)))
+        return kq, mq
 
     def set_penalty(self, penal, vd_penal=1.0):
         self._check(lib().tofe_set_penalty(self.h, penal, vd_penal))

@@ -277,6 +277,79 @@ __global__ __launch_bounds__(NT) void k_sensitivity(const double *__restrict__ U
   g[el] = (-dEdz[el] * s);
 }
 
+// Lumped mass diagonal (Frequency::DiagMassFnc, Dynamic.cpp:181-236): the element blocks
+// V_e m0 I are added in ascending element order, then the lumped point masses
+// (MatDiagonalSet ADD_VALUES, 233); Dirichlet DOFs get 1e-8 (MatZeroRowsColumns, 44-45).
+__global__ __launch_bounds__(NT) void k_mass_diag(const double *__restrict__ V, int nx, int ny,
+                                                  int nz, double m0, const double *__restrict__ mlump,
+                                                  const uint8_t *__restrict__ fix, double *M)
+{
+  const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
+  const int64_t nn = (int64_t)Nx * Ny * Nz;
+  const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (n >= nn) return;
+  const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+  double s = 0.0;
+  for (int dz = -1; dz <= 0; ++dz)
+    for (int dy = -1; dy <= 0; ++dy)
+      for (int dx = -1; dx <= 0; ++dx) {
+        const int ex = i + dx, ey = j + dy, ez = k + dz;
+        if (ex < 0 || ex >= nx || ey < 0 || ey >= ny || ez < 0 || ez >= nz) continue;
+        s = s + V[ex + (int64_t)nx * (ey + (int64_t)ny * ez)] * m0;
+      }
+  const uint8_t fm = fix ? fix[n] : 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double v = mlump ? s + mlump[3 * n + c] : s;
+    if ((fm >> c) & 1) v = 1e-8;
+    M[3 * n + c] = v;
+  }
+}
+
+__global__ void k_diag_mul(const double *__restrict__ d, const double *__restrict__ x, double *y,
+                           int64_t n, int nrhs)
+{
+  const int64_t tot = n * nrhs;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < tot; i += (int64_t)gridDim.x * NT)
+    y[i] = d[i % n] * x[i];
+}
+
+// Element quadratic forms of nvec vectors (the element terms of the eigenvalue
+// sensitivities, Dynamic.cpp:113-154): kq = phi_e^T KE phi_e, mq = sum over the element's
+// 24 DOFs of phi^2.  One thread per (element, vector).
+__global__ __launch_bounds__(NT) void k_elem_quad(const double *__restrict__ phi,
+                                                  const double *__restrict__ KE, int nx, int ny,
+                                                  int nz, int64_t ndof, int nvec, double *kq,
+                                                  double *mq)
+{
+  const int64_t ne = (int64_t)nx * ny * nz;
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= ne * nvec) return;
+  const int64_t el = t % ne;
+  const int v = (int)(t / ne);
+  const double *U = phi + (int64_t)v * ndof;
+  const int ex = (int)(el % nx), ey = (int)((el / nx) % ny), ez = (int)(el / ((int64_t)nx * ny));
+  const int Nx = nx + 1, Ny = ny + 1;
+  double u[24];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    const int64_t n = (ex + lnx(l)) + (int64_t)Nx * ((ey + lny(l)) + (int64_t)Ny * (ez + lnz(l)));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[3 * l + c] = U[3 * n + c];
+  }
+  double s = 0.0, m = 0.0;
+#pragma unroll
+  for (int r = 0; r < 24; ++r) {
+    double tr = 0.0;
+#pragma unroll
+    for (int c = 0; c < 24; ++c) tr = tr + KE[r * 24 + c] * u[c];
+    s = s + u[r] * tr;
+    m = m + u[r] * u[r];
+  }
+  kq[t] = s;
+  if (mq) mq[t] = m;
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + NT - 1) / NT); }
 
 }  // namespace
@@ -372,6 +445,29 @@ void sensitivity(hipStream_t s, const double *U, const double *dEdz, const doubl
 {
   const int64_t ne = (int64_t)nx * ny * nz;
   k_sensitivity<<<dim3(blocks_for(ne)), dim3(NT), 0, s>>>(U, dEdz, KE, nx, ny, nz, g);
+  TOFE_LAUNCH_CHECK();
+}
+
+void mass_diag(hipStream_t s, const double *V, int nx, int ny, int nz, double m0,
+               const double *mlump, const uint8_t *fix, double *M)
+{
+  const int64_t nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  k_mass_diag<<<dim3(blocks_for(nn)), dim3(NT), 0, s>>>(V, nx, ny, nz, m0, mlump, fix, M);
+  TOFE_LAUNCH_CHECK();
+}
+
+void diag_mul(hipStream_t s, const double *d, const double *x, double *y, int64_t n, int nrhs)
+{
+  k_diag_mul<<<dim3(stride_blocks(n * nrhs)), dim3(NT), 0, s>>>(d, x, y, n, nrhs);
+  TOFE_LAUNCH_CHECK();
+}
+
+void elem_quad(hipStream_t s, const double *phi, const double *KE, int nx, int ny, int nz,
+               int64_t ndof, int nvec, double *kq, double *mq)
+{
+  const int64_t ne = (int64_t)nx * ny * nz;
+  k_elem_quad<<<dim3(blocks_for(ne * nvec)), dim3(NT), 0, s>>>(phi, KE, nx, ny, nz, ndof, nvec,
+                                                              kq, mq);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -102,6 +102,9 @@ struct tofe_ctx {
   std::vector<int> rtpl_off;
   DBuf<int> drtpl_off;
   DBuf<double> drtpl_w, REdge, y, rhoq, ebuf3;
+  // lumped mass diagonal (Frequency::DiagMassFnc)
+  DBuf<double> Mdiag;
+  bool has_mass = false;
 
   std::vector<std::unique_ptr<Level<double>>> lv64;
   std::vector<std::unique_ptr<Level<float>>> lv32;
@@ -971,6 +974,11 @@ int tofe_get_field(const tofe_ctx *cc, int field, double *out)
       case TOFE_FIELD_ES: src = c->Es.p; break;
       case TOFE_FIELD_DESDZ: src = c->dEsdz.p; break;
       case TOFE_FIELD_Y: src = c->y.p; break;
+      case TOFE_FIELD_MDIAG:
+        if (!c->has_mass) throw Error(TOFE_ERR_STATE, "no mass matrix (call tofe_assemble_mass)");
+        src = c->Mdiag.p;
+        n = c->ndof;
+        break;
       case TOFE_FIELD_RHOQ: src = c->rhoq.p; break;
       case TOFE_FIELD_REDGE:
         if (!c->has_vd) throw Error(TOFE_ERR_STATE, "no maximum length scale (vd_min <= 0)");
@@ -1044,6 +1052,70 @@ int tofe_apply_K(tofe_ctx *c, const double *X, double *Y, int nrhs)
       level_apply<double>(c, 0, c->vp.p, EPI_APPLY, ep);
       d2h(c, Y + (size_t)i * c->ndof, c->vq.p, c->ndof);
     }
+  });
+}
+
+int tofe_assemble_mass(tofe_ctx *c, double density, const double *mlump)
+{
+  return guarded(c, [&] {
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    if (!(density > 0)) throw Error(TOFE_ERR_ARG, "density must be positive");
+    // elemSize = 1 * dx * dy * dz (RecMesh.cpp:529-533); m0 = 1/2^3/3 * elemSize * density
+    double elem = 1.0;
+    elem *= c->desc.hx;
+    elem *= c->desc.hy;
+    elem *= c->desc.hz;
+    const double m0 = 1.0 / std::pow(2.0, 3) / 3 * elem * density;
+    if (!c->Mdiag.p) c->Mdiag.alloc(c->ndof);
+    const double *dl = nullptr;
+    if (mlump) {
+      h2d(c, c->vq.p, mlump, c->ndof);
+      dl = c->vq.p;
+    }
+    mass_diag(c->stream, c->rho.p, c->desc.nx, c->desc.ny, c->desc.nz, m0, dl,
+              c->has_fix ? c->dfix.p : nullptr, c->Mdiag.p);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+    c->has_mass = true;
+  });
+}
+
+int tofe_apply_M_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
+{
+  return guarded(c, [&] {
+    if (!c->has_mass) throw Error(TOFE_ERR_STATE, "no mass matrix (call tofe_assemble_mass)");
+    if (!dX || !dY || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
+    diag_mul(c->stream, c->Mdiag.p, dX, dY, c->ndof, nrhs);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tofe_apply_M(tofe_ctx *c, const double *X, double *Y, int nrhs)
+{
+  return guarded(c, [&] {
+    if (!c->has_mass) throw Error(TOFE_ERR_STATE, "no mass matrix (call tofe_assemble_mass)");
+    if (!X || !Y || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
+    for (int i = 0; i < nrhs; ++i) {
+      h2d(c, c->vp.p, X + (size_t)i * c->ndof, c->ndof);
+      diag_mul(c->stream, c->Mdiag.p, c->vp.p, c->vq.p, c->ndof, 1);
+      d2h(c, Y + (size_t)i * c->ndof, c->vq.p, c->ndof);
+    }
+  });
+}
+
+int tofe_element_quadratic(tofe_ctx *c, const double *phi, int nvec, double *kq, double *mq)
+{
+  return guarded(c, [&] {
+    if (!phi || !kq || nvec < 1) throw Error(TOFE_ERR_ARG, "bad arguments");
+    const size_t nd = (size_t)c->ndof * nvec, ne = (size_t)c->ne * nvec;
+    DBuf<double> dphi, dk, dm;
+    dphi.alloc(nd);
+    dk.alloc(ne);
+    if (mq) dm.alloc(ne);
+    h2d(c, dphi.p, phi, nd);
+    elem_quad(c->stream, dphi.p, c->dKE.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ndof, nvec, dk.p,
+              mq ? dm.p : nullptr);
+    d2h(c, kq, dk.p, ne);
+    if (mq) d2h(c, mq, dm.p, ne);
   });
 }
 

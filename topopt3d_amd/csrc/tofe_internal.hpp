@@ -180,6 +180,11 @@ void simp(hipStream_t s, const double *rho, const double *y, int64_t ne, const S
 void chain_mid(hipStream_t s, const double *dfdE, const double *y, const double *rhoq,
                const double *RtT, double *out, int64_t ne);
 void mul(hipStream_t s, const double *a, const double *b, double *c, int64_t n);
+void mass_diag(hipStream_t s, const double *V, int nx, int ny, int nz, double m0,
+               const double *mlump, const uint8_t *fix, double *M);
+void diag_mul(hipStream_t s, const double *d, const double *x, double *y, int64_t n, int nrhs);
+void elem_quad(hipStream_t s, const double *phi, const double *KE, int nx, int ny, int nz,
+               int64_t ndof, int nvec, double *kq, double *mq);
 void sensitivity(hipStream_t s, const double *U, const double *dEdz, const double *KE, int nx,
                  int ny, int nz, double *g);
 

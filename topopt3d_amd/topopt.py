@@ -29,6 +29,10 @@ class TopOpt:
         self.E0 = E0
         self.Nu0 = Nu0
         self.penal = penal
+        self.vdPenal = 1.0          # void penalty q (TopOpt.h:167)
+        self.vdMin = -1.0           # minimum void count within Rmax; <= 0 disables (TopOpt.h:175)
+        self.interpolation = _lib.INTERP_SIMP
+        self.interp_param = [0.0, 0.0]
         self.verbose = verbose
         self.eps = 1e-10  # Material_Interpolation.cpp:28
         self.ctx = None
@@ -45,9 +49,9 @@ class TopOpt:
         self.its = 0
 
     # -- mesh (RecMesh.cpp:426-597, single rank, no reordering) ----------------
-    def CreateMesh(self, dimensions, Nel, Rmin):
-        """dimensions = (x0, x1, y0, y1, z0, z1); Nel = elements per direction; Rmin is
-        the filter radius in units of length."""
+    def CreateMesh(self, dimensions, Nel, Rmin, Rmax=0.0):
+        """dimensions = (x0, x1, y0, y1, z0, z1); Nel = elements per direction; Rmin and
+        Rmax are the density-filter and maximum-length-scale radii in units of length."""
         self.dimensions = np.asarray(dimensions, np.float64)
         self.Nel = tuple(int(n) for n in Nel)
         nx, ny, nz = self.Nel
@@ -67,6 +71,7 @@ class TopOpt:
                                  base + NxNy, base + 1 + NxNy, base + 1 + Nx + NxNy,
                                  base + Nx + NxNy], axis=1)
         self.Rmin = float(Rmin)
+        self.Rmax = float(Rmax)
 
     # -- boundary conditions (Inputs.cpp:105-223, SUPPORT and LOAD) -------------
     def Set_BC(self, center, radius, limits, values, TYPE):
@@ -111,17 +116,26 @@ class TopOpt:
         self.ctx = Context(self.Nel, h=self.dx, E0=self.E0, nu=self.Nu0, Emin=self.eps,
                            penal=self.penal, rmin=self.Rmin, mg_levels=self.mg_levels,
                            smoother=self.smoother, precision=self.precision,
-                           fixed_dof=self.fixed_dof)
+                           fixed_dof=self.fixed_dof, rmax=self.Rmax, vd_min=self.vdMin,
+                           vd_penal=self.vdPenal, interpolation=int(self.interpolation),
+                           interp_param=tuple(self.interp_param))
         self.ke = [self.ctx.ke()]
 
     # -- design iteration ---------------------------------------------------------
     def MatIntFnc(self, design):
-        """rho = P x and the SIMP interpolation (Material_Interpolation.cpp:20-140)."""
+        """rho = P x, the maximum-length-scale factor y and the SIMP / SIMP_CUT /
+        SIMP_LOGISTIC interpolation (Material_Interpolation.cpp:20-140).  The current
+        penal and vdPenal apply (penalty continuation, Main.cpp:131-133)."""
+        self.ctx.set_penalty(self.penal, self.vdPenal)
         self.ctx.set_design(design)
         self.x = np.asarray(design, np.float64)
         self.rho = self.ctx.field(_lib.FIELD_RHO)
+        self.y = self.ctx.field(_lib.FIELD_Y)
+        self.rhoq = self.ctx.field(_lib.FIELD_RHOQ)
         self.E = self.ctx.field(_lib.FIELD_E)
         self.dEdz = self.ctx.field(_lib.FIELD_DEDZ)
+        self.Es = self.ctx.field(_lib.FIELD_ES)
+        self.dEsdz = self.ctx.field(_lib.FIELD_DESDZ)
         self.V = self.rho
         self.dVdrho = np.ones_like(self.rho)
 
@@ -135,11 +149,9 @@ class TopOpt:
         return self.KUF_reason
 
     def Chain_Filter(self, dfdV=None, dfdE=None):
-        """dfdV <- P^T dfdV; dfdE <- P^T (dfdE .* y), y = 1 for the plain density filter
-        (Material_Interpolation.cpp:150-187)."""
-        outV = None if dfdV is None else self.ctx.filter_T(dfdV)
-        outE = None if dfdE is None else self.ctx.filter_T(dfdE)
-        return outV, outE
+        """dfdV <- P^T dfdV; dfdE <- P^T [dfdE .* y - rhoq .* (y < 1) .* R^T(dfdE .* rho)]
+        (Material_Interpolation.cpp:150-187); returns the two results."""
+        return self.ctx.chain_filter(dfdV, dfdE)
 
 
 class Compliance:
