@@ -242,6 +242,33 @@ int tofe_get_level_lambda(const tofe_ctx *ctx, int level, double *lambda);
 int tofe_time_operator(tofe_ctx *ctx, int level, int precision, int epilogue, int nrep,
                        double *ms);
 
+/* ---- on-disk formats ------------------------------------------------------------
+ * PETSc binary files as PetscViewerBinaryOpen + VecView / MatView write them (big-endian;
+ * Vec: classid 1211214, n, n doubles; sequential AIJ: classid 1211216, rows, cols, nnz,
+ * row lengths, column indices, values).  index_bytes is sizeof(PetscInt): 4, or 8 for a
+ * PETSc configured with 64-bit indices. */
+int tofe_petsc_vec_write(const char *path, const double *v, int64_t n, int index_bytes);
+/* v == NULL: *n <- length; otherwise *n is the capacity of v and becomes the length. */
+int tofe_petsc_vec_read(const char *path, double *v, int64_t *n, int index_bytes);
+int tofe_petsc_aij_write(const char *path, int64_t rows, int64_t cols, const int64_t *rowptr,
+                         const int32_t *col, const double *val, int index_bytes);
+/* rowptr == NULL: sizes only; otherwise *rows / *nnz are the capacities of rowptr (rows+1)
+ * and col/val, and col/val may be NULL to read the row pointers alone. */
+int tofe_petsc_aij_read(const char *path, int64_t *rows, int64_t *cols, int64_t *nnz,
+                        int64_t *rowptr, int32_t *col, double *val, int index_bytes);
+
+/* TopOpt::MeshOut for one rank (TopOpt.cpp:97-360): Element_Distribution.bin,
+ * Node_Distribution.bin, elements.bin, nodes.bin, active.bin (raw native-endian),
+ * Filter.bin, Max_Filter.bin and Void_Edge_Volume.bin when rmax > 0, and P<l>.bin for the
+ * multigrid interpolations (PETSc binary) under dir.  origin = the mesh's lower corner
+ * (may be NULL for 0).  The boundary-condition files belong to the host facade. */
+int tofe_mesh_out(tofe_ctx *ctx, const char *dir, const double origin[3], int index_bytes);
+
+/* TopOpt::PrintVals (TopOpt.cpp:442-480): U<suffix>.bin (if U is not NULL), x, V, Es and E
+ * of the current design as PETSc binary Vecs under dir. */
+int tofe_print_vals(tofe_ctx *ctx, const char *dir, const char *suffix, const double *U,
+                    int index_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,7 +41,8 @@ EXPORTS = (
     "tofe_filter", "tofe_filter_T", "tofe_filter_pattern", "tofe_set_precision",
     "tofe_get_stats", "tofe_get_level_stencil", "tofe_get_level_lambda", "tofe_time_operator",
     "tofe_set_penalty", "tofe_chain_filter", "tofe_assemble_mass", "tofe_apply_M",
-    "tofe_apply_M_dev", "tofe_element_quadratic",
+    "tofe_apply_M_dev", "tofe_element_quadratic", "tofe_petsc_vec_write", "tofe_petsc_vec_read",
+    "tofe_petsc_aij_write", "tofe_petsc_aij_read", "tofe_mesh_out", "tofe_print_vals",
 )
 
 
@@ -123,6 +124,13 @@ def lib():
         "tofe_apply_M": ([vp, dp, dp, C.c_int], C.c_int),
         "tofe_apply_M_dev": ([vp, vp, vp, C.c_int], C.c_int),
         "tofe_element_quadratic": ([vp, dp, C.c_int, dp, dp], C.c_int),
+        "tofe_petsc_vec_write": ([C.c_char_p, dp, C.c_int64, C.c_int], C.c_int),
+        "tofe_petsc_vec_read": ([C.c_char_p, dp, i64p, C.c_int], C.c_int),
+        "tofe_petsc_aij_write": ([C.c_char_p, C.c_int64, C.c_int64, i64p, i32p, dp, C.c_int],
+                                 C.c_int),
+        "tofe_petsc_aij_read": ([C.c_char_p, i64p, i64p, i64p, i64p, i32p, dp, C.c_int], C.c_int),
+        "tofe_mesh_out": ([vp, C.c_char_p, dp, C.c_int], C.c_int),
+        "tofe_print_vals": ([vp, C.c_char_p, C.c_char_p, dp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -138,6 +146,49 @@ def _dp(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _chk(rc, what):
+    if rc != TOFE_OK:
+        raise TofeError(rc, what)
+
+
+def petsc_vec_write(path, v, index_bytes=4):
+    v = _f64(v)
+    _chk(lib().tofe_petsc_vec_write(os.fsencode(path), _dp(v), v.size, index_bytes), path)
+
+
+def petsc_vec_read(path, index_bytes=4):
+    n = C.c_int64(0)
+    _chk(lib().tofe_petsc_vec_read(os.fsencode(path), None, C.byref(n), index_bytes), path)
+    v = np.empty(n.value)
+    _chk(lib().tofe_petsc_vec_read(os.fsencode(path), _dp(v), C.byref(n), index_bytes), path)
+    return v
+
+
+def petsc_aij_write(path, rows, cols, rowptr, col, val, index_bytes=4):
+    rp = np.ascontiguousarray(rowptr, np.int64)
+    cl = np.ascontiguousarray(col, np.int32)
+    vl = _f64(val)
+    _chk(lib().tofe_petsc_aij_write(os.fsencode(path), rows, cols,
+                                    rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    cl.ctypes.data_as(C.POINTER(C.c_int32)), _dp(vl), index_bytes),
+         path)
+
+
+def petsc_aij_read(path, index_bytes=4):
+    """(rows, cols, rowptr, col, val) of a PETSc binary AIJ file."""
+    m, n, z = C.c_int64(), C.c_int64(), C.c_int64()
+    L = lib()
+    _chk(L.tofe_petsc_aij_read(os.fsencode(path), C.byref(m), C.byref(n), C.byref(z), None, None,
+                               None, index_bytes), path)
+    rp = np.empty(m.value + 1, np.int64)
+    cl = np.empty(z.value, np.int32)
+    vl = np.empty(z.value)
+    _chk(L.tofe_petsc_aij_read(os.fsencode(path), C.byref(m), C.byref(n), C.byref(z),
+                               rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                               cl.ctypes.data_as(C.POINTER(C.c_int32)), _dp(vl), index_bytes), path)
+    return m.value, n.value, rp, cl, vl
 
 
 class Context:
@@ -307,6 +358,15 @@ class Context:
         mq = np.empty((nvec, self.nelem)) if mass else None
         self._check(lib().tofe_element_quadratic(self.h, _dp(phi), nvec, _dp(kq), _dp(mq)))
         return kq, mq
+
+    def mesh_out(self, directory, origin=(0.0, 0.0, 0.0), index_bytes=4):
+        o = _f64(origin)
+        self._check(lib().tofe_mesh_out(self.h, os.fsencode(directory), _dp(o), index_bytes))
+
+    def print_vals(self, directory, suffix="", U=None, index_bytes=4):
+        U = None if U is None else _f64(U)
+        self._check(lib().tofe_print_vals(self.h, os.fsencode(directory), suffix.encode(), _dp(U),
+                                          index_bytes))
 
     def set_penalty(self, penal, vd_penal=1.0):
         self._check(lib().tofe_set_penalty(self.h, penal, vd_penal))

@@ -170,4 +170,56 @@ double tridiag_max_eig(int k, const double *a, const double *b)
   return hi;
 }
 
+void build_interpolation(int Nfx, int Nfy, int Nfz, std::vector<int64_t> &rowptr,
+                         std::vector<int32_t> &col, std::vector<double> &val)
+{
+  static const double vals[3] = {0.5, 1.0, 0.5};
+  const int64_t Ncx = (Nfx + 1) / 2, Ncy = (Nfy + 1) / 2, Ncz = (Nfz + 1) / 2;
+  const int64_t nf = (int64_t)Nfx * Nfy * Nfz;
+  rowptr.assign(3 * nf + 1, 0);
+  col.clear();
+  val.clear();
+  col.reserve(3 * nf * 8);
+  val.reserve(3 * nf * 8);
+  auto parents = [](int64_t i, int64_t Nc, int64_t *J, int *t) {
+    int n = 0;
+    for (int64_t j = i / 2 - 1; j <= i / 2 + 1; ++j) {
+      const int64_t d = i - 2 * j;
+      if (j < 0 || j >= Nc || d < -1 || d > 1) continue;
+      J[n] = j;
+      t[n] = (int)d + 1;
+      ++n;
+    }
+    return n;
+  };
+  for (int64_t f = 0; f < nf; ++f) {
+    const int64_t ix = f % Nfx, iy = (f / Nfx) % Nfy, iz = f / ((int64_t)Nfx * Nfy);
+    int64_t Jx[3], Jy[3], Jz[3];
+    int tx[3], ty[3], tz[3];
+    const int nxp = parents(ix, Ncx, Jx, tx), nyp = parents(iy, Ncy, Jy, ty),
+              nzp = parents(iz, Ncz, Jz, tz);
+    int64_t cn[8];
+    double w[8];
+    int c = 0;
+    for (int a = 0; a < nzp; ++a)
+      for (int b = 0; b < nyp; ++b)
+        for (int d = 0; d < nxp; ++d) {
+          cn[c] = Jx[d] + Ncx * (Jy[b] + Ncy * Jz[a]);
+          w[c] = vals[tx[d]] * vals[ty[b]] * vals[tz[a]];
+          ++c;
+        }
+    // MatGetRowSum in column order, then the row times the reciprocal
+    double s = 0.0;
+    for (int q = 0; q < c; ++q) s += w[q];
+    const double inv = 1.0 / s;
+    for (int comp = 0; comp < 3; ++comp) {
+      for (int q = 0; q < c; ++q) {
+        col.push_back((int32_t)(3 * cn[q] + comp));
+        val.push_back(w[q] * inv);
+      }
+      rowptr[3 * f + comp + 1] = (int64_t)col.size();
+    }
+  }
+}
+
 }  // namespace tofe

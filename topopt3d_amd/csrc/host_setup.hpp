@@ -1,5 +1,7 @@
 #pragma once
 
+#include <cstdint>
+#include <string>
 #include <vector>
 
 namespace tofe {
@@ -22,5 +24,15 @@ void galerkin_forms(const double KE[576], double Qforms[27 * 64], double KEC[27 
 // Largest eigenvalue of a symmetric tridiagonal matrix (diagonal a, off-diagonal b) by
 // Sturm-sequence bisection.
 double tridiag_max_eig(int k, const double *a, const double *b);
+
+// DOF-level interpolation from a coarse node grid (Nf+1)/2 to the fine grid Nf:
+// weights (0.5, 1, 0.5) per direction for coarse nodes at even fine indices, each row
+// divided by its sum (Interpolation.cpp:123-169, 322-347); CSR with ascending columns.
+void build_interpolation(int Nfx, int Nfy, int Nfz, std::vector<int64_t> &rowptr,
+                         std::vector<int32_t> &col, std::vector<double> &val);
+
+// Raw mesh files of TopOpt::MeshOut for one rank (TopOpt.cpp:97-360).
+int write_mesh_files(const std::string &dir, int nx, int ny, int nz, double hx, double hy,
+                     double hz, const double origin[3], int index_bytes);
 
 }  // namespace tofe

@@ -717,6 +717,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
   if (d.interpolation < TOFE_INTERP_SIMP || d.interpolation > TOFE_INTERP_SIMP_LOGISTIC)
     return TOFE_ERR_ARG;
   if (d.vd_min > 0 && !(d.rmax > 0)) return TOFE_ERR_ARG;
+  if (d.rmax < 0) return TOFE_ERR_ARG;
   c->sp.penal = d.penal;
   c->sp.eps = d.Emin;
   c->sp.vd_penal = d.vd_penal;
@@ -779,7 +780,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     filter_rowsum_inv(c->stream, filter_tpl(c.get()), c->filt_inv.p);
     c->y.alloc(c->ne);
     c->rhoq.alloc(c->ne);
-    if (c->has_vd) {
+    if (d.rmax > 0) {
       // R: the same neighbourhood rule with radius Rmax and unit weights (RecMesh.cpp:544,
       // 706-718); REdge = max_i rowsum_i - rowsum (Material_Interpolation.cpp:37-41)
       std::vector<double> rw;
@@ -981,7 +982,7 @@ int tofe_get_field(const tofe_ctx *cc, int field, double *out)
         break;
       case TOFE_FIELD_RHOQ: src = c->rhoq.p; break;
       case TOFE_FIELD_REDGE:
-        if (!c->has_vd) throw Error(TOFE_ERR_STATE, "no maximum length scale (vd_min <= 0)");
+        if (!c->REdge.p) throw Error(TOFE_ERR_STATE, "no maximum-length-scale filter (rmax <= 0)");
         src = c->REdge.p;
         break;
       case TOFE_FIELD_DIAG:
@@ -1241,32 +1242,117 @@ int tofe_filter_T(tofe_ctx *c, const double *g, double *gx)
   });
 }
 
+namespace {
+
+// CSR of a template filter on the host: row counts, prefix sums, then the device fill
+// (weights w_q * inv_e; inv == nullptr -> unit weights).
+void template_pattern(tofe_ctx *c, const FilterTemplate &t, const double *inv, int64_t *rowptr,
+                      int32_t *col, double *w)
+{
+  const int64_t ne = c->ne;
+  DBuf<int64_t> cnt;
+  cnt.alloc(ne + 1);
+  filter_count(c->stream, t, cnt.p);
+  std::vector<int64_t> h(ne + 1);
+  TOFE_HIP(hipMemcpyAsync(h.data() + 1, cnt.p, ne * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  TOFE_HIP(hipStreamSynchronize(c->stream));
+  h[0] = 0;
+  for (int64_t e = 0; e < ne; ++e) h[e + 1] += h[e];
+  std::memcpy(rowptr, h.data(), (ne + 1) * sizeof(int64_t));
+  if (col && w) {
+    const int64_t nnz = h[ne];
+    DBuf<int32_t> dcol;
+    DBuf<double> dw, ones;
+    dcol.alloc(nnz);
+    dw.alloc(nnz);
+    if (!inv) {
+      ones.alloc(ne);
+      fill_d(c->stream, ones.p, 1.0, ne);
+      inv = ones.p;
+    }
+    TOFE_HIP(hipMemcpyAsync(cnt.p, h.data(), (ne + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    filter_fill(c->stream, t, inv, cnt.p, dcol.p, dw.p);
+    TOFE_HIP(hipMemcpyAsync(col, dcol.p, nnz * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    TOFE_HIP(hipMemcpyAsync(w, dw.p, nnz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+  }
+}
+
+void write_template_matrix(tofe_ctx *c, const std::string &path, const FilterTemplate &t,
+                           const double *inv, int index_bytes)
+{
+  std::vector<int64_t> rp(c->ne + 1);
+  template_pattern(c, t, inv, rp.data(), nullptr, nullptr);
+  std::vector<int32_t> col(rp[c->ne]);
+  std::vector<double> w(rp[c->ne]);
+  template_pattern(c, t, inv, rp.data(), col.data(), w.data());
+  const int rc = tofe_petsc_aij_write(path.c_str(), c->ne, c->ne, rp.data(), col.data(), w.data(),
+                                      index_bytes);
+  if (rc) throw Error(rc, "cannot write " + path);
+}
+
+void write_vec(tofe_ctx *c, const std::string &path, const double *dsrc, int64_t n, int index_bytes)
+{
+  std::vector<double> h(n);
+  d2h(c, h.data(), dsrc, n);
+  const int rc = tofe_petsc_vec_write(path.c_str(), h.data(), n, index_bytes);
+  if (rc) throw Error(rc, "cannot write " + path);
+}
+
+}  // namespace
+
 int tofe_filter_pattern(tofe_ctx *c, int64_t *rowptr, int32_t *col, double *w)
 {
   return guarded(c, [&] {
     if (!rowptr) throw Error(TOFE_ERR_ARG, "null rowptr");
-    const int64_t ne = c->ne;
-    DBuf<int64_t> cnt;
-    cnt.alloc(ne + 1);
-    filter_count(c->stream, filter_tpl(c), cnt.p);
-    std::vector<int64_t> h(ne + 1);
-    TOFE_HIP(hipMemcpyAsync(h.data() + 1, cnt.p, ne * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
-    TOFE_HIP(hipStreamSynchronize(c->stream));
-    h[0] = 0;
-    for (int64_t e = 0; e < ne; ++e) h[e + 1] += h[e];
-    std::memcpy(rowptr, h.data(), (ne + 1) * sizeof(int64_t));
-    if (col && w) {
-      const int64_t nnz = h[ne];
-      DBuf<int32_t> dcol;
-      DBuf<double> dw;
-      dcol.alloc(nnz);
-      dw.alloc(nnz);
-      TOFE_HIP(hipMemcpyAsync(cnt.p, h.data(), (ne + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-      filter_fill(c->stream, filter_tpl(c), c->filt_inv.p, cnt.p, dcol.p, dw.p);
-      TOFE_HIP(hipMemcpyAsync(col, dcol.p, nnz * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-      TOFE_HIP(hipMemcpyAsync(w, dw.p, nnz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      TOFE_HIP(hipStreamSynchronize(c->stream));
+    template_pattern(c, filter_tpl(c), c->filt_inv.p, rowptr, col, w);
+  });
+}
+
+int tofe_mesh_out(tofe_ctx *c, const char *dir, const double origin[3], int index_bytes)
+{
+  return guarded(c, [&] {
+    if (!dir || (index_bytes != 4 && index_bytes != 8)) throw Error(TOFE_ERR_ARG, "bad arguments");
+    const std::string d(dir);
+    const double zero[3] = {0, 0, 0};
+    const tofe_desc &q = c->desc;
+    int rc = write_mesh_files(d, q.nx, q.ny, q.nz, q.hx, q.hy, q.hz, origin ? origin : zero,
+                              index_bytes);
+    if (rc) throw Error(rc, "cannot write the mesh files under " + d);
+    write_template_matrix(c, d + "/Filter.bin", filter_tpl(c), c->filt_inv.p, index_bytes);
+    if (c->REdge.p) {
+      write_template_matrix(c, d + "/Max_Filter.bin", rfilter_tpl(c), nullptr, index_bytes);
+      write_vec(c, d + "/Void_Edge_Volume.bin", c->REdge.p, c->ne, index_bytes);
     }
+    for (int l = 0; l + 1 < c->L; ++l) {
+      const Grid &g = c->grids[l];
+      std::vector<int64_t> rp;
+      std::vector<int32_t> col;
+      std::vector<double> val;
+      build_interpolation(g.Nx, g.Ny, g.Nz, rp, col, val);
+      const std::string path = d + "/P" + std::to_string(l) + ".bin";
+      rc = tofe_petsc_aij_write(path.c_str(), 3 * g.nn(), c->grids[l + 1].ndof(), rp.data(),
+                                col.data(), val.data(), index_bytes);
+      if (rc) throw Error(rc, "cannot write " + path);
+    }
+  });
+}
+
+int tofe_print_vals(tofe_ctx *c, const char *dir, const char *suffix, const double *U,
+                    int index_bytes)
+{
+  return guarded(c, [&] {
+    if (!dir || (index_bytes != 4 && index_bytes != 8)) throw Error(TOFE_ERR_ARG, "bad arguments");
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    const std::string d(dir), sfx(suffix ? suffix : "");
+    if (U) {
+      const int rc = tofe_petsc_vec_write((d + "/U" + sfx + ".bin").c_str(), U, c->ndof, index_bytes);
+      if (rc) throw Error(rc, "cannot write U");
+    }
+    write_vec(c, d + "/x" + sfx + ".bin", c->x.p, c->ne, index_bytes);
+    write_vec(c, d + "/V" + sfx + ".bin", c->rho.p, c->ne, index_bytes);
+    write_vec(c, d + "/Es" + sfx + ".bin", c->Es.p, c->ne, index_bytes);
+    write_vec(c, d + "/E" + sfx + ".bin", c->E.p, c->ne, index_bytes);
   });
 }
 
