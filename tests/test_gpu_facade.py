@@ -42,10 +42,11 @@ def test_python_facade_matches_api_and_oracle(oracle, tofe):
     comp.Function(t)
     vol.Function(t)
     assert vol.values[0] == pytest.approx(0.3, rel=1e-14)
-    assert np.allclose(vol.gradients, 1.0 / t.nElem, rtol=1e-13)
     # oracle
     KE = oracle.ke()
     Pf = oracle.filter_csr(*nel, 1.5)
+    # dV/dx = P^T (1/nElem): the filter is row- not column-normalised (Volume.cpp:35-41)
+    assert np.array_equal(vol.gradients, Pf.rmatvec(np.full(t.nElem, 1.0 / t.nElem)))
     E, dEdz, _, _ = oracle.simp(Pf.matvec(x))
     K = oracle.assemble_K(*nel, KE, E, pb.fixed_dof)
     mg = oracle.MG(K, tuple(n + 1 for n in nel), t.ctx.levels)

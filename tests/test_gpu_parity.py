@@ -208,3 +208,32 @@ def test_config1_against_sparse_direct_fixture(tofe):
     gs = ctx.sensitivity(U)
     assert np.allclose(gs[fx["el_samples"]], fx["g_samples"], rtol=1e-9, atol=0)
     ctx.close()
+
+
+@pytest.mark.parametrize("nel,L", [((8, 4, 4), 3), ((9, 6, 5), 3), ((16, 8, 8), 3)])
+def test_pc_apply_matches_oracle_vcycle(oracle, tofe, nel, L):
+    """One V-cycle on the GPU equals the oracle's V-cycle (same Galerkin levels, Chebyshev
+    bounds from the same Lanczos estimate, exact coarse solve)."""
+    pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, L)
+    ctx.assemble()
+    mg = oracle.MG(K, tuple(n + 1 for n in nel), L)
+    r = np.random.default_rng(4).standard_normal(pb.ndof)
+    r[pb.fixed_dof.astype(bool)] = 0
+    z = ctx.pc_apply(r)
+    zo = mg.vcycle(r)
+    assert rel(z, zo) <= 1e-11
+    ctx.close()
+
+
+@pytest.mark.parametrize("nel", [(8, 4, 4), (10, 6, 5)])
+def test_single_level_is_exact_coarse_solve(oracle, tofe, nel):
+    """With one level the preconditioner is the blocked Cholesky inverse of K (several
+    64-column diagonal blocks, the last one partial): M^-1 r = K^-1 r."""
+    pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, 1)
+    ctx.assemble()
+    assert ctx.levels == 1 and pb.ndof % 64 != 0 and pb.ndof > 3 * 64
+    r = np.random.default_rng(6).standard_normal(pb.ndof)
+    z = ctx.pc_apply(r)
+    x = np.linalg.solve(K.to_scipy().toarray(), r)
+    assert rel(z, x) <= 1e-10
+    ctx.close()

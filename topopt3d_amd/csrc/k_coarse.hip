@@ -178,57 +178,57 @@ __global__ __launch_bounds__(NT) void k_gemv(const double *__restrict__ A, int64
 // dimension lda, kb <= kCholNB): L_kk overwrites the lower triangle of the block and its
 // inverse goes to W (leading dimension kCholNB).  info[0] = global pivot index + 1 of the
 // first non-positive pivot (0 if none).
-__global__ __launch_bounds__(NT) void k_chol_block(double *A, int lda, int kb, int k0, double *W,
+// One wavefront: lane i owns row i of the block during the factorisation and column i of
+// W = L^-1 during the inversion, so the inversion needs no barrier at all and the
+// factorisation one per column.  a is column-major (a[i + 64 j]); w is row-major
+// (w[c + 64 i]) so that the 64 lanes touch consecutive words.
+__global__ __launch_bounds__(64) void k_chol_block(double *A, int lda, int kb, int k0, double *W,
                                                    int *info)
 {
-  extern __shared__ double sm[];  // a[i + j*kCholNB], then x[i + c*kCholNB]
-  double *a = sm;
-  const int tid = threadIdx.x;
-  for (int v = tid; v < kb * kb; v += NT) {
-    const int i = v % kb, j = v / kb;
-    a[i + j * kCholNB] = (i >= j) ? A[i + (int64_t)j * lda] : 0.0;
-  }
+  __shared__ double a[kCholNB * kCholNB];
+  __shared__ double w[kCholNB * kCholNB];
+  const int i = threadIdx.x;
+  for (int j = 0; j < kb; ++j) a[i + kCholNB * j] = (i < kb && i >= j) ? A[i + (int64_t)j * lda] : 0.0;
   __syncthreads();
   for (int j = 0; j < kb; ++j) {
-    if (tid == 0) {
-      double d = a[j + j * kCholNB];
-      if (!(d > 0)) {
-        if (info[0] == 0) info[0] = k0 + j + 1;
-        d = 1.0;
-      }
-      a[j + j * kCholNB] = sqrt(d);
+    double d = a[j + kCholNB * j];
+    if (!(d > 0)) {
+      if (i == 0 && info[0] == 0) info[0] = k0 + j + 1;
+      d = 1.0;
     }
-    __syncthreads();
-    const double djj = a[j + j * kCholNB];
-    for (int i = j + 1 + tid; i < kb; i += NT) a[i + j * kCholNB] /= djj;
-    __syncthreads();
-    const int m = kb - j - 1;  // trailing lower triangle, (i, l) with j < l <= i
-    for (int v = tid; v < m * m; v += NT) {
-      const int i = j + 1 + v % m, l = j + 1 + v / m;
-      if (l <= i) a[i + l * kCholNB] -= a[i + j * kCholNB] * a[l + j * kCholNB];
+    d = sqrt(d);
+    double lij = 0.0;
+    if (i > j && i < kb) lij = a[i + kCholNB * j] / d;
+    __syncthreads();  // every lane has read column j and the pivot
+    if (i == j) a[j + kCholNB * j] = d;
+    if (i > j && i < kb) a[i + kCholNB * j] = lij;
+    // trailing update of row i: a(i, l) -= l(i, j) l(l, j) for j < l <= i
+    for (int l = j + 1; l < kb; ++l) {
+      const double llj = __shfl(lij, l, 64);
+      if (i >= l && i < kb) a[i + kCholNB * l] -= lij * llj;
     }
     __syncthreads();
   }
-  for (int v = tid; v < kb * kb; v += NT) {
-    const int i = v % kb, j = v / kb;
-    if (i >= j) A[i + (int64_t)j * lda] = a[i + j * kCholNB];
+  for (int j = 0; j < kb; ++j)
+    if (i < kb && i >= j) A[i + (int64_t)j * lda] = a[i + kCholNB * j];
+  // W = L^-1: lane c solves L w = e_c by forward substitution on its own column
+  const int c = i;
+  for (int r = 0; r < kb; ++r) w[c + kCholNB * r] = (r == c) ? 1.0 : 0.0;
+  for (int q = 0; q < kb; ++q) {
+    const double wq = w[c + kCholNB * q] / a[q + kCholNB * q];
+    w[c + kCholNB * q] = wq;
+    for (int r = q + 1; r < kb; ++r) w[c + kCholNB * r] -= a[r + kCholNB * q] * wq;
   }
-  // W = L^-1 by forward substitution, one column per thread
-  for (int c = tid; c < kb; c += NT) {
-    for (int i = 0; i < kb; ++i) {
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int q = c; q < i; ++q) s -= a[i + q * kCholNB] * W[q + c * kCholNB];
-      W[i + c * kCholNB] = (i >= c) ? s / a[i + i * kCholNB] : 0.0;
-    }
-  }
+  if (c < kCholNB)
+    for (int r = 0; r < kCholNB; ++r)
+      W[r + kCholNB * c] = (r < kb && c < kb && r >= c) ? w[c + kCholNB * r] : 0.0;
 }
 
 }  // namespace
 
 void chol_block(hipStream_t s, double *A, int lda, int kb, int k0, double *W, int *info)
 {
-  k_chol_block<<<dim3(1), dim3(NT), (size_t)kCholNB * kCholNB * sizeof(double), s>>>(A, lda, kb, k0,
-                                                                                     W, info);
+  k_chol_block<<<dim3(1), dim3(kCholNB), 0, s>>>(A, lda, kb, k0, W, info);
   TOFE_LAUNCH_CHECK();
 }
 

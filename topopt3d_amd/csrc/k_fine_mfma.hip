@@ -26,17 +26,22 @@ constexpr int OWNX = ETX - 1, OWNY = ETY - 1; // nodes whose four in-plane eleme
 constexpr int PL = NTX * NTY * 3;             // values per staged node plane
 constexpr int ZL = 12;                        // node planes per z chunk
 
+// Operand layouts (lane l of 64): A(16x4) element (l % 16, l / 16), B(4x16) element
+// (l / 16, l % 16).  Output item j of lane l is D(row(l, j), l % 16) with
+// row = 4 (l / 16) + j for the FP32 instruction and row = l / 16 + 4 j for the FP64 one.
 template <typename T>
 struct Mfma;
 template <>
 struct Mfma<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
   __device__ static v4 mac(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int kk, int j) { return 4 * kk + j; }
 };
 template <>
 struct Mfma<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
   __device__ static v4 mac(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int kk, int j) { return kk + 4 * j; }
 };
 
 struct MfmaGrid {
@@ -100,14 +105,17 @@ __global__ __launch_bounds__(NTH) void k_fine_mfma(FineOp<T> op, const T *__rest
     goff[s] = (ly * NTX + lx) * 3 + c;
     gup[s] = L >= 4 ? 1 : 0;
   }
-  // scatter targets of the D rows this lane holds: r = 16t + 4kk + j
+  // scatter targets of the D rows this lane holds: r = 16t + row(kk, j); rows 24..31 are
+  // padding (sval = false) and are never scattered
   int soff[2][4], sup[2][4];
+  bool sval[2][4];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      int r = 16 * t + 4 * kk + j;
-      if (r > 23) r = 23;  // padding rows are never scattered
+      int r = 16 * t + Mfma<T>::row(kk, j);
+      sval[t][j] = r < 24;
+      if (r > 23) r = 23;
       const int L = r / 3, c = r - 3 * (r / 3);
       const int lx = (L == 1 || L == 2 || L == 5 || L == 6) ? 1 : 0;
       const int ly = (L == 2 || L == 3 || L == 6 || L == 7) ? 1 : 0;
@@ -163,10 +171,9 @@ __global__ __launch_bounds__(NTH) void k_fine_mfma(FineOp<T> op, const T *__rest
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) sacc[sup[0][j] ? hi : lo][base + soff[0][j]] += d0[j];
-      if (kk < 2) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) sacc[sup[1][j] ? hi : lo][base + soff[1][j]] += d1[j];
-      }
+      for (int j = 0; j < 4; ++j)
+        if (sval[1][j]) sacc[sup[1][j] ? hi : lo][base + soff[1][j]] += d1[j];
       __syncthreads();
     }
     if (ez >= k0) {

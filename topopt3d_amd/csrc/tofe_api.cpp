@@ -129,7 +129,7 @@ struct tofe_ctx {
   DBuf<double> ebuf, ebuf2;
 
   bool assembled = false;
-  bool fine_vector = false;  // TOFE_FINE_VECTOR=1: VALU fine operator instead of MFMA
+  bool fine_mfma = false;  // TOFE_FINE_MFMA=1: element-centric MFMA fine operator
   int precision = TOFE_PREC_FP64;
   int built_precision = -1;
   tofe_stats stats{};
@@ -198,7 +198,7 @@ template <typename T>
 void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
 {
   if (l == 0) {
-    if (c->fine_vector) fine_op<T>(c->stream, fine_of<T>(c), u, epi, ep);
+    if (!c->fine_mfma) fine_op<T>(c->stream, fine_of<T>(c), u, epi, ep);
     else fine_mfma<T>(c->stream, fine_of<T>(c), u, epi, ep);
   } else {
     StencilOp<T> op;
@@ -212,7 +212,7 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
 template <typename T>
 int level_blocks(tofe_ctx *c, int l)
 {
-  if (l == 0) return c->fine_vector ? fine_op_blocks<T>(fine_of<T>(c)) : fine_mfma_blocks<T>(fine_of<T>(c));
+  if (l == 0) return c->fine_mfma ? fine_mfma_blocks<T>(fine_of<T>(c)) : fine_op_blocks<T>(fine_of<T>(c));
   StencilOp<T> op;
   op.Nx = c->grids[l].Nx; op.Ny = c->grids[l].Ny; op.Nz = c->grids[l].Nz;
   return stencil_op_blocks<T>(op);
@@ -730,7 +730,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
   c->precision = (desc->precision == TOFE_PREC_MIXED) ? TOFE_PREC_MIXED : TOFE_PREC_FP64;
   try {
     TOFE_HIP(hipGetDevice(&c->device));
-    if (const char *fv = std::getenv("TOFE_FINE_VECTOR")) c->fine_vector = fv[0] == '1';
+    if (const char *fm = std::getenv("TOFE_FINE_MFMA")) c->fine_mfma = fm[0] == '1';
     // level count: floor(log2(min nel)) unless given (RecMesh.cpp:552-554), truncated
     // once a coarse grid has <= 10 DOFs (Interpolation.cpp:70-71)
     int L = d.mg_levels;
