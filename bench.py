@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="48x24x24 instead of 192x96x96")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only launch the dominant kernel 5 times and a 512 MiB device copy, "
+                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box in this "
@@ -109,6 +112,21 @@ def main():
     ctx = _lib.Context(pb.nel, Emin=pb.Emin, penal=pb.penal, rmin=pb.rmin,
                        mg_levels=pb.mg_levels, fixed_dof=pb.fixed_dof)
     ctx.set_design(pb.x)
+    if args.pmc_probe:
+        # calibration copy: 512 MiB read + 512 MiB written by one copy kernel
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        ctx.set_precision(_lib.PREC_MIXED)
+        ctx.assemble()
+        ms = ctx.time_operator(0, _lib.PREC_MIXED, 1, 5)
+        log(f"bench: pmc probe: copy 536870912 B read + written; fine operator fp32 cheb "
+            f"x5 {ms * 1e3:.1f} us each, algorithmic bytes "
+            f"{4.0 * (2 * pb.ndof + pb.nelem) + 16.0 * pb.ndof:.0f}")
+        ctx.close()
+        return
     F = torch.from_numpy(pb.F).to(dev)
     U = torch.zeros_like(F)
     r = torch.zeros_like(F)

@@ -187,6 +187,32 @@ int tofe_apply_M_dev(tofe_ctx *ctx, const double *d_X, double *d_Y, int nrhs);
  * dMdy = dVdrho m0 I). */
 int tofe_element_quadratic(tofe_ctx *ctx, const double *phi, int nvec, double *kq, double *mq);
 
+/* Linearised buckling (Stability::Function, Buckling.cpp:15-414).
+ * tofe_set_stress stores the stress state of the displacement U (StressFnc, 283-345):
+ * per element H_e = sum_q W detJ grad N^T s(D B_q u_e) grad N, so that the stress
+ * stiffness of the current design is Ks = sum_e (-Es_e) H_e (x) I_3 (Es from the last
+ * tofe_set_design). */
+int tofe_set_stress(tofe_ctx *ctx, const double *U);
+int tofe_set_stress_dev(tofe_ctx *ctx, const double *d_U);
+
+/* Y = Ks X with the Dirichlet rows and columns of MatZeroRowsColumns(Ks, fixed, 0.0)
+ * (Buckling.cpp:28-29): MatMult(Ks) of the eigensolvers (operators set at Buckling.cpp:67). */
+int tofe_apply_Ks(tofe_ctx *ctx, const double *X, double *Y, int nrhs);
+int tofe_apply_Ks_dev(tofe_ctx *ctx, const double *d_X, double *d_Y, int nrhs);
+
+/* Eigenvalue-sensitivity terms of nvec buckling modes phi (ndof each), either output may be
+ * NULL: pksp[j*nelem+e] = phi_e^T ks_e phi_e (times -dEsdz_e this is the pdKsp term,
+ * Buckling.cpp:239-242, 324); dksdu[j*ndof+d] = sum over the elements holding d of
+ * Es_e phi_e^T (d ks_e / d u_d) phi_e with fixed rows zero -- the right-hand sides of the
+ * adjoint solves (Buckling.cpp:130-191). */
+int tofe_stability_terms(tofe_ctx *ctx, const double *phi, int nvec, double *pksp,
+                         double *dksdu);
+
+/* out[j*nelem+e] = a_j,e^T KE b_e for nvec vectors a against one vector b (the
+ * v^T dK/dy U term of the buckling sensitivity without dEdz, Buckling.cpp:245-246). */
+int tofe_element_bilinear(tofe_ctx *ctx, const double *A, int nvec, const double *B,
+                          double *out);
+
 /* Z = M^-1 R, one multigrid V-cycle; PCApply of the KUF preconditioner
  * (LOPGMRES.cpp:125). */
 int tofe_pc_apply(tofe_ctx *ctx, const double *R, double *Z, int nrhs);

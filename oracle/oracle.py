@@ -93,6 +93,11 @@ def lib():
     L.orc_mg_setup_seconds.argtypes = [vp]
     L.orc_dot.restype = C.c_double
     L.orc_dot.argtypes = [C.c_int64, _f64p, _f64p]
+    L.orc_assemble_Ks.restype = vp
+    L.orc_assemble_Ks.argtypes = [C.c_int] * 3 + [C.c_double] * 5 + [_f64p, _f64p, vp]
+    L.orc_ks_quadratic.argtypes = [C.c_int] * 3 + [C.c_double] * 5 + [_f64p, _f64p, C.c_int, _f64p]
+    L.orc_stability_rhs.argtypes = [C.c_int] * 3 + [C.c_double] * 5 + [_f64p, _f64p, C.c_int, vp,
+                                                                       _f64p]
     L.orc_mass_diag.argtypes = [C.c_int] * 3 + [_f64p, C.c_double, vp, vp, _f64p]
     L.orc_sensitivity.argtypes = [C.c_int] * 3 + [_f64p, _f64p, _f64p, _f64p]
     L.orc_power_start.argtypes = [C.c_int64, _f64p]
@@ -256,6 +261,36 @@ def mass_diag(nx, ny, nz, V, density, h=(1.0, 1.0, 1.0), mlump=None, fixed_dof=N
                         None if ml is None else ml.ctypes.data_as(C.c_void_p),
                         None if fx is None else fx.ctypes.data_as(C.c_void_p), M)
     return M
+
+
+def assemble_Ks(nx, ny, nz, U, Es, fixed_dof=None, E0=1.0, nu=0.3, h=(1.0, 1.0, 1.0)):
+    """Stress stiffness Ks = sum_e (-Es_e) ks_e(u_e) with zeroed fixed rows/columns."""
+    fx = None if fixed_dof is None else np.ascontiguousarray(fixed_dof, np.uint8)
+    return CSR(lib().orc_assemble_Ks(nx, ny, nz, E0, nu, h[0], h[1], h[2],
+                                     np.ascontiguousarray(U, np.float64),
+                                     np.ascontiguousarray(Es, np.float64),
+                                     None if fx is None else fx.ctypes.data_as(C.c_void_p)))
+
+
+def ks_quadratic(nx, ny, nz, U, phi, E0=1.0, nu=0.3, h=(1.0, 1.0, 1.0)):
+    phi = np.ascontiguousarray(phi, np.float64)
+    ne = nx * ny * nz
+    nvec = phi.size // (3 * (nx + 1) * (ny + 1) * (nz + 1))
+    out = np.empty(nvec * ne)
+    lib().orc_ks_quadratic(nx, ny, nz, E0, nu, h[0], h[1], h[2], np.ascontiguousarray(U, np.float64),
+                           phi, nvec, out)
+    return out.reshape(nvec, ne)
+
+
+def stability_rhs(nx, ny, nz, Es, phi, fixed_dof=None, E0=1.0, nu=0.3, h=(1.0, 1.0, 1.0)):
+    phi = np.ascontiguousarray(phi, np.float64)
+    ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    nvec = phi.size // ndof
+    out = np.empty(nvec * ndof)
+    fx = None if fixed_dof is None else np.ascontiguousarray(fixed_dof, np.uint8)
+    lib().orc_stability_rhs(nx, ny, nz, E0, nu, h[0], h[1], h[2], np.ascontiguousarray(Es, np.float64),
+                            phi, nvec, None if fx is None else fx.ctypes.data_as(C.c_void_p), out)
+    return out.reshape(nvec, ndof)
 
 
 def power_start(n):

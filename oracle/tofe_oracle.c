@@ -1073,3 +1073,233 @@ void orc_sensitivity(int nx, int ny, int nz, const double KE[576], const double 
     g[el] = -dEdz[el] * s;
   }
 }
+
+/* ======================================================================
+ * Linearised buckling: stress stiffness and its sensitivities
+ * (Stability::Function / StressFnc / sigtos, Buckling.cpp:15-414)
+ * ==================================================================== */
+
+/* dN/dx (3 x 8) at the 8 Gauss points and det(J) of the axis-aligned box element
+ * (LocalK FEAnalysis.cpp:869-892, GaussPoints 900-926, dN 936-990). */
+void orc_shape_grad(double hx, double hy, double hz, double dNdx[8][3][8], double *detJ)
+{
+  const double g = 1 / sqrt(3.0);
+  const double h[3] = {hx, hy, hz};
+  for (int q = 0; q < 8; q++) {
+    double xi = (2 * LN[q][0] - 1) * g, eta = (2 * LN[q][1] - 1) * g,
+           zeta = (2 * LN[q][2] - 1) * g;
+    double dNdxi[3][8];
+    for (int n = 0; n < 8; n++) {
+      double sx = 2 * LN[n][0] - 1, sy = 2 * LN[n][1] - 1, sz = 2 * LN[n][2] - 1;
+      dNdxi[0][n] = sx / 8 * (1 + sy * eta) * (1 + sz * zeta);
+      dNdxi[1][n] = sy / 8 * (1 + sx * xi) * (1 + sz * zeta);
+      dNdxi[2][n] = sz / 8 * (1 + sx * xi) * (1 + sy * eta);
+    }
+    /* J = dNdxi * coords is diagonal (h/2) for the box; invert it the general way */
+    double J[3][3];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        double s = 0;
+        for (int n = 0; n < 8; n++) s += dNdxi[a][n] * (h[b] * LN[n][b]);
+        J[a][b] = s;
+      }
+    double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                 J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                 J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    double inv[3][3];
+    inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+    inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+    inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    for (int a = 0; a < 3; a++)
+      for (int n = 0; n < 8; n++) {
+        double s = 0;
+        for (int b = 0; b < 3; b++) s += inv[a][b] * dNdxi[b][n];
+        dNdx[q][a][n] = s;
+      }
+    *detJ = det;
+  }
+}
+
+/* B (6x24, AssignB FEAnalysis.cpp:1024-1046) and G (9x24, AssignG 1062-1081:
+ * G[3i+j][3n+i] = dN_n/dx_j) of one Gauss point. */
+static void bg_matrices(const double dNdx[3][8], double B[6][24], double G[9][24])
+{
+  memset(B, 0, 6 * 24 * sizeof(double));
+  memset(G, 0, 9 * 24 * sizeof(double));
+  for (int n = 0; n < 8; n++) {
+    B[0][3 * n + 0] = dNdx[0][n];
+    B[1][3 * n + 1] = dNdx[1][n];
+    B[2][3 * n + 2] = dNdx[2][n];
+    B[3][3 * n + 0] = dNdx[1][n];
+    B[3][3 * n + 1] = dNdx[0][n];
+    B[4][3 * n + 1] = dNdx[2][n];
+    B[4][3 * n + 2] = dNdx[1][n];
+    B[5][3 * n + 0] = dNdx[2][n];
+    B[5][3 * n + 2] = dNdx[0][n];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) G[3 * i + j][3 * n + i] = dNdx[j][n];
+  }
+}
+
+/* ks(u) = sum_q W G_q^T S(D B_q u) G_q detJ, W = 1 (StressFnc Buckling.cpp:313-317); S is
+ * the 9x9 block diagonal of three copies of the 3x3 stress tensor built from the Voigt
+ * vector [xx,yy,zz,xy,yz,xz] (sigtos, 371-414). */
+static void stress_stiffness(const double D[36], const double dNdx[8][3][8], double detJ,
+                             const double u[24], double ks[576])
+{
+  memset(ks, 0, 576 * sizeof(double));
+  for (int q = 0; q < 8; q++) {
+    double B[6][24], G[9][24];
+    bg_matrices(dNdx[q], B, G);
+    double eps[6], sig[6];
+    for (int a = 0; a < 6; a++) {
+      double s = 0;
+      for (int c = 0; c < 24; c++) s += B[a][c] * u[c];
+      eps[a] = s;
+    }
+    for (int a = 0; a < 6; a++) {
+      double s = 0;
+      for (int b = 0; b < 6; b++) s += D[a * 6 + b] * eps[b];
+      sig[a] = s;
+    }
+    double S3[3][3] = {{sig[0], sig[3], sig[5]}, {sig[3], sig[1], sig[4]}, {sig[5], sig[4], sig[2]}};
+    double S[9][9];
+    memset(S, 0, sizeof(S));
+    for (int blk = 0; blk < 3; blk++)
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) S[3 * blk + a][3 * blk + b] = S3[a][b];
+    double SG[9][24];
+    for (int a = 0; a < 9; a++)
+      for (int c = 0; c < 24; c++) {
+        double s = 0;
+        for (int b = 0; b < 9; b++) s += S[a][b] * G[b][c];
+        SG[a][c] = s;
+      }
+    for (int r = 0; r < 24; r++)
+      for (int c = 0; c < 24; c++) {
+        double s = 0;
+        for (int a = 0; a < 9; a++) s += G[a][r] * SG[a][c];
+        ks[r * 24 + c] += 1.0 * s * detJ;
+      }
+  }
+}
+
+static void element_dofs(int nx, int ny, int64_t e, int64_t dof[24])
+{
+  int64_t Nx = nx + 1, Ny = ny + 1;
+  int64_t ex = e % nx, ey = (e / nx) % ny, ez = e / ((int64_t)nx * ny);
+  for (int l = 0; l < 8; l++) {
+    int64_t n = (ex + LN[l][0]) + Nx * ((ey + LN[l][1]) + Ny * (ez + LN[l][2]));
+    for (int c = 0; c < 3; c++) dof[3 * l + c] = 3 * n + c;
+  }
+}
+
+orc_csr *orc_assemble_Ks(int nx, int ny, int nz, double E0, double nu, double hx, double hy,
+                         double hz, const double *U, const double *Es, const uint8_t *fixed)
+{
+  double D[36], dNdx[8][3][8], detJ;
+  orc_dmat(E0, nu, D);
+  orc_shape_grad(hx, hy, hz, dNdx, &detJ);
+  int64_t ne = (int64_t)nx * ny * nz;
+  int64_t ndof = 3 * (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  /* dense accumulation per row in a CSR with the 81-entry node pattern: build through
+   * triplets into a dense-per-row map keyed by column */
+  orc_csr *K = orc_assemble_K(nx, ny, nz, (double[576]){0}, Es, NULL); /* pattern only */
+  for (int64_t p = 0; p < K->nnz; p++) K->val[p] = 0.0;
+  for (int64_t e = 0; e < ne; e++) {
+    int64_t dof[24];
+    element_dofs(nx, ny, e, dof);
+    double u[24], ks[576];
+    for (int c = 0; c < 24; c++) u[c] = U[dof[c]];
+    stress_stiffness(D, dNdx, detJ, u, ks);
+    for (int r = 0; r < 24; r++)
+      for (int c = 0; c < 24; c++) {
+        double v = -Es[e] * ks[r * 24 + c];    /* ks *= -p_Es[el] (Buckling.cpp:331) */
+        int64_t row = dof[r];
+        for (int64_t p = K->rowptr[row]; p < K->rowptr[row + 1]; p++)
+          if (K->col[p] == dof[c]) { K->val[p] += v; break; }
+      }
+  }
+  if (fixed)  /* MatZeroRowsColumns(Ks, fixed, 0.0) (Buckling.cpp:28-29) */
+    for (int64_t r = 0; r < ndof; r++)
+      for (int64_t p = K->rowptr[r]; p < K->rowptr[r + 1]; p++)
+        if (fixed[r] || fixed[K->col[p]]) K->val[p] = 0.0;
+  return K;
+}
+
+void orc_ks_quadratic(int nx, int ny, int nz, double E0, double nu, double hx, double hy,
+                      double hz, const double *U, const double *phi, int nvec, double *out)
+{
+  double D[36], dNdx[8][3][8], detJ;
+  orc_dmat(E0, nu, D);
+  orc_shape_grad(hx, hy, hz, dNdx, &detJ);
+  int64_t ne = (int64_t)nx * ny * nz;
+  int64_t ndof = 3 * (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  for (int64_t e = 0; e < ne; e++) {
+    int64_t dof[24];
+    element_dofs(nx, ny, e, dof);
+    double u[24], ks[576];
+    for (int c = 0; c < 24; c++) u[c] = U[dof[c]];
+    stress_stiffness(D, dNdx, detJ, u, ks);
+    for (int j = 0; j < nvec; j++) {
+      const double *ph = phi + (int64_t)j * ndof;
+      double s = 0;
+      for (int r = 0; r < 24; r++) {
+        double t = 0;
+        for (int c = 0; c < 24; c++) t += ks[r * 24 + c] * ph[dof[c]];
+        s += ph[dof[r]] * t;
+      }
+      out[(int64_t)j * ne + e] = s;
+    }
+  }
+}
+
+/* dKsdu[d] = sum_q W G^T S(D B e_d) G detJ for the 24 unit element displacements
+ * (Buckling.cpp:132-147); the adjoint right-hand side of eigenvector j is
+ * dKsdU[d_global] += Es_e phi_e^T dKsdu[d_local] phi_e over the elements holding the DOF
+ * (155-172), with fixed rows zeroed (187-191). */
+void orc_stability_rhs(int nx, int ny, int nz, double E0, double nu, double hx, double hy,
+                       double hz, const double *Es, const double *phi, int nvec,
+                       const uint8_t *fixed, double *out)
+{
+  double D[36], dNdx[8][3][8], detJ;
+  orc_dmat(E0, nu, D);
+  orc_shape_grad(hx, hy, hz, dNdx, &detJ);
+  static double dks[24][576];
+  for (int d = 0; d < 24; d++) {
+    double du[24] = {0};
+    du[d] = 1;
+    stress_stiffness(D, dNdx, detJ, du, dks[d]);
+  }
+  int64_t ne = (int64_t)nx * ny * nz;
+  int64_t ndof = 3 * (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  memset(out, 0, (size_t)nvec * ndof * sizeof(double));
+  for (int64_t e = 0; e < ne; e++) {
+    int64_t dof[24];
+    element_dofs(nx, ny, e, dof);
+    for (int j = 0; j < nvec; j++) {
+      const double *ph = phi + (int64_t)j * ndof;
+      double pl[24];
+      for (int c = 0; c < 24; c++) pl[c] = ph[dof[c]];
+      for (int d = 0; d < 24; d++) {
+        double s = 0;
+        for (int r = 0; r < 24; r++) {
+          double t = 0;
+          for (int c = 0; c < 24; c++) t += Es[e] * dks[d][r * 24 + c] * pl[c];
+          s += pl[r] * t;
+        }
+        out[(int64_t)j * ndof + dof[d]] += s;
+      }
+    }
+  }
+  if (fixed)
+    for (int j = 0; j < nvec; j++)
+      for (int64_t r = 0; r < ndof; r++)
+        if (fixed[r]) out[(int64_t)j * ndof + r] = 0.0;
+}

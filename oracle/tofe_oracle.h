@@ -114,6 +114,19 @@ double orc_lanczos_lambda(const orc_csr *A, const double *dinv, int its);
 
 int orc_num_threads(void);
 
+/* ---- linearised buckling (Buckling.cpp:15-414) ---- */
+void orc_shape_grad(double hx, double hy, double hz, double dNdx[8][3][8], double *detJ);
+/* Ks = sum_e (-Es_e) ks_e(u_e), MatZeroRowsColumns(Ks, fixed, 0.0) */
+orc_csr *orc_assemble_Ks(int nx, int ny, int nz, double E0, double nu, double hx, double hy,
+                         double hz, const double *U, const double *Es, const uint8_t *fixed);
+/* out[j*ne + e] = phi_j,e^T ks_e(u_e) phi_j,e (without the -Es_e factor) */
+void orc_ks_quadratic(int nx, int ny, int nz, double E0, double nu, double hx, double hy,
+                      double hz, const double *U, const double *phi, int nvec, double *out);
+/* adjoint right-hand sides dKsdU (nvec x ndof) */
+void orc_stability_rhs(int nx, int ny, int nz, double E0, double nu, double hx, double hy,
+                       double hz, const double *Es, const double *phi, int nvec,
+                       const uint8_t *fixed, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,179 @@
+"""Pins of the oracle's linearised-buckling pieces (orc_assemble_Ks, orc_ks_quadratic,
+orc_stability_rhs; Stability::StressFnc / sigtos / Function, Buckling.cpp:15-414)
+against what the mathematics fixes:
+
+- uniform stress: for u = a x (uniaxial) and u = g y e_x (shear) the element geometric
+  stiffness is block diagonal over the displacement components with the scalar matrix
+  sum_ab s_ab int dN_k/dx_a dN_m/dx_b, built here from exact 1-D integrals (2-point Gauss
+  is exact for these degrees), so B, D, G and sigtos are checked against closed forms;
+- Ks is symmetric, annihilates rigid translations, zeroes fixed rows and columns;
+- linearity in u: dKsdU . du = -phi^T Ks(du) phi (the adjoint right-hand side is the
+  derivative of the quadratic form), and sum_e -Es_e phi_e^T ks_e phi_e = phi^T Ks phi.
+"""
+import numpy as np
+import pytest
+
+from topopt3d_amd import configs
+
+LN = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+
+
+def one_d(h):
+    k = np.array([[1.0, -1.0], [-1.0, 1.0]]) / h            # int N_a' N_b'
+    m = np.array([[h / 3, h / 6], [h / 6, h / 3]])           # int N_a N_b
+    g = np.array([[-0.5, -0.5], [0.5, 0.5]])                 # int N_a' N_b
+    return k, m, g
+
+
+def closed_form_ks(s, h):
+    """24x24 ks for a uniform 3x3 stress tensor s on a box element of size h."""
+    ops = [one_d(hd) for hd in h]
+    H = np.zeros((8, 8))
+    for a in range(3):
+        for b in range(3):
+            if s[a][b] == 0:
+                continue
+            for k_ in range(8):
+                for m_ in range(8):
+                    v = 1.0
+                    for d in range(3):
+                        kd, md, gd = ops[d]
+                        lk, lm = LN[k_][d], LN[m_][d]
+                        if d == a and d == b:
+                            v *= kd[lk, lm]
+                        elif d == a:
+                            v *= gd[lk, lm]
+                        elif d == b:
+                            v *= gd[lm, lk]
+                        else:
+                            v *= md[lk, lm]
+                    H[k_, m_] += s[a][b] * v
+    return np.kron(H, np.eye(3))
+
+
+def dmat(E0=1.0, nu=0.3):
+    c = E0 / ((1 + nu) * (1 - 2 * nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = nu * c
+    np.fill_diagonal(D[:3, :3], (1 - nu) * c)
+    D[3, 3] = D[4, 4] = D[5, 5] = E0 / (2 * (1 + nu))
+    return D
+
+
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 1.5, 2.0)])
+@pytest.mark.parametrize("case", ["uniaxial", "shear"])
+def test_uniform_stress_closed_form(oracle, h, case):
+    # one element: global node g = x + 2 y + 4 z, local node l at LN[l]
+    glob = [l[0] + 2 * l[1] + 4 * l[2] for l in LN]
+    coords = np.array([[h[0] * (g % 2), h[1] * ((g // 2) % 2), h[2] * (g // 4)] for g in range(8)])
+    u = np.zeros(24)
+    strain = np.zeros(6)
+    if case == "uniaxial":
+        u[0::3] = 0.01 * coords[:, 0]       # u_x = a x
+        strain[0] = 0.01
+    else:
+        u[0::3] = 0.02 * coords[:, 1]       # u_x = g y: engineering shear xy = g
+        strain[3] = 0.02
+    sig = dmat() @ strain
+    s = [[sig[0], sig[3], sig[5]], [sig[3], sig[1], sig[4]], [sig[5], sig[4], sig[2]]]
+    K = oracle.assemble_Ks(1, 1, 1, u, np.ones(1), h=h).to_scipy().toarray()
+    dof = np.array([3 * g + c for g in glob for c in range(3)])
+    ref = np.zeros((24, 24))
+    ref[np.ix_(dof, dof)] = -closed_form_ks(s, h)
+    assert np.abs(K - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_symmetry_translation_and_fixed_rows(oracle):
+    nel = (4, 3, 2)
+    pb = configs.cantilever(nel, seed=1, lo=0.1, hi=1.0)
+    rng = np.random.default_rng(0)
+    U = rng.standard_normal(pb.ndof)
+    Es = configs.uniform(3, pb.nelem, 0.1, 1.0)
+    K = oracle.assemble_Ks(*nel, U, Es).to_scipy()
+    assert abs(K - K.T).max() <= 1e-13 * abs(K).max()
+    for c in range(3):
+        t = np.zeros(pb.ndof)
+        t[c::3] = 1.0
+        assert np.abs(K @ t).max() <= 1e-13 * abs(K).max()
+    Kf = oracle.assemble_Ks(*nel, U, Es, pb.fixed_dof).to_scipy().toarray()
+    fx = pb.fixed_dof.astype(bool)
+    assert (Kf[fx] == 0).all() and (Kf[:, fx] == 0).all()
+    assert np.array_equal(Kf[np.ix_(~fx, ~fx)], K.toarray()[np.ix_(~fx, ~fx)])
+
+
+def test_adjoint_rhs_is_derivative_of_quadratic_form(oracle):
+    nel = (4, 3, 3)
+    pb = configs.cantilever(nel, seed=2, lo=0.1, hi=1.0)
+    rng = np.random.default_rng(3)
+    Es = configs.uniform(5, pb.nelem, 0.1, 1.0)
+    phi = rng.standard_normal((2, pb.ndof))
+    fx = pb.fixed_dof.astype(bool)
+    phi[:, fx] = 0
+    rhs = oracle.stability_rhs(*nel, Es, phi, pb.fixed_dof)
+    assert (rhs[:, fx] == 0).all()
+    du = rng.standard_normal(pb.ndof)
+    du[fx] = 0
+    Kd = oracle.assemble_Ks(*nel, du, Es).to_scipy()
+    for j in range(2):
+        lhs = rhs[j] @ du
+        quad = phi[j] @ (Kd @ phi[j])
+        assert lhs == pytest.approx(-quad, rel=1e-12)
+    # element quadratic forms sum to the assembled one
+    U = rng.standard_normal(pb.ndof)
+    q = oracle.ks_quadratic(*nel, U, phi)
+    KU = oracle.assemble_Ks(*nel, U, Es).to_scipy()
+    for j in range(2):
+        assert (-Es * q[j]).sum() == pytest.approx(phi[j] @ (KU @ phi[j]), rel=1e-12)
+
+
+def _buckling_value(oracle, nel, pb, x, nev, p):
+    """Aggregate (sum mu_j^p)^(1/p) of the nev largest mu of Ks phi = mu K phi and the
+    pieces of the reference's sensitivity formula (Buckling.cpp:80-262), dense."""
+    import scipy.linalg as sla
+
+    P = oracle.filter_csr(*nel, 1.5)
+    rho = P.matvec(x)
+    E, dEdz, Es, dEsdz = oracle.simp(rho)
+    KE = oracle.ke()
+    K = oracle.assemble_K(*nel, KE, E, pb.fixed_dof).to_scipy().toarray()
+    U = np.linalg.solve(K, pb.F)
+    Ks = oracle.assemble_Ks(*nel, U, Es, pb.fixed_dof).to_scipy().toarray()
+    mu, V = sla.eigh(Ks, K)
+    mu, V = mu[::-1][:nev], V[:, ::-1][:, :nev]   # largest first, K-normalised
+    val = np.sum(mu ** p) ** (1 / p)
+    return val, dict(P=P, rho=rho, E=E, dEdz=dEdz, Es=Es, dEsdz=dEsdz, KE=KE, K=K, U=U, mu=mu, V=V)
+
+
+def test_buckling_sensitivity_chain_against_finite_differences(oracle):
+    """The reference's buckling sensitivity (eigenvalue term with dKsdy = -dEsdz ks, the
+    material term, and the adjoint term v^T dK/dy U with K v = dKsdU) matches central
+    differences of the aggregated eigenvalues, so every piece has the right sign."""
+    nel, nev, p = (6, 2, 2), 2, 8.0
+    pb = configs.cantilever(nel, seed=4, lo=0.3, hi=1.0)
+    val, s = _buckling_value(oracle, nel, pb, pb.x, nev, p)
+    assert (s["mu"] > 0).all() and s["mu"][0] > 1.01 * s["mu"][1]
+    phi = s["V"].T.copy()
+    q = oracle.ks_quadratic(*nel, s["U"], phi)                       # phi^T ks phi
+    rhs = oracle.stability_rhs(*nel, s["Es"], phi, pb.fixed_dof)     # dKsdU
+    ones = np.ones(pb.nelem)
+    pKp = np.stack([-oracle.sensitivity(*nel, s["KE"], phi[j], ones) for j in range(nev)])
+    grad = np.zeros(pb.nelem)
+    for j in range(nev):
+        v = np.linalg.solve(s["K"], rhs[j])
+        # v_e^T KE U_e by polarisation of the element quadratic form
+        qa = -oracle.sensitivity(*nel, s["KE"], v + s["U"], ones)
+        qb = -oracle.sensitivity(*nel, s["KE"], v - s["U"], ones)
+        vKu = (qa - qb) / 4
+        pdKsp = -s["dEsdz"] * q[j]
+        pdKp = s["dEdz"] * pKp[j]
+        grad += s["mu"][j] ** (p - 1) * (pdKsp - s["mu"][j] * pdKp + s["dEdz"] * vKu)
+    grad *= val ** (1 - p)
+    grad = s["P"].rmatvec(grad)
+    h = 1e-6
+    for e in (0, 5, 13, 22):
+        xp, xm = pb.x.copy(), pb.x.copy()
+        xp[e] += h
+        xm[e] -= h
+        fd = (_buckling_value(oracle, nel, pb, xp, nev, p)[0] -
+              _buckling_value(oracle, nel, pb, xm, nev, p)[0]) / (2 * h)
+        assert fd == pytest.approx(grad[e], rel=1e-5, abs=1e-9 * np.abs(grad).max())

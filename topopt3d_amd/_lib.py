@@ -43,6 +43,8 @@ EXPORTS = (
     "tofe_set_penalty", "tofe_chain_filter", "tofe_assemble_mass", "tofe_apply_M",
     "tofe_apply_M_dev", "tofe_element_quadratic", "tofe_petsc_vec_write", "tofe_petsc_vec_read",
     "tofe_petsc_aij_write", "tofe_petsc_aij_read", "tofe_mesh_out", "tofe_print_vals",
+    "tofe_set_stress", "tofe_set_stress_dev", "tofe_apply_Ks", "tofe_apply_Ks_dev",
+    "tofe_stability_terms", "tofe_element_bilinear",
 )
 
 
@@ -131,6 +133,12 @@ def lib():
         "tofe_petsc_aij_read": ([C.c_char_p, i64p, i64p, i64p, i64p, i32p, dp, C.c_int], C.c_int),
         "tofe_mesh_out": ([vp, C.c_char_p, dp, C.c_int], C.c_int),
         "tofe_print_vals": ([vp, C.c_char_p, C.c_char_p, dp, C.c_int], C.c_int),
+        "tofe_set_stress": ([vp, dp], C.c_int),
+        "tofe_set_stress_dev": ([vp, vp], C.c_int),
+        "tofe_apply_Ks": ([vp, dp, dp, C.c_int], C.c_int),
+        "tofe_apply_Ks_dev": ([vp, vp, vp, C.c_int], C.c_int),
+        "tofe_stability_terms": ([vp, dp, C.c_int, dp, dp], C.c_int),
+        "tofe_element_bilinear": ([vp, dp, C.c_int, dp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -367,6 +375,32 @@ class Context:
         U = None if U is None else _f64(U)
         self._check(lib().tofe_print_vals(self.h, os.fsencode(directory), suffix.encode(), _dp(U),
                                           index_bytes))
+
+    def set_stress(self, U):
+        self._check(lib().tofe_set_stress(self.h, _dp(_f64(U))))
+
+    def apply_Ks(self, X):
+        X = _f64(X)
+        nrhs = X.size // self.ndof
+        Y = np.empty_like(X)
+        self._check(lib().tofe_apply_Ks(self.h, _dp(X), _dp(Y), nrhs))
+        return Y
+
+    def stability_terms(self, phi, quadratic=True, rhs=True):
+        """(pksp (nvec, nelem), dksdu (nvec, ndof)) of buckling modes phi."""
+        phi = _f64(phi)
+        nvec = phi.size // self.ndof
+        q = np.empty((nvec, self.nelem)) if quadratic else None
+        r = np.empty((nvec, self.ndof)) if rhs else None
+        self._check(lib().tofe_stability_terms(self.h, _dp(phi), nvec, _dp(q), _dp(r)))
+        return q, r
+
+    def element_bilinear(self, A, B):
+        A, B = _f64(A), _f64(B)
+        nvec = A.size // self.ndof
+        out = np.empty((nvec, self.nelem))
+        self._check(lib().tofe_element_bilinear(self.h, _dp(A), nvec, _dp(B), _dp(out)))
+        return out
 
     def set_penalty(self, penal, vd_penal=1.0):
         self._check(lib().tofe_set_penalty(self.h, penal, vd_penal))

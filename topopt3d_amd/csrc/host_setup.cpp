@@ -12,17 +12,21 @@ namespace tofe {
 static const int kLN[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                               {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
 
-void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE[576])
+void dmatrix(double E0, double nu, double D[36])
 {
   // Isotropic constitutive matrix, Voigt [xx,yy,zz,xy,yz,xz] (RecMesh.cpp:453-462).
   const double c = E0 / ((1 + nu) * (1 - 2 * nu));
   const double G = E0 / (2 * (1 + nu));
   const double t1 = (1 - nu) * c, t2 = G, t3 = nu * c;
-  double D[6][6] = {{t1, t3, t3, 0, 0, 0}, {t3, t1, t3, 0, 0, 0}, {t3, t3, t1, 0, 0, 0},
-                    {0, 0, 0, t2, 0, 0},   {0, 0, 0, 0, t2, 0},   {0, 0, 0, 0, 0, t2}};
+  const double v[36] = {t1, t3, t3, 0, 0, 0, t3, t1, t3, 0, 0, 0, t3, t3, t1, 0, 0, 0,
+                        0,  0,  0,  t2, 0, 0, 0,  0,  0,  0, t2, 0, 0,  0,  0,  0, 0, t2};
+  std::memcpy(D, v, sizeof(v));
+}
+
+void shape_gradients(double hx, double hy, double hz, double dNdx[8][3][8], double *detJ)
+{
   const double g = 1 / std::sqrt(3.0);
   const double h[3] = {hx, hy, hz};
-  std::memset(KE, 0, 576 * sizeof(double));
   // 2x2x2 Gauss points at +-1/sqrt(3), unit weights, in local-node sign order
   // (GaussPoints FEAnalysis.cpp:900-926; LocalK 869-892).
   for (int q = 0; q < 8; q++) {
@@ -53,13 +57,25 @@ void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE
     inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
     inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
     inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
-    double dNdx[3][8];
     for (int a = 0; a < 3; a++)
       for (int n = 0; n < 8; n++) {
         double s = 0;
         for (int b = 0; b < 3; b++) s += inv[a][b] * dNdxi[b][n];
-        dNdx[a][n] = s;
+        dNdx[q][a][n] = s;
       }
+    *detJ = det;
+  }
+}
+
+void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE[576])
+{
+  double Dm[36];
+  dmatrix(E0, nu, Dm);
+  double dN[8][3][8], det;
+  shape_gradients(hx, hy, hz, dN, &det);
+  std::memset(KE, 0, 576 * sizeof(double));
+  for (int q = 0; q < 8; q++) {
+    const double(&dNdx)[3][8] = dN[q];
     // Strain-displacement matrix B (AssignB, 1024-1046)
     double B[6][24];
     std::memset(B, 0, sizeof(B));
@@ -78,7 +94,7 @@ void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE
     for (int a = 0; a < 6; a++)
       for (int cc = 0; cc < 24; cc++) {
         double s = 0;
-        for (int b = 0; b < 6; b++) s += D[a][b] * B[b][cc];
+        for (int b = 0; b < 6; b++) s += Dm[a * 6 + b] * B[b][cc];
         DB[a][cc] = s;
       }
     for (int r = 0; r < 24; r++)

@@ -10,6 +10,11 @@ namespace tofe {
 // (TopOpt::LocalK, FEAnalysis.cpp:869-1050).
 void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE[576]);
 
+// Isotropic D (6x6 row-major, Voigt xx,yy,zz,xy,yz,xz) and the shape-function gradients
+// dN_n/dx_a at the 8 Gauss points of the box element with det(J) (unit weights).
+void dmatrix(double E0, double nu, double D[36]);
+void shape_gradients(double hx, double hy, double hz, double dNdx[8][3][8], double *detJ);
+
 // Density-filter neighbourhood: offsets (i,j,k) walked k, j, i ascending with
 // dist < R, and weights 1 - dist/R (TopOpt::RecFilter, Filter.cpp:21-94).
 int filter_template(double dx, double dy, double dz, double R, std::vector<int> &off,

@@ -57,66 +57,74 @@ __global__ __launch_bounds__(NT) void k_stencil_op(StencilOp<T> op, const T *__r
   }
 }
 
-// One thread per (coarse node I, coarse offset o >= 13, block entry q = 3a + b): the
-// entry of P^T A P coupling I with J = I + off(o), summed over fine pairs (i, i+d) with
-// nonzero transfer weights; the transposed entry goes to (J, 26-o), so the operator is
-// exactly symmetric.  Nine threads per block keep the small coarse levels busy.
-__global__ __launch_bounds__(NT) void k_galerkin(const double *__restrict__ Sf, int Nfx, int Nfy,
-                                                 int Nfz, double *__restrict__ Sc, int Ncx,
-                                                 int Ncy, int Ncz)
+// P^T A P for a 27-point 3x3-block stencil field, one axis at a time.  P is a tensor
+// product of the 1-D row-scaled interpolations w1d, so the triple sum over fine nodes and
+// fine offsets factorises: restricting x, then y, then z, each step is a 1-D Galerkin
+// product of a stencil field on a grid (n0, n1, n2) whose `axis` has Nf nodes into one
+// with Nc = (Nf + 1) / 2 nodes along that axis:
+//   out(node', o', q) = sum_{i, d} w(i, I) w(i + d, I + D) in(node' with i, o' with d, q)
+// where I, D are node' and o' along the axis and the other coordinates are unchanged.
+// One thread per output value; every output value is written exactly once.  In the last
+// step (upper != 0) offsets o' >= 13 are computed and mirrored to (J, 26 - o', b, a), so
+// the coarse operator is exactly symmetric; o' < 13 threads only write the zeros of
+// couplings that leave the grid.
+__global__ __launch_bounds__(NT) void k_galerkin_axis(const double *__restrict__ in, int n0,
+                                                      int n1, int n2, int axis,
+                                                      double *__restrict__ out, int upper)
 {
-  const int64_t ncn = (int64_t)Ncx * Ncy * Ncz, nfn = (int64_t)Nfx * Nfy * Nfz;
+  int m[3] = {n0, n1, n2};
+  const int Nf = m[axis];
+  const int Nc = (Nf + 1) / 2;
+  m[axis] = Nc;
+  const int64_t nin = (int64_t)n0 * n1 * n2, nout = (int64_t)m[0] * m[1] * m[2];
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
-  if (t >= 126 * ncn) return;
-  const int oq = (int)(t / ncn);
-  const int64_t I = t - (int64_t)oq * ncn;
-  const int o = 13 + oq / 9, q = oq % 9;
-  const int Dx = o % 3 - 1, Dy = (o / 3) % 3 - 1, Dz = o / 9 - 1;
-  const int Ix = (int)(I % Ncx), Iy = (int)((I / Ncx) % Ncy), Iz = (int)(I / ((int64_t)Ncx * Ncy));
-  const int Jx = Ix + Dx, Jy = Iy + Dy, Jz = Iz + Dz;
-  if (Jx < 0 || Jx >= Ncx || Jy < 0 || Jy >= Ncy || Jz < 0 || Jz >= Ncz) return;
-
-  int li[3][9], ld[3][9];
-  double lw[3][9];
-  int cnt[3];
-  const int Ic[3] = {Ix, Iy, Iz}, Jc[3] = {Jx, Jy, Jz}, Nf[3] = {Nfx, Nfy, Nfz};
+  if (t >= 243 * nout) return;
+  const int oq = (int)(t / nout);
+  const int64_t node = t - (int64_t)oq * nout;
+  const int o = oq / 9, q = oq % 9;
+  int c[3];
+  c[0] = (int)(node % m[0]);
+  c[1] = (int)((node / m[0]) % m[1]);
+  c[2] = (int)(node / ((int64_t)m[0] * m[1]));
+  const int off[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+  // the coupled output node must exist
+  bool inside = true;
 #pragma unroll
-  for (int dim = 0; dim < 3; ++dim) {
-    int c = 0;
-    for (int i = 2 * Ic[dim] - 1; i <= 2 * Ic[dim] + 1; ++i) {
-      if (i < 0 || i >= Nf[dim]) continue;
-      const double wi = w1d(i, Ic[dim], Nf[dim]);
-      if (wi == 0.0) continue;
-      for (int d = -1; d <= 1; ++d) {
-        const int jj = i + d;
-        if (jj < 0 || jj >= Nf[dim]) continue;
-        const double wj = w1d(jj, Jc[dim], Nf[dim]);
-        if (wj == 0.0) continue;
-        li[dim][c] = i;
-        ld[dim][c] = d;
-        lw[dim][c] = wi * wj;
-        ++c;
-      }
-    }
-    cnt[dim] = c;
+  for (int k = 0; k < 3; ++k) {
+    const int cc = c[k] + off[k];
+    if (cc < 0 || cc >= m[k]) inside = false;
   }
-  const double *Sq = Sf + (int64_t)q * nfn;
+  if (!inside) {
+    out[t] = 0.0;
+    return;
+  }
+  if (upper && o < 13) return;  // written by the mirror of (J, 26 - o)
+  const int I = c[axis], D = off[axis];
+  const int sa = axis == 0 ? 1 : (axis == 1 ? n0 : n0 * n1);  // input node stride along axis
+  const int so = axis == 0 ? 1 : (axis == 1 ? 3 : 9);         // offset-index stride of axis
+  int cin[3] = {c[0], c[1], c[2]};
+  cin[axis] = 0;
+  const int64_t base_node = cin[0] + (int64_t)n0 * (cin[1] + (int64_t)n1 * cin[2]);
+  const int o_base = o - (D + 1) * so;
   double acc = 0.0;
-  for (int cz = 0; cz < cnt[2]; ++cz)
-    for (int cy = 0; cy < cnt[1]; ++cy) {
-      const double wyz = lw[1][cy] * lw[2][cz];
-      const int64_t rowyz = (int64_t)Nfx * (li[1][cy] + (int64_t)Nfy * li[2][cz]);
-      const int odyz = 3 * (ld[1][cy] + 1) + 9 * (ld[2][cz] + 1);
-      for (int cx = 0; cx < cnt[0]; ++cx) {
-        const int od = (ld[0][cx] + 1) + odyz;
-        acc += lw[0][cx] * wyz * Sq[(int64_t)(od * 9) * nfn + li[0][cx] + rowyz];
-      }
+  for (int i = 2 * I - 1; i <= 2 * I + 1; ++i) {
+    if (i < 0 || i >= Nf) continue;
+    const double wi = w1d(i, I, Nf);
+    if (wi == 0.0) continue;
+    for (int d = -1; d <= 1; ++d) {
+      const int j = i + d;
+      if (j < 0 || j >= Nf) continue;
+      const double wj = w1d(j, I + D, Nf);
+      if (wj == 0.0) continue;
+      const int oin = o_base + (d + 1) * so;
+      acc += wi * wj * in[(int64_t)(oin * 9 + q) * nin + base_node + (int64_t)i * sa];
     }
-  Sc[(int64_t)(o * 9 + q) * ncn + I] = acc;
-  if (o != 13) {
-    const int64_t J = Jx + (int64_t)Ncx * (Jy + (int64_t)Ncy * Jz);
+  }
+  out[t] = acc;
+  if (upper && o != 13) {
+    const int64_t J = (c[0] + off[0]) + (int64_t)m[0] * ((c[1] + off[1]) + (int64_t)m[1] * (c[2] + off[2]));
     const int a = q / 3, b = q % 3;
-    Sc[(int64_t)((26 - o) * 9 + b * 3 + a) * ncn + J] = acc;
+    out[(int64_t)((26 - o) * 9 + b * 3 + a) * nout + J] = acc;
   }
 }
 
@@ -254,15 +262,29 @@ void stencil_op(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, cons
   TOFE_LAUNCH_CHECK();
 }
 
-void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double *Sc, int Ncx,
-              int Ncy, int Ncz)
+void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double *Sc, double *T1,
+              double *T2)
 {
-  const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
-  TOFE_HIP(hipMemsetAsync(Sc, 0, sizeof(double) * 243 * ncn, s));
-  const int64_t nt = 126 * ncn;
-  k_galerkin<<<dim3((unsigned)((nt + NT - 1) / NT)), dim3(NT), 0, s>>>(Sf, Nfx, Nfy, Nfz, Sc, Ncx,
-                                                                      Ncy, Ncz);
+  // x: (Nfx, Nfy, Nfz) -> (Ncx, Nfy, Nfz); y: -> (Ncx, Ncy, Nfz); z: -> (Ncx, Ncy, Ncz)
+  const int Ncx = (Nfx + 1) / 2, Ncy = (Nfy + 1) / 2, Ncz = (Nfz + 1) / 2;
+  const int64_t n1 = (int64_t)Ncx * Nfy * Nfz, n2 = (int64_t)Ncx * Ncy * Nfz,
+                n3 = (int64_t)Ncx * Ncy * Ncz;
+  k_galerkin_axis<<<dim3((unsigned)((243 * n1 + NT - 1) / NT)), dim3(NT), 0, s>>>(Sf, Nfx, Nfy, Nfz,
+                                                                                  0, T1, 0);
   TOFE_LAUNCH_CHECK();
+  k_galerkin_axis<<<dim3((unsigned)((243 * n2 + NT - 1) / NT)), dim3(NT), 0, s>>>(T1, Ncx, Nfy, Nfz,
+                                                                                  1, T2, 0);
+  TOFE_LAUNCH_CHECK();
+  k_galerkin_axis<<<dim3((unsigned)((243 * n3 + NT - 1) / NT)), dim3(NT), 0, s>>>(T2, Ncx, Ncy, Nfz,
+                                                                                  2, Sc, 1);
+  TOFE_LAUNCH_CHECK();
+}
+
+int64_t galerkin_scratch(int Nfx, int Nfy, int Nfz, int64_t *n2)
+{
+  const int Ncx = (Nfx + 1) / 2, Ncy = (Nfy + 1) / 2;
+  *n2 = (int64_t)243 * Ncx * Ncy * Nfz;
+  return (int64_t)243 * Ncx * Nfy * Nfz;
 }
 
 void stencil_diag(hipStream_t s, const double *S, int64_t nn, double *diag, double *dinv)

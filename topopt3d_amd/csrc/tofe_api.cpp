@@ -105,6 +105,10 @@ struct tofe_ctx {
   // lumped mass diagonal (Frequency::DiagMassFnc)
   DBuf<double> Mdiag;
   bool has_mass = false;
+  // stress state of the buckling analysis: H_e per element (Stability::StressFnc)
+  ShapeData shape{};
+  DBuf<double> Hs;
+  bool has_stress = false;
 
   std::vector<std::unique_ptr<Level<double>>> lv64;
   std::vector<std::unique_ptr<Level<float>>> lv32;
@@ -115,6 +119,7 @@ struct tofe_ctx {
   DBuf<int64_t> dme_elem;
   DBuf<uint32_t> dme_mask;
   int n_masked = 0;
+  DBuf<double> gal_t1, gal_t2;  // axis-by-axis Galerkin scratch
   DBuf<double> Ainv;
   DBuf<int> dinfo;
   DBuf<double> chol_w, chol_p, chol_x;
@@ -130,6 +135,7 @@ struct tofe_ctx {
 
   bool assembled = false;
   bool fine_mfma = false;  // TOFE_FINE_MFMA=1: element-centric MFMA fine operator
+  bool fine_pipe = false;  // TOFE_FINE_PIPE=1: register-prefetched z march
   int precision = TOFE_PREC_FP64;
   int built_precision = -1;
   tofe_stats stats{};
@@ -198,8 +204,9 @@ template <typename T>
 void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
 {
   if (l == 0) {
-    if (!c->fine_mfma) fine_op<T>(c->stream, fine_of<T>(c), u, epi, ep);
-    else fine_mfma<T>(c->stream, fine_of<T>(c), u, epi, ep);
+    if (c->fine_mfma) fine_mfma<T>(c->stream, fine_of<T>(c), u, epi, ep);
+    else if (c->fine_pipe) fine_op_pipe<T>(c->stream, fine_of<T>(c), u, epi, ep);
+    else fine_op<T>(c->stream, fine_of<T>(c), u, epi, ep);
   } else {
     StencilOp<T> op;
     auto &lv = *levels<T>(c)[l];
@@ -556,7 +563,12 @@ void assemble(tofe_ctx *c)
   }
   for (int l = 1; l + 1 < L; ++l) {
     const Grid &gf = c->grids[l], &gc = c->grids[l + 1];
-    galerkin(s, c->lv64[l]->S.p, gf.Nx, gf.Ny, gf.Nz, c->lv64[l + 1]->S.p, gc.Nx, gc.Ny, gc.Nz);
+    (void)gc;
+    int64_t n2 = 0;
+    const int64_t n1 = galerkin_scratch(gf.Nx, gf.Ny, gf.Nz, &n2);
+    if (c->gal_t1.n < (size_t)n1) c->gal_t1.alloc(n1);
+    if (c->gal_t2.n < (size_t)n2) c->gal_t2.alloc(n2);
+    galerkin(s, c->lv64[l]->S.p, gf.Nx, gf.Ny, gf.Nz, c->lv64[l + 1]->S.p, c->gal_t1.p, c->gal_t2.p);
     stencil_diag(s, c->lv64[l + 1]->S.p, gc.nn(), nullptr, c->lv64[l + 1]->dinv.p);
   }
   // coarsest level: dense Cholesky factor and explicit inverse (FP64)
@@ -731,6 +743,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
   try {
     TOFE_HIP(hipGetDevice(&c->device));
     if (const char *fm = std::getenv("TOFE_FINE_MFMA")) c->fine_mfma = fm[0] == '1';
+    if (const char *fp = std::getenv("TOFE_FINE_PIPE")) c->fine_pipe = fp[0] == '1';
     // level count: floor(log2(min nel)) unless given (RecMesh.cpp:552-554), truncated
     // once a coarse grid has <= 10 DOFs (Interpolation.cpp:70-71)
     int L = d.mg_levels;
@@ -762,6 +775,8 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     rocblas_set_stream(c->rb, c->stream);
 
     compute_ke(d.E0, d.nu, d.hx, d.hy, d.hz, c->KE);
+    dmatrix(d.E0, d.nu, c->shape.D);
+    shape_gradients(d.hx, d.hy, d.hz, c->shape.dN, &c->shape.detJ);
     float ke32[576];
     for (int i = 0; i < 576; ++i) ke32[i] = (float)c->KE[i];
     c->dKE.alloc(576);
@@ -1117,6 +1132,100 @@ int tofe_element_quadratic(tofe_ctx *c, const double *phi, int nvec, double *kq,
               mq ? dm.p : nullptr);
     d2h(c, kq, dk.p, ne);
     if (mq) d2h(c, mq, dm.p, ne);
+  });
+}
+
+int tofe_set_stress_dev(tofe_ctx *c, const double *dU)
+{
+  return guarded(c, [&] {
+    if (!dU) throw Error(TOFE_ERR_ARG, "null displacement");
+    if (!c->Hs.p) c->Hs.alloc((size_t)64 * c->ne);
+    stress_h(c->stream, dU, c->desc.nx, c->desc.ny, c->desc.nz, c->shape, c->Hs.p);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+    c->has_stress = true;
+  });
+}
+
+int tofe_set_stress(tofe_ctx *c, const double *U)
+{
+  return guarded(c, [&] {
+    if (!U) throw Error(TOFE_ERR_ARG, "null displacement");
+    h2d(c, c->vp.p, U, c->ndof);
+    if (!c->Hs.p) c->Hs.alloc((size_t)64 * c->ne);
+    stress_h(c->stream, c->vp.p, c->desc.nx, c->desc.ny, c->desc.nz, c->shape, c->Hs.p);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+    c->has_stress = true;
+  });
+}
+
+int tofe_apply_Ks_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
+{
+  return guarded(c, [&] {
+    if (!c->has_stress) throw Error(TOFE_ERR_STATE, "no stress state (call tofe_set_stress)");
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    if (!dX || !dY || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
+    for (int i = 0; i < nrhs; ++i)
+      apply_ks(c->stream, c->Hs.p, c->Es.p, c->has_fix ? c->dfix.p : nullptr, c->desc.nx,
+               c->desc.ny, c->desc.nz, dX + (size_t)i * c->ndof, dY + (size_t)i * c->ndof);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tofe_apply_Ks(tofe_ctx *c, const double *X, double *Y, int nrhs)
+{
+  return guarded(c, [&] {
+    if (!c->has_stress) throw Error(TOFE_ERR_STATE, "no stress state (call tofe_set_stress)");
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    if (!X || !Y || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
+    for (int i = 0; i < nrhs; ++i) {
+      h2d(c, c->vp.p, X + (size_t)i * c->ndof, c->ndof);
+      apply_ks(c->stream, c->Hs.p, c->Es.p, c->has_fix ? c->dfix.p : nullptr, c->desc.nx,
+               c->desc.ny, c->desc.nz, c->vp.p, c->vq.p);
+      d2h(c, Y + (size_t)i * c->ndof, c->vq.p, c->ndof);
+    }
+  });
+}
+
+int tofe_stability_terms(tofe_ctx *c, const double *phi, int nvec, double *pksp, double *dksdu)
+{
+  return guarded(c, [&] {
+    if (!phi || nvec < 1) throw Error(TOFE_ERR_ARG, "bad arguments");
+    if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
+    const size_t nd = (size_t)c->ndof * nvec, ne = (size_t)c->ne * nvec;
+    DBuf<double> dphi;
+    dphi.alloc(nd);
+    h2d(c, dphi.p, phi, nd);
+    if (pksp) {
+      if (!c->has_stress) throw Error(TOFE_ERR_STATE, "no stress state (call tofe_set_stress)");
+      DBuf<double> dq;
+      dq.alloc(ne);
+      ks_quad(c->stream, c->Hs.p, dphi.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ndof, nvec, dq.p);
+      d2h(c, pksp, dq.p, ne);
+    }
+    if (dksdu) {
+      DBuf<double> dr;
+      dr.alloc(nd);
+      for (int j = 0; j < nvec; ++j)
+        stab_rhs(c->stream, dphi.p + (size_t)j * c->ndof, c->Es.p, c->has_fix ? c->dfix.p : nullptr,
+                 c->desc.nx, c->desc.ny, c->desc.nz, c->shape, dr.p + (size_t)j * c->ndof);
+      d2h(c, dksdu, dr.p, nd);
+    }
+  });
+}
+
+int tofe_element_bilinear(tofe_ctx *c, const double *A, int nvec, const double *B, double *out)
+{
+  return guarded(c, [&] {
+    if (!A || !B || !out || nvec < 1) throw Error(TOFE_ERR_ARG, "bad arguments");
+    const size_t nd = (size_t)c->ndof * nvec, ne = (size_t)c->ne * nvec;
+    DBuf<double> da, dout;
+    da.alloc(nd);
+    dout.alloc(ne);
+    h2d(c, da.p, A, nd);
+    h2d(c, c->vq.p, B, c->ndof);
+    elem_bilinear(c->stream, da.p, c->vq.p, c->dKE.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ndof,
+                  nvec, dout.p);
+    d2h(c, out, dout.p, ne);
   });
 }
 

@@ -77,6 +77,8 @@ template <typename T>
 int fine_op_blocks(const FineOp<T> &op);
 template <typename T>
 void fine_op(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
+template <typename T>
+void fine_op_pipe(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
 void fine_stencil_build(hipStream_t s, const FineOp<double> &op, double *S);
 // ---- fine operator on the matrix cores (k_fine_mfma.hip) -------------------
 template <typename T>
@@ -89,8 +91,11 @@ template <typename T>
 int stencil_op_blocks(const StencilOp<T> &op);
 template <typename T>
 void stencil_op(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
-void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double *Sc, int Ncx,
-              int Ncy, int Ncz);
+// Coarse stencil of P^T A P, restricted axis by axis through the scratch fields T1 and T2
+// (sizes from galerkin_scratch: values in T1, and in T2 through n2).
+void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double *Sc, double *T1,
+              double *T2);
+int64_t galerkin_scratch(int Nfx, int Nfy, int Nfz, int64_t *n2);
 void stencil_diag(hipStream_t s, const double *S, int64_t nn, double *diag, double *dinv);
 void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A);
 void symmetrize_lower(hipStream_t s, double *A, int64_t n);
@@ -180,6 +185,22 @@ void simp(hipStream_t s, const double *rho, const double *y, int64_t ne, const S
 void chain_mid(hipStream_t s, const double *dfdE, const double *y, const double *rhoq,
                const double *RtT, double *out, int64_t ne);
 void mul(hipStream_t s, const double *a, const double *b, double *c, int64_t n);
+// Element geometry for the stress stiffness: D (6x6), dN_n/dx_a at the Gauss points, det J.
+struct ShapeData {
+  double D[36];
+  double dN[8][3][8];
+  double detJ;
+};
+void stress_h(hipStream_t s, const double *U, int nx, int ny, int nz, const ShapeData &sd,
+              double *H);
+void apply_ks(hipStream_t s, const double *H, const double *Es, const uint8_t *fix, int nx,
+              int ny, int nz, const double *x, double *y);
+void ks_quad(hipStream_t s, const double *H, const double *phi, int nx, int ny, int nz,
+             int64_t ndof, int nvec, double *q);
+void stab_rhs(hipStream_t s, const double *phi, const double *Es, const uint8_t *fix, int nx,
+              int ny, int nz, const ShapeData &sd, double *r);
+void elem_bilinear(hipStream_t s, const double *A, const double *B, const double *KE, int nx,
+                   int ny, int nz, int64_t ndof, int nvec, double *out);
 void mass_diag(hipStream_t s, const double *V, int nx, int ny, int nz, double m0,
                const double *mlump, const uint8_t *fix, double *M);
 void diag_mul(hipStream_t s, const double *d, const double *x, double *y, int64_t n, int nrhs);
