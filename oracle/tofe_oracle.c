@@ -1081,7 +1081,7 @@ void orc_sensitivity(int nx, int ny, int nz, const double KE[576], const double 
 
 /* dN/dx (3 x 8) at the 8 Gauss points and det(J) of the axis-aligned box element
  * (LocalK FEAnalysis.cpp:869-892, GaussPoints 900-926, dN 936-990). */
-void orc_shape_grad(double hx, double hy, double hz, double dNdx[8][3][8], double *detJ)
+void orc_shape_grad(double hx, double hy, double hz, double dNdx[8][3][8], double detJ[8])
 {
   const double g = 1 / sqrt(3.0);
   const double h[3] = {hx, hy, hz};
@@ -1122,7 +1122,7 @@ void orc_shape_grad(double hx, double hy, double hz, double dNdx[8][3][8], doubl
         for (int b = 0; b < 3; b++) s += inv[a][b] * dNdxi[b][n];
         dNdx[q][a][n] = s;
       }
-    *detJ = det;
+    detJ[q] = det;
   }
 }
 
@@ -1150,7 +1150,7 @@ static void bg_matrices(const double dNdx[3][8], double B[6][24], double G[9][24
 /* ks(u) = sum_q W G_q^T S(D B_q u) G_q detJ, W = 1 (StressFnc Buckling.cpp:313-317); S is
  * the 9x9 block diagonal of three copies of the 3x3 stress tensor built from the Voigt
  * vector [xx,yy,zz,xy,yz,xz] (sigtos, 371-414). */
-static void stress_stiffness(const double D[36], const double dNdx[8][3][8], double detJ,
+static void stress_stiffness(const double D[36], const double dNdx[8][3][8], const double detJ[8],
                              const double u[24], double ks[576])
 {
   memset(ks, 0, 576 * sizeof(double));
@@ -1185,7 +1185,7 @@ static void stress_stiffness(const double D[36], const double dNdx[8][3][8], dou
       for (int c = 0; c < 24; c++) {
         double s = 0;
         for (int a = 0; a < 9; a++) s += G[a][r] * SG[a][c];
-        ks[r * 24 + c] += 1.0 * s * detJ;
+        ks[r * 24 + c] += 1.0 * s * detJ[q];
       }
   }
 }
@@ -1203,9 +1203,9 @@ static void element_dofs(int nx, int ny, int64_t e, int64_t dof[24])
 orc_csr *orc_assemble_Ks(int nx, int ny, int nz, double E0, double nu, double hx, double hy,
                          double hz, const double *U, const double *Es, const uint8_t *fixed)
 {
-  double D[36], dNdx[8][3][8], detJ;
+  double D[36], dNdx[8][3][8], detJ[8];
   orc_dmat(E0, nu, D);
-  orc_shape_grad(hx, hy, hz, dNdx, &detJ);
+  orc_shape_grad(hx, hy, hz, dNdx, detJ);
   int64_t ne = (int64_t)nx * ny * nz;
   int64_t ndof = 3 * (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
   /* dense accumulation per row in a CSR with the 81-entry node pattern: build through
@@ -1236,9 +1236,9 @@ orc_csr *orc_assemble_Ks(int nx, int ny, int nz, double E0, double nu, double hx
 void orc_ks_quadratic(int nx, int ny, int nz, double E0, double nu, double hx, double hy,
                       double hz, const double *U, const double *phi, int nvec, double *out)
 {
-  double D[36], dNdx[8][3][8], detJ;
+  double D[36], dNdx[8][3][8], detJ[8];
   orc_dmat(E0, nu, D);
-  orc_shape_grad(hx, hy, hz, dNdx, &detJ);
+  orc_shape_grad(hx, hy, hz, dNdx, detJ);
   int64_t ne = (int64_t)nx * ny * nz;
   int64_t ndof = 3 * (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
   for (int64_t e = 0; e < ne; e++) {
@@ -1268,9 +1268,9 @@ void orc_stability_rhs(int nx, int ny, int nz, double E0, double nu, double hx, 
                        double hz, const double *Es, const double *phi, int nvec,
                        const uint8_t *fixed, double *out)
 {
-  double D[36], dNdx[8][3][8], detJ;
+  double D[36], dNdx[8][3][8], detJ[8];
   orc_dmat(E0, nu, D);
-  orc_shape_grad(hx, hy, hz, dNdx, &detJ);
+  orc_shape_grad(hx, hy, hz, dNdx, detJ);
   static double dks[24][576];
   for (int d = 0; d < 24; d++) {
     double du[24] = {0};

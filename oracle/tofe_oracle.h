@@ -115,7 +115,7 @@ double orc_lanczos_lambda(const orc_csr *A, const double *dinv, int its);
 int orc_num_threads(void);
 
 /* ---- linearised buckling (Buckling.cpp:15-414) ---- */
-void orc_shape_grad(double hx, double hy, double hz, double dNdx[8][3][8], double *detJ);
+void orc_shape_grad(double hx, double hy, double hz, double dNdx[8][3][8], double detJ[8]);
 /* Ks = sum_e (-Es_e) ks_e(u_e), MatZeroRowsColumns(Ks, fixed, 0.0) */
 orc_csr *orc_assemble_Ks(int nx, int ny, int nz, double E0, double nu, double hx, double hy,
                          double hz, const double *U, const double *Es, const uint8_t *fixed);

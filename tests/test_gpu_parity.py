@@ -55,6 +55,14 @@ def test_filter_and_simp_bit_exact(oracle, tofe, nel, L):
         ctx.close()
 
 
+@pytest.mark.parametrize("h", [(1.0, 1.0, 1.0), (0.5, 0.75, 1.25)])
+def test_element_matrix_bit_exact(oracle, tofe, h):
+    """The library's KE (host quadrature, det J per Gauss point) is the oracle's bit for bit."""
+    ctx = tofe.Context((2, 2, 2), h=h, E0=2.5, nu=0.27, mg_levels=1)
+    assert np.array_equal(ctx.ke(), oracle.ke(2.5, 0.27, *h))
+    ctx.close()
+
+
 @pytest.mark.parametrize("nel,L", CASES)
 def test_operator_matches_assembled_K(oracle, tofe, nel, L):
     for fixed in (True, False):

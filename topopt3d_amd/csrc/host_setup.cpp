@@ -23,7 +23,7 @@ void dmatrix(double E0, double nu, double D[36])
   std::memcpy(D, v, sizeof(v));
 }
 
-void shape_gradients(double hx, double hy, double hz, double dNdx[8][3][8], double *detJ)
+void shape_gradients(double hx, double hy, double hz, double dNdx[8][3][8], double detJ[8])
 {
   const double g = 1 / std::sqrt(3.0);
   const double h[3] = {hx, hy, hz};
@@ -63,7 +63,7 @@ void shape_gradients(double hx, double hy, double hz, double dNdx[8][3][8], doub
         for (int b = 0; b < 3; b++) s += inv[a][b] * dNdxi[b][n];
         dNdx[q][a][n] = s;
       }
-    *detJ = det;
+    detJ[q] = det;  // rounding of J differs between Gauss points
   }
 }
 
@@ -71,8 +71,8 @@ void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE
 {
   double Dm[36];
   dmatrix(E0, nu, Dm);
-  double dN[8][3][8], det;
-  shape_gradients(hx, hy, hz, dN, &det);
+  double dN[8][3][8], det[8];
+  shape_gradients(hx, hy, hz, dN, det);
   std::memset(KE, 0, 576 * sizeof(double));
   for (int q = 0; q < 8; q++) {
     const double(&dNdx)[3][8] = dN[q];
@@ -101,7 +101,7 @@ void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE
       for (int cc = 0; cc < 24; cc++) {
         double s = 0;
         for (int a = 0; a < 6; a++) s += B[a][r] * DB[a][cc];
-        KE[r * 24 + cc] += s * det;
+        KE[r * 24 + cc] += s * det[q];
       }
   }
 }

@@ -11,9 +11,9 @@ namespace tofe {
 void compute_ke(double E0, double nu, double hx, double hy, double hz, double KE[576]);
 
 // Isotropic D (6x6 row-major, Voigt xx,yy,zz,xy,yz,xz) and the shape-function gradients
-// dN_n/dx_a at the 8 Gauss points of the box element with det(J) (unit weights).
+// dN_n/dx_a at the 8 Gauss points of the box element with det(J) per point (unit weights).
 void dmatrix(double E0, double nu, double D[36]);
-void shape_gradients(double hx, double hy, double hz, double dNdx[8][3][8], double *detJ);
+void shape_gradients(double hx, double hy, double hz, double dNdx[8][3][8], double detJ[8]);
 
 // Density-filter neighbourhood: offsets (i,j,k) walked k, j, i ascending with
 // dist < R, and weights 1 - dist/R (TopOpt::RecFilter, Filter.cpp:21-94).

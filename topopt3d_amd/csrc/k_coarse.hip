@@ -186,50 +186,60 @@ __global__ __launch_bounds__(NT) void k_gemv(const double *__restrict__ A, int64
 // dimension lda, kb <= kCholNB): L_kk overwrites the lower triangle of the block and its
 // inverse goes to W (leading dimension kCholNB).  info[0] = global pivot index + 1 of the
 // first non-positive pivot (0 if none).
-// One wavefront: lane i owns row i of the block during the factorisation and column i of
-// W = L^-1 during the inversion, so the inversion needs no barrier at all and the
-// factorisation one per column.  a is column-major (a[i + 64 j]); w is row-major
-// (w[c + 64 i]) so that the 64 lanes touch consecutive words.
+// Value of a double held by lane `lane` (uniform) of the wavefront.
+__device__ inline double lane_value(double v, int lane)
+{
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// One wavefront, everything in registers: lane i holds row i of the block while it is
+// factorised and column i of W = L^-1 while that is formed; the other lanes' values come
+// through v_readlane with compile-time lane numbers, so there are no LDS round trips and
+// no barriers.  A partial block (kb < 64) is padded with the identity, which factors and
+// inverts to itself and leaves the leading kb x kb part unchanged.
 __global__ __launch_bounds__(64) void k_chol_block(double *A, int lda, int kb, int k0, double *W,
                                                    int *info)
 {
-  __shared__ double a[kCholNB * kCholNB];
-  __shared__ double w[kCholNB * kCholNB];
   const int i = threadIdx.x;
-  for (int j = 0; j < kb; ++j) a[i + kCholNB * j] = (i < kb && i >= j) ? A[i + (int64_t)j * lda] : 0.0;
-  __syncthreads();
-  for (int j = 0; j < kb; ++j) {
-    double d = a[j + kCholNB * j];
+  double r[kCholNB];
+#pragma unroll
+  for (int j = 0; j < kCholNB; ++j)
+    r[j] = (i < kb && j < kb) ? (i >= j ? A[i + (int64_t)j * lda] : 0.0) : (i == j ? 1.0 : 0.0);
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < kCholNB; ++j) {
+    double d = lane_value(r[j], j);
     if (!(d > 0)) {
-      if (i == 0 && info[0] == 0) info[0] = k0 + j + 1;
+      if (bad == 0) bad = k0 + j + 1;
       d = 1.0;
     }
     d = sqrt(d);
-    double lij = 0.0;
-    if (i > j && i < kb) lij = a[i + kCholNB * j] / d;
-    __syncthreads();  // every lane has read column j and the pivot
-    if (i == j) a[j + kCholNB * j] = d;
-    if (i > j && i < kb) a[i + kCholNB * j] = lij;
-    // trailing update of row i: a(i, l) -= l(i, j) l(l, j) for j < l <= i
-    for (int l = j + 1; l < kb; ++l) {
-      const double llj = __shfl(lij, l, 64);
-      if (i >= l && i < kb) a[i + kCholNB * l] -= lij * llj;
-    }
-    __syncthreads();
+    const double lij = (i > j) ? r[j] / d : (i == j ? d : 0.0);
+    r[j] = lij;
+    // trailing update of row i; entries above the diagonal are never read as L values
+#pragma unroll
+    for (int l = j + 1; l < kCholNB; ++l) r[l] -= lij * lane_value(lij, l);
   }
-  for (int j = 0; j < kb; ++j)
-    if (i < kb && i >= j) A[i + (int64_t)j * lda] = a[i + kCholNB * j];
-  // W = L^-1: lane c solves L w = e_c by forward substitution on its own column
+  if (bad != 0 && i == 0 && info[0] == 0) info[0] = bad;
+#pragma unroll
+  for (int j = 0; j < kCholNB; ++j)
+    if (i < kb && j < kb && i >= j) A[i + (int64_t)j * lda] = r[j];
+  // W = L^-1: lane c solves L w = e_c; L(q, p) is r[p] of lane q
   const int c = i;
-  for (int r = 0; r < kb; ++r) w[c + kCholNB * r] = (r == c) ? 1.0 : 0.0;
-  for (int q = 0; q < kb; ++q) {
-    const double wq = w[c + kCholNB * q] / a[q + kCholNB * q];
-    w[c + kCholNB * q] = wq;
-    for (int r = q + 1; r < kb; ++r) w[c + kCholNB * r] -= a[r + kCholNB * q] * wq;
+  double w[kCholNB];
+#pragma unroll
+  for (int q = 0; q < kCholNB; ++q) w[q] = (q == c) ? 1.0 : 0.0;
+#pragma unroll
+  for (int q = 0; q < kCholNB; ++q) {
+    w[q] = w[q] / lane_value(r[q], q);
+#pragma unroll
+    for (int p = q + 1; p < kCholNB; ++p) w[p] -= lane_value(r[q], p) * w[q];
   }
-  if (c < kCholNB)
-    for (int r = 0; r < kCholNB; ++r)
-      W[r + kCholNB * c] = (r < kb && c < kb && r >= c) ? w[c + kCholNB * r] : 0.0;
+#pragma unroll
+  for (int q = 0; q < kCholNB; ++q)
+    W[q + kCholNB * c] = (q < kb && c < kb && q >= c) ? w[q] : 0.0;
 }
 
 }  // namespace

@@ -239,201 +239,6 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
   }
 }
 
-// Epilogue operands of one node, loaded before the element work of a step so their
-// latency overlaps it.
-template <typename T, int EPI>
-struct EpiIn {
-  T uval[3], b[3], dinv[3], d[3];
-  __device__ void load(const EpiArgs<T> &ep, const T *u, int64_t dof0)
-  {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      uval[c] = u[dof0 + c];
-      if constexpr (EPI == EPI_RESID || EPI == EPI_CHEB || EPI == EPI_APPLY_DOT2) b[c] = ep.b[dof0 + c];
-      if constexpr (EPI == EPI_CHEB) {
-        dinv[c] = ep.dinv[dof0 + c];
-        if (ep.c1 != T(0)) d[c] = ep.d[dof0 + c];
-      }
-    }
-  }
-  __device__ void store(const EpiArgs<T> &ep, int64_t dof0, const T (&acc)[3], uint8_t fm,
-                        double &dsum, double &dsum2) const
-  {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int64_t dof = dof0 + c;
-      const T Au = ((fm >> c) & 1) ? uval[c] : acc[c];
-      if constexpr (EPI == EPI_APPLY) {
-        ep.y[dof] = Au;
-      } else if constexpr (EPI == EPI_APPLY_DOT) {
-        ep.y[dof] = Au;
-        dsum += (double)uval[c] * (double)Au;
-      } else if constexpr (EPI == EPI_APPLY_DOT2) {
-        ep.y[dof] = Au;
-        dsum += (double)uval[c] * (double)Au;
-        dsum2 += (double)uval[c] * (double)b[c];
-      } else if constexpr (EPI == EPI_RESID) {
-        ep.y[dof] = b[c] - Au;
-      } else if constexpr (EPI == EPI_CHEB) {
-        const T r = dinv[c] * (b[c] - Au);
-        const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * d[c] + ep.c2 * r;
-        ep.d[dof] = dn;
-        ep.xout[dof] = uval[c] + dn;
-      }
-    }
-  }
-};
-
-// Software-pipelined z march.  LDS holds a ring of three node planes of u (one array per
-// component, so the 32 lanes of a tile row read consecutive words) and two
-// element layers of E.  Step ez (element layer between node planes ez and ez+1):
-//   1. the registers fetched in the previous step (plane ez+1, layer ez) go to LDS;
-//   2. global loads of plane ez+2 and layer ez+1 are issued into registers;
-//   3. one barrier; the epilogue operands of node (i, j, ez) are loaded;
-//   4. the layer's four elements around the node column are gathered from LDS and
-//      multiplied by their KE rows; node (i, j, ez) is then complete and written.
-// Writes of step ez+1 go to the ring slot read last in step ez-1, so one barrier per
-// step orders them.
-template <typename T, int EPI>
-__global__ __launch_bounds__(NT) void k_fine_op_pipe(FineOp<T> op, const T *__restrict__ u,
-                                                EpiArgs<T> ep, int tiles_x, int tiles_y,
-                                                int nblocks)
-{
-  constexpr int PLN = UW * UH;               // nodes per staged plane
-  constexpr int UPT = (UPL + NT - 1) / NT;   // staged u values per thread
-  constexpr int EPT = (EPL + NT - 1) / NT;   // staged E values per thread
-  __shared__ T su[3][3][PLN];
-  __shared__ T sE[2][EPL];
-  __shared__ double sred[NT / 64];
-
-  const int nx = op.nx, ny = op.ny, nz = op.nz;
-  const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
-  const int bid = xcd_remap(blockIdx.x, nblocks);
-  const int txy = tiles_x * tiles_y;
-  const int bz = bid / txy;
-  const int rem = bid - bz * txy;
-  const int by = rem / tiles_x, bx = rem - by * tiles_x;
-  const int x0 = bx * TX, y0 = by * TY;
-  const int k0 = bz * ZLEN, k1 = min(k0 + ZLEN, Nz);
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  const int i = x0 + tx, j = y0 + ty;
-  const bool own = (i < Nx) && (j < Ny);
-  const T *KEbase = op.KE;
-  const T *__restrict__ E = op.E;
-  const uint8_t *__restrict__ fix = op.fix;
-
-  T pu[UPT], pe[EPT];
-  // fetch plane gz of u (Dirichlet entries zeroed) and layer ez of E into registers
-  auto fetch = [&](int gz, int ez) {
-    const bool zin = gz >= 0 && gz < Nz;
-#pragma unroll
-    for (int r = 0; r < UPT; ++r) {
-      const int v = threadIdx.x + r * NT;
-      T val = 0;
-      if (v < UPL) {
-        const int nl = v / 3, c = v - 3 * nl;  // node-major: coalesced global reads
-        const int ly = nl / UW, lx = nl - ly * UW;
-        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-        if (zin && gx >= 0 && gx < Nx && gy >= 0 && gy < Ny) {
-          const int64_t n = gx + (int64_t)Nx * (gy + (int64_t)Ny * gz);
-          val = u[3 * n + c];
-          if (fix && ((fix[n] >> c) & 1)) val = 0;
-        }
-      }
-      pu[r] = val;
-    }
-    const bool ein = ez >= 0 && ez < nz;
-#pragma unroll
-    for (int r = 0; r < EPT; ++r) {
-      const int v = threadIdx.x + r * NT;
-      T val = 0;
-      if (v < EPL) {
-        const int ly = v / EW, lx = v - ly * EW;
-        const int ex = x0 - 1 + lx, ey = y0 - 1 + ly;
-        if (ein && ex >= 0 && ex < nx && ey >= 0 && ey < ny)
-          val = E[ex + (int64_t)nx * (ey + (int64_t)ny * ez)];
-      }
-      pe[r] = val;
-    }
-  };
-  auto commit = [&](int uslot, int eslot) {
-#pragma unroll
-    for (int r = 0; r < UPT; ++r) {
-      const int v = threadIdx.x + r * NT;
-      if (v < UPL) {
-        const int nl = v / 3, c = v - 3 * nl;
-        su[uslot][c][nl] = pu[r];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < EPT; ++r) {
-      const int v = threadIdx.x + r * NT;
-      if (v < EPL) sE[eslot][v] = pe[r];
-    }
-  };
-
-  T alo[3], ahi[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) ahi[c] = 0;
-  double dsum = 0, dsum2 = 0;
-
-  // prologue: plane k0-1 into slot 0; plane k0 and layer k0-1 into registers
-  fetch(k0 - 1, -1);
-  commit(0, 1);
-  fetch(k0, k0 - 1);
-
-  int st = 0;
-  for (int ez = k0 - 1; ez < k1; ++ez, ++st) {
-    const int slo = st % 3, shi = (st + 1) % 3, es = st & 1;
-    commit(shi, es);
-    fetch(ez + 2, ez + 1);
-    __syncthreads();
-    const bool emit = own && ez >= k0;
-    const int64_t n = i + (int64_t)Nx * (j + (int64_t)Ny * ez);
-    EpiIn<T, EPI> ein;
-    uint8_t fm = 0;
-    if (emit) {
-      ein.load(ep, u, 3 * n);
-      fm = fix ? fix[n] : 0;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      alo[c] = ahi[c];
-      ahi[c] = 0;
-    }
-    auto element = [&](auto A_, auto B_) {
-      constexpr int A = decltype(A_)::value, B = decltype(B_)::value;
-      // no reuse of LDS values across elements: keeps the gathered vector short-lived
-      asm volatile("" ::: "memory");
-      T ue[24];
-#pragma unroll
-      for (int l = 0; l < 8; ++l) {
-        const int nl = (ty + B + lny(l)) * UW + tx + A + lnx(l);
-        const int slot = lnz(l) ? shi : slo;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) ue[3 * l + c] = su[slot][c][nl];
-      }
-      const T Ee = sE[es][(ty + B) * EW + tx + A];
-      element_rows<T, A, B>(KEbase, ue, Ee, alo, ahi);
-    };
-    element(std::integral_constant<int, 0>(), std::integral_constant<int, 0>());
-    element(std::integral_constant<int, 1>(), std::integral_constant<int, 0>());
-    element(std::integral_constant<int, 0>(), std::integral_constant<int, 1>());
-    element(std::integral_constant<int, 1>(), std::integral_constant<int, 1>());
-    if (emit) ein.store(ep, 3 * n, alo, fm, dsum, dsum2);
-  }
-  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2) {
-    __syncthreads();
-    const double t = block_sum<NT>(dsum, sred);
-    if (threadIdx.x == 0) ep.partial[blockIdx.x] = t;
-    if constexpr (EPI == EPI_APPLY_DOT2) {
-      __syncthreads();
-      const double t2 = block_sum<NT>(dsum2, sred);
-      if (threadIdx.x == 0) ep.partial2[blockIdx.x] = t2;
-    }
-  }
-}
-
 // Assembled fine stencil: for node n and neighbour m = n + off(o), block(a,b) =
 // sum over shared elements (ascending element order) of E_e * KE, each product
 // rounded before it is added exactly as FEAssemble's ke = E*ke[0] then ADD_VALUES
@@ -527,31 +332,6 @@ void fine_op(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiA
   TOFE_LAUNCH_CHECK();
 }
 
-template <typename T>
-void fine_op_pipe(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep)
-{
-  const FineGrid<T> g = fine_grid(op);
-  dim3 grid(g.nblocks), block(NT);
-  switch (epi) {
-    case EPI_APPLY:
-      k_fine_op_pipe<T, EPI_APPLY><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-    case EPI_APPLY_DOT:
-      k_fine_op_pipe<T, EPI_APPLY_DOT><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-    case EPI_APPLY_DOT2:
-      k_fine_op_pipe<T, EPI_APPLY_DOT2><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-    case EPI_RESID:
-      k_fine_op_pipe<T, EPI_RESID><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-    case EPI_CHEB:
-      k_fine_op_pipe<T, EPI_CHEB><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-  }
-  TOFE_LAUNCH_CHECK();
-}
-
 void fine_stencil_build(hipStream_t s, const FineOp<double> &op, double *S)
 {
   const int64_t nn = (int64_t)(op.nx + 1) * (op.ny + 1) * (op.nz + 1);
@@ -567,9 +347,5 @@ template void fine_op<double>(hipStream_t, const FineOp<double> &, const double 
                               const EpiArgs<double> &);
 template void fine_op<float>(hipStream_t, const FineOp<float> &, const float *, Epi,
                              const EpiArgs<float> &);
-template void fine_op_pipe<double>(hipStream_t, const FineOp<double> &, const double *, Epi,
-                                   const EpiArgs<double> &);
-template void fine_op_pipe<float>(hipStream_t, const FineOp<float> &, const float *, Epi,
-                                  const EpiArgs<float> &);
 
 }  // namespace tofe

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(NT) void k_stress_h(const double *__restrict__ U, i
       double t = 0.0;
 #pragma unroll
       for (int b = 0; b < 6; ++b) t += sd.D[a * 6 + b] * eps[b];
-      sig[a] = t * sd.detJ;  // unit Gauss weight
+      sig[a] = t * sd.detJ[q];  // unit Gauss weight
     }
     const double s[3][3] = {{sig[0], sig[3], sig[5]}, {sig[3], sig[1], sig[4]}, {sig[5], sig[4], sig[2]}};
 #pragma unroll
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(NT) void k_stab_rhs(const double *__restrict__ phi,
             double v = 0.0;
 #pragma unroll
             for (int b = 0; b < 6; ++b) v += sd.D[a * 6 + b] * tau[b];
-            Dt[a] = v * sd.detJ;
+            Dt[a] = v * sd.detJ[q];
           }
           // B^T rows of the node's three DOFs (AssignB row pattern)
           const double nx_ = sd.dN[q][0][kl], ny_ = sd.dN[q][1][kl], nz_ = sd.dN[q][2][kl];

@@ -135,7 +135,6 @@ struct tofe_ctx {
 
   bool assembled = false;
   bool fine_mfma = false;  // TOFE_FINE_MFMA=1: element-centric MFMA fine operator
-  bool fine_pipe = false;  // TOFE_FINE_PIPE=1: register-prefetched z march
   int precision = TOFE_PREC_FP64;
   int built_precision = -1;
   tofe_stats stats{};
@@ -205,7 +204,6 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
 {
   if (l == 0) {
     if (c->fine_mfma) fine_mfma<T>(c->stream, fine_of<T>(c), u, epi, ep);
-    else if (c->fine_pipe) fine_op_pipe<T>(c->stream, fine_of<T>(c), u, epi, ep);
     else fine_op<T>(c->stream, fine_of<T>(c), u, epi, ep);
   } else {
     StencilOp<T> op;
@@ -743,7 +741,6 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
   try {
     TOFE_HIP(hipGetDevice(&c->device));
     if (const char *fm = std::getenv("TOFE_FINE_MFMA")) c->fine_mfma = fm[0] == '1';
-    if (const char *fp = std::getenv("TOFE_FINE_PIPE")) c->fine_pipe = fp[0] == '1';
     // level count: floor(log2(min nel)) unless given (RecMesh.cpp:552-554), truncated
     // once a coarse grid has <= 10 DOFs (Interpolation.cpp:70-71)
     int L = d.mg_levels;
@@ -776,7 +773,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
 
     compute_ke(d.E0, d.nu, d.hx, d.hy, d.hz, c->KE);
     dmatrix(d.E0, d.nu, c->shape.D);
-    shape_gradients(d.hx, d.hy, d.hz, c->shape.dN, &c->shape.detJ);
+    shape_gradients(d.hx, d.hy, d.hz, c->shape.dN, c->shape.detJ);
     float ke32[576];
     for (int i = 0; i < 576; ++i) ke32[i] = (float)c->KE[i];
     c->dKE.alloc(576);

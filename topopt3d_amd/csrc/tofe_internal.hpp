@@ -77,8 +77,6 @@ template <typename T>
 int fine_op_blocks(const FineOp<T> &op);
 template <typename T>
 void fine_op(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
-template <typename T>
-void fine_op_pipe(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
 void fine_stencil_build(hipStream_t s, const FineOp<double> &op, double *S);
 // ---- fine operator on the matrix cores (k_fine_mfma.hip) -------------------
 template <typename T>
@@ -189,7 +187,7 @@ void mul(hipStream_t s, const double *a, const double *b, double *c, int64_t n);
 struct ShapeData {
   double D[36];
   double dN[8][3][8];
-  double detJ;
+  double detJ[8];
 };
 void stress_h(hipStream_t s, const double *U, int nx, int ny, int nz, const ShapeData &sd,
               double *H);
