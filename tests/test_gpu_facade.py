@@ -112,3 +112,26 @@ def test_python_facade_max_length_scale(oracle, tofe):
         oV, oE = oracle.chain_filter(P, R, o["rho"], o["y"], o["rhoq"], g, g)
         assert np.array_equal(dV, oV) and np.array_equal(dE, oE)
     t.ctx.close()
+
+
+def test_stability_function_matches_oracle(oracle, tofe):
+    """Stability.Function (LOBPCG over the GPU K, Ks and V-cycle) gives the oracle's
+    aggregated buckling eigenvalues and sensitivities (dense generalized eigh)."""
+    from test_oracle_buckling import buckling_gradient
+
+    from topopt3d_amd.topopt import Stability
+
+    nel, nev, p = (6, 2, 2), 2, 8.0
+    pb = configs.cantilever(nel, seed=4, lo=0.3, hi=1.0)
+    t = facade_cantilever(nel)
+    t.MatIntFnc(pb.x)
+    t.FEAssemble()
+    t.rtol = 1e-12
+    assert t.FESolve() == 2
+    st = Stability(nvals=nev, p=p, tol=1e-10)
+    st.Function(t)
+    val, grad, s = buckling_gradient(oracle, nel, pb, pb.x, nev, p)
+    assert np.allclose(st.lam, s["mu"], rtol=1e-7)
+    assert st.values[0] == pytest.approx(val, rel=1e-7)
+    assert np.abs(st.gradients - grad).max() <= 1e-5 * np.abs(grad).max()
+    t.ctx.close()

@@ -112,7 +112,8 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
                                                 EpiArgs<T> ep, int tiles_x, int tiles_y,
                                                 int nblocks)
 {
-  __shared__ T su[2][UPL];
+  constexpr int PLN = UW * UH;  // nodes per staged plane
+  __shared__ T su[3][3][PLN];   // ring of three node planes, one array per component
   __shared__ T sE[2][EPL];
   __shared__ double sred[NT / 64];
 
@@ -132,7 +133,9 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
   const T *__restrict__ E = op.E;
   const uint8_t *__restrict__ fix = op.fix;
 
-  auto stage_u = [&](int buf, int gz) {
+  // plane gz of u into ring slot `slot` (Dirichlet entries zeroed); global reads in node
+  // order (coalesced), LDS writes per component
+  auto stage_u = [&](int slot, int gz) {
     const bool zin = gz >= 0 && gz < Nz;
     for (int v = threadIdx.x; v < UPL; v += NT) {
       const int nl = v / 3, c = v - 3 * nl;
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
         val = u[3 * n + c];
         if (fix && ((fix[n] >> c) & 1)) val = 0;
       }
-      su[buf][v] = val;
+      su[slot][c][nl] = val;
     }
   };
   auto stage_E = [&](int buf, int ez) {
@@ -159,62 +162,44 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
     }
   };
 
-  T lo[9][3], hi[9][3];
   T alo[3], ahi[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) ahi[c] = 0;
   double dsum = 0, dsum2 = 0;
 
   stage_u(0, k0 - 1);
-  __syncthreads();
-#pragma unroll
-  for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-    for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) hi[dy * 3 + dx][c] = su[0][((ty + dy) * UW + tx + dx) * 3 + c];
-
-  int buf = 0;
-  for (int ez = k0 - 1; ez < k1; ++ez) {
-    buf ^= 1;
-    stage_u(buf, ez + 1);
-    stage_E(buf, ez);
+  // Step st (element layer ez = k0 - 1 + st) reads node planes ez (slot st % 3) and ez + 1
+  // (slot (st + 1) % 3); the slot written in step st + 1 was last read in step st - 1, so
+  // the single barrier of step st orders it.
+  int st = 0;
+  for (int ez = k0 - 1; ez < k1; ++ez, ++st) {
+    const int slo = st % 3, shi = (st + 1) % 3, eb = st & 1;
+    stage_u(shi, ez + 1);
+    stage_E(eb, ez);
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 9; ++q)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) lo[q][c] = hi[q][c];
-#pragma unroll
-    for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-      for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          hi[dy * 3 + dx][c] = su[buf][((ty + dy) * UW + tx + dx) * 3 + c];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       alo[c] = ahi[c];
       ahi[c] = 0;
     }
-    auto gather = [&](int a, int b, T(&ue)[24]) {
-#pragma unroll
-      for (int l = 0; l < 8; ++l)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          ue[3 * l + c] = lnz(l) ? hi[(a + lnx(l)) + 3 * (b + lny(l))][c]
-                                 : lo[(a + lnx(l)) + 3 * (b + lny(l))][c];
-    };
-    {
+    // the four elements of the layer around the node column, each gathered from LDS
+    // just before use so only one element vector is live
+    auto element = [&](auto A_, auto B_) {
+      constexpr int A = decltype(A_)::value, B = decltype(B_)::value;
+      asm volatile("" ::: "memory");
       T ue[24];
-      gather(0, 0, ue);
-      element_rows<T, 0, 0>(KEbase, ue, sE[buf][(ty + 0) * EW + tx + 0], alo, ahi);
-      gather(1, 0, ue);
-      element_rows<T, 1, 0>(KEbase, ue, sE[buf][(ty + 0) * EW + tx + 1], alo, ahi);
-      gather(0, 1, ue);
-      element_rows<T, 0, 1>(KEbase, ue, sE[buf][(ty + 1) * EW + tx + 0], alo, ahi);
-      gather(1, 1, ue);
-      element_rows<T, 1, 1>(KEbase, ue, sE[buf][(ty + 1) * EW + tx + 1], alo, ahi);
-    }
+#pragma unroll
+      for (int l = 0; l < 8; ++l) {
+        const int nl = (ty + B + lny(l)) * UW + tx + A + lnx(l);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ue[3 * l + c] = su[lnz(l) ? shi : slo][c][nl];
+      }
+      element_rows<T, A, B>(KEbase, ue, sE[eb][(ty + B) * EW + tx + A], alo, ahi);
+    };
+    element(std::integral_constant<int, 0>(), std::integral_constant<int, 0>());
+    element(std::integral_constant<int, 1>(), std::integral_constant<int, 0>());
+    element(std::integral_constant<int, 0>(), std::integral_constant<int, 1>());
+    element(std::integral_constant<int, 1>(), std::integral_constant<int, 1>());
     if (own && ez >= k0) {
       const int64_t n = i + (int64_t)Nx * (j + (int64_t)Ny * ez);
       const uint8_t fm = fix ? fix[n] : 0;

@@ -183,3 +183,63 @@ class Volume:
         self.values[0] = topOpt.V.sum() / topOpt.nElem
         self.gradients = topOpt.ctx.filter_T(np.full(topOpt.nElem, 1.0 / topOpt.nElem))
         return 0
+
+
+class Stability:
+    """Linearised buckling (Stability::Function, Buckling.cpp:15-262): the nvals largest
+    eigenvalues mu of Ks phi = mu K phi for the current design and displacement,
+    aggregated as (sum mu^p)^(1/p), with the sensitivity of the reference: per element
+    sum_j mu_j^(p-1) (phi^T dKs/dy phi - mu_j phi^T dK/dy phi + v_j^T dK/dy U), K v_j = dKsdU_j,
+    scaled by value^(1-p) and passed through Chain_Filter.
+
+    The eigenproblem is solved on the host with scipy's LOBPCG over the GPU operators
+    (tofe_apply_Ks, tofe_apply_K, tofe_pc_apply as preconditioner); the reference uses its
+    own LOPGMRES over the same three operators."""
+
+    def __init__(self, nvals=4, p=8.0, calc_gradient=True, tol=1e-9, maxit=500, seed=0):
+        self.nvals = nvals
+        self.p = p
+        self.calc_gradient = calc_gradient
+        self.tol = tol
+        self.maxit = maxit
+        self.seed = seed
+        self.values = np.zeros(1)
+        self.gradients = None
+        self.lam = None
+        self.phi = None
+
+    def Function(self, topOpt: TopOpt):
+        from scipy.sparse.linalg import LinearOperator, lobpcg
+
+        ctx = topOpt.ctx
+        n = ctx.ndof
+        ctx.set_stress(topOpt.U)
+        Ks = LinearOperator((n, n), matvec=lambda x: ctx.apply_Ks(np.ravel(x)),
+                            matmat=lambda X: ctx.apply_Ks(np.ascontiguousarray(X.T)).T)
+        K = LinearOperator((n, n), matvec=lambda x: ctx.apply_K(np.ravel(x)),
+                           matmat=lambda X: ctx.apply_K(np.ascontiguousarray(X.T)).T)
+        M = LinearOperator((n, n), matvec=lambda x: ctx.pc_apply(np.ravel(x)),
+                           matmat=lambda X: ctx.pc_apply(np.ascontiguousarray(X.T)).T)
+        X0 = np.random.default_rng(self.seed).standard_normal((n, self.nvals))
+        X0[topOpt.fixed_dof.astype(bool)] = 0
+        mu, V = lobpcg(Ks, X0, B=K, M=M, largest=True, tol=self.tol, maxiter=self.maxit)
+        order = np.argsort(mu)[::-1]
+        mu, V = mu[order], V[:, order]
+        self.lam, self.phi = mu, V.T.copy()
+        p = self.p
+        self.values[0] = np.sum(mu ** p) ** (1 / p)
+        if not self.calc_gradient:
+            return 0
+        phi = self.phi
+        pksp, rhs = ctx.stability_terms(phi)
+        pkp, _ = ctx.element_quadratic(phi, mass=False)
+        v = np.stack([ctx.solve(rhs[j], rtol=1e-10, maxit=topOpt.maxit)[0]
+                      for j in range(len(mu))])
+        vku = ctx.element_bilinear(v, topOpt.U)
+        dEdz, dEsdz = ctx.field(_lib.FIELD_DEDZ), ctx.field(_lib.FIELD_DESDZ)
+        g = np.zeros(topOpt.nElem)
+        for j, m in enumerate(mu):
+            g += m ** (p - 1) * (-dEsdz * pksp[j] - m * dEdz * pkp[j] + dEdz * vku[j])
+        g *= self.values[0] ** (1 - p)
+        _, self.gradients = topOpt.Chain_Filter(None, g)
+        return 0
