@@ -83,6 +83,10 @@ struct tofe_ctx {
   int64_t ne = 0, nn = 0, ndof = 0;
   hipStream_t stream = nullptr;
   rocblas_handle rb = nullptr;
+  // second stream: the coarsest-level factorisation runs beside the Lanczos estimates
+  hipStream_t stream2 = nullptr;
+  rocblas_handle rb2 = nullptr;
+  hipEvent_t ev_dense = nullptr, ev_inv = nullptr;
   double KE[576];
   std::vector<int> tpl_off;
   std::vector<double> tpl_w;
@@ -141,6 +145,21 @@ struct tofe_ctx {
   std::string err;
 
   int kmax() const { return std::max(std::max(desc.nu_pre, desc.nu_post), 1); }
+
+  tofe_ctx() = default;
+  tofe_ctx(const tofe_ctx &) = delete;
+  tofe_ctx &operator=(const tofe_ctx &) = delete;
+  ~tofe_ctx()
+  {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (stream2) (void)hipStreamSynchronize(stream2);
+    if (rb) rocblas_destroy_handle(rb);
+    if (rb2) rocblas_destroy_handle(rb2);
+    if (ev_dense) (void)hipEventDestroy(ev_dense);
+    if (ev_inv) (void)hipEventDestroy(ev_inv);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
+  }
 };
 
 namespace {
@@ -495,9 +514,9 @@ void rb_check(rocblas_status st, const char *what)
 // (diagonal blocks factored and inverted in LDS, panels L21 = A21 L11^-T and the trailing
 // update A22 -= L21 L21^T with rocBLAS), then L^-1 (rocBLAS trtri) and A^-1 = L^-T L^-1
 // (rocBLAS syrk), symmetrised for the per-V-cycle GEMV.
-void coarse_inverse(tofe_ctx *c)
+void coarse_inverse_enqueue(tofe_ctx *c)
 {
-  const hipStream_t s = c->stream;
+  const hipStream_t s = c->stream2;
   const int n = (int)c->nc;
   double *A = c->Ainv.p;
   c->chol_w.alloc((size_t)kCholNB * kCholNB);
@@ -512,28 +531,33 @@ void coarse_inverse(tofe_ctx *c)
     const int m = n - k0 - kb;
     if (m <= 0) break;
     double *A21 = Akk + kb;
-    rb_check(rocblas_dgemm(c->rb, rocblas_operation_none, rocblas_operation_transpose, m, kb, kb, &one,
+    rb_check(rocblas_dgemm(c->rb2, rocblas_operation_none, rocblas_operation_transpose, m, kb, kb, &one,
                            A21, n, c->chol_w.p, kCholNB, &zero, c->chol_p.p, m),
              "rocblas_dgemm");
     TOFE_HIP(hipMemcpy2DAsync(A21, (size_t)n * sizeof(double), c->chol_p.p, (size_t)m * sizeof(double),
                               (size_t)m * sizeof(double), kb, hipMemcpyDeviceToDevice, s));
-    rb_check(rocblas_dsyrk(c->rb, rocblas_fill_lower, rocblas_operation_none, m, kb, &mone, A21, n,
+    rb_check(rocblas_dsyrk(c->rb2, rocblas_fill_lower, rocblas_operation_none, m, kb, &mone, A21, n,
                            &one, A21 + (size_t)kb * n, n),
              "rocblas_dsyrk");
   }
-  int info[2] = {0, 0};
-  TOFE_HIP(hipMemcpyAsync(info, c->dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  TOFE_HIP(hipStreamSynchronize(s));
-  if (info[0] != 0)
-    throw Error(TOFE_ERR_NUMERIC, "coarse operator is not positive definite (pivot " +
-                                      std::to_string(info[0]) + ")");
+  // a failed factorisation is reported by coarse_inverse_finish; the inverse is then unused
   TOFE_HIP(hipMemsetAsync(c->chol_x.p, 0, (size_t)n * n * sizeof(double), s));
-  rb_check(rocblas_dtrtri(c->rb, rocblas_fill_lower, rocblas_diagonal_non_unit, n, A, n, c->chol_x.p, n),
+  rb_check(rocblas_dtrtri(c->rb2, rocblas_fill_lower, rocblas_diagonal_non_unit, n, A, n, c->chol_x.p, n),
            "rocblas_dtrtri");
-  rb_check(rocblas_dsyrk(c->rb, rocblas_fill_lower, rocblas_operation_transpose, n, n, &one,
+  rb_check(rocblas_dsyrk(c->rb2, rocblas_fill_lower, rocblas_operation_transpose, n, n, &one,
                          c->chol_x.p, n, &zero, A, n),
            "rocblas_dsyrk");
   symmetrize_lower(s, A, n);
+}
+
+void coarse_inverse_finish(tofe_ctx *c)
+{
+  int info[2] = {0, 0};
+  TOFE_HIP(hipMemcpyAsync(info, c->dinfo.p, sizeof(int), hipMemcpyDeviceToHost, c->stream2));
+  TOFE_HIP(hipStreamSynchronize(c->stream2));
+  if (info[0] != 0)
+    throw Error(TOFE_ERR_NUMERIC, "coarse operator is not positive definite (pivot " +
+                                      std::to_string(info[0]) + ")");
 }
 
 void assemble(tofe_ctx *c)
@@ -576,7 +600,12 @@ void assemble(tofe_ctx *c)
   c->Ainv.alloc((size_t)c->nc * c->nc);
   c->dinfo.alloc(2);
   stencil_to_dense(s, SL, gL.Nx, gL.Ny, gL.Nz, c->Ainv.p);
-  coarse_inverse(c);
+  // the coarse factorisation (one wavefront per diagonal block plus small GEMMs) runs on
+  // the second stream while the Lanczos estimates fill the device
+  TOFE_HIP(hipEventRecord(c->ev_dense, s));
+  TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
+  coarse_inverse_enqueue(c);
+  TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
   // Chebyshev bounds from lambda_max(D^-1 A) per smoothed level
   lanczos_lambdas(c);
   chebyshev_coefficients(c);
@@ -598,6 +627,8 @@ void assemble(tofe_ctx *c)
     }
     convert<double, float>(s, c->E.p, c->E32.p, c->ne);
   }
+  TOFE_HIP(hipStreamWaitEvent(s, c->ev_inv, 0));
+  coarse_inverse_finish(c);
   TOFE_HIP(hipStreamSynchronize(s));
   c->built_precision = c->precision;
   c->assembled = true;
@@ -767,9 +798,14 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
                                     " DOFs; use more multigrid levels");
 
     TOFE_HIP(hipStreamCreate(&c->stream));
-    if (rocblas_create_handle(&c->rb) != rocblas_status_success)
+    TOFE_HIP(hipStreamCreate(&c->stream2));
+    TOFE_HIP(hipEventCreateWithFlags(&c->ev_dense, hipEventDisableTiming));
+    TOFE_HIP(hipEventCreateWithFlags(&c->ev_inv, hipEventDisableTiming));
+    if (rocblas_create_handle(&c->rb) != rocblas_status_success ||
+        rocblas_create_handle(&c->rb2) != rocblas_status_success)
       throw Error(TOFE_ERR_HIP, "rocblas_create_handle failed");
     rocblas_set_stream(c->rb, c->stream);
+    rocblas_set_stream(c->rb2, c->stream2);
 
     compute_ke(d.E0, d.nu, d.hx, d.hy, d.hz, c->KE);
     dmatrix(d.E0, d.nu, c->shape.D);
@@ -915,11 +951,7 @@ int tofe_destroy(tofe_ctx *c)
 {
   if (!c) return TOFE_OK;
   (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->rb) rocblas_destroy_handle(c->rb);
-  hipStream_t s = c->stream;
-  delete c;
-  if (s) (void)hipStreamDestroy(s);
+  delete c;  // ~tofe_ctx synchronises and releases streams, events and handles
   return TOFE_OK;
 }
 
