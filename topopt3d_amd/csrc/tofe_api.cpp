@@ -13,7 +13,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "host_setup.hpp"
@@ -457,12 +459,17 @@ void lanczos_coefficients(tofe_ctx *c, int l)
   }
 }
 
+void lanczos_enqueue(tofe_ctx *c)
+{
+  const int L = c->L;
+  c->eig_ab.alloc((size_t)2 * L * kMaxEig);
+  for (int l = 0; l + 1 < L; ++l) lanczos_coefficients(c, l);
+}
+
 void lanczos_lambdas(tofe_ctx *c)
 {
   const int L = c->L;
   const int its = std::min(c->desc.eig_its, kMaxEig);
-  c->eig_ab.alloc((size_t)2 * L * kMaxEig);
-  for (int l = 0; l + 1 < L; ++l) lanczos_coefficients(c, l);
   std::vector<double> ab((size_t)2 * L * kMaxEig);
   TOFE_HIP(hipMemcpyAsync(ab.data(), c->eig_ab.p, ab.size() * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
@@ -524,11 +531,15 @@ void coarse_inverse_enqueue(tofe_ctx *c)
   c->chol_x.alloc((size_t)n * n);
   TOFE_HIP(hipMemsetAsync(c->dinfo.p, 0, 2 * sizeof(int), s));
   const double one = 1.0, zero = 0.0, mone = -1.0;
+  // The coarse operator couples DOFs at most bw apart (27-point stencil, x-fastest nodes),
+  // and Cholesky fill stays inside that band, so each panel has at most bw nonzero rows.
+  const Grid &gL = c->grids[c->L - 1];
+  const int bw = 3 * (gL.Nx * gL.Ny + gL.Nx + 1) + 2;
   for (int k0 = 0; k0 < n; k0 += kCholNB) {
     const int kb = std::min(kCholNB, n - k0);
     double *Akk = A + k0 + (size_t)k0 * n;
     chol_block(s, Akk, n, kb, k0, c->chol_w.p, c->dinfo.p);
-    const int m = n - k0 - kb;
+    const int m = std::min(n - k0 - kb, bw);
     if (m <= 0) break;
     double *A21 = Akk + kb;
     rb_check(rocblas_dgemm(c->rb2, rocblas_operation_none, rocblas_operation_transpose, m, kb, kb, &one,
@@ -603,11 +614,29 @@ void assemble(tofe_ctx *c)
   // the coarse factorisation (one wavefront per diagonal block plus small GEMMs) runs on
   // the second stream while the Lanczos estimates fill the device
   TOFE_HIP(hipEventRecord(c->ev_dense, s));
-  TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
-  coarse_inverse_enqueue(c);
-  TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
-  // Chebyshev bounds from lambda_max(D^-1 A) per smoothed level
-  lanczos_lambdas(c);
+  // The coarse factorisation is queued on the second stream by a helper host thread while
+  // this thread queues the Lanczos steps (Chebyshev bounds from lambda_max(D^-1 A) per
+  // smoothed level) on the main stream, so both streams fill the device at once.
+  std::exception_ptr coarse_err;
+  std::thread coarse([&] {
+    try {
+      TOFE_HIP(hipSetDevice(c->device));
+      TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
+      coarse_inverse_enqueue(c);
+      TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
+    } catch (...) {
+      coarse_err = std::current_exception();
+    }
+  });
+  try {
+    lanczos_enqueue(c);
+    lanczos_lambdas(c);
+  } catch (...) {
+    coarse.join();
+    throw;
+  }
+  coarse.join();
+  if (coarse_err) std::rethrow_exception(coarse_err);
   chebyshev_coefficients(c);
   // FP32 copies for the mixed-precision V-cycle
   if (c->precision == TOFE_PREC_MIXED) {
