@@ -79,16 +79,23 @@ __device__ inline T ke_row_dot(const T *base, const T (&ue)[24])
   return t;
 }
 
+// The rows of element (A, B) of the layer for the thread's lower node (want_lo) and upper
+// node (want_hi); the flags are uniform across the block.
 template <typename T, int A, int B>
-__device__ inline void element_rows(const T *base, const T (&ue)[24], T Ee, T (&alo)[3], T (&ahi)[3])
+__device__ inline void element_rows(const T *base, const T (&ue)[24], T Ee, T (&alo)[3], T (&ahi)[3],
+                                    bool want_lo, bool want_hi)
 {
   constexpr int Llo = lidx(1 - A, 1 - B, 0), Lhi = lidx(1 - A, 1 - B, 1);
-  alo[0] += Ee * ke_row_dot<T, Llo, 0>(base, ue);
-  alo[1] += Ee * ke_row_dot<T, Llo, 1>(base, ue);
-  alo[2] += Ee * ke_row_dot<T, Llo, 2>(base, ue);
-  ahi[0] += Ee * ke_row_dot<T, Lhi, 0>(base, ue);
-  ahi[1] += Ee * ke_row_dot<T, Lhi, 1>(base, ue);
-  ahi[2] += Ee * ke_row_dot<T, Lhi, 2>(base, ue);
+  if (want_lo) {
+    alo[0] += Ee * ke_row_dot<T, Llo, 0>(base, ue);
+    alo[1] += Ee * ke_row_dot<T, Llo, 1>(base, ue);
+    alo[2] += Ee * ke_row_dot<T, Llo, 2>(base, ue);
+  }
+  if (want_hi) {
+    ahi[0] += Ee * ke_row_dot<T, Lhi, 0>(base, ue);
+    ahi[1] += Ee * ke_row_dot<T, Lhi, 1>(base, ue);
+    ahi[2] += Ee * ke_row_dot<T, Lhi, 2>(base, ue);
+  }
 }
 
 template <typename T>
@@ -182,6 +189,11 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
       alo[c] = ahi[c];
       ahi[c] = 0;
     }
+    // Lower-node rows are needed once node plane ez is emitted (not in the chunk's first
+    // step), upper-node rows only while plane ez + 1 belongs to the chunk (not in its last
+    // step); layers outside the mesh contribute nothing.
+    const bool layer_in = ez >= 0 && ez < nz;
+    const bool want_lo = layer_in && ez >= k0, want_hi = layer_in && ez + 1 < k1;
     // the four elements of the layer around the node column, each gathered from LDS
     // just before use so only one element vector is live
     auto element = [&](auto A_, auto B_) {
@@ -194,7 +206,7 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
 #pragma unroll
         for (int c = 0; c < 3; ++c) ue[3 * l + c] = su[lnz(l) ? shi : slo][c][nl];
       }
-      element_rows<T, A, B>(KEbase, ue, sE[eb][(ty + B) * EW + tx + A], alo, ahi);
+      element_rows<T, A, B>(KEbase, ue, sE[eb][(ty + B) * EW + tx + A], alo, ahi, want_lo, want_hi);
     };
     element(std::integral_constant<int, 0>(), std::integral_constant<int, 0>());
     element(std::integral_constant<int, 1>(), std::integral_constant<int, 0>());
