@@ -128,6 +128,68 @@ __global__ __launch_bounds__(NT) void k_galerkin_axis(const double *__restrict__
   }
 }
 
+// The same operator from the upper half of an exactly symmetric stencil: offsets
+// o >= 13 are stored at the node (Su[((o-13)*9 + a*3 + b)*nn + n], 126 values per node) and
+// the block of a lower offset o < 13 is the transpose of the neighbour's block 26 - o.  Each
+// stored block is read by its node and by one neighbour, so the HBM traffic is about half of
+// the full stencil's.
+template <typename T, int EPI>
+__global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T *__restrict__ u,
+                                                       EpiArgs<T> ep, int64_t nn)
+{
+  __shared__ double sred[NT / 64];
+  const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
+  const T *__restrict__ S = op.S;
+  double dsum = 0, dsum2 = 0;
+  if (n < nn) {
+    const int Nx = op.Nx, Ny = op.Ny, Nz = op.Nz;
+    const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+    T acc[3] = {0, 0, 0};
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+      const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+      const int mi = i + dx, mj = j + dy, mk = k + dz;
+      if (mi >= 0 && mi < Nx && mj >= 0 && mj < Ny && mk >= 0 && mk < Nz) {
+        const int64_t m = n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz);
+        const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+        if (o >= 13) {
+          const T *B = S + (int64_t)((o - 13) * 9) * nn + n;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            acc[a] += B[(a * 3 + 0) * nn] * u0 + B[(a * 3 + 1) * nn] * u1 + B[(a * 3 + 2) * nn] * u2;
+        } else {
+          const T *B = S + (int64_t)((13 - o) * 9) * nn + m;  // block (m, 26 - o), transposed
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            acc[a] += B[(0 * 3 + a) * nn] * u0 + B[(1 * 3 + a) * nn] * u1 + B[(2 * 3 + a) * nn] * u2;
+        }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int64_t dof = 3 * n + a;
+      apply_epilogue<T, EPI>(ep, dof, acc[a], u[dof], dsum, dsum2);
+    }
+  }
+  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2) {
+    const double t = block_sum<NT>(dsum, sred);
+    if (threadIdx.x == 0) ep.partial[blockIdx.x] = t;
+    if constexpr (EPI == EPI_APPLY_DOT2) {
+      __syncthreads();
+      const double t2 = block_sum<NT>(dsum2, sred);
+      if (threadIdx.x == 0) ep.partial2[blockIdx.x] = t2;
+    }
+  }
+}
+
+// FP64 full stencil -> FP32 upper half (offsets 13..26).
+__global__ void k_stencil_upper32(const double *__restrict__ S, int64_t nn, float *Su)
+{
+  const int64_t tot = 126 * nn;
+  for (int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x; t < tot; t += (int64_t)gridDim.x * NT)
+    Su[t] = (float)S[13 * 9 * nn + t];
+}
+
 __global__ void k_stencil_diag(const double *__restrict__ S, int64_t nn, double *diag,
                                double *dinv)
 {
@@ -272,6 +334,29 @@ void stencil_op(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, cons
   TOFE_LAUNCH_CHECK();
 }
 
+template <typename T>
+void stencil_op_sym(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep)
+{
+  const int64_t nn = (int64_t)op.Nx * op.Ny * op.Nz;
+  dim3 grid((unsigned)((nn + NT - 1) / NT)), block(NT);
+  switch (epi) {
+    case EPI_APPLY: k_stencil_op_sym<T, EPI_APPLY><<<grid, block, 0, s>>>(op, u, ep, nn); break;
+    case EPI_APPLY_DOT: k_stencil_op_sym<T, EPI_APPLY_DOT><<<grid, block, 0, s>>>(op, u, ep, nn); break;
+    case EPI_APPLY_DOT2: k_stencil_op_sym<T, EPI_APPLY_DOT2><<<grid, block, 0, s>>>(op, u, ep, nn); break;
+    case EPI_RESID: k_stencil_op_sym<T, EPI_RESID><<<grid, block, 0, s>>>(op, u, ep, nn); break;
+    case EPI_CHEB: k_stencil_op_sym<T, EPI_CHEB><<<grid, block, 0, s>>>(op, u, ep, nn); break;
+  }
+  TOFE_LAUNCH_CHECK();
+}
+
+void stencil_upper32(hipStream_t s, const double *S, int64_t nn, float *Su)
+{
+  unsigned nb = (unsigned)((126 * nn + NT - 1) / NT);
+  if (nb > 4096) nb = 4096;
+  k_stencil_upper32<<<dim3(nb), dim3(NT), 0, s>>>(S, nn, Su);
+  TOFE_LAUNCH_CHECK();
+}
+
 void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double *Sc, double *T1,
               double *T2)
 {
@@ -333,4 +418,6 @@ template void stencil_op<float>(hipStream_t, const StencilOp<float> &, const flo
 template void dense_gemv<double, double>(hipStream_t, const double *, int64_t, const double *, double *);
 template void dense_gemv<float, float>(hipStream_t, const double *, int64_t, const float *, float *);
 
+template void stencil_op_sym<float>(hipStream_t, const StencilOp<float> &, const float *, Epi,
+                                    const EpiArgs<float> &);
 }  // namespace tofe

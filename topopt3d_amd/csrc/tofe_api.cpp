@@ -231,7 +231,9 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
     auto &lv = *levels<T>(c)[l];
     op.S = lv.S.p;
     op.Nx = lv.g.Nx; op.Ny = lv.g.Ny; op.Nz = lv.g.Nz;
-    stencil_op<T>(c->stream, op, u, epi, ep);
+    // FP32 levels keep the upper half of the (exactly symmetric) Galerkin stencil
+    if constexpr (sizeof(T) == 4) stencil_op_sym<T>(c->stream, op, u, epi, ep);
+    else stencil_op<T>(c->stream, op, u, epi, ep);
   }
 }
 
@@ -262,7 +264,7 @@ void alloc_level32(tofe_ctx *c, Level<float> &lv, int l)
 {
   lv.g = c->grids[l];
   const int64_t nd = lv.g.ndof();
-  if (l > 0) lv.S.alloc((size_t)243 * lv.g.nn());
+  if (l > 0) lv.S.alloc((size_t)126 * lv.g.nn());  // upper half of the symmetric stencil
   lv.dinv.alloc(nd);
   lv.b.alloc(nd);
   lv.x.alloc(nd);
@@ -652,7 +654,7 @@ void assemble(tofe_ctx *c)
       const int64_t nd = c->grids[l].ndof();
       convert<double, float>(s, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, nd);
       if (l > 0)
-        convert<double, float>(s, c->lv64[l]->S.p, c->lv32[l]->S.p, (int64_t)243 * c->grids[l].nn());
+        stencil_upper32(s, c->lv64[l]->S.p, c->grids[l].nn(), c->lv32[l]->S.p);
     }
     convert<double, float>(s, c->E.p, c->E32.p, c->ne);
   }

@@ -91,6 +91,11 @@ template <typename T>
 void stencil_op(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
 // Coarse stencil of P^T A P, restricted axis by axis through the scratch fields T1 and T2
 // (sizes from galerkin_scratch: values in T1, and in T2 through n2).
+// Stencil operator from the upper half of a symmetric stencil (126 values per node) and
+// the FP64 full -> FP32 upper-half conversion.
+template <typename T>
+void stencil_op_sym(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
+void stencil_upper32(hipStream_t s, const double *S, int64_t nn, float *Su);
 void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double *Sc, double *T1,
               double *T2);
 int64_t galerkin_scratch(int Nfx, int Nfy, int Nfz, int64_t *n2);
