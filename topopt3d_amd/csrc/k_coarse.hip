@@ -220,15 +220,6 @@ __global__ void k_stencil_dense(const double *__restrict__ S, int Nx, int Ny, in
     for (int b = 0; b < 3; ++b) A[(3 * I + a) * n + 3 * J + b] = S[(int64_t)(o * 9 + a * 3 + b) * nn + I];
 }
 
-// column-major lower triangle -> full symmetric
-__global__ void k_symmetrize(double *A, int64_t n)
-{
-  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
-  if (t >= n * n) return;
-  const int64_t c = t / n, r = t - c * n;
-  if (r < c) A[r + c * n] = A[c + r * n];
-}
-
 // y = A x for a symmetric dense A: one wavefront per row, fixed-order reduction.
 template <typename Tin, typename Tout>
 __global__ __launch_bounds__(NT) void k_gemv(const double *__restrict__ A, int64_t n,
@@ -244,10 +235,12 @@ __global__ __launch_bounds__(NT) void k_gemv(const double *__restrict__ A, int64
   if (lane == 0) y[row] = (Tout)s;
 }
 
-// Cholesky factor of one kb x kb diagonal block held in LDS (column-major A, leading
-// dimension lda, kb <= kCholNB): L_kk overwrites the lower triangle of the block and its
-// inverse goes to W (leading dimension kCholNB).  info[0] = global pivot index + 1 of the
-// first non-positive pivot (0 if none).
+// ---- dense coarse solve: banded blocked Cholesky, X = L^-1, A^-1 = X^T X ------------
+// The coarse matrix is n x n, column-major, SPD, and couples DOFs at most bw apart, so
+// the Cholesky factor L stays inside that band.  Tiles are 64 x 64 (kCholNB).
+
+constexpr int CB = kCholNB;
+
 // Value of a double held by lane `lane` (uniform) of the wavefront.
 __device__ inline double lane_value(double v, int lane)
 {
@@ -256,22 +249,33 @@ __device__ inline double lane_value(double v, int lane)
   return __hiloint2double(hi, lo);
 }
 
-// One wavefront, everything in registers: lane i holds row i of the block while it is
-// factorised and column i of W = L^-1 while that is formed; the other lanes' values come
-// through v_readlane with compile-time lane numbers, so there are no LDS round trips and
-// no barriers.  A partial block (kb < 64) is padded with the identity, which factors and
-// inverts to itself and leaves the leading kb x kb part unchanged.
-__global__ __launch_bounds__(64) void k_chol_block(double *A, int lda, int kb, int k0, double *W,
-                                                   int *info)
+// Ordering point for LDS traffic inside one wavefront: stores by any lane issued before
+// it are seen by loads issued after it (a wavefront's LDS operations complete in order).
+__device__ inline void wave_lds_sync()
 {
-  const int i = threadIdx.x;
-  double r[kCholNB];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Cholesky factor of the kb x kb diagonal block at (k0, k0) of A (leading dimension lda),
+// padded to 64 with the identity, in one wavefront: lane i keeps row i in registers, each
+// finished column j of L is stored once to Ls (column-major) and the other lanes read the
+// entries they need from there as broadcasts.  dinv[j] = 1 / L(j,j).  Returns the global
+// pivot index + 1 of the first non-positive pivot (0 if none); that pivot is replaced by 1
+// so the factorisation runs to the end.
+__device__ inline int chol_diag(const double *A, int64_t lda, int k0, int kb, double *Ls,
+                                double *dinv)
+{
+  const int i = threadIdx.x & 63;
+  double r[CB];
 #pragma unroll
-  for (int j = 0; j < kCholNB; ++j)
-    r[j] = (i < kb && j < kb) ? (i >= j ? A[i + (int64_t)j * lda] : 0.0) : (i == j ? 1.0 : 0.0);
+  for (int j = 0; j < CB; ++j)
+    r[j] = (i < kb && j < kb) ? (i >= j ? A[(k0 + i) + (int64_t)(k0 + j) * lda] : 0.0)
+                              : (i == j ? 1.0 : 0.0);
   int bad = 0;
 #pragma unroll
-  for (int j = 0; j < kCholNB; ++j) {
+  for (int j = 0; j < CB; ++j) {
     double d = lane_value(r[j], j);
     if (!(d > 0)) {
       if (bad == 0) bad = k0 + j + 1;
@@ -279,36 +283,255 @@ __global__ __launch_bounds__(64) void k_chol_block(double *A, int lda, int kb, i
     }
     d = sqrt(d);
     const double lij = (i > j) ? r[j] / d : (i == j ? d : 0.0);
-    r[j] = lij;
-    // trailing update of row i; entries above the diagonal are never read as L values
+    Ls[j * CB + i] = lij;
+    if (i == 0) dinv[j] = 1.0 / d;
+    wave_lds_sync();
+    // trailing update of row i; entries above the diagonal are never read as L values.
+    // The empty asm pins each updated value here: left to itself the compiler defers the
+    // updates to each entry's first use and keeps every broadcast column live, which
+    // spills.
 #pragma unroll
-    for (int l = j + 1; l < kCholNB; ++l) r[l] -= lij * lane_value(lij, l);
+    for (int l = j + 1; l < CB; ++l) r[l] -= lij * Ls[j * CB + l];
+#pragma unroll
+    for (int l = j + 1; l < CB; ++l) asm volatile("" : "+v"(r[l]));
   }
-  if (bad != 0 && i == 0 && info[0] == 0) info[0] = bad;
+  return bad;
+}
+
+// Lower tile (I, J), I >= J, of a triangle of tiles from its linear index.
+__device__ inline void tile_ij(int b, int &I, int &J)
+{
+  int i = 0;
+  while (b > i) {
+    b -= i + 1;
+    ++i;
+  }
+  I = i;
+  J = b;
+}
+
+// One step of the banded right-looking Cholesky for the column block at k0 (kb columns),
+// with m band rows below it.  Every workgroup factors the diagonal block itself, so no
+// workgroup waits for another: it then solves the panel rows P = A21 L11^-T of its two row
+// blocks I and J of the trailing window and applies C(I,J) -= P_I P_J^T to its tile of A.
+// L goes to Lb only (the diagonal block from workgroup 0, the panel rows from the diagonal
+// tiles), and the panel of A is only read, so no workgroup reads what another one writes.
+__global__ __launch_bounds__(256) void k_chol_step(double *A, double *Lb, int n, int k0, int kb,
+                                                   int m, int *info)
+{
+  __shared__ double Ls[CB * CB];
+  __shared__ double dinv[CB];
+  __shared__ double Pt[2][CB * CB];  // panel rows, transposed: Pt[s][q * 64 + row]
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  int I, J;
+  tile_ij(blockIdx.x, I, J);
+  if (w == 0) {
+    const int bad = chol_diag(A, n, k0, kb, Ls, dinv);
+    if (blockIdx.x == 0 && lane == 0 && bad != 0 && info[0] == 0) info[0] = bad;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    for (int e = t; e < CB * CB; e += 256) {
+      const int r = e & 63, cc = e >> 6;
+      if (r < kb && cc < kb) Lb[(k0 + r) + (int64_t)(k0 + cc) * n] = Ls[cc * CB + r];
+    }
+  }
+  if (m <= 0) return;
+  // m > 0 means a full diagonal block (kb == 64) with rows below it
+  const int w0 = k0 + kb;  // first row and column of the trailing window
+  if (t < 128 && (t < 64 || J != I)) {
+    const int s = t >> 6;
+    const int row = 64 * (s ? J : I) + lane;  // row of the window
+    const bool live = row < m;
+    double x[CB];
 #pragma unroll
-  for (int j = 0; j < kCholNB; ++j)
-    if (i < kb && j < kb && i >= j) A[i + (int64_t)j * lda] = r[j];
-  // W = L^-1: lane c solves L w = e_c; L(q, p) is r[p] of lane q
-  const int c = i;
-  double w[kCholNB];
+    for (int q = 0; q < CB; ++q) x[q] = live ? A[(w0 + row) + (int64_t)(k0 + q) * n] : 0.0;
+    // x L11^T = a: column by column, each finished x[q] updates the later entries
 #pragma unroll
-  for (int q = 0; q < kCholNB; ++q) w[q] = (q == c) ? 1.0 : 0.0;
+    for (int q = 0; q < CB; ++q) {
+      x[q] *= dinv[q];
 #pragma unroll
-  for (int q = 0; q < kCholNB; ++q) {
-    w[q] = w[q] / lane_value(r[q], q);
+      for (int l = q + 1; l < CB; ++l) x[l] -= x[q] * Ls[q * CB + l];
 #pragma unroll
-    for (int p = q + 1; p < kCholNB; ++p) w[p] -= lane_value(r[q], p) * w[q];
+      for (int l = q + 1; l < CB; ++l) asm volatile("" : "+v"(x[l]));
+    }
+#pragma unroll
+    for (int q = 0; q < CB; ++q) Pt[s][q * CB + lane] = x[q];
+    if (I == J && live) {
+#pragma unroll
+      for (int q = 0; q < CB; ++q) Lb[(w0 + row) + (int64_t)(k0 + q) * n] = x[q];
+    }
+  }
+  __syncthreads();
+  const double *PI = Pt[0], *PJ = (I == J) ? Pt[0] : Pt[1];
+  const int ra = (t & 15) * 4, cb = (t >> 4) * 4;
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+#pragma unroll 8
+  for (int q = 0; q < CB; ++q) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = PI[q * CB + ra + u];
+      b[u] = PJ[q * CB + cb + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
   }
 #pragma unroll
-  for (int q = 0; q < kCholNB; ++q)
-    W[q + kCholNB * c] = (q < kb && c < kb && q >= c) ? w[q] : 0.0;
+  for (int u = 0; u < 4; ++u) {
+    const int row = 64 * I + ra + u;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int col = 64 * J + cb + v;
+      if (row < m && col < m && (I != J || row >= col))
+        A[(w0 + row) + (int64_t)(w0 + col) * n] -= acc[u][v];
+    }
+  }
+}
+
+// X = L^-1 by column blocks: workgroup (kc, part) owns CW = 4 CPW columns of block kc and
+// sweeps the row blocks i = kc, kc+1, ...: T = E - sum_j L(i,j) X(j,kc) over the row
+// blocks j < i inside the band (at most jb of them), then X(i,kc) = L(i,i)^-1 T by forward
+// substitution, wave w solving CPW columns with lane = row.  The JB most recent X tiles
+// stay in an LDS ring (JB > jb), so a workgroup reads back only what it wrote, from LDS.
+template <int CPW, int JB>
+__global__ __launch_bounds__(256) void k_tri_inv(const double *Lb, double *X, int n, int nbk,
+                                                 int jb)
+{
+  constexpr int CW = 4 * CPW;
+  __shared__ double Ld[CB * CB];        // L(i,i), column-major, identity-padded
+  __shared__ double ring[JB][CB * CW];  // X(j,kc) tiles, [row][column]
+  const int kc = blockIdx.x / (CB / CW), part = blockIdx.x % (CB / CW);
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int c0 = CB * kc + CW * part + CPW * w;  // first global column of this wave
+  for (int i = kc; i < nbk; ++i) {
+    const int64_t grow = (int64_t)CB * i + lane;
+    double T[CPW];
+#pragma unroll
+    for (int v = 0; v < CPW; ++v) T[v] = (grow == c0 + v) ? 1.0 : 0.0;
+    const int jlo = (i - jb > kc) ? i - jb : kc;
+    for (int j = jlo; j < i; ++j) {
+      const double *Xj = ring[j % JB];
+      const double *Lrow = Lb + grow + (int64_t)CB * j * n;
+#pragma unroll 8
+      for (int p = 0; p < CB; ++p) {
+        const double l = (grow < n) ? Lrow[(int64_t)p * n] : 0.0;
+#pragma unroll
+        for (int v = 0; v < CPW; ++v) T[v] -= l * Xj[p * CW + CPW * w + v];
+      }
+    }
+    __syncthreads();  // previous step's readers of Ld are done
+    for (int e = t; e < CB * CB; e += 256) {
+      const int r = e & 63, cc = e >> 6;
+      const int64_t gr = (int64_t)CB * i + r, gc = (int64_t)CB * i + cc;
+      Ld[e] = (gr < n && gc < n) ? Lb[gr + gc * n] : (r == cc ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int p = 0; p < CB; ++p) {
+      const double rl = 1.0 / Ld[p * CB + p];
+      const double lrp = Ld[p * CB + lane];
+#pragma unroll
+      for (int v = 0; v < CPW; ++v) {
+        const double xp = lane_value(T[v], p) * rl;
+        if (lane == p) T[v] = xp;
+        else if (lane > p) T[v] -= lrp * xp;
+      }
+    }
+    double *Xi = ring[i % JB];
+#pragma unroll
+    for (int v = 0; v < CPW; ++v) {
+      Xi[lane * CW + CPW * w + v] = T[v];
+      if (grow < n && c0 + v < n) X[grow + (int64_t)(c0 + v) * n] = T[v];
+    }
+    __syncthreads();  // the new ring tile is complete before the next step reads it
+  }
+}
+
+// A^-1 = X^T X: workgroup per lower tile (I, J) sums X(K,I)^T X(K,J) over K >= I and
+// writes the tile and, off the diagonal, its transpose.
+__global__ __launch_bounds__(256) void k_xtx(const double *X, double *Ai, int n, int nbk)
+{
+  __shared__ double XI[CB * (CB + 1)], XJ[CB * (CB + 1)];  // [row][column], padded rows
+  const int t = threadIdx.x;
+  int I, J;
+  tile_ij(blockIdx.x, I, J);
+  const int ra = (t & 15) * 4, cb = (t >> 4) * 4;
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int K = I; K < nbk; ++K) {
+    __syncthreads();
+    for (int e = t; e < CB * CB; e += 256) {
+      const int r = e & 63, cc = e >> 6;
+      const int64_t gr = (int64_t)CB * K + r;
+      const int64_t gi = (int64_t)CB * I + cc, gj = (int64_t)CB * J + cc;
+      XI[r * (CB + 1) + cc] = (gr < n && gi < n) ? X[gr + gi * n] : 0.0;
+      XJ[r * (CB + 1) + cc] = (gr < n && gj < n) ? X[gr + gj * n] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int r = 0; r < CB; ++r) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = XI[r * (CB + 1) + ra + u];
+        b[u] = XJ[r * (CB + 1) + cb + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t row = (int64_t)CB * I + ra + u;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t col = (int64_t)CB * J + cb + v;
+      if (row < n && col < n) {
+        Ai[row + col * n] = acc[u][v];
+        if (I != J) Ai[col + row * n] = acc[u][v];
+      }
+    }
+  }
 }
 
 }  // namespace
 
-void chol_block(hipStream_t s, double *A, int lda, int kb, int k0, double *W, int *info)
+void coarse_factor_invert(hipStream_t s, double *A, double *Lb, double *X, int n, int bw,
+                          int *info)
 {
-  k_chol_block<<<dim3(1), dim3(kCholNB), 0, s>>>(A, lda, kb, k0, W, info);
+  const int nbk = (n + CB - 1) / CB;
+  const int jb = std::min((bw + CB - 1) / CB, nbk);
+  if (jb >= 32)
+    throw Error(TOFE_ERR_ARG, "coarse grid too wide for the dense coarse solve (band " +
+                                  std::to_string(bw) + "); use more multigrid levels");
+  TOFE_HIP(hipMemsetAsync(Lb, 0, sizeof(double) * (size_t)n * n, s));
+  for (int k0 = 0; k0 < n; k0 += CB) {
+    const int kb = std::min(CB, n - k0);
+    const int m = std::min(n - k0 - kb, bw);
+    const int nI = (m + CB - 1) / CB;
+    const int tiles = m > 0 ? nI * (nI + 1) / 2 : 1;
+    k_chol_step<<<dim3(tiles), dim3(256), 0, s>>>(A, Lb, n, k0, kb, m, info);
+    TOFE_LAUNCH_CHECK();
+  }
+  if (jb < 8)
+    k_tri_inv<4, 8><<<dim3(nbk * 4), dim3(256), 0, s>>>(Lb, X, n, nbk, jb);
+  else if (jb < 16)
+    k_tri_inv<2, 16><<<dim3(nbk * 8), dim3(256), 0, s>>>(Lb, X, n, nbk, jb);
+  else
+    k_tri_inv<1, 32><<<dim3(nbk * 16), dim3(256), 0, s>>>(Lb, X, n, nbk, jb);
+  TOFE_LAUNCH_CHECK();
+  k_xtx<<<dim3(nbk * (nbk + 1) / 2), dim3(256), 0, s>>>(X, A, n, nbk);
   TOFE_LAUNCH_CHECK();
 }
 
@@ -393,12 +616,6 @@ void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, do
   const int64_t nn = (int64_t)Nx * Ny * Nz;
   TOFE_HIP(hipMemsetAsync(A, 0, sizeof(double) * 9 * nn * nn, s));
   k_stencil_dense<<<dim3((unsigned)((27 * nn + NT - 1) / NT)), dim3(NT), 0, s>>>(S, Nx, Ny, Nz, A);
-  TOFE_LAUNCH_CHECK();
-}
-
-void symmetrize_lower(hipStream_t s, double *A, int64_t n)
-{
-  k_symmetrize<<<dim3((unsigned)((n * n + NT - 1) / NT)), dim3(NT), 0, s>>>(A, n);
   TOFE_LAUNCH_CHECK();
 }
 

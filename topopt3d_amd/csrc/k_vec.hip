@@ -193,6 +193,66 @@ __global__ void k_record(const double *scal, int slot, double *dst)
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = scal[slot];
 }
 
+// Sum of nparts per-block partial sums, formed by every block of the consuming kernel
+// in the order k_reduce_final uses (thread-strided, wave butterfly, waves in order), so
+// the value is the same bit for bit without a separate reduction launch.  Valid in all
+// threads.
+__device__ inline double all_sum(const double *__restrict__ partial, int nparts, double *smem)
+{
+  double s = 0;
+  for (int i = threadIdx.x; i < nparts; i += NT) s += partial[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double t = 0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) t += smem[i];
+  __syncthreads();
+  return t;
+}
+
+// Jacobi-CG step of the eigenvalue estimate, first half: alpha = rz_old / p.Ap from the
+// operator's partials; r -= alpha q; partial r.(D^-1 r).  Block 0 records alpha.
+__global__ __launch_bounds__(NT) void k_lz_r(double *__restrict__ r, const double *__restrict__ q,
+                                             const double *__restrict__ dinv, int64_t n,
+                                             const double *__restrict__ pq_part, int npq,
+                                             const double *__restrict__ scal, int rz_slot,
+                                             double *rec_alpha, double *partial)
+{
+  __shared__ double sred[NT / 64];
+  const double alpha = scal[rz_slot] / all_sum(pq_part, npq, sred);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *rec_alpha = alpha;
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const double zi = dinv[i] * ri;
+    s += ri * zi;
+  }
+  const double t = block_sum<NT>(s, sred);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// Second half: rz = sum of the partials, beta = rz / rz_old, p = D^-1 r + beta p.  Block 0
+// records beta and stores rz in the other slot for the next step.
+__global__ __launch_bounds__(NT) void k_lz_p(double *__restrict__ p, const double *__restrict__ r,
+                                             const double *__restrict__ dinv, int64_t n,
+                                             const double *__restrict__ rz_part, int nrz,
+                                             double *scal, int rz_old, int rz_new, double *rec_beta)
+{
+  __shared__ double sred[NT / 64];
+  const double rz = all_sum(rz_part, nrz, sred);
+  const double beta = rz / scal[rz_old];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *rec_beta = beta;
+    scal[rz_new] = rz;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const double zi = dinv[i] * r[i];
+    p[i] = zi + beta * p[i];
+  }
+}
+
 inline unsigned stride_blocks(int64_t n)
 {
   const int64_t b = (n + NT - 1) / NT;
@@ -285,6 +345,23 @@ void jacobi_dot(hipStream_t s, const double *r, const double *dinv, double *z, i
 void record_scalar(hipStream_t s, const double *scal, int slot, double *dst)
 {
   k_record<<<dim3(1), dim3(64), 0, s>>>(scal, slot, dst);
+  TOFE_LAUNCH_CHECK();
+}
+
+void lanczos_r(hipStream_t s, double *r, const double *q, const double *dinv, int64_t n,
+               const double *pq_part, int npq, const double *scal, int rz_slot, double *rec_alpha,
+               double *partial)
+{
+  k_lz_r<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, q, dinv, n, pq_part, npq, scal, rz_slot,
+                                               rec_alpha, partial);
+  TOFE_LAUNCH_CHECK();
+}
+
+void lanczos_p(hipStream_t s, double *p, const double *r, const double *dinv, int64_t n,
+               const double *rz_part, double *scal, int rz_old, int rz_new, double *rec_beta)
+{
+  k_lz_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, r, dinv, n, rz_part, kRedBlocks, scal,
+                                                     rz_old, rz_new, rec_beta);
   TOFE_LAUNCH_CHECK();
 }
 

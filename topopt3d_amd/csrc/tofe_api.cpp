@@ -6,7 +6,6 @@
 // Mixed path: the same V-cycle in FP32 (FP32 operator, stencils, vectors) inside an
 // FP64 flexible CG (truncated FCG(1)); the outer residual, the dot products and the
 // coarse solve stay FP64.
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <chrono>
@@ -84,10 +83,8 @@ struct tofe_ctx {
   std::vector<Grid> grids;
   int64_t ne = 0, nn = 0, ndof = 0;
   hipStream_t stream = nullptr;
-  rocblas_handle rb = nullptr;
   // second stream: the coarsest-level factorisation runs beside the Lanczos estimates
   hipStream_t stream2 = nullptr;
-  rocblas_handle rb2 = nullptr;
   hipEvent_t ev_dense = nullptr, ev_inv = nullptr;
   double KE[576];
   std::vector<int> tpl_off;
@@ -128,7 +125,7 @@ struct tofe_ctx {
   DBuf<double> gal_t1, gal_t2;  // axis-by-axis Galerkin scratch
   DBuf<double> Ainv;
   DBuf<int> dinfo;
-  DBuf<double> chol_w, chol_p, chol_x;
+  DBuf<double> chol_l, chol_x;
   int64_t nc = 0;
   DBuf<double> partial, partial2, scal, eig_ab;
   int npart = 0;
@@ -155,8 +152,6 @@ struct tofe_ctx {
   {
     if (stream) (void)hipStreamSynchronize(stream);
     if (stream2) (void)hipStreamSynchronize(stream2);
-    if (rb) rocblas_destroy_handle(rb);
-    if (rb2) rocblas_destroy_handle(rb2);
     if (ev_dense) (void)hipEventDestroy(ev_dense);
     if (ev_inv) (void)hipEventDestroy(ev_inv);
     if (stream) (void)hipStreamDestroy(stream);
@@ -439,25 +434,26 @@ void lanczos_coefficients(tofe_ctx *c, int l)
   const hipStream_t s = c->stream;
   Level<double> &lv = *c->lv64[l];
   const int64_t nd = lv.g.ndof();
-  double *r = c->vr.p, *z = c->vz.p, *p = c->vp.p, *q = c->vq.p, *x = c->vU.p;
+  double *r = c->vr.p, *p = c->vp.p, *q = c->vq.p;
   const int nb = level_blocks<double>(c, l);
   const int its = std::min(c->desc.eig_its, kMaxEig);
+  double *alpha = c->eig_ab.p + (size_t)(2 * l) * kMaxEig;
+  double *beta = c->eig_ab.p + (size_t)(2 * l + 1) * kMaxEig;
+  // Three launches per step: the operator with its p.Ap partials, the r update with the
+  // r.z partials, the p update; each consumer sums the producer's partials itself.  The
+  // iterate x of the CG is never needed for the estimate and is not formed.
   power_start(s, r, nd);
-  jacobi_dot(s, r, lv.dinv.p, z, nd, c->partial.p);
-  reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RZ, POST_NONE, c->scal.p);
-  TOFE_HIP(hipMemcpyAsync(p, z, nd * sizeof(double), hipMemcpyDeviceToDevice, s));
+  jacobi_dot(s, r, lv.dinv.p, p, nd, c->partial.p);
+  reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RZA, POST_NONE, c->scal.p);
   for (int k = 0; k < its; ++k) {
+    const int rz_old = (k & 1) ? SL_RZB : SL_RZA, rz_new = (k & 1) ? SL_RZA : SL_RZB;
     EpiArgs<double> ep;
     ep.y = q;
     ep.partial = c->partial.p;
     level_apply<double>(c, l, p, EPI_APPLY_DOT, ep);
-    reduce_final(s, c->partial.p, nb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p);
-    record_scalar(s, c->scal.p, SL_ALPHA, c->eig_ab.p + (size_t)(2 * l) * kMaxEig + k);
-    cg_update_xr(s, x, r, p, q, nd, c->scal.p, SL_ALPHA, c->partial2.p);
-    jacobi_dot(s, r, lv.dinv.p, z, nd, c->partial.p);
-    reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_TMP, POST_BETA_CG, c->scal.p);
-    record_scalar(s, c->scal.p, SL_BETA, c->eig_ab.p + (size_t)(2 * l + 1) * kMaxEig + k);
-    cg_update_p(s, p, z, nd, c->scal.p, SL_BETA);
+    lanczos_r(s, r, q, lv.dinv.p, nd, c->partial.p, nb, c->scal.p, rz_old, alpha + k,
+              c->partial2.p);
+    lanczos_p(s, p, r, lv.dinv.p, nd, c->partial2.p, c->scal.p, rz_old, rz_new, beta + k);
   }
 }
 
@@ -513,54 +509,21 @@ void chebyshev_coefficients(tofe_ctx *c)
   }
 }
 
-void rb_check(rocblas_status st, const char *what)
-{
-  if (st != rocblas_status_success)
-    throw Error(TOFE_ERR_HIP, std::string(what) + " failed: " + rocblas_status_to_string(st));
-}
-
-// A (dense SPD, column-major, in c->Ainv) -> A^-1 in place: blocked right-looking Cholesky
-// (diagonal blocks factored and inverted in LDS, panels L21 = A21 L11^-T and the trailing
-// update A22 -= L21 L21^T with rocBLAS), then L^-1 (rocBLAS trtri) and A^-1 = L^-T L^-1
-// (rocBLAS syrk), symmetrised for the per-V-cycle GEMV.
+// A (dense SPD, column-major, in c->Ainv) -> A^-1 in place on the second stream: banded
+// blocked Cholesky (one launch per 64-column step), X = L^-1, A^-1 = X^T X, stored full
+// for the per-V-cycle GEMV.  The coarse operator couples DOFs at most bw apart (27-point
+// stencil, x-fastest nodes) and the factor's fill stays inside that band.
 void coarse_inverse_enqueue(tofe_ctx *c)
 {
   const hipStream_t s = c->stream2;
   const int n = (int)c->nc;
-  double *A = c->Ainv.p;
-  c->chol_w.alloc((size_t)kCholNB * kCholNB);
-  c->chol_p.alloc((size_t)n * kCholNB);
+  c->chol_l.alloc((size_t)n * n);
   c->chol_x.alloc((size_t)n * n);
   TOFE_HIP(hipMemsetAsync(c->dinfo.p, 0, 2 * sizeof(int), s));
-  const double one = 1.0, zero = 0.0, mone = -1.0;
-  // The coarse operator couples DOFs at most bw apart (27-point stencil, x-fastest nodes),
-  // and Cholesky fill stays inside that band, so each panel has at most bw nonzero rows.
   const Grid &gL = c->grids[c->L - 1];
   const int bw = 3 * (gL.Nx * gL.Ny + gL.Nx + 1) + 2;
-  for (int k0 = 0; k0 < n; k0 += kCholNB) {
-    const int kb = std::min(kCholNB, n - k0);
-    double *Akk = A + k0 + (size_t)k0 * n;
-    chol_block(s, Akk, n, kb, k0, c->chol_w.p, c->dinfo.p);
-    const int m = std::min(n - k0 - kb, bw);
-    if (m <= 0) break;
-    double *A21 = Akk + kb;
-    rb_check(rocblas_dgemm(c->rb2, rocblas_operation_none, rocblas_operation_transpose, m, kb, kb, &one,
-                           A21, n, c->chol_w.p, kCholNB, &zero, c->chol_p.p, m),
-             "rocblas_dgemm");
-    TOFE_HIP(hipMemcpy2DAsync(A21, (size_t)n * sizeof(double), c->chol_p.p, (size_t)m * sizeof(double),
-                              (size_t)m * sizeof(double), kb, hipMemcpyDeviceToDevice, s));
-    rb_check(rocblas_dsyrk(c->rb2, rocblas_fill_lower, rocblas_operation_none, m, kb, &mone, A21, n,
-                           &one, A21 + (size_t)kb * n, n),
-             "rocblas_dsyrk");
-  }
   // a failed factorisation is reported by coarse_inverse_finish; the inverse is then unused
-  TOFE_HIP(hipMemsetAsync(c->chol_x.p, 0, (size_t)n * n * sizeof(double), s));
-  rb_check(rocblas_dtrtri(c->rb2, rocblas_fill_lower, rocblas_diagonal_non_unit, n, A, n, c->chol_x.p, n),
-           "rocblas_dtrtri");
-  rb_check(rocblas_dsyrk(c->rb2, rocblas_fill_lower, rocblas_operation_transpose, n, n, &one,
-                         c->chol_x.p, n, &zero, A, n),
-           "rocblas_dsyrk");
-  symmetrize_lower(s, A, n);
+  coarse_factor_invert(s, c->Ainv.p, c->chol_l.p, c->chol_x.p, n, bw, c->dinfo.p);
 }
 
 void coarse_inverse_finish(tofe_ctx *c)
@@ -824,19 +787,18 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     c->ne = (int64_t)d.nx * d.ny * d.nz;
     c->nn = c->grids[0].nn();
     c->ndof = 3 * c->nn;
-    if (c->grids[c->L - 1].ndof() > 20000)
-      throw Error(TOFE_ERR_ARG, "coarsest level has " + std::to_string(c->grids[c->L - 1].ndof()) +
-                                    " DOFs; use more multigrid levels");
+    // dense direct solve on the coarsest level: n^2 doubles, half-bandwidth below 31 tiles
+    const Grid &gc = c->grids[c->L - 1];
+    if (gc.ndof() > 20000 || 3 * (gc.Nx * gc.Ny + gc.Nx + 1) + 2 > 31 * kCholNB - 63)
+      throw Error(TOFE_ERR_ARG, "coarsest level has " + std::to_string(gc.ndof()) +
+                                    " DOFs (" + std::to_string(gc.Nx) + " x " + std::to_string(gc.Ny) +
+                                    " x " + std::to_string(gc.Nz) +
+                                    " nodes); use more multigrid levels");
 
     TOFE_HIP(hipStreamCreate(&c->stream));
     TOFE_HIP(hipStreamCreate(&c->stream2));
     TOFE_HIP(hipEventCreateWithFlags(&c->ev_dense, hipEventDisableTiming));
     TOFE_HIP(hipEventCreateWithFlags(&c->ev_inv, hipEventDisableTiming));
-    if (rocblas_create_handle(&c->rb) != rocblas_status_success ||
-        rocblas_create_handle(&c->rb2) != rocblas_status_success)
-      throw Error(TOFE_ERR_HIP, "rocblas_create_handle failed");
-    rocblas_set_stream(c->rb, c->stream);
-    rocblas_set_stream(c->rb2, c->stream2);
 
     compute_ke(d.E0, d.nu, d.hx, d.hy, d.hz, c->KE);
     dmatrix(d.E0, d.nu, c->shape.D);

@@ -101,9 +101,12 @@ void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double
 int64_t galerkin_scratch(int Nfx, int Nfy, int Nfz, int64_t *n2);
 void stencil_diag(hipStream_t s, const double *S, int64_t nn, double *diag, double *dinv);
 void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A);
-void symmetrize_lower(hipStream_t s, double *A, int64_t n);
-constexpr int kCholNB = 64;  // diagonal block of the coarse Cholesky factorisation
-void chol_block(hipStream_t s, double *A, int lda, int kb, int k0, double *W, int *info);
+constexpr int kCholNB = 64;  // tile of the coarse Cholesky factorisation
+// A (n x n SPD, column-major, half-bandwidth bw) -> A^-1 in place: banded blocked
+// Cholesky into Lb, X = L^-1 into X, then A^-1 = X^T X.  info[0] = first non-positive
+// pivot index + 1 (0 if none).  Lb and X are n x n scratch.
+void coarse_factor_invert(hipStream_t s, double *A, double *Lb, double *X, int n, int bw,
+                          int *info);
 template <typename Tin, typename Tout>
 void dense_gemv(hipStream_t s, const double *A, int64_t n, const Tin *x, Tout *y);
 
@@ -144,11 +147,20 @@ void power_start(hipStream_t s, double *v, int64_t n);
 void jacobi_dot(hipStream_t s, const double *r, const double *dinv, double *z, int64_t n,
                 double *partial);
 void record_scalar(hipStream_t s, const double *scal, int slot, double *dst);
+// Fused Jacobi-CG steps of the eigenvalue estimate (no separate reduction launches):
+// lanczos_r: alpha = scal[rz_slot] / sum(pq_part[0..npq)), recorded; r -= alpha q; partials
+// of r.(D^-1 r).  lanczos_p: beta = sum(rz_part) / scal[rz_old], recorded; scal[rz_new] =
+// the sum; p = D^-1 r + beta p.
+void lanczos_r(hipStream_t s, double *r, const double *q, const double *dinv, int64_t n,
+               const double *pq_part, int npq, const double *scal, int rz_slot, double *rec_alpha,
+               double *partial);
+void lanczos_p(hipStream_t s, double *p, const double *r, const double *dinv, int64_t n,
+               const double *rz_part, double *scal, int rz_old, int rz_new, double *rec_beta);
 
 // device scalar slots (ctx->scal)
 enum Slot : int {
   SL_RZ = 0, SL_PQ = 1, SL_RR = 2, SL_ALPHA = 3, SL_BETA = 4, SL_BB = 5, SL_PR = 6, SL_ZQ = 7,
-  SL_TMP = 11, SL_COUNT = 16
+  SL_RZA = 8, SL_RZB = 9, SL_TMP = 11, SL_COUNT = 16
 };
 // reduce_final post-processing
 enum Post : int {
