@@ -264,15 +264,29 @@ __device__ inline void wave_lds_sync()
 // entries they need from there as broadcasts.  dinv[j] = 1 / L(j,j).  Returns the global
 // pivot index + 1 of the first non-positive pivot (0 if none); that pivot is replaced by 1
 // so the factorisation runs to the end.
+template <bool FULL>
 __device__ inline int chol_diag(const double *A, int64_t lda, int k0, int kb, double *Ls,
                                 double *dinv)
 {
   const int i = threadIdx.x & 63;
   double r[CB];
+  if constexpr (FULL) {
+    // The dense coarse matrix is stored full (symmetric), so the whole block is loaded
+    // unconditionally, all 64 loads in flight together.  Entries above the diagonal then
+    // go through the same arithmetic as the rest but are never read as L values: lane i
+    // writes column j's entry only for j <= i, later columns read Ls only below the
+    // diagonal, and the result is written out with zeros above it.
 #pragma unroll
-  for (int j = 0; j < CB; ++j)
-    r[j] = (i < kb && j < kb) ? (i >= j ? A[(k0 + i) + (int64_t)(k0 + j) * lda] : 0.0)
-                              : (i == j ? 1.0 : 0.0);
+    for (int j = 0; j < CB; ++j) r[j] = A[(k0 + i) + (int64_t)(k0 + j) * lda];
+  } else {
+    // last, partial block: padded with the identity (clamped loads, selected afterwards)
+    const int ic = i < kb ? i : 0;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      const double a = A[(k0 + ic) + (int64_t)(k0 + (j < kb ? j : 0)) * lda];
+      r[j] = (i < kb && j < kb) ? (i >= j ? a : 0.0) : (i == j ? 1.0 : 0.0);
+    }
+  }
   int bad = 0;
 #pragma unroll
   for (int j = 0; j < CB; ++j) {
@@ -282,16 +296,24 @@ __device__ inline int chol_diag(const double *A, int64_t lda, int k0, int kb, do
       d = 1.0;
     }
     d = sqrt(d);
-    const double lij = (i > j) ? r[j] / d : (i == j ? d : 0.0);
+    const double rd = 1.0 / d;
+    int ii = i;
+    asm volatile("" : "+v"(ii));  // keeps the lane test here instead of 64 hoisted masks
+    const double lij = (ii == j) ? d : r[j] * rd;
     Ls[j * CB + i] = lij;
-    if (i == 0) dinv[j] = 1.0 / d;
+    dinv[j] = rd;  // same value from every lane
     wave_lds_sync();
-    // trailing update of row i; entries above the diagonal are never read as L values.
-    // The empty asm pins each updated value here: left to itself the compiler defers the
-    // updates to each entry's first use and keeps every broadcast column live, which
-    // spills.
+    // trailing update of row i.  The empty asms pin the column (so its LDS reads are
+    // issued together rather than one read and wait at a time) and each updated value
+    // (left to itself the compiler defers the updates to each entry's first use and keeps
+    // every broadcast column live, which spills).
+    double lc[CB];
 #pragma unroll
-    for (int l = j + 1; l < CB; ++l) r[l] -= lij * Ls[j * CB + l];
+    for (int l = j + 1; l < CB; ++l) lc[l] = Ls[j * CB + l];
+#pragma unroll
+    for (int l = j + 1; l < CB; ++l) asm volatile("" : "+v"(lc[l]));
+#pragma unroll
+    for (int l = j + 1; l < CB; ++l) r[l] -= lij * lc[l];
 #pragma unroll
     for (int l = j + 1; l < CB; ++l) asm volatile("" : "+v"(r[l]));
   }
@@ -316,6 +338,7 @@ __device__ inline void tile_ij(int b, int &I, int &J)
 // blocks I and J of the trailing window and applies C(I,J) -= P_I P_J^T to its tile of A.
 // L goes to Lb only (the diagonal block from workgroup 0, the panel rows from the diagonal
 // tiles), and the panel of A is only read, so no workgroup reads what another one writes.
+template <bool FULL>
 __global__ __launch_bounds__(256) void k_chol_step(double *A, double *Lb, int n, int k0, int kb,
                                                    int m, int *info)
 {
@@ -326,14 +349,14 @@ __global__ __launch_bounds__(256) void k_chol_step(double *A, double *Lb, int n,
   int I, J;
   tile_ij(blockIdx.x, I, J);
   if (w == 0) {
-    const int bad = chol_diag(A, n, k0, kb, Ls, dinv);
+    const int bad = chol_diag<FULL>(A, n, k0, kb, Ls, dinv);
     if (blockIdx.x == 0 && lane == 0 && bad != 0 && info[0] == 0) info[0] = bad;
   }
   __syncthreads();
   if (blockIdx.x == 0) {
     for (int e = t; e < CB * CB; e += 256) {
       const int r = e & 63, cc = e >> 6;
-      if (r < kb && cc < kb) Lb[(k0 + r) + (int64_t)(k0 + cc) * n] = Ls[cc * CB + r];
+      if (r < kb && cc < kb) Lb[(k0 + r) + (int64_t)(k0 + cc) * n] = r >= cc ? Ls[cc * CB + r] : 0.0;
     }
   }
   if (m <= 0) return;
@@ -343,15 +366,24 @@ __global__ __launch_bounds__(256) void k_chol_step(double *A, double *Lb, int n,
     const int s = t >> 6;
     const int row = 64 * (s ? J : I) + lane;  // row of the window
     const bool live = row < m;
+    const int rowc = live ? row : 0;
     double x[CB];
 #pragma unroll
-    for (int q = 0; q < CB; ++q) x[q] = live ? A[(w0 + row) + (int64_t)(k0 + q) * n] : 0.0;
+    for (int q = 0; q < CB; ++q) {
+      const double a = A[(w0 + rowc) + (int64_t)(k0 + q) * n];
+      x[q] = live ? a : 0.0;
+    }
     // x L11^T = a: column by column, each finished x[q] updates the later entries
 #pragma unroll
     for (int q = 0; q < CB; ++q) {
       x[q] *= dinv[q];
+      double lc[CB];  // pinned as in chol_diag
 #pragma unroll
-      for (int l = q + 1; l < CB; ++l) x[l] -= x[q] * Ls[q * CB + l];
+      for (int l = q + 1; l < CB; ++l) lc[l] = Ls[q * CB + l];
+#pragma unroll
+      for (int l = q + 1; l < CB; ++l) asm volatile("" : "+v"(lc[l]));
+#pragma unroll
+      for (int l = q + 1; l < CB; ++l) x[l] -= x[q] * lc[l];
 #pragma unroll
       for (int l = q + 1; l < CB; ++l) asm volatile("" : "+v"(x[l]));
     }
@@ -416,31 +448,65 @@ __global__ __launch_bounds__(256) void k_tri_inv(const double *Lb, double *X, in
 #pragma unroll
     for (int v = 0; v < CPW; ++v) T[v] = (grow == c0 + v) ? 1.0 : 0.0;
     const int jlo = (i - jb > kc) ? i - jb : kc;
+    const bool rok = grow < n;
+    const int64_t growc = rok ? grow : 0;
     for (int j = jlo; j < i; ++j) {
-      const double *Xj = ring[j % JB];
-      const double *Lrow = Lb + grow + (int64_t)CB * j * n;
-#pragma unroll 8
-      for (int p = 0; p < CB; ++p) {
-        const double l = (grow < n) ? Lrow[(int64_t)p * n] : 0.0;
+      const double *Xj = ring[j % JB] + CPW * w;
+      const double *Lrow = Lb + growc + (int64_t)CB * j * n;
+      double lv[CB];  // row of L(i,j): all 64 loads in flight at once
 #pragma unroll
-        for (int v = 0; v < CPW; ++v) T[v] -= l * Xj[p * CW + CPW * w + v];
+      for (int p = 0; p < CB; ++p) lv[p] = Lrow[(int64_t)p * n];
+#pragma unroll
+      for (int p0 = 0; p0 < CB; p0 += 16) {
+        // 16 rows of this wave's columns of X(j,kc), read together (pinned) from LDS
+        double xv[16][CPW];
+#pragma unroll
+        for (int p = 0; p < 16; ++p)
+#pragma unroll
+          for (int v = 0; v < CPW; ++v) xv[p][v] = Xj[(p0 + p) * CW + v];
+#pragma unroll
+        for (int p = 0; p < 16; ++p)
+#pragma unroll
+          for (int v = 0; v < CPW; ++v) asm volatile("" : "+v"(xv[p][v]));
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+          const double l = rok ? lv[p0 + p] : 0.0;
+#pragma unroll
+          for (int v = 0; v < CPW; ++v) T[v] -= l * xv[p][v];
+        }
       }
     }
     __syncthreads();  // previous step's readers of Ld are done
-    for (int e = t; e < CB * CB; e += 256) {
-      const int r = e & 63, cc = e >> 6;
+#pragma unroll
+    for (int e0 = 0; e0 < CB * CB; e0 += 256) {
+      const int e = e0 + t, r = e & 63, cc = e >> 6;
       const int64_t gr = (int64_t)CB * i + r, gc = (int64_t)CB * i + cc;
-      Ld[e] = (gr < n && gc < n) ? Lb[gr + gc * n] : (r == cc ? 1.0 : 0.0);
+      const bool ok = gr < n && gc < n;
+      const double v = Lb[(ok ? gr : 0) + (ok ? gc : 0) * n];
+      Ld[e] = ok ? v : (r == cc ? 1.0 : 0.0);
     }
     __syncthreads();
+    // Forward substitution with L(i,i) scaled to a unit diagonal row by row (lane r
+    // divides its row and right-hand side by L(r,r)), so no step divides: at step p lane p
+    // holds x_p, and the lanes below subtract L(r,p)/L(r,r) x_p.
+    const double rown = 1.0 / Ld[lane * CB + lane];
+    double lr[CB];
+#pragma unroll
+    for (int p = 0; p < CB; ++p) lr[p] = Ld[p * CB + lane];
+#pragma unroll
+    for (int p = 0; p < CB; ++p) asm volatile("" : "+v"(lr[p]));
+#pragma unroll
+    for (int v = 0; v < CPW; ++v) T[v] *= rown;
+#pragma unroll
     for (int p = 0; p < CB; ++p) {
-      const double rl = 1.0 / Ld[p * CB + p];
-      const double lrp = Ld[p * CB + lane];
+      int ii = lane;
+      asm volatile("" : "+v"(ii));  // keeps the lane test here instead of hoisted masks
+      const double lp = lr[p] * rown;
 #pragma unroll
       for (int v = 0; v < CPW; ++v) {
-        const double xp = lane_value(T[v], p) * rl;
-        if (lane == p) T[v] = xp;
-        else if (lane > p) T[v] -= lrp * xp;
+        const double xp = lane_value(T[v], p);
+        const double u = T[v] - lp * xp;
+        T[v] = (ii > p) ? u : T[v];
       }
     }
     double *Xi = ring[i % JB];
@@ -467,28 +533,52 @@ __global__ __launch_bounds__(256) void k_xtx(const double *X, double *Ai, int n,
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-  for (int K = I; K < nbk; ++K) {
-    __syncthreads();
-    for (int e = t; e < CB * CB; e += 256) {
-      const int r = e & 63, cc = e >> 6;
+  // the next K's tiles are loaded into registers while the current ones are used
+  constexpr int PT = CB * CB / 256;
+  double vi[PT], vj[PT];
+  auto load = [&](int K) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int e = 256 * k + t, r = e & 63, cc = e >> 6;
       const int64_t gr = (int64_t)CB * K + r;
       const int64_t gi = (int64_t)CB * I + cc, gj = (int64_t)CB * J + cc;
-      XI[r * (CB + 1) + cc] = (gr < n && gi < n) ? X[gr + gi * n] : 0.0;
-      XJ[r * (CB + 1) + cc] = (gr < n && gj < n) ? X[gr + gj * n] : 0.0;
+      const bool oki = gr < n && gi < n, okj = gr < n && gj < n;
+      const double a = X[(oki ? gr : 0) + (oki ? gi : 0) * n];
+      const double b = X[(okj ? gr : 0) + (okj ? gj : 0) * n];
+      vi[k] = oki ? a : 0.0;
+      vj[k] = okj ? b : 0.0;
+    }
+  };
+  load(I);
+  for (int K = I; K < nbk; ++K) {
+    __syncthreads();  // previous step's readers of XI / XJ are done
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int e = 256 * k + t, r = e & 63, cc = e >> 6;
+      XI[r * (CB + 1) + cc] = vi[k];
+      XJ[r * (CB + 1) + cc] = vj[k];
     }
     __syncthreads();
-#pragma unroll 4
-    for (int r = 0; r < CB; ++r) {
-      double a[4], b[4];
+    if (K + 1 < nbk) load(K + 1);
+    for (int r0 = 0; r0 < CB; r0 += 8) {
+      double a[8][4], b[8][4];  // 8 rows read together (pinned), then 128 FMAs
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a[u] = XI[r * (CB + 1) + ra + u];
-        b[u] = XJ[r * (CB + 1) + cb + u];
-      }
+      for (int r = 0; r < 8; ++r)
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 4; ++u) {
+          a[r][u] = XI[(r0 + r) * (CB + 1) + ra + u];
+          b[r][u] = XJ[(r0 + r) * (CB + 1) + cb + u];
+        }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) asm volatile("" : "+v"(a[r][u]), "+v"(b[r][u]));
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u][v] += a[r][u] * b[r][v];
     }
   }
 #pragma unroll
@@ -521,7 +611,10 @@ void coarse_factor_invert(hipStream_t s, double *A, double *Lb, double *X, int n
     const int m = std::min(n - k0 - kb, bw);
     const int nI = (m + CB - 1) / CB;
     const int tiles = m > 0 ? nI * (nI + 1) / 2 : 1;
-    k_chol_step<<<dim3(tiles), dim3(256), 0, s>>>(A, Lb, n, k0, kb, m, info);
+    if (kb == CB)
+      k_chol_step<true><<<dim3(tiles), dim3(256), 0, s>>>(A, Lb, n, k0, kb, m, info);
+    else
+      k_chol_step<false><<<dim3(tiles), dim3(256), 0, s>>>(A, Lb, n, k0, kb, m, info);
     TOFE_LAUNCH_CHECK();
   }
   if (jb < 8)

@@ -14,7 +14,6 @@
 #include <cstring>
 #include <exception>
 #include <memory>
-#include <thread>
 #include <vector>
 
 #include "host_setup.hpp"
@@ -85,7 +84,10 @@ struct tofe_ctx {
   hipStream_t stream = nullptr;
   // second stream: the coarsest-level factorisation runs beside the Lanczos estimates
   hipStream_t stream2 = nullptr;
-  hipEvent_t ev_dense = nullptr, ev_inv = nullptr;
+  // third stream: the Lanczos estimates of the coarser smoothed levels run beside the
+  // fine level's
+  hipStream_t stream3 = nullptr;
+  hipEvent_t ev_dense = nullptr, ev_inv = nullptr, ev_ops = nullptr, ev_lz = nullptr;
   double KE[576];
   std::vector<int> tpl_off;
   std::vector<double> tpl_w;
@@ -128,6 +130,7 @@ struct tofe_ctx {
   DBuf<double> chol_l, chol_x;
   int64_t nc = 0;
   DBuf<double> partial, partial2, scal, eig_ab;
+  DBuf<double> partial_b, partial2_b, scal_b;  // reductions of the third stream
   int npart = 0;
   std::vector<double> lam;
   std::vector<double> c1, c2;  // Chebyshev coefficients, per level * kmax
@@ -152,10 +155,12 @@ struct tofe_ctx {
   {
     if (stream) (void)hipStreamSynchronize(stream);
     if (stream2) (void)hipStreamSynchronize(stream2);
-    if (ev_dense) (void)hipEventDestroy(ev_dense);
-    if (ev_inv) (void)hipEventDestroy(ev_inv);
+    if (stream3) (void)hipStreamSynchronize(stream3);
+    for (hipEvent_t e : {ev_dense, ev_inv, ev_ops, ev_lz})
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
+    if (stream3) (void)hipStreamDestroy(stream3);
   }
 };
 
@@ -216,19 +221,21 @@ FineOp<float> fine_of<float>(tofe_ctx *c)
 }
 
 template <typename T>
-void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
+void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
+                 hipStream_t s = nullptr)
 {
+  if (!s) s = c->stream;
   if (l == 0) {
-    if (c->fine_mfma) fine_mfma<T>(c->stream, fine_of<T>(c), u, epi, ep);
-    else fine_op<T>(c->stream, fine_of<T>(c), u, epi, ep);
+    if (c->fine_mfma) fine_mfma<T>(s, fine_of<T>(c), u, epi, ep);
+    else fine_op<T>(s, fine_of<T>(c), u, epi, ep);
   } else {
     StencilOp<T> op;
     auto &lv = *levels<T>(c)[l];
     op.S = lv.S.p;
     op.Nx = lv.g.Nx; op.Ny = lv.g.Ny; op.Nz = lv.g.Nz;
     // FP32 levels keep the upper half of the (exactly symmetric) Galerkin stencil
-    if constexpr (sizeof(T) == 4) stencil_op_sym<T>(c->stream, op, u, epi, ep);
-    else stencil_op<T>(c->stream, op, u, epi, ep);
+    if constexpr (sizeof(T) == 4) stencil_op_sym<T>(s, op, u, epi, ep);
+    else stencil_op<T>(s, op, u, epi, ep);
   }
 }
 
@@ -429,12 +436,11 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
 // coefficients alpha_k, beta_k (recorded on the device, read back once per assemble)
 // form the Lanczos tridiagonal whose top eigenvalue is the estimate; PETSc's Chebyshev
 // smoother estimates the same quantity (KSPChebyshevEstEigSet 0, 0.1, 0, 1.1).
-void lanczos_coefficients(tofe_ctx *c, int l)
+void lanczos_coefficients(tofe_ctx *c, int l, hipStream_t s, double *r, double *p, double *q,
+                          double *part, double *part2, double *scal)
 {
-  const hipStream_t s = c->stream;
   Level<double> &lv = *c->lv64[l];
   const int64_t nd = lv.g.ndof();
-  double *r = c->vr.p, *p = c->vp.p, *q = c->vq.p;
   const int nb = level_blocks<double>(c, l);
   const int its = std::min(c->desc.eig_its, kMaxEig);
   double *alpha = c->eig_ab.p + (size_t)(2 * l) * kMaxEig;
@@ -443,25 +449,36 @@ void lanczos_coefficients(tofe_ctx *c, int l)
   // r.z partials, the p update; each consumer sums the producer's partials itself.  The
   // iterate x of the CG is never needed for the estimate and is not formed.
   power_start(s, r, nd);
-  jacobi_dot(s, r, lv.dinv.p, p, nd, c->partial.p);
-  reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RZA, POST_NONE, c->scal.p);
+  jacobi_dot(s, r, lv.dinv.p, p, nd, part);
+  reduce_final(s, part, kRedBlocks, scal + SL_RZA, POST_NONE, scal);
   for (int k = 0; k < its; ++k) {
     const int rz_old = (k & 1) ? SL_RZB : SL_RZA, rz_new = (k & 1) ? SL_RZA : SL_RZB;
     EpiArgs<double> ep;
     ep.y = q;
-    ep.partial = c->partial.p;
-    level_apply<double>(c, l, p, EPI_APPLY_DOT, ep);
-    lanczos_r(s, r, q, lv.dinv.p, nd, c->partial.p, nb, c->scal.p, rz_old, alpha + k,
-              c->partial2.p);
-    lanczos_p(s, p, r, lv.dinv.p, nd, c->partial2.p, c->scal.p, rz_old, rz_new, beta + k);
+    ep.partial = part;
+    level_apply<double>(c, l, p, EPI_APPLY_DOT, ep, s);
+    lanczos_r(s, r, q, lv.dinv.p, nd, part, nb, scal, rz_old, alpha + k, part2);
+    lanczos_p(s, p, r, lv.dinv.p, nd, part2, scal, rz_old, rz_new, beta + k);
   }
 }
 
+// The smoothed levels' estimates are independent: levels 1.. run in order on the third
+// stream (each on its own V-cycle work vectors) while the fine level runs on the main
+// stream; the main stream then waits for the third.
 void lanczos_enqueue(tofe_ctx *c)
 {
   const int L = c->L;
   c->eig_ab.alloc((size_t)2 * L * kMaxEig);
-  for (int l = 0; l + 1 < L; ++l) lanczos_coefficients(c, l);
+  TOFE_HIP(hipEventRecord(c->ev_ops, c->stream));
+  TOFE_HIP(hipStreamWaitEvent(c->stream3, c->ev_ops, 0));
+  for (int l = 1; l + 1 < L; ++l) {
+    Level<double> &lv = *c->lv64[l];
+    lanczos_coefficients(c, l, c->stream3, lv.r.p, lv.x.p, lv.x2.p, c->partial_b.p,
+                         c->partial2_b.p, c->scal_b.p);
+  }
+  if (L > 1)
+    lanczos_coefficients(c, 0, c->stream, c->vr.p, c->vp.p, c->vq.p, c->partial.p, c->partial2.p,
+                         c->scal.p);
 }
 
 void lanczos_lambdas(tofe_ctx *c)
@@ -576,34 +593,14 @@ void assemble(tofe_ctx *c)
   c->Ainv.alloc((size_t)c->nc * c->nc);
   c->dinfo.alloc(2);
   stencil_to_dense(s, SL, gL.Nx, gL.Ny, gL.Nz, c->Ainv.p);
-  // the coarse factorisation (one wavefront per diagonal block plus small GEMMs) runs on
-  // the second stream while the Lanczos estimates fill the device
   TOFE_HIP(hipEventRecord(c->ev_dense, s));
-  // The coarse factorisation is queued on the second stream by a helper host thread while
-  // this thread queues the Lanczos steps (Chebyshev bounds from lambda_max(D^-1 A) per
-  // smoothed level) on the main stream, so both streams fill the device at once.
-  std::exception_ptr coarse_err;
-  std::thread coarse([&] {
-    try {
-      TOFE_HIP(hipSetDevice(c->device));
-      TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
-      coarse_inverse_enqueue(c);
-      TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
-    } catch (...) {
-      coarse_err = std::current_exception();
-    }
-  });
-  try {
-    lanczos_enqueue(c);
-    lanczos_lambdas(c);
-  } catch (...) {
-    coarse.join();
-    throw;
-  }
-  coarse.join();
-  if (coarse_err) std::rethrow_exception(coarse_err);
-  chebyshev_coefficients(c);
-  // FP32 copies for the mixed-precision V-cycle
+  // Three streams from here: the coarsest level's factorisation and inverse on the second,
+  // the Lanczos estimates of levels 1.. and the FP32 copies on the third, the fine level's
+  // Lanczos estimate on the main stream, which then joins the other two.
+  TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
+  coarse_inverse_enqueue(c);
+  TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
+  lanczos_enqueue(c);
   if (c->precision == TOFE_PREC_MIXED) {
     if (c->lv32.empty()) {
       for (int l = 0; l < L; ++l) {
@@ -613,14 +610,19 @@ void assemble(tofe_ctx *c)
       c->vr32.alloc(c->ndof);
       c->vz32.alloc(c->ndof);
     }
+    const hipStream_t s3 = c->stream3;
     for (int l = 0; l < L; ++l) {
       const int64_t nd = c->grids[l].ndof();
-      convert<double, float>(s, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, nd);
+      convert<double, float>(s3, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, nd);
       if (l > 0)
-        stencil_upper32(s, c->lv64[l]->S.p, c->grids[l].nn(), c->lv32[l]->S.p);
+        stencil_upper32(s3, c->lv64[l]->S.p, c->grids[l].nn(), c->lv32[l]->S.p);
     }
-    convert<double, float>(s, c->E.p, c->E32.p, c->ne);
+    convert<double, float>(s3, c->E.p, c->E32.p, c->ne);
   }
+  TOFE_HIP(hipEventRecord(c->ev_lz, c->stream3));
+  TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz, 0));
+  lanczos_lambdas(c);
+  chebyshev_coefficients(c);
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_inv, 0));
   coarse_inverse_finish(c);
   TOFE_HIP(hipStreamSynchronize(s));
@@ -797,8 +799,9 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
 
     TOFE_HIP(hipStreamCreate(&c->stream));
     TOFE_HIP(hipStreamCreate(&c->stream2));
-    TOFE_HIP(hipEventCreateWithFlags(&c->ev_dense, hipEventDisableTiming));
-    TOFE_HIP(hipEventCreateWithFlags(&c->ev_inv, hipEventDisableTiming));
+    TOFE_HIP(hipStreamCreate(&c->stream3));
+    for (hipEvent_t *e : {&c->ev_dense, &c->ev_inv, &c->ev_ops, &c->ev_lz})
+      TOFE_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
 
     compute_ke(d.E0, d.nu, d.hx, d.hy, d.hz, c->KE);
     dmatrix(d.E0, d.nu, c->shape.D);
@@ -928,6 +931,10 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     c->partial2.alloc(np);
     c->scal.alloc(SL_COUNT);
     c->scal.zero(c->stream);
+    c->partial_b.alloc(np);
+    c->partial2_b.alloc(np);
+    c->scal_b.alloc(SL_COUNT);
+    c->scal_b.zero(c->stream);
     TOFE_HIP(hipStreamSynchronize(c->stream));
     c->stats.levels = c->L;
     c->stats.coarse_size = c->grids[c->L - 1].ndof();
