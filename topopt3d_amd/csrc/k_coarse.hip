@@ -235,6 +235,38 @@ __global__ __launch_bounds__(NT) void k_gemv(const double *__restrict__ A, int64
   if (lane == 0) y[row] = (Tout)s;
 }
 
+// FP32 copy of the symmetric inverse, rows padded to ld floats (multiple of 256)
+__global__ void k_pad32(const double *__restrict__ A, int n, int ld, float *__restrict__ B)
+{
+  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (t >= (int64_t)n * ld) return;
+  const int64_t row = t / ld, col = t - row * ld;
+  B[t] = col < n ? (float)A[col + row * (int64_t)n] : 0.0f;
+}
+
+// y = A x with the padded FP32 inverse: four rows per block (one per wavefront), x staged
+// in LDS, 16-byte loads along the row, products summed in FP64 in a fixed order.
+__global__ __launch_bounds__(NT) void k_gemv32(const float *__restrict__ A, int n, int ld,
+                                               const float *__restrict__ x, float *__restrict__ y)
+{
+  extern __shared__ float xs[];
+  for (int i = threadIdx.x; i < ld; i += NT) xs[i] = i < n ? x[i] : 0.0f;
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const float4 *a = reinterpret_cast<const float4 *>(A + row * ld);
+  const float4 *xv = reinterpret_cast<const float4 *>(xs);
+  double s = 0;
+#pragma unroll 8
+  for (int c = lane; c < ld / 4; c += 64) {
+    const float4 v = a[c], w = xv[c];
+    s += (double)(v.x * w.x) + (double)(v.y * w.y) + (double)(v.z * w.z) + (double)(v.w * w.w);
+  }
+  s = wave_sum(s);
+  if (lane == 0) y[row] = (float)s;
+}
+
 // ---- dense coarse solve: banded blocked Cholesky, X = L^-1, A^-1 = X^T X ------------
 // The coarse matrix is n x n, column-major, SPD, and couples DOFs at most bw apart, so
 // the Cholesky factor L stays inside that band.  Tiles are 64 x 64 (kCholNB).
@@ -716,6 +748,18 @@ template <typename Tin, typename Tout>
 void dense_gemv(hipStream_t s, const double *A, int64_t n, const Tin *x, Tout *y)
 {
   k_gemv<Tin, Tout><<<dim3((unsigned)((n + 3) / 4)), dim3(NT), 0, s>>>(A, n, x, y);
+  TOFE_LAUNCH_CHECK();
+}
+
+void pad32(hipStream_t s, const double *A, int n, int ld, float *B)
+{
+  k_pad32<<<dim3((unsigned)(((int64_t)n * ld + NT - 1) / NT)), dim3(NT), 0, s>>>(A, n, ld, B);
+  TOFE_LAUNCH_CHECK();
+}
+
+void dense_gemv32(hipStream_t s, const float *A, int n, int ld, const float *x, float *y)
+{
+  k_gemv32<<<dim3((unsigned)((n + 3) / 4)), dim3(NT), sizeof(float) * ld, s>>>(A, n, ld, x, y);
   TOFE_LAUNCH_CHECK();
 }
 

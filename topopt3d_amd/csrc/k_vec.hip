@@ -211,6 +211,53 @@ __device__ inline double all_sum(const double *__restrict__ partial, int nparts,
   return t;
 }
 
+// Flexible-CG step without separate reduction launches, first half: alpha = p.r / p.Ap
+// from the operator's two sets of partials (block 0 stores p.r, p.Ap and alpha in scal);
+// x += alpha p, r -= alpha q, partials of r.r.
+__global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ x, double *__restrict__ r,
+                                               const double *__restrict__ p,
+                                               const double *__restrict__ q, int64_t n,
+                                               const double *__restrict__ pq_part,
+                                               const double *__restrict__ pr_part, int nparts,
+                                               double *scal, double *partial)
+{
+  __shared__ double sred[NT / 64];
+  const double pr = all_sum(pr_part, nparts, sred);
+  const double pq = all_sum(pq_part, nparts, sred);
+  const double alpha = pr / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[SL_PR] = pr;
+    scal[SL_PQ] = pq;
+    scal[SL_ALPHA] = alpha;
+  }
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  const double t = block_sum<NT>(s, sred);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// Second half: beta = -z.Ap / p.Ap from the z.q partials (block 0 stores z.Ap and beta);
+// p = z + beta p.
+__global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const double *__restrict__ z,
+                                              int64_t n, const double *__restrict__ zq_part,
+                                              double *scal)
+{
+  __shared__ double sred[NT / 64];
+  const double zq = all_sum(zq_part, kRedBlocks, sred);
+  const double beta = -zq / scal[SL_PQ];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[SL_ZQ] = zq;
+    scal[SL_BETA] = beta;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+    p[i] = z[i] + beta * p[i];
+}
+
 // Jacobi-CG step of the eigenvalue estimate, first half: alpha = rz_old / p.Ap from the
 // operator's partials; r -= alpha q; partial r.(D^-1 r).  Block 0 records alpha.
 __global__ __launch_bounds__(NT) void k_lz_r(double *__restrict__ r, const double *__restrict__ q,
@@ -345,6 +392,22 @@ void jacobi_dot(hipStream_t s, const double *r, const double *dinv, double *z, i
 void record_scalar(hipStream_t s, const double *scal, int slot, double *dst)
 {
   k_record<<<dim3(1), dim3(64), 0, s>>>(scal, slot, dst);
+  TOFE_LAUNCH_CHECK();
+}
+
+void fcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
+                   int64_t n, const double *pq_part, const double *pr_part, int nparts,
+                   double *scal, double *partial)
+{
+  k_fcg_xr<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(x, r, p, q, n, pq_part, pr_part, nparts, scal,
+                                                 partial);
+  TOFE_LAUNCH_CHECK();
+}
+
+void fcg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *zq_part,
+                  double *scal)
+{
+  k_fcg_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, n, zq_part, scal);
   TOFE_LAUNCH_CHECK();
 }
 

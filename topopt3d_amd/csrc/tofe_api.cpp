@@ -88,6 +88,8 @@ struct tofe_ctx {
   // fine level's
   hipStream_t stream3 = nullptr;
   hipEvent_t ev_dense = nullptr, ev_inv = nullptr, ev_ops = nullptr, ev_lz = nullptr;
+  hipEvent_t ev_scal = nullptr;
+  double *hscal = nullptr;  // pinned host copy of the device scalars
   double KE[576];
   std::vector<int> tpl_off;
   std::vector<double> tpl_w;
@@ -126,10 +128,12 @@ struct tofe_ctx {
   int n_masked = 0;
   DBuf<double> gal_t1, gal_t2;  // axis-by-axis Galerkin scratch
   DBuf<double> Ainv;
+  DBuf<float> Ainv32;  // FP32 copy for the mixed-precision V-cycle, rows padded to ld32
+  int ld32 = 0;
   DBuf<int> dinfo;
   DBuf<double> chol_l, chol_x;
   int64_t nc = 0;
-  DBuf<double> partial, partial2, scal, eig_ab;
+  DBuf<double> partial, partial2, partial3, scal, eig_ab;
   DBuf<double> partial_b, partial2_b, scal_b;  // reductions of the third stream
   int npart = 0;
   std::vector<double> lam;
@@ -156,8 +160,9 @@ struct tofe_ctx {
     if (stream) (void)hipStreamSynchronize(stream);
     if (stream2) (void)hipStreamSynchronize(stream2);
     if (stream3) (void)hipStreamSynchronize(stream3);
-    for (hipEvent_t e : {ev_dense, ev_inv, ev_ops, ev_lz})
+    for (hipEvent_t e : {ev_dense, ev_inv, ev_ops, ev_lz, ev_scal})
       if (e) (void)hipEventDestroy(e);
+    if (hscal) (void)hipHostFree(hscal);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream3) (void)hipStreamDestroy(stream3);
@@ -276,6 +281,14 @@ void alloc_level32(tofe_ctx *c, Level<float> &lv, int l)
   lv.d.zero(c->stream);
 }
 
+// Exact solve with the coarsest operator: its explicit inverse, in the V-cycle's precision.
+template <typename T>
+void coarse_apply(tofe_ctx *c, const T *b, T *x)
+{
+  if constexpr (sizeof(T) == 4) dense_gemv32(c->stream, c->Ainv32.p, (int)c->nc, c->ld32, b, x);
+  else dense_gemv<double, double>(c->stream, c->Ainv.p, c->nc, b, x);
+}
+
 // Multigrid V-cycle: x0 = M^-1 b0 (PCMG V-cycle, Chebyshev-Jacobi smoothers,
 // Galerkin coarse operators, direct coarse solve).
 template <typename T>
@@ -285,7 +298,7 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0)
   const int L = c->L;
   const hipStream_t s = c->stream;
   if (L == 1) {
-    dense_gemv<T, T>(s, c->Ainv.p, c->nc, b0, x0);
+    coarse_apply<T>(c, b0, x0);
     return;
   }
   const int nu1 = c->desc.nu_pre, nu2 = c->desc.nu_post, km = c->kmax();
@@ -319,7 +332,7 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0)
     oth[l] = xo;
   }
   Level<T> &lc = *LV[L - 1];
-  dense_gemv<T, T>(s, c->Ainv.p, c->nc, lc.b.p, lc.x.p);
+  coarse_apply<T>(c, lc.b.p, lc.x.p);
   cur[L - 1] = lc.x.p;
   for (int l = L - 2; l >= 0; --l) {
     Level<T> &lv = *LV[l];
@@ -357,10 +370,25 @@ void dot_into(tofe_ctx *c, const double *a, const double *b, int64_t n, int slot
   reduce_final(c->stream, c->partial.p, kRedBlocks, c->scal.p + slot, post, c->scal.p);
 }
 
+// Device scalars to the host through the pinned buffer: queue the copy and an event ...
+void scal_to_host_async(tofe_ctx *c)
+{
+  TOFE_HIP(hipMemcpyAsync(c->hscal, c->scal.p, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  TOFE_HIP(hipEventRecord(c->ev_scal, c->stream));
+}
+
+// ... and wait for them.
+void scal_wait(tofe_ctx *c, double *h)
+{
+  TOFE_HIP(hipEventSynchronize(c->ev_scal));
+  std::memcpy(h, c->hscal, SL_COUNT * sizeof(double));
+}
+
 void read_scal(tofe_ctx *c, double *h)
 {
-  TOFE_HIP(hipMemcpyAsync(h, c->scal.p, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  TOFE_HIP(hipStreamSynchronize(c->stream));
+  scal_to_host_async(c);
+  scal_wait(c, h);
 }
 
 // CG / flexible CG on the fine level with the KSPConvergedDefault test
@@ -395,6 +423,21 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
     dot_into(c, r, z, n, SL_RZ);
     TOFE_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
+  // The next preconditioner application and search direction depend on the device
+  // scalars only, so they are queued before the host reads this step's residual norm
+  // whenever the step is predicted (from the last contraction rate) not to converge: the
+  // device then never waits for the host's test.  A converged step leaves them unused.
+  double rn_prev = rn, rate = 0.5;
+  auto next_direction = [&] {
+    precondition(c, r, z);
+    if (flex) {
+      dot_partial<double>(s, z, q, n, c->partial.p);
+      fcg_update_p(s, p, z, n, c->partial.p, c->scal.p);
+    } else {
+      dot_into(c, r, z, n, SL_TMP, POST_BETA_CG);
+      cg_update_p(s, p, z, n, c->scal.p, SL_BETA);
+    }
+  };
   while (!reason) {
     if (k >= maxit) { reason = TOFE_DIVERGED_ITS; break; }
     EpiArgs<double> ep;
@@ -403,15 +446,18 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
       ep.b = r;
       ep.partial2 = c->partial2.p;
       level_apply<double>(c, 0, p, EPI_APPLY_DOT2, ep);
-      reduce_final(s, c->partial2.p, nfb, c->scal.p + SL_PR, POST_NONE, c->scal.p);
-      reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_FCG, c->scal.p);
+      fcg_update_xr(s, U, r, p, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p);
+      reduce_final(s, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p);
     } else {
       level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
       reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p);
+      cg_update_xr(s, U, r, p, q, n, c->scal.p, SL_ALPHA, c->partial.p);
+      reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p);
     }
-    cg_update_xr(s, U, r, p, q, n, c->scal.p, SL_ALPHA, c->partial.p);
-    reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p);
-    read_scal(c, h);
+    scal_to_host_async(c);
+    const bool ahead = k + 1 < maxit && rn_prev * std::min(rate, 1.0) > 3 * ttol;
+    if (ahead) next_direction();
+    scal_wait(c, h);
     ++k;
     if (!flex && h[SL_RZ] < 0) { reason = TOFE_DIVERGED_INDEFINITE_PC; break; }
     if (!(h[SL_PQ] > 0)) {
@@ -422,10 +468,9 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
     if (std::isnan(rn) || std::isinf(rn)) { reason = TOFE_DIVERGED_NANORINF; break; }
     if (rn <= ttol) { reason = conv; break; }
     if (rn >= 1e5 * rnorm0) { reason = TOFE_DIVERGED_DTOL; break; }
-    precondition(c, r, z);
-    if (flex) dot_into(c, z, q, n, SL_ZQ, POST_BETA_FCG);
-    else dot_into(c, r, z, n, SL_TMP, POST_BETA_CG);
-    cg_update_p(s, p, z, n, c->scal.p, SL_BETA);
+    if (rn_prev > 0) rate = rn / rn_prev;
+    rn_prev = rn;
+    if (!ahead) next_direction();
   }
   *its_out = k;
   *rn_out = rn;
@@ -541,6 +586,11 @@ void coarse_inverse_enqueue(tofe_ctx *c)
   const int bw = 3 * (gL.Nx * gL.Ny + gL.Nx + 1) + 2;
   // a failed factorisation is reported by coarse_inverse_finish; the inverse is then unused
   coarse_factor_invert(s, c->Ainv.p, c->chol_l.p, c->chol_x.p, n, bw, c->dinfo.p);
+  if (c->precision == TOFE_PREC_MIXED) {
+    c->ld32 = (n + 255) / 256 * 256;
+    c->Ainv32.alloc((size_t)n * c->ld32);
+    pad32(s, c->Ainv.p, n, c->ld32, c->Ainv32.p);
+  }
 }
 
 void coarse_inverse_finish(tofe_ctx *c)
@@ -800,8 +850,9 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     TOFE_HIP(hipStreamCreate(&c->stream));
     TOFE_HIP(hipStreamCreate(&c->stream2));
     TOFE_HIP(hipStreamCreate(&c->stream3));
-    for (hipEvent_t *e : {&c->ev_dense, &c->ev_inv, &c->ev_ops, &c->ev_lz})
+    for (hipEvent_t *e : {&c->ev_dense, &c->ev_inv, &c->ev_ops, &c->ev_lz, &c->ev_scal})
       TOFE_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    TOFE_HIP(hipHostMalloc((void **)&c->hscal, SL_COUNT * sizeof(double), hipHostMallocDefault));
 
     compute_ke(d.E0, d.nu, d.hx, d.hy, d.hz, c->KE);
     dmatrix(d.E0, d.nu, c->shape.D);
@@ -929,6 +980,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     c->npart = np;
     c->partial.alloc(np);
     c->partial2.alloc(np);
+    c->partial3.alloc(np);
     c->scal.alloc(SL_COUNT);
     c->scal.zero(c->stream);
     c->partial_b.alloc(np);

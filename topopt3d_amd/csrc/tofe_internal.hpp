@@ -109,6 +109,10 @@ void coarse_factor_invert(hipStream_t s, double *A, double *Lb, double *X, int n
                           int *info);
 template <typename Tin, typename Tout>
 void dense_gemv(hipStream_t s, const double *A, int64_t n, const Tin *x, Tout *y);
+// FP32 copy of the n x n symmetric coarse inverse with rows padded to ld (multiple of 256)
+// floats, and y = A x with it.
+void pad32(hipStream_t s, const double *A, int n, int ld, float *B);
+void dense_gemv32(hipStream_t s, const float *A, int n, int ld, const float *x, float *y);
 
 // ---- level-1 Galerkin operator from E and KE (k_galerkin.hip) -------------
 struct MaskedElems {
@@ -147,6 +151,15 @@ void power_start(hipStream_t s, double *v, int64_t n);
 void jacobi_dot(hipStream_t s, const double *r, const double *dinv, double *z, int64_t n,
                 double *partial);
 void record_scalar(hipStream_t s, const double *scal, int slot, double *dst);
+// Flexible-CG updates that sum the producers' partials themselves: fcg_update_xr forms
+// alpha = sum(pr_part) / sum(pq_part) (stored with both sums in scal), x += alpha p,
+// r -= alpha q, partials of r.r; fcg_update_p forms beta = -sum(zq_part) / scal[PQ] and
+// p = z + beta p.
+void fcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
+                   int64_t n, const double *pq_part, const double *pr_part, int nparts,
+                   double *scal, double *partial);
+void fcg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *zq_part,
+                  double *scal);
 // Fused Jacobi-CG steps of the eigenvalue estimate (no separate reduction launches):
 // lanczos_r: alpha = scal[rz_slot] / sum(pq_part[0..npq)), recorded; r -= alpha q; partials
 // of r.(D^-1 r).  lanczos_p: beta = sum(rz_part) / scal[rz_old], recorded; scal[rz_new] =
