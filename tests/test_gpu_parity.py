@@ -106,6 +106,19 @@ def test_galerkin_levels_match_ptap(oracle, tofe, nel, L):
     ctx.close()
 
 
+@pytest.mark.parametrize("nel,L", [((16, 8, 8), 3), ((48, 24, 24), 4)])
+def test_mixed_lambdas_from_fp32_operators(oracle, tofe, nel, L):
+    """The mixed-precision build estimates lambda_max(D^-1 A) with the same Jacobi-CG steps
+    on its FP32 operators: within FP32 rounding of the FP64 (oracle-matched) estimates."""
+    pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, L)
+    mg = oracle.MG(K, tuple(n + 1 for n in nel), L)
+    ctx.set_precision(tofe.PREC_MIXED)
+    ctx.assemble()
+    for l in range(L - 1):
+        assert ctx.level_lambda(l) == pytest.approx(mg.lam(l), rel=1e-4)
+    ctx.close()
+
+
 @pytest.mark.parametrize("nel,L", CASES)
 def test_solve_compliance_sensitivity(oracle, tofe, nel, L):
     pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, L)

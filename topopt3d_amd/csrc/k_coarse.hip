@@ -319,16 +319,20 @@ __device__ inline int chol_diag(const double *A, int64_t lda, int k0, int kb, do
       r[j] = (i < kb && j < kb) ? (i >= j ? a : 0.0) : (i == j ? 1.0 : 0.0);
     }
   }
+  // Software-pipelined: the pivot, square root and reciprocal of column j+1 are formed
+  // right after entry j+1 is updated, ahead of the rest of column j's update, so their
+  // latency overlaps that work.  A non-positive pivot is replaced by 1 (and reported).
   int bad = 0;
+  double d, rd;
+  auto pivot = [&](double p, int j) {
+    const bool ok = p > 0;
+    bad = (bad == 0 && !ok) ? k0 + j + 1 : bad;
+    d = sqrt(ok ? p : 1.0);
+    rd = 1.0 / d;
+  };
+  pivot(lane_value(r[0], 0), 0);
 #pragma unroll
   for (int j = 0; j < CB; ++j) {
-    double d = lane_value(r[j], j);
-    if (!(d > 0)) {
-      if (bad == 0) bad = k0 + j + 1;
-      d = 1.0;
-    }
-    d = sqrt(d);
-    const double rd = 1.0 / d;
     int ii = i;
     asm volatile("" : "+v"(ii));  // keeps the lane test here instead of 64 hoisted masks
     const double lij = (ii == j) ? d : r[j] * rd;
@@ -344,10 +348,15 @@ __device__ inline int chol_diag(const double *A, int64_t lda, int k0, int kb, do
     for (int l = j + 1; l < CB; ++l) lc[l] = Ls[j * CB + l];
 #pragma unroll
     for (int l = j + 1; l < CB; ++l) asm volatile("" : "+v"(lc[l]));
+    if (j + 1 < CB) {
+      r[j + 1] -= lij * lc[j + 1];
+      asm volatile("" : "+v"(r[j + 1]));
+      pivot(lane_value(r[j + 1], j + 1), j + 1);
+    }
 #pragma unroll
-    for (int l = j + 1; l < CB; ++l) r[l] -= lij * lc[l];
+    for (int l = j + 2; l < CB; ++l) r[l] -= lij * lc[l];
 #pragma unroll
-    for (int l = j + 1; l < CB; ++l) asm volatile("" : "+v"(r[l]));
+    for (int l = j + 2; l < CB; ++l) asm volatile("" : "+v"(r[l]));
   }
   return bad;
 }

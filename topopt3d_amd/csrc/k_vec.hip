@@ -164,25 +164,26 @@ __global__ void k_cg_p(double *__restrict__ p, const double *__restrict__ z, int
 }
 
 // v_i = 0.5 + frac(i * 2654435761 / 2^32): deterministic start vector of the estimate
-__global__ void k_power_start(double *v, int64_t n)
+template <typename T>
+__global__ void k_power_start(T *v, int64_t n)
 {
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
     const uint32_t h = (uint32_t)((uint64_t)i * 2654435761ull);
-    v[i] = 0.5 + (double)h / 4294967296.0;
+    v[i] = (T)(0.5 + (double)h / 4294967296.0);
   }
 }
 
 // z = dinv r, partial r.z (Jacobi-preconditioned CG of the eigenvalue estimate)
-__global__ __launch_bounds__(NT) void k_jacobi_dot(const double *__restrict__ r,
-                                                   const double *__restrict__ dinv, double *z,
-                                                   int64_t n, double *partial)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_jacobi_dot(const T *__restrict__ r, const T *__restrict__ dinv,
+                                                   T *z, int64_t n, double *partial)
 {
   __shared__ double sred[NT / 64];
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-    const double zi = dinv[i] * r[i];
+    const T zi = dinv[i] * r[i];
     z[i] = zi;
-    s += r[i] * zi;
+    s += (double)r[i] * (double)zi;
   }
   const double t = block_sum<NT>(s, sred);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -260,8 +261,9 @@ __global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const doub
 
 // Jacobi-CG step of the eigenvalue estimate, first half: alpha = rz_old / p.Ap from the
 // operator's partials; r -= alpha q; partial r.(D^-1 r).  Block 0 records alpha.
-__global__ __launch_bounds__(NT) void k_lz_r(double *__restrict__ r, const double *__restrict__ q,
-                                             const double *__restrict__ dinv, int64_t n,
+template <typename T>
+__global__ __launch_bounds__(NT) void k_lz_r(T *__restrict__ r, const T *__restrict__ q,
+                                             const T *__restrict__ dinv, int64_t n,
                                              const double *__restrict__ pq_part, int npq,
                                              const double *__restrict__ scal, int rz_slot,
                                              double *rec_alpha, double *partial)
@@ -269,12 +271,13 @@ __global__ __launch_bounds__(NT) void k_lz_r(double *__restrict__ r, const doubl
   __shared__ double sred[NT / 64];
   const double alpha = scal[rz_slot] / all_sum(pq_part, npq, sred);
   if (blockIdx.x == 0 && threadIdx.x == 0) *rec_alpha = alpha;
+  const T a = (T)alpha;
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-    const double ri = r[i] - alpha * q[i];
+    const T ri = r[i] - a * q[i];
     r[i] = ri;
-    const double zi = dinv[i] * ri;
-    s += ri * zi;
+    const T zi = dinv[i] * ri;
+    s += (double)ri * (double)zi;
   }
   const double t = block_sum<NT>(s, sred);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -282,8 +285,9 @@ __global__ __launch_bounds__(NT) void k_lz_r(double *__restrict__ r, const doubl
 
 // Second half: rz = sum of the partials, beta = rz / rz_old, p = D^-1 r + beta p.  Block 0
 // records beta and stores rz in the other slot for the next step.
-__global__ __launch_bounds__(NT) void k_lz_p(double *__restrict__ p, const double *__restrict__ r,
-                                             const double *__restrict__ dinv, int64_t n,
+template <typename T>
+__global__ __launch_bounds__(NT) void k_lz_p(T *__restrict__ p, const T *__restrict__ r,
+                                             const T *__restrict__ dinv, int64_t n,
                                              const double *__restrict__ rz_part, int nrz,
                                              double *scal, int rz_old, int rz_new, double *rec_beta)
 {
@@ -294,9 +298,10 @@ __global__ __launch_bounds__(NT) void k_lz_p(double *__restrict__ p, const doubl
     *rec_beta = beta;
     scal[rz_new] = rz;
   }
+  const T b = (T)beta;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-    const double zi = dinv[i] * r[i];
-    p[i] = zi + beta * p[i];
+    const T zi = dinv[i] * r[i];
+    p[i] = zi + b * p[i];
   }
 }
 
@@ -376,16 +381,17 @@ void cg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const dou
   TOFE_LAUNCH_CHECK();
 }
 
-void power_start(hipStream_t s, double *v, int64_t n)
+template <typename T>
+void power_start(hipStream_t s, T *v, int64_t n)
 {
-  k_power_start<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(v, n);
+  k_power_start<T><<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(v, n);
   TOFE_LAUNCH_CHECK();
 }
 
-void jacobi_dot(hipStream_t s, const double *r, const double *dinv, double *z, int64_t n,
-                double *partial)
+template <typename T>
+void jacobi_dot(hipStream_t s, const T *r, const T *dinv, T *z, int64_t n, double *partial)
 {
-  k_jacobi_dot<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, dinv, z, n, partial);
+  k_jacobi_dot<T><<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, dinv, z, n, partial);
   TOFE_LAUNCH_CHECK();
 }
 
@@ -411,19 +417,20 @@ void fcg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const do
   TOFE_LAUNCH_CHECK();
 }
 
-void lanczos_r(hipStream_t s, double *r, const double *q, const double *dinv, int64_t n,
-               const double *pq_part, int npq, const double *scal, int rz_slot, double *rec_alpha,
-               double *partial)
+template <typename T>
+void lanczos_r(hipStream_t s, T *r, const T *q, const T *dinv, int64_t n, const double *pq_part,
+               int npq, const double *scal, int rz_slot, double *rec_alpha, double *partial)
 {
-  k_lz_r<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, q, dinv, n, pq_part, npq, scal, rz_slot,
+  k_lz_r<T><<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, q, dinv, n, pq_part, npq, scal, rz_slot,
                                                rec_alpha, partial);
   TOFE_LAUNCH_CHECK();
 }
 
-void lanczos_p(hipStream_t s, double *p, const double *r, const double *dinv, int64_t n,
-               const double *rz_part, double *scal, int rz_old, int rz_new, double *rec_beta)
+template <typename T>
+void lanczos_p(hipStream_t s, T *p, const T *r, const T *dinv, int64_t n, const double *rz_part,
+               double *scal, int rz_old, int rz_new, double *rec_beta)
 {
-  k_lz_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, r, dinv, n, rz_part, kRedBlocks, scal,
+  k_lz_p<T><<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, r, dinv, n, rz_part, kRedBlocks, scal,
                                                      rz_old, rz_new, rec_beta);
   TOFE_LAUNCH_CHECK();
 }
@@ -436,6 +443,15 @@ void lanczos_p(hipStream_t s, double *p, const double *r, const double *dinv, in
 
 TOFE_INST_T(double)
 TOFE_INST_T(float)
+#define TOFE_INST_LZ(T)                                                                      \
+  template void power_start<T>(hipStream_t, T *, int64_t);                                  \
+  template void jacobi_dot<T>(hipStream_t, const T *, const T *, T *, int64_t, double *);   \
+  template void lanczos_r<T>(hipStream_t, T *, const T *, const T *, int64_t, const double *, \
+                             int, const double *, int, double *, double *);                 \
+  template void lanczos_p<T>(hipStream_t, T *, const T *, const T *, int64_t, const double *, \
+                             double *, int, int, double *);
+TOFE_INST_LZ(double)
+TOFE_INST_LZ(float)
 template void convert<double, float>(hipStream_t, const double *, float *, int64_t);
 template void convert<float, double>(hipStream_t, const float *, double *, int64_t);
 

@@ -67,6 +67,7 @@ struct Level {
 };
 
 constexpr int kMaxEig = 64;
+constexpr int kCoarseCUs = 32;  // compute units left to the coarse factorisation during set-up
 
 double now_s()
 {
@@ -87,8 +88,12 @@ struct tofe_ctx {
   // third stream: the Lanczos estimates of the coarser smoothed levels run beside the
   // fine level's
   hipStream_t stream3 = nullptr;
+  // fourth stream: the fine level's Lanczos estimate.  The third and fourth streams are
+  // kept off the first kCoarseCUs compute units, which the latency-bound coarse
+  // factorisation on the second stream then has to itself.
+  hipStream_t stream4 = nullptr;
   hipEvent_t ev_dense = nullptr, ev_inv = nullptr, ev_ops = nullptr, ev_lz = nullptr;
-  hipEvent_t ev_scal = nullptr;
+  hipEvent_t ev_scal = nullptr, ev_lz0 = nullptr;
   double *hscal = nullptr;  // pinned host copy of the device scalars
   double KE[576];
   std::vector<int> tpl_off;
@@ -160,12 +165,14 @@ struct tofe_ctx {
     if (stream) (void)hipStreamSynchronize(stream);
     if (stream2) (void)hipStreamSynchronize(stream2);
     if (stream3) (void)hipStreamSynchronize(stream3);
-    for (hipEvent_t e : {ev_dense, ev_inv, ev_ops, ev_lz, ev_scal})
+    if (stream4) (void)hipStreamSynchronize(stream4);
+    for (hipEvent_t e : {ev_dense, ev_inv, ev_ops, ev_lz, ev_scal, ev_lz0})
       if (e) (void)hipEventDestroy(e);
     if (hscal) (void)hipHostFree(hscal);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream3) (void)hipStreamDestroy(stream3);
+    if (stream4) (void)hipStreamDestroy(stream4);
   }
 };
 
@@ -481,49 +488,84 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
 // coefficients alpha_k, beta_k (recorded on the device, read back once per assemble)
 // form the Lanczos tridiagonal whose top eigenvalue is the estimate; PETSc's Chebyshev
 // smoother estimates the same quantity (KSPChebyshevEstEigSet 0, 0.1, 0, 1.1).
-void lanczos_coefficients(tofe_ctx *c, int l, hipStream_t s, double *r, double *p, double *q,
-                          double *part, double *part2, double *scal)
+template <typename T>
+void lanczos_coefficients(tofe_ctx *c, int l, hipStream_t s, T *r, T *p, T *q, double *part,
+                          double *part2, double *scal)
 {
-  Level<double> &lv = *c->lv64[l];
+  Level<T> &lv = *levels<T>(c)[l];
   const int64_t nd = lv.g.ndof();
-  const int nb = level_blocks<double>(c, l);
+  const int nb = level_blocks<T>(c, l);
   const int its = std::min(c->desc.eig_its, kMaxEig);
   double *alpha = c->eig_ab.p + (size_t)(2 * l) * kMaxEig;
   double *beta = c->eig_ab.p + (size_t)(2 * l + 1) * kMaxEig;
   // Three launches per step: the operator with its p.Ap partials, the r update with the
   // r.z partials, the p update; each consumer sums the producer's partials itself.  The
   // iterate x of the CG is never needed for the estimate and is not formed.
-  power_start(s, r, nd);
-  jacobi_dot(s, r, lv.dinv.p, p, nd, part);
+  power_start<T>(s, r, nd);
+  jacobi_dot<T>(s, r, lv.dinv.p, p, nd, part);
   reduce_final(s, part, kRedBlocks, scal + SL_RZA, POST_NONE, scal);
   for (int k = 0; k < its; ++k) {
     const int rz_old = (k & 1) ? SL_RZB : SL_RZA, rz_new = (k & 1) ? SL_RZA : SL_RZB;
-    EpiArgs<double> ep;
+    EpiArgs<T> ep;
     ep.y = q;
     ep.partial = part;
-    level_apply<double>(c, l, p, EPI_APPLY_DOT, ep, s);
-    lanczos_r(s, r, q, lv.dinv.p, nd, part, nb, scal, rz_old, alpha + k, part2);
-    lanczos_p(s, p, r, lv.dinv.p, nd, part2, scal, rz_old, rz_new, beta + k);
+    level_apply<T>(c, l, p, EPI_APPLY_DOT, ep, s);
+    lanczos_r<T>(s, r, q, lv.dinv.p, nd, part, nb, scal, rz_old, alpha + k, part2);
+    lanczos_p<T>(s, p, r, lv.dinv.p, nd, part2, scal, rz_old, rz_new, beta + k);
   }
 }
 
 // The smoothed levels' estimates are independent: levels 1.. run in order on the third
-// stream (each on its own V-cycle work vectors) while the fine level runs on the main
-// stream; the main stream then waits for the third.
+// stream and the fine level on the fourth, each on its level's V-cycle work vectors, once
+// the operators are built on the main stream.  In the mixed-precision build they run on
+// the FP32 operators of the V-cycle (scalars and recorded coefficients stay FP64), whose
+// copies are made on the same streams first.  ev_lz / ev_lz0 mark their ends.
 void lanczos_enqueue(tofe_ctx *c)
 {
   const int L = c->L;
+  const hipStream_t s3 = c->stream3, s4 = c->stream4;
   c->eig_ab.alloc((size_t)2 * L * kMaxEig);
   TOFE_HIP(hipEventRecord(c->ev_ops, c->stream));
-  TOFE_HIP(hipStreamWaitEvent(c->stream3, c->ev_ops, 0));
-  for (int l = 1; l + 1 < L; ++l) {
-    Level<double> &lv = *c->lv64[l];
-    lanczos_coefficients(c, l, c->stream3, lv.r.p, lv.x.p, lv.x2.p, c->partial_b.p,
-                         c->partial2_b.p, c->scal_b.p);
+  TOFE_HIP(hipStreamWaitEvent(s3, c->ev_ops, 0));
+  TOFE_HIP(hipStreamWaitEvent(s4, c->ev_ops, 0));
+  if (c->precision == TOFE_PREC_MIXED) {
+    if (c->lv32.empty()) {
+      for (int l = 0; l < L; ++l) {
+        c->lv32.emplace_back(new Level<float>());
+        alloc_level32(c, *c->lv32.back(), l);
+      }
+      c->vr32.alloc(c->ndof);
+      c->vz32.alloc(c->ndof);
+      TOFE_HIP(hipStreamSynchronize(c->stream));
+    }
+    convert<double, float>(s4, c->lv64[0]->dinv.p, c->lv32[0]->dinv.p, c->ndof);
+    convert<double, float>(s4, c->E.p, c->E32.p, c->ne);
+    for (int l = 1; l < L; ++l) {
+      convert<double, float>(s3, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, c->grids[l].ndof());
+      stencil_upper32(s3, c->lv64[l]->S.p, c->grids[l].nn(), c->lv32[l]->S.p);
+    }
+    for (int l = 1; l + 1 < L; ++l) {
+      Level<float> &lv = *c->lv32[l];
+      lanczos_coefficients<float>(c, l, s3, lv.r.p, lv.x.p, lv.x2.p, c->partial_b.p,
+                                  c->partial2_b.p, c->scal_b.p);
+    }
+    if (L > 1) {
+      Level<float> &lv = *c->lv32[0];
+      lanczos_coefficients<float>(c, 0, s4, lv.r.p, lv.x.p, lv.x2.p, c->partial.p,
+                                  c->partial2.p, c->scal.p);
+    }
+  } else {
+    for (int l = 1; l + 1 < L; ++l) {
+      Level<double> &lv = *c->lv64[l];
+      lanczos_coefficients<double>(c, l, s3, lv.r.p, lv.x.p, lv.x2.p, c->partial_b.p,
+                                   c->partial2_b.p, c->scal_b.p);
+    }
+    if (L > 1)
+      lanczos_coefficients<double>(c, 0, s4, c->vr.p, c->vp.p, c->vq.p, c->partial.p,
+                                   c->partial2.p, c->scal.p);
   }
-  if (L > 1)
-    lanczos_coefficients(c, 0, c->stream, c->vr.p, c->vp.p, c->vq.p, c->partial.p, c->partial2.p,
-                         c->scal.p);
+  TOFE_HIP(hipEventRecord(c->ev_lz, s3));
+  TOFE_HIP(hipEventRecord(c->ev_lz0, s4));
 }
 
 void lanczos_lambdas(tofe_ctx *c)
@@ -644,33 +686,15 @@ void assemble(tofe_ctx *c)
   c->dinfo.alloc(2);
   stencil_to_dense(s, SL, gL.Nx, gL.Ny, gL.Nz, c->Ainv.p);
   TOFE_HIP(hipEventRecord(c->ev_dense, s));
-  // Three streams from here: the coarsest level's factorisation and inverse on the second,
-  // the Lanczos estimates of levels 1.. and the FP32 copies on the third, the fine level's
-  // Lanczos estimate on the main stream, which then joins the other two.
+  // Four streams from here: the coarsest level's factorisation and inverse on the second,
+  // the Lanczos estimates of levels 1.. on the third and of the fine level on the fourth;
+  // the main stream then joins the other three.
   TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
   coarse_inverse_enqueue(c);
   TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
   lanczos_enqueue(c);
-  if (c->precision == TOFE_PREC_MIXED) {
-    if (c->lv32.empty()) {
-      for (int l = 0; l < L; ++l) {
-        c->lv32.emplace_back(new Level<float>());
-        alloc_level32(c, *c->lv32.back(), l);
-      }
-      c->vr32.alloc(c->ndof);
-      c->vz32.alloc(c->ndof);
-    }
-    const hipStream_t s3 = c->stream3;
-    for (int l = 0; l < L; ++l) {
-      const int64_t nd = c->grids[l].ndof();
-      convert<double, float>(s3, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, nd);
-      if (l > 0)
-        stencil_upper32(s3, c->lv64[l]->S.p, c->grids[l].nn(), c->lv32[l]->S.p);
-    }
-    convert<double, float>(s3, c->E.p, c->E32.p, c->ne);
-  }
-  TOFE_HIP(hipEventRecord(c->ev_lz, c->stream3));
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz, 0));
+  TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz0, 0));
   lanczos_lambdas(c);
   chebyshev_coefficients(c);
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_inv, 0));
@@ -849,8 +873,20 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
 
     TOFE_HIP(hipStreamCreate(&c->stream));
     TOFE_HIP(hipStreamCreate(&c->stream2));
-    TOFE_HIP(hipStreamCreate(&c->stream3));
-    for (hipEvent_t *e : {&c->ev_dense, &c->ev_inv, &c->ev_ops, &c->ev_lz, &c->ev_scal})
+    {
+      int ncu = 0;
+      TOFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+      if (ncu >= 4 * kCoarseCUs) {
+        std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+        for (int cu = kCoarseCUs; cu < ncu; ++cu) mask[cu / 32] |= 1u << (cu % 32);
+        TOFE_HIP(hipExtStreamCreateWithCUMask(&c->stream3, (uint32_t)mask.size(), mask.data()));
+        TOFE_HIP(hipExtStreamCreateWithCUMask(&c->stream4, (uint32_t)mask.size(), mask.data()));
+      } else {
+        TOFE_HIP(hipStreamCreate(&c->stream3));
+        TOFE_HIP(hipStreamCreate(&c->stream4));
+      }
+    }
+    for (hipEvent_t *e : {&c->ev_dense, &c->ev_inv, &c->ev_ops, &c->ev_lz, &c->ev_scal, &c->ev_lz0})
       TOFE_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     TOFE_HIP(hipHostMalloc((void **)&c->hscal, SL_COUNT * sizeof(double), hipHostMallocDefault));
 

@@ -147,9 +147,10 @@ void cg_update_xr(hipStream_t s, double *x, double *r, const double *p, const do
                   int64_t n, const double *scal, int alpha_slot, double *partial);
 void cg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *scal,
                  int beta_slot);
-void power_start(hipStream_t s, double *v, int64_t n);
-void jacobi_dot(hipStream_t s, const double *r, const double *dinv, double *z, int64_t n,
-                double *partial);
+template <typename T>
+void power_start(hipStream_t s, T *v, int64_t n);
+template <typename T>
+void jacobi_dot(hipStream_t s, const T *r, const T *dinv, T *z, int64_t n, double *partial);
 void record_scalar(hipStream_t s, const double *scal, int slot, double *dst);
 // Flexible-CG updates that sum the producers' partials themselves: fcg_update_xr forms
 // alpha = sum(pr_part) / sum(pq_part) (stored with both sums in scal), x += alpha p,
@@ -164,11 +165,12 @@ void fcg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const do
 // lanczos_r: alpha = scal[rz_slot] / sum(pq_part[0..npq)), recorded; r -= alpha q; partials
 // of r.(D^-1 r).  lanczos_p: beta = sum(rz_part) / scal[rz_old], recorded; scal[rz_new] =
 // the sum; p = D^-1 r + beta p.
-void lanczos_r(hipStream_t s, double *r, const double *q, const double *dinv, int64_t n,
-               const double *pq_part, int npq, const double *scal, int rz_slot, double *rec_alpha,
-               double *partial);
-void lanczos_p(hipStream_t s, double *p, const double *r, const double *dinv, int64_t n,
-               const double *rz_part, double *scal, int rz_old, int rz_new, double *rec_beta);
+template <typename T>
+void lanczos_r(hipStream_t s, T *r, const T *q, const T *dinv, int64_t n, const double *pq_part,
+               int npq, const double *scal, int rz_slot, double *rec_alpha, double *partial);
+template <typename T>
+void lanczos_p(hipStream_t s, T *p, const T *r, const T *dinv, int64_t n, const double *rz_part,
+               double *scal, int rz_old, int rz_new, double *rec_beta);
 
 // device scalar slots (ctx->scal)
 enum Slot : int {
