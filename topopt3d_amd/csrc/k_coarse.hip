@@ -13,40 +13,69 @@ namespace {
 
 constexpr int NT = 256;
 
-template <typename T, int EPI>
+// SPLIT (small grids): one thread per DOF instead of per node, component-major over the
+// blocks (a = blockIdx / node blocks), so a few thousand nodes still fill many waves; each
+// DOF's sum runs in the same order either way.
+// Blocks are remapped so each XCD works on a contiguous range of nodes, whose
+// neighbours' stencil blocks (read again as the transposed lower half) then sit in the
+// same L2.
+template <bool SPLIT>
+__device__ inline void split_index(int64_t nn, int64_t &n, int &a0)
+{
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  if constexpr (SPLIT) {
+    const int nbn = (int)((nn + NT - 1) / NT);
+    a0 = b / nbn;
+    n = (int64_t)(b - a0 * nbn) * NT + threadIdx.x;
+  } else {
+    a0 = 0;
+    n = (int64_t)b * NT + threadIdx.x;
+  }
+}
+
+template <typename T, int EPI, bool SPLIT>
 __global__ __launch_bounds__(NT) void k_stencil_op(StencilOp<T> op, const T *__restrict__ u,
                                                    EpiArgs<T> ep, int64_t nn)
 {
   __shared__ double sred[NT / 64];
-  const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
+  int64_t n;
+  int a0;
+  split_index<SPLIT>(nn, n, a0);
   const T *__restrict__ S = op.S;
   double dsum = 0, dsum2 = 0;
   if (n < nn) {
     const int Nx = op.Nx, Ny = op.Ny, Nz = op.Nz;
     const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
-    T acc[3] = {0, 0, 0};
+    constexpr int NA = SPLIT ? 1 : 3;  // components summed by this thread
+    T acc[NA];
+#pragma unroll
+    for (int c = 0; c < NA; ++c) acc[c] = 0;
 #pragma unroll
     for (int o = 0; o < 27; ++o) {
       const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
       const int mi = i + dx, mj = j + dy, mk = k + dz;
-      if (mi >= 0 && mi < Nx && mj >= 0 && mj < Ny && mk >= 0 && mk < Nz) {
-        const int64_t m = n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz);
-        const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+      // Loads are unconditional (a missing neighbour reads node n) and the contribution is
+      // selected afterwards, so no branch separates the 27 neighbours' loads.
+      const bool in = mi >= 0 && mi < Nx && mj >= 0 && mj < Ny && mk >= 0 && mk < Nz;
+      const int64_t m = in ? n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz) : n;
+      const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          acc[a] += S[(int64_t)(o * 9 + a * 3 + 0) * nn + n] * u0;
-          acc[a] += S[(int64_t)(o * 9 + a * 3 + 1) * nn + n] * u1;
-          acc[a] += S[(int64_t)(o * 9 + a * 3 + 2) * nn + n] * u2;
-        }
+      for (int c = 0; c < NA; ++c) {
+        const int a = SPLIT ? a0 : c;
+        T t = acc[c];
+        t += S[(int64_t)(o * 9 + a * 3 + 0) * nn + n] * u0;
+        t += S[(int64_t)(o * 9 + a * 3 + 1) * nn + n] * u1;
+        t += S[(int64_t)(o * 9 + a * 3 + 2) * nn + n] * u2;
+        acc[c] = in ? t : acc[c];
       }
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const int64_t dof = 3 * n + a;
-      apply_epilogue<T, EPI>(ep, dof, acc[a], u[dof], dsum, dsum2);
+    for (int c = 0; c < NA; ++c) {
+      const int64_t dof = 3 * n + (SPLIT ? a0 : c);
+      apply_epilogue<T, EPI>(ep, dof, acc[c], u[dof], dsum, dsum2);
     }
   }
-  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2) {
+  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2 || EPI == EPI_CHEB_OUT) {
     const double t = block_sum<NT>(dsum, sred);
     if (threadIdx.x == 0) ep.partial[blockIdx.x] = t;
     if constexpr (EPI == EPI_APPLY_DOT2) {
@@ -133,45 +162,52 @@ __global__ __launch_bounds__(NT) void k_galerkin_axis(const double *__restrict__
 // the block of a lower offset o < 13 is the transpose of the neighbour's block 26 - o.  Each
 // stored block is read by its node and by one neighbour, so the HBM traffic is about half of
 // the full stencil's.
-template <typename T, int EPI>
+template <typename T, int EPI, bool SPLIT>
 __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T *__restrict__ u,
                                                        EpiArgs<T> ep, int64_t nn)
 {
   __shared__ double sred[NT / 64];
-  const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
+  int64_t n;
+  int a0;
+  split_index<SPLIT>(nn, n, a0);
   const T *__restrict__ S = op.S;
   double dsum = 0, dsum2 = 0;
   if (n < nn) {
     const int Nx = op.Nx, Ny = op.Ny, Nz = op.Nz;
     const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
-    T acc[3] = {0, 0, 0};
+    constexpr int NA = SPLIT ? 1 : 3;
+    T acc[NA];
+#pragma unroll
+    for (int c = 0; c < NA; ++c) acc[c] = 0;
 #pragma unroll
     for (int o = 0; o < 27; ++o) {
       const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
       const int mi = i + dx, mj = j + dy, mk = k + dz;
-      if (mi >= 0 && mi < Nx && mj >= 0 && mj < Ny && mk >= 0 && mk < Nz) {
-        const int64_t m = n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz);
-        const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+      // unconditional loads, selected contribution (as in k_stencil_op)
+      const bool in = mi >= 0 && mi < Nx && mj >= 0 && mj < Ny && mk >= 0 && mk < Nz;
+      const int64_t m = in ? n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz) : n;
+      const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+#pragma unroll
+      for (int c = 0; c < NA; ++c) {
+        const int a = SPLIT ? a0 : c;
+        T t;
         if (o >= 13) {
           const T *B = S + (int64_t)((o - 13) * 9) * nn + n;
-#pragma unroll
-          for (int a = 0; a < 3; ++a)
-            acc[a] += B[(a * 3 + 0) * nn] * u0 + B[(a * 3 + 1) * nn] * u1 + B[(a * 3 + 2) * nn] * u2;
+          t = B[(a * 3 + 0) * nn] * u0 + B[(a * 3 + 1) * nn] * u1 + B[(a * 3 + 2) * nn] * u2;
         } else {
           const T *B = S + (int64_t)((13 - o) * 9) * nn + m;  // block (m, 26 - o), transposed
-#pragma unroll
-          for (int a = 0; a < 3; ++a)
-            acc[a] += B[(0 * 3 + a) * nn] * u0 + B[(1 * 3 + a) * nn] * u1 + B[(2 * 3 + a) * nn] * u2;
+          t = B[(0 * 3 + a) * nn] * u0 + B[(1 * 3 + a) * nn] * u1 + B[(2 * 3 + a) * nn] * u2;
         }
+        acc[c] = in ? acc[c] + t : acc[c];
       }
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const int64_t dof = 3 * n + a;
-      apply_epilogue<T, EPI>(ep, dof, acc[a], u[dof], dsum, dsum2);
+    for (int c = 0; c < NA; ++c) {
+      const int64_t dof = 3 * n + (SPLIT ? a0 : c);
+      apply_epilogue<T, EPI>(ep, dof, acc[c], u[dof], dsum, dsum2);
     }
   }
-  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2) {
+  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2 || EPI == EPI_CHEB_OUT) {
     const double t = block_sum<NT>(dsum, sred);
     if (threadIdx.x == 0) ep.partial[blockIdx.x] = t;
     if constexpr (EPI == EPI_APPLY_DOT2) {
@@ -669,41 +705,59 @@ void coarse_factor_invert(hipStream_t s, double *A, double *Lb, double *X, int n
   TOFE_LAUNCH_CHECK();
 }
 
+// grids below this many nodes run one thread per DOF (see split_index)
+constexpr int64_t kSplitNodes = 131072;
+
 template <typename T>
 int stencil_op_blocks(const StencilOp<T> &op)
 {
   const int64_t nn = (int64_t)op.Nx * op.Ny * op.Nz;
-  return (int)((nn + NT - 1) / NT);
+  const int nb = (int)((nn + NT - 1) / NT);
+  return nn < kSplitNodes ? 3 * nb : nb;
 }
+
+#define TOFE_STENCIL_SWITCH(KERNEL)                                                         \
+  do {                                                                                      \
+    const int64_t nn = (int64_t)op.Nx * op.Ny * op.Nz;                                      \
+    const bool split = nn < kSplitNodes;                                                    \
+    dim3 grid((unsigned)stencil_op_blocks<T>(op)), block(NT);                               \
+    switch (epi) {                                                                          \
+      case EPI_APPLY:                                                                       \
+        if (split) KERNEL<T, EPI_APPLY, true><<<grid, block, 0, s>>>(op, u, ep, nn);        \
+        else KERNEL<T, EPI_APPLY, false><<<grid, block, 0, s>>>(op, u, ep, nn);             \
+        break;                                                                              \
+      case EPI_APPLY_DOT:                                                                   \
+        if (split) KERNEL<T, EPI_APPLY_DOT, true><<<grid, block, 0, s>>>(op, u, ep, nn);    \
+        else KERNEL<T, EPI_APPLY_DOT, false><<<grid, block, 0, s>>>(op, u, ep, nn);         \
+        break;                                                                              \
+      case EPI_APPLY_DOT2:                                                                  \
+        if (split) KERNEL<T, EPI_APPLY_DOT2, true><<<grid, block, 0, s>>>(op, u, ep, nn);   \
+        else KERNEL<T, EPI_APPLY_DOT2, false><<<grid, block, 0, s>>>(op, u, ep, nn);        \
+        break;                                                                              \
+      case EPI_RESID:                                                                       \
+        if (split) KERNEL<T, EPI_RESID, true><<<grid, block, 0, s>>>(op, u, ep, nn);        \
+        else KERNEL<T, EPI_RESID, false><<<grid, block, 0, s>>>(op, u, ep, nn);             \
+        break;                                                                              \
+      case EPI_CHEB:                                                                        \
+        if (split) KERNEL<T, EPI_CHEB, true><<<grid, block, 0, s>>>(op, u, ep, nn);         \
+        else KERNEL<T, EPI_CHEB, false><<<grid, block, 0, s>>>(op, u, ep, nn);              \
+        break;                                                                              \
+      default:                                                                              \
+        throw Error(TOFE_ERR_ARG, "epilogue not available for stencil levels");             \
+    }                                                                                       \
+    TOFE_LAUNCH_CHECK();                                                                    \
+  } while (0)
 
 template <typename T>
 void stencil_op(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep)
 {
-  const int64_t nn = (int64_t)op.Nx * op.Ny * op.Nz;
-  dim3 grid((unsigned)((nn + NT - 1) / NT)), block(NT);
-  switch (epi) {
-    case EPI_APPLY: k_stencil_op<T, EPI_APPLY><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-    case EPI_APPLY_DOT: k_stencil_op<T, EPI_APPLY_DOT><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-    case EPI_APPLY_DOT2: k_stencil_op<T, EPI_APPLY_DOT2><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-    case EPI_RESID: k_stencil_op<T, EPI_RESID><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-    case EPI_CHEB: k_stencil_op<T, EPI_CHEB><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-  }
-  TOFE_LAUNCH_CHECK();
+  TOFE_STENCIL_SWITCH(k_stencil_op);
 }
 
 template <typename T>
 void stencil_op_sym(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep)
 {
-  const int64_t nn = (int64_t)op.Nx * op.Ny * op.Nz;
-  dim3 grid((unsigned)((nn + NT - 1) / NT)), block(NT);
-  switch (epi) {
-    case EPI_APPLY: k_stencil_op_sym<T, EPI_APPLY><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-    case EPI_APPLY_DOT: k_stencil_op_sym<T, EPI_APPLY_DOT><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-    case EPI_APPLY_DOT2: k_stencil_op_sym<T, EPI_APPLY_DOT2><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-    case EPI_RESID: k_stencil_op_sym<T, EPI_RESID><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-    case EPI_CHEB: k_stencil_op_sym<T, EPI_CHEB><<<grid, block, 0, s>>>(op, u, ep, nn); break;
-  }
-  TOFE_LAUNCH_CHECK();
+  TOFE_STENCIL_SWITCH(k_stencil_op_sym);
 }
 
 void stencil_upper32(hipStream_t s, const double *S, int64_t nn, float *Su)

@@ -78,6 +78,12 @@ __device__ inline void apply_epilogue(const EpiArgs<T> &ep, int64_t dof, T Au, T
     const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * ep.d[dof] + ep.c2 * r;
     ep.d[dof] = dn;
     ep.xout[dof] = uval + dn;
+  } else if constexpr (EPI == EPI_CHEB_OUT) {
+    const T r = ep.dinv[dof] * (ep.b[dof] - Au);
+    const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * ep.d[dof] + ep.c2 * r;
+    const double x = (double)(uval + dn);
+    ep.xout64[dof] = x;
+    dsum += x * ep.q64[dof];
   }
 }
 

@@ -224,7 +224,7 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
       }
     }
   }
-  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2) {
+  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2 || EPI == EPI_CHEB_OUT) {
     __syncthreads();
     const double t = block_sum<NT>(dsum, sred);
     if (threadIdx.x == 0) ep.partial[blockIdx.x] = t;
@@ -324,6 +324,9 @@ void fine_op(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiA
       break;
     case EPI_CHEB:
       k_fine_op<T, EPI_CHEB><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
+      break;
+    case EPI_CHEB_OUT:
+      k_fine_op<T, EPI_CHEB_OUT><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
       break;
   }
   TOFE_LAUNCH_CHECK();

@@ -196,7 +196,7 @@ __global__ __launch_bounds__(NTH) void k_fine_mfma(FineOp<T> op, const T *__rest
     __syncthreads();
     for (int v = tid; v < PL; v += NTH) sacc[lo][v] = 0;
   }
-  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2) {
+  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2 || EPI == EPI_CHEB_OUT) {
     __syncthreads();
     const double t = block_sum<NTH>(dsum, sred);
     if (tid == 0) ep.partial[blockIdx.x] = t;
@@ -236,6 +236,9 @@ void fine_mfma(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const Ep
       break;
     case EPI_CHEB:
       k_fine_mfma<T, EPI_CHEB><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
+      break;
+    case EPI_CHEB_OUT:
+      k_fine_mfma<T, EPI_CHEB_OUT><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
       break;
   }
   TOFE_LAUNCH_CHECK();

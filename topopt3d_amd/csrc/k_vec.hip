@@ -57,6 +57,19 @@ __global__ void k_cheb_zero(const T *__restrict__ b, const T *__restrict__ dinv,
   }
 }
 
+__global__ void k_cheb_zero_from64(const double *__restrict__ b64, float *__restrict__ b32,
+                                   const float *__restrict__ dinv, float c2, float *d, float *x,
+                                   int64_t n)
+{
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const float b = (float)b64[i];
+    b32[i] = b;
+    const float v = c2 * (dinv[i] * b);
+    d[i] = v;
+    x[i] = v;
+  }
+}
+
 // b_c(I) = sum over fine nodes i within one fine step of 2I of w(i,I) r_f(i)  (P^T r)
 template <typename T>
 __global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int Nfx, int Nfy,
@@ -246,10 +259,10 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ x, double *_
 // p = z + beta p.
 __global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const double *__restrict__ z,
                                               int64_t n, const double *__restrict__ zq_part,
-                                              double *scal)
+                                              int nparts, double *scal)
 {
   __shared__ double sred[NT / 64];
-  const double zq = all_sum(zq_part, kRedBlocks, sred);
+  const double zq = all_sum(zq_part, nparts, sred);
   const double beta = -zq / scal[SL_PQ];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     scal[SL_ZQ] = zq;
@@ -334,6 +347,13 @@ void cheb_zero(hipStream_t s, const T *b, const T *dinv, T c2, T *d, T *x, int64
   TOFE_LAUNCH_CHECK();
 }
 
+void cheb_zero_from64(hipStream_t s, const double *b64, float *b32, const float *dinv, float c2,
+                      float *d, float *x, int64_t n)
+{
+  k_cheb_zero_from64<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(b64, b32, dinv, c2, d, x, n);
+  TOFE_LAUNCH_CHECK();
+}
+
 template <typename T>
 void restrict_op(hipStream_t s, const T *rf, int Nfx, int Nfy, int Nfz, T *bc, int Ncx, int Ncy,
                  int Ncz)
@@ -411,9 +431,9 @@ void fcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const d
 }
 
 void fcg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *zq_part,
-                  double *scal)
+                  int nparts, double *scal)
 {
-  k_fcg_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, n, zq_part, scal);
+  k_fcg_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, n, zq_part, nparts, scal);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -296,15 +296,28 @@ void coarse_apply(tofe_ctx *c, const T *b, T *x)
   else dense_gemv<double, double>(c->stream, c->Ainv.p, c->nc, b, x);
 }
 
+// FP64 ends of the mixed-precision V-cycle, fused into its first and last fine-level
+// kernels: b64 is converted while the first Chebyshev step is formed, and the last step
+// writes x in FP64 to x64 with the partial sums of x.q64 (one per fine-operator block).
+struct VcycleIO {
+  const double *b64 = nullptr;
+  double *x64 = nullptr;
+  const double *q64 = nullptr;
+  double *partial = nullptr;
+  bool fused_out = false;  // set when the last step wrote x64 and the partials
+};
+
 // Multigrid V-cycle: x0 = M^-1 b0 (PCMG V-cycle, Chebyshev-Jacobi smoothers,
 // Galerkin coarse operators, direct coarse solve).
 template <typename T>
-void vcycle(tofe_ctx *c, const T *b0, T *x0)
+void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
 {
   auto &LV = levels<T>(c);
   const int L = c->L;
   const hipStream_t s = c->stream;
+  const bool mixed_io = io && sizeof(T) == 4;
   if (L == 1) {
+    if (mixed_io && io->b64) convert<double, T>(s, io->b64, const_cast<T *>(b0), c->ndof);
     coarse_apply<T>(c, b0, x0);
     return;
   }
@@ -318,8 +331,18 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0)
     T *xc = (flips & 1) ? xb : xa, *xo = (xc == xa) ? xb : xa;
     const int64_t nd = lv.g.ndof();
     const double *c1 = &c->c1[(size_t)l * km], *c2 = &c->c2[(size_t)l * km];
+    const bool from64 = l == 0 && mixed_io && io->b64;
+    if (from64 && nu1 == 0) convert<double, T>(s, io->b64, const_cast<T *>(b0), nd);
     if (nu1 > 0) {
-      cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd);
+      if constexpr (sizeof(T) == 4) {
+        if (from64)
+          cheb_zero_from64(s, io->b64, const_cast<float *>(b0), lv.dinv.p, (float)c2[0], lv.d.p,
+                           xc, nd);
+        else
+          cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd);
+      } else {
+        cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd);
+      }
       for (int k = 1; k < nu1; ++k) {
         EpiArgs<T> ep;
         ep.b = b; ep.dinv = lv.dinv.p; ep.d = lv.d.p; ep.xout = xo;
@@ -352,6 +375,12 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0)
       EpiArgs<T> ep;
       ep.b = b; ep.dinv = lv.dinv.p; ep.d = lv.d.p; ep.xout = xo;
       ep.c1 = (T)(k == 0 ? 0.0 : c1[k]); ep.c2 = (T)c2[k];
+      if (l == 0 && k == nu2 - 1 && mixed_io && io->x64) {
+        ep.xout64 = io->x64; ep.q64 = io->q64; ep.partial = io->partial;
+        level_apply<T>(c, l, xc, EPI_CHEB_OUT, ep);
+        io->fused_out = true;
+        return;
+      }
       level_apply<T>(c, l, xc, EPI_CHEB, ep);
       std::swap(xc, xo);
     }
@@ -360,15 +389,26 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0)
   if (cur[0] != x0) TOFE_HIP(hipMemcpyAsync(x0, cur[0], c->ndof * sizeof(T), hipMemcpyDeviceToDevice, s));
 }
 
-void precondition(tofe_ctx *c, const double *r, double *z)
+// z = M^-1 r.  With q given (flexible CG), also the partial sums of z.q into c->partial;
+// returns how many there are.
+int precondition(tofe_ctx *c, const double *r, double *z, const double *q = nullptr)
 {
+  const hipStream_t s = c->stream;
   if (c->built_precision == TOFE_PREC_MIXED) {
-    convert<double, float>(c->stream, r, c->vr32.p, c->ndof);
-    vcycle<float>(c, c->vr32.p, c->vz32.p);
-    convert<float, double>(c->stream, c->vz32.p, z, c->ndof);
+    VcycleIO io;
+    io.b64 = r;
+    if (q && c->L > 1 && !c->fine_mfma) {
+      io.x64 = z; io.q64 = q; io.partial = c->partial.p;
+    }
+    vcycle<float>(c, c->vr32.p, c->vz32.p, &io);
+    if (io.fused_out) return level_blocks<float>(c, 0);
+    convert<float, double>(s, c->vz32.p, z, c->ndof);
   } else {
     vcycle<double>(c, r, z);
   }
+  if (!q) return 0;
+  dot_partial<double>(s, z, q, c->ndof, c->partial.p);
+  return kRedBlocks;
 }
 
 void dot_into(tofe_ctx *c, const double *a, const double *b, int64_t n, int slot, int post = POST_NONE)
@@ -436,11 +476,11 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
   // device then never waits for the host's test.  A converged step leaves them unused.
   double rn_prev = rn, rate = 0.5;
   auto next_direction = [&] {
-    precondition(c, r, z);
     if (flex) {
-      dot_partial<double>(s, z, q, n, c->partial.p);
-      fcg_update_p(s, p, z, n, c->partial.p, c->scal.p);
+      const int nparts = precondition(c, r, z, q);
+      fcg_update_p(s, p, z, n, c->partial.p, nparts, c->scal.p);
     } else {
+      precondition(c, r, z);
       dot_into(c, r, z, n, SL_TMP, POST_BETA_CG);
       cg_update_p(s, p, z, n, c->scal.p, SL_BETA);
     }

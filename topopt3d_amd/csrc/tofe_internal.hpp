@@ -39,7 +39,9 @@ enum Epi : int {
   EPI_APPLY_DOT = 1,  // y = Au, partial += u.Au
   EPI_RESID = 2,      // y = b - Au
   EPI_CHEB = 3,       // r = dinv (b - Au); d = c1 d + c2 r; xout = u + d
-  EPI_APPLY_DOT2 = 4  // y = Au, partial += u.Au, partial2 += u.b
+  EPI_APPLY_DOT2 = 4, // y = Au, partial += u.Au, partial2 += u.b
+  EPI_CHEB_OUT = 5    // last Chebyshev step: x = u + (c1 d + c2 dinv (b - Au)) written in
+                      // FP64 to xout64, partial += x.q64 (d is not updated)
 };
 
 template <typename T>
@@ -51,6 +53,8 @@ struct EpiArgs {
   T *xout = nullptr;
   double *partial = nullptr;
   double *partial2 = nullptr;
+  double *xout64 = nullptr;
+  const double *q64 = nullptr;
   T c1 = 0, c2 = 0;
 };
 
@@ -134,6 +138,9 @@ void reduce_final(hipStream_t s, const double *partial, int nparts, double *out,
                   double *scal);
 template <typename T>
 void cheb_zero(hipStream_t s, const T *b, const T *dinv, T c2, T *d, T *x, int64_t n);
+// first Chebyshev step from an FP64 right-hand side: b32 = (float) b64, d = x = c2 dinv b32
+void cheb_zero_from64(hipStream_t s, const double *b64, float *b32, const float *dinv, float c2,
+                      float *d, float *x, int64_t n);
 template <typename T>
 void restrict_op(hipStream_t s, const T *rf, int Nfx, int Nfy, int Nfz, T *bc, int Ncx, int Ncy,
                  int Ncz);
@@ -160,7 +167,7 @@ void fcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const d
                    int64_t n, const double *pq_part, const double *pr_part, int nparts,
                    double *scal, double *partial);
 void fcg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *zq_part,
-                  double *scal);
+                  int nparts, double *scal);
 // Fused Jacobi-CG steps of the eigenvalue estimate (no separate reduction launches):
 // lanczos_r: alpha = scal[rz_slot] / sum(pq_part[0..npq)), recorded; r -= alpha q; partials
 // of r.(D^-1 r).  lanczos_p: beta = sum(rz_part) / scal[rz_old], recorded; scal[rz_new] =
