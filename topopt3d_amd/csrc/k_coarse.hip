@@ -409,28 +409,39 @@ __device__ inline void tile_ij(int b, int &I, int &J)
   J = b;
 }
 
-// One step of the banded right-looking Cholesky for the column block at k0 (kb columns),
-// with m band rows below it.  Every workgroup factors the diagonal block itself, so no
-// workgroup waits for another: it then solves the panel rows P = A21 L11^-T of its two row
-// blocks I and J of the trailing window and applies C(I,J) -= P_I P_J^T to its tile of A.
-// L goes to Lb only (the diagonal block from workgroup 0, the panel rows from the diagonal
-// tiles), and the panel of A is only read, so no workgroup reads what another one writes.
+// ---- one launch of the pipelined coarse factorisation and inversion -------------------
+// Launch k (0 <= k <= nbk + 1) runs three independent parts in one grid:
+//  * step k of the banded right-looking Cholesky (k < nbk), one workgroup per lower tile
+//    of the trailing window;
+//  * row block k-1 of X = L^-1 (1 <= k <= nbk), one workgroup per quarter column block;
+//  * the contributions X(k-2, I)^T X(k-2, J) to the lower tiles (I, J) of A^-1 (k >= 2).
+// Each part reads only what earlier launches completed (row block k-1 of L is complete
+// after launch k-1; X row block k-2 after launch k-1), so the three chains run as one
+// pipeline and X and A^-1 are finished two launches after the last factorisation step.
+// A^-1 accumulates in place of A: its tile (I, J) lies in column blocks that the
+// factorisation has finished with, and the upper triangle is written at the last
+// contribution, after the last factorisation step.
+constexpr int XCPW = 4, XCW = 4 * XCPW, XPARTS = CB / XCW;
+constexpr int kCoarseLds = CB * CB + CB + 2 * CB * CB;  // doubles
+
+// Factorisation step: diagonal block (every workgroup, redundantly, so none waits for
+// another), the panel rows P = A21 L11^-T of its row blocks I and J of the window, and
+// C(I,J) -= P_I P_J^T.  L goes to Lb (the diagonal block from workgroup 0, the panel rows
+// from the diagonal tiles); the panel of A is only read.
 template <bool FULL>
-__global__ __launch_bounds__(256) void k_chol_step(double *A, double *Lb, int n, int k0, int kb,
-                                                   int m, int *info)
+__device__ void chol_part(double *A, double *Lb, int n, int k0, int kb, int m, int *info, int b,
+                          double *lds)
 {
-  __shared__ double Ls[CB * CB];
-  __shared__ double dinv[CB];
-  __shared__ double Pt[2][CB * CB];  // panel rows, transposed: Pt[s][q * 64 + row]
+  double *Ls = lds, *dinv = lds + CB * CB, *Pt0 = lds + CB * CB + CB, *Pt1 = Pt0 + CB * CB;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   int I, J;
-  tile_ij(blockIdx.x, I, J);
+  tile_ij(b, I, J);
   if (w == 0) {
     const int bad = chol_diag<FULL>(A, n, k0, kb, Ls, dinv);
-    if (blockIdx.x == 0 && lane == 0 && bad != 0 && info[0] == 0) info[0] = bad;
+    if (b == 0 && lane == 0 && bad != 0 && info[0] == 0) info[0] = bad;
   }
   __syncthreads();
-  if (blockIdx.x == 0) {
+  if (b == 0) {
     for (int e = t; e < CB * CB; e += 256) {
       const int r = e & 63, cc = e >> 6;
       if (r < kb && cc < kb) Lb[(k0 + r) + (int64_t)(k0 + cc) * n] = r >= cc ? Ls[cc * CB + r] : 0.0;
@@ -441,6 +452,7 @@ __global__ __launch_bounds__(256) void k_chol_step(double *A, double *Lb, int n,
   const int w0 = k0 + kb;  // first row and column of the trailing window
   if (t < 128 && (t < 64 || J != I)) {
     const int s = t >> 6;
+    double *Pt = s ? Pt1 : Pt0;
     const int row = 64 * (s ? J : I) + lane;  // row of the window
     const bool live = row < m;
     const int rowc = live ? row : 0;
@@ -465,14 +477,14 @@ __global__ __launch_bounds__(256) void k_chol_step(double *A, double *Lb, int n,
       for (int l = q + 1; l < CB; ++l) asm volatile("" : "+v"(x[l]));
     }
 #pragma unroll
-    for (int q = 0; q < CB; ++q) Pt[s][q * CB + lane] = x[q];
+    for (int q = 0; q < CB; ++q) Pt[q * CB + lane] = x[q];
     if (I == J && live) {
 #pragma unroll
       for (int q = 0; q < CB; ++q) Lb[(w0 + row) + (int64_t)(k0 + q) * n] = x[q];
     }
   }
   __syncthreads();
-  const double *PI = Pt[0], *PJ = (I == J) ? Pt[0] : Pt[1];
+  const double *PI = Pt0, *PJ = (I == J) ? Pt0 : Pt1;
   const int ra = (t & 15) * 4, cb = (t >> 4) * 4;
   double acc[4][4];
 #pragma unroll
@@ -481,16 +493,16 @@ __global__ __launch_bounds__(256) void k_chol_step(double *A, double *Lb, int n,
     for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
 #pragma unroll 8
   for (int q = 0; q < CB; ++q) {
-    double a[4], b[4];
+    double a[4], bb[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       a[u] = PI[q * CB + ra + u];
-      b[u] = PJ[q * CB + cb + u];
+      bb[u] = PJ[q * CB + cb + u];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+      for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * bb[v];
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -504,159 +516,147 @@ __global__ __launch_bounds__(256) void k_chol_step(double *A, double *Lb, int n,
   }
 }
 
-// X = L^-1 by column blocks: workgroup (kc, part) owns CW = 4 CPW columns of block kc and
-// sweeps the row blocks i = kc, kc+1, ...: T = E - sum_j L(i,j) X(j,kc) over the row
-// blocks j < i inside the band (at most jb of them), then X(i,kc) = L(i,i)^-1 T by forward
-// substitution, wave w solving CPW columns with lane = row.  The JB most recent X tiles
-// stay in an LDS ring (JB > jb), so a workgroup reads back only what it wrote, from LDS.
-template <int CPW, int JB>
-__global__ __launch_bounds__(256) void k_tri_inv(const double *Lb, double *X, int n, int nbk,
-                                                 int jb)
+// Row block ix of X = L^-1, columns [CB kc + XCW part, + XCW): T = E - sum_j L(ix,j) X(j,kc)
+// over the row blocks kc <= j < ix inside the band (X(j,kc) staged through LDS), then
+// X(ix,kc) = L(ix,ix)^-1 T by forward substitution with lane = row, XCPW columns per wave.
+__device__ void x_part(const double *Lb, double *X, int n, int jb, int ix, int kc, int part,
+                       double *lds)
 {
-  constexpr int CW = 4 * CPW;
-  __shared__ double Ld[CB * CB];        // L(i,i), column-major, identity-padded
-  __shared__ double ring[JB][CB * CW];  // X(j,kc) tiles, [row][column]
-  const int kc = blockIdx.x / (CB / CW), part = blockIdx.x % (CB / CW);
+  double *Ld = lds, *Xs = lds + CB * CB;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int c0 = CB * kc + CW * part + CPW * w;  // first global column of this wave
-  for (int i = kc; i < nbk; ++i) {
-    const int64_t grow = (int64_t)CB * i + lane;
-    double T[CPW];
+  const int cw0 = CB * kc + XCW * part;  // first column of the workgroup
+  const int c0 = cw0 + XCPW * w;         // first column of this wave
+  const int64_t grow = (int64_t)CB * ix + lane;
+  const bool rok = grow < n;
+  const int64_t growc = rok ? grow : 0;
+  double T[XCPW];
 #pragma unroll
-    for (int v = 0; v < CPW; ++v) T[v] = (grow == c0 + v) ? 1.0 : 0.0;
-    const int jlo = (i - jb > kc) ? i - jb : kc;
-    const bool rok = grow < n;
-    const int64_t growc = rok ? grow : 0;
-    for (int j = jlo; j < i; ++j) {
-      const double *Xj = ring[j % JB] + CPW * w;
-      const double *Lrow = Lb + growc + (int64_t)CB * j * n;
-      double lv[CB];  // row of L(i,j): all 64 loads in flight at once
+  for (int v = 0; v < XCPW; ++v) T[v] = (grow == c0 + v) ? 1.0 : 0.0;
+  const int jlo = (ix - jb > kc) ? ix - jb : kc;
+  for (int j = jlo; j < ix; ++j) {
+    __syncthreads();  // previous readers of Xs are done
 #pragma unroll
-      for (int p = 0; p < CB; ++p) lv[p] = Lrow[(int64_t)p * n];
-#pragma unroll
-      for (int p0 = 0; p0 < CB; p0 += 16) {
-        // 16 rows of this wave's columns of X(j,kc), read together (pinned) from LDS
-        double xv[16][CPW];
-#pragma unroll
-        for (int p = 0; p < 16; ++p)
-#pragma unroll
-          for (int v = 0; v < CPW; ++v) xv[p][v] = Xj[(p0 + p) * CW + v];
-#pragma unroll
-        for (int p = 0; p < 16; ++p)
-#pragma unroll
-          for (int v = 0; v < CPW; ++v) asm volatile("" : "+v"(xv[p][v]));
-#pragma unroll
-        for (int p = 0; p < 16; ++p) {
-          const double l = rok ? lv[p0 + p] : 0.0;
-#pragma unroll
-          for (int v = 0; v < CPW; ++v) T[v] -= l * xv[p][v];
-        }
-      }
-    }
-    __syncthreads();  // previous step's readers of Ld are done
-#pragma unroll
-    for (int e0 = 0; e0 < CB * CB; e0 += 256) {
+    for (int e0 = 0; e0 < CB * XCW; e0 += 256) {
       const int e = e0 + t, r = e & 63, cc = e >> 6;
-      const int64_t gr = (int64_t)CB * i + r, gc = (int64_t)CB * i + cc;
-      const bool ok = gr < n && gc < n;
-      const double v = Lb[(ok ? gr : 0) + (ok ? gc : 0) * n];
-      Ld[e] = ok ? v : (r == cc ? 1.0 : 0.0);
+      const int64_t gc = cw0 + cc, gr = (int64_t)CB * j + r;  // gr < n: j < ix
+      const bool ok = gc < n;
+      const double v = X[gr + (ok ? gc : 0) * (int64_t)n];
+      Xs[r * XCW + cc] = ok ? v : 0.0;
     }
     __syncthreads();
-    // Forward substitution with L(i,i) scaled to a unit diagonal row by row (lane r
-    // divides its row and right-hand side by L(r,r)), so no step divides: at step p lane p
-    // holds x_p, and the lanes below subtract L(r,p)/L(r,r) x_p.
-    const double rown = 1.0 / Ld[lane * CB + lane];
-    double lr[CB];
+    const double *Lrow = Lb + growc + (int64_t)CB * j * n;
+    double lv[CB];  // row of L(ix,j): all 64 loads in flight at once
 #pragma unroll
-    for (int p = 0; p < CB; ++p) lr[p] = Ld[p * CB + lane];
+    for (int p = 0; p < CB; ++p) lv[p] = Lrow[(int64_t)p * n];
 #pragma unroll
-    for (int p = 0; p < CB; ++p) asm volatile("" : "+v"(lr[p]));
+    for (int p0 = 0; p0 < CB; p0 += 16) {
+      double xv[16][XCPW];  // 16 rows of this wave's columns, read together (pinned)
 #pragma unroll
-    for (int v = 0; v < CPW; ++v) T[v] *= rown;
+      for (int p = 0; p < 16; ++p)
 #pragma unroll
-    for (int p = 0; p < CB; ++p) {
-      int ii = lane;
-      asm volatile("" : "+v"(ii));  // keeps the lane test here instead of hoisted masks
-      const double lp = lr[p] * rown;
+        for (int v = 0; v < XCPW; ++v) xv[p][v] = Xs[(p0 + p) * XCW + XCPW * w + v];
 #pragma unroll
-      for (int v = 0; v < CPW; ++v) {
-        const double xp = lane_value(T[v], p);
-        const double u = T[v] - lp * xp;
-        T[v] = (ii > p) ? u : T[v];
+      for (int p = 0; p < 16; ++p)
+#pragma unroll
+        for (int v = 0; v < XCPW; ++v) asm volatile("" : "+v"(xv[p][v]));
+#pragma unroll
+      for (int p = 0; p < 16; ++p) {
+        const double l = rok ? lv[p0 + p] : 0.0;
+#pragma unroll
+        for (int v = 0; v < XCPW; ++v) T[v] -= l * xv[p][v];
       }
     }
-    double *Xi = ring[i % JB];
-#pragma unroll
-    for (int v = 0; v < CPW; ++v) {
-      Xi[lane * CW + CPW * w + v] = T[v];
-      if (grow < n && c0 + v < n) X[grow + (int64_t)(c0 + v) * n] = T[v];
-    }
-    __syncthreads();  // the new ring tile is complete before the next step reads it
   }
+  __syncthreads();  // Xs readers done before Ld overwrites the start of the buffer
+#pragma unroll
+  for (int e0 = 0; e0 < CB * CB; e0 += 256) {
+    const int e = e0 + t, r = e & 63, cc = e >> 6;
+    const int64_t gr = (int64_t)CB * ix + r, gc = (int64_t)CB * ix + cc;
+    const bool ok = gr < n && gc < n;
+    const double v = Lb[(ok ? gr : 0) + (ok ? gc : 0) * n];
+    Ld[e] = ok ? v : (r == cc ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  // Forward substitution with L(ix,ix) scaled to a unit diagonal row by row (lane r
+  // divides its row and right-hand side by L(r,r)), so no step divides: at step p lane p
+  // holds x_p, and the lanes below subtract L(r,p)/L(r,r) x_p.
+  const double rown = 1.0 / Ld[lane * CB + lane];
+  double lr[CB];
+#pragma unroll
+  for (int p = 0; p < CB; ++p) lr[p] = Ld[p * CB + lane];
+#pragma unroll
+  for (int p = 0; p < CB; ++p) asm volatile("" : "+v"(lr[p]));
+#pragma unroll
+  for (int v = 0; v < XCPW; ++v) T[v] *= rown;
+#pragma unroll
+  for (int p = 0; p < CB; ++p) {
+    int ii = lane;
+    asm volatile("" : "+v"(ii));  // keeps the lane test here instead of hoisted masks
+    const double lp = lr[p] * rown;
+#pragma unroll
+    for (int v = 0; v < XCPW; ++v) {
+      const double xp = lane_value(T[v], p);
+      const double u = T[v] - lp * xp;
+      T[v] = (ii > p) ? u : T[v];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < XCPW; ++v)
+    if (rok && c0 + v < n) X[grow + (int64_t)(c0 + v) * n] = T[v];
 }
 
-// A^-1 = X^T X: workgroup per lower tile (I, J) sums X(K,I)^T X(K,J) over K >= I and
-// writes the tile and, off the diagonal, its transpose.
-__global__ __launch_bounds__(256) void k_xtx(const double *X, double *Ai, int n, int nbk)
+// Contribution of X row block K to tile (I, J) of A^-1 (J <= I <= K): the first one
+// (K == I) initialises the tile, the last (K == nbk - 1) also writes its transpose.
+__device__ void xtx_part(const double *X, double *Ai, int n, int nbk, int K, int I, int J,
+                         double *lds)
 {
-  __shared__ double XI[CB * (CB + 1)], XJ[CB * (CB + 1)];  // [row][column], padded rows
+  double *XI = lds, *XJ = lds + CB * (CB + 1);  // [row][column], padded rows
   const int t = threadIdx.x;
-  int I, J;
-  tile_ij(blockIdx.x, I, J);
+  constexpr int PT = CB * CB / 256;
+  double vi[PT], vj[PT];  // all 32 loads in flight at once
+#pragma unroll
+  for (int k = 0; k < PT; ++k) {
+    const int e = 256 * k + t, r = e & 63, cc = e >> 6;
+    const int64_t gr = (int64_t)CB * K + r;
+    const int64_t gi = (int64_t)CB * I + cc, gj = (int64_t)CB * J + cc;
+    const bool oki = gr < n && gi < n, okj = gr < n && gj < n;
+    const double a = X[(oki ? gr : 0) + (oki ? gi : 0) * n];
+    const double b = X[(okj ? gr : 0) + (okj ? gj : 0) * n];
+    vi[k] = oki ? a : 0.0;
+    vj[k] = okj ? b : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < PT; ++k) {
+    const int e = 256 * k + t, r = e & 63, cc = e >> 6;
+    XI[r * (CB + 1) + cc] = vi[k];
+    XJ[r * (CB + 1) + cc] = vj[k];
+  }
+  __syncthreads();
   const int ra = (t & 15) * 4, cb = (t >> 4) * 4;
   double acc[4][4];
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-  // the next K's tiles are loaded into registers while the current ones are used
-  constexpr int PT = CB * CB / 256;
-  double vi[PT], vj[PT];
-  auto load = [&](int K) {
+  for (int r0 = 0; r0 < CB; r0 += 8) {
+    double a[8][4], b[8][4];  // 8 rows read together (pinned), then 128 FMAs
 #pragma unroll
-    for (int k = 0; k < PT; ++k) {
-      const int e = 256 * k + t, r = e & 63, cc = e >> 6;
-      const int64_t gr = (int64_t)CB * K + r;
-      const int64_t gi = (int64_t)CB * I + cc, gj = (int64_t)CB * J + cc;
-      const bool oki = gr < n && gi < n, okj = gr < n && gj < n;
-      const double a = X[(oki ? gr : 0) + (oki ? gi : 0) * n];
-      const double b = X[(okj ? gr : 0) + (okj ? gj : 0) * n];
-      vi[k] = oki ? a : 0.0;
-      vj[k] = okj ? b : 0.0;
-    }
-  };
-  load(I);
-  for (int K = I; K < nbk; ++K) {
-    __syncthreads();  // previous step's readers of XI / XJ are done
+    for (int r = 0; r < 8; ++r)
 #pragma unroll
-    for (int k = 0; k < PT; ++k) {
-      const int e = 256 * k + t, r = e & 63, cc = e >> 6;
-      XI[r * (CB + 1) + cc] = vi[k];
-      XJ[r * (CB + 1) + cc] = vj[k];
-    }
-    __syncthreads();
-    if (K + 1 < nbk) load(K + 1);
-    for (int r0 = 0; r0 < CB; r0 += 8) {
-      double a[8][4], b[8][4];  // 8 rows read together (pinned), then 128 FMAs
+      for (int u = 0; u < 4; ++u) {
+        a[r][u] = XI[(r0 + r) * (CB + 1) + ra + u];
+        b[r][u] = XJ[(r0 + r) * (CB + 1) + cb + u];
+      }
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
+    for (int r = 0; r < 8; ++r)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          a[r][u] = XI[(r0 + r) * (CB + 1) + ra + u];
-          b[r][u] = XJ[(r0 + r) * (CB + 1) + cb + u];
-        }
+      for (int u = 0; u < 4; ++u) asm volatile("" : "+v"(a[r][u]), "+v"(b[r][u]));
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
+    for (int r = 0; r < 8; ++r)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) asm volatile("" : "+v"(a[r][u]), "+v"(b[r][u]));
+      for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[u][v] += a[r][u] * b[r][v];
-    }
+        for (int v = 0; v < 4; ++v) acc[u][v] += a[r][u] * b[r][v];
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -665,10 +665,30 @@ __global__ __launch_bounds__(256) void k_xtx(const double *X, double *Ai, int n,
     for (int v = 0; v < 4; ++v) {
       const int64_t col = (int64_t)CB * J + cb + v;
       if (row < n && col < n) {
-        Ai[row + col * n] = acc[u][v];
-        if (I != J) Ai[col + row * n] = acc[u][v];
+        const double val = (K == I) ? acc[u][v] : Ai[row + col * n] + acc[u][v];
+        Ai[row + col * n] = val;
+        if (K == nbk - 1 && I != J) Ai[col + row * n] = val;
       }
     }
+  }
+}
+
+template <bool FULL>
+__global__ __launch_bounds__(256) void k_coarse_launch(double *A, double *Lb, double *X, int n,
+                                                       int nbk, int jb, int k, int kb, int m,
+                                                       int nchol, int nx, int *info)
+{
+  __shared__ double lds[kCoarseLds];
+  const int b = blockIdx.x;
+  if (b < nchol) {
+    chol_part<FULL>(A, Lb, n, CB * k, kb, m, info, b, lds);
+  } else if (b < nchol + nx) {
+    const int xb = b - nchol;
+    x_part(Lb, X, n, jb, k - 1, xb / XPARTS, xb % XPARTS, lds);
+  } else {
+    int I, J;
+    tile_ij(b - nchol - nx, I, J);
+    xtx_part(X, A, n, nbk, k - 2, I, J, lds);
   }
 }
 
@@ -679,30 +699,25 @@ void coarse_factor_invert(hipStream_t s, double *A, double *Lb, double *X, int n
 {
   const int nbk = (n + CB - 1) / CB;
   const int jb = std::min((bw + CB - 1) / CB, nbk);
-  if (jb >= 32)
-    throw Error(TOFE_ERR_ARG, "coarse grid too wide for the dense coarse solve (band " +
-                                  std::to_string(bw) + "); use more multigrid levels");
   TOFE_HIP(hipMemsetAsync(Lb, 0, sizeof(double) * (size_t)n * n, s));
-  for (int k0 = 0; k0 < n; k0 += CB) {
-    const int kb = std::min(CB, n - k0);
-    const int m = std::min(n - k0 - kb, bw);
-    const int nI = (m + CB - 1) / CB;
-    const int tiles = m > 0 ? nI * (nI + 1) / 2 : 1;
-    if (kb == CB)
-      k_chol_step<true><<<dim3(tiles), dim3(256), 0, s>>>(A, Lb, n, k0, kb, m, info);
+  for (int k = 0; k <= nbk + 1; ++k) {
+    int kb = 0, m = 0, nchol = 0;
+    if (k < nbk) {
+      kb = std::min(CB, n - CB * k);
+      m = std::min(n - CB * k - kb, bw);
+      const int nI = (m + CB - 1) / CB;
+      nchol = m > 0 ? nI * (nI + 1) / 2 : 1;
+    }
+    const int nx = (k >= 1 && k <= nbk) ? k * XPARTS : 0;
+    const int K = k - 2;
+    const int nxtx = (K >= 0) ? (K + 1) * (K + 2) / 2 : 0;
+    const dim3 grid((unsigned)(nchol + nx + nxtx)), block(256);
+    if (kb == CB || k >= nbk)
+      k_coarse_launch<true><<<grid, block, 0, s>>>(A, Lb, X, n, nbk, jb, k, kb, m, nchol, nx, info);
     else
-      k_chol_step<false><<<dim3(tiles), dim3(256), 0, s>>>(A, Lb, n, k0, kb, m, info);
+      k_coarse_launch<false><<<grid, block, 0, s>>>(A, Lb, X, n, nbk, jb, k, kb, m, nchol, nx, info);
     TOFE_LAUNCH_CHECK();
   }
-  if (jb < 8)
-    k_tri_inv<4, 8><<<dim3(nbk * 4), dim3(256), 0, s>>>(Lb, X, n, nbk, jb);
-  else if (jb < 16)
-    k_tri_inv<2, 16><<<dim3(nbk * 8), dim3(256), 0, s>>>(Lb, X, n, nbk, jb);
-  else
-    k_tri_inv<1, 32><<<dim3(nbk * 16), dim3(256), 0, s>>>(Lb, X, n, nbk, jb);
-  TOFE_LAUNCH_CHECK();
-  k_xtx<<<dim3(nbk * (nbk + 1) / 2), dim3(256), 0, s>>>(X, A, n, nbk);
-  TOFE_LAUNCH_CHECK();
 }
 
 // grids below this many nodes run one thread per DOF (see split_index)
