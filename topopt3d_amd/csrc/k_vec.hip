@@ -70,6 +70,11 @@ __global__ void k_cheb_zero_from64(const double *__restrict__ b64, float *__rest
   }
 }
 
+// Both kernels below visit their (up to 27 or 8) partners unconditionally: an index outside
+// the grid is clamped to a valid node and its weight is zero, so no branch separates the
+// loads; the sums run in the same ascending order as a skip-based loop and the zero terms
+// leave them unchanged.
+
 // b_c(I) = sum over fine nodes i within one fine step of 2I of w(i,I) r_f(i)  (P^T r)
 template <typename T>
 __global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int Nfx, int Nfy,
@@ -80,24 +85,33 @@ __global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int N
   const int64_t I = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (I >= ncn) return;
   const int Ix = (int)(I % Ncx), Iy = (int)((I / Ncx) % Ncy), Iz = (int)(I / ((int64_t)Ncx * Ncy));
-  double acc[3] = {0, 0, 0};
-  for (int iz = 2 * Iz - 1; iz <= 2 * Iz + 1; ++iz) {
-    if (iz < 0 || iz >= Nfz) continue;
-    const double wz = w1d(iz, Iz, Nfz);
-    for (int iy = 2 * Iy - 1; iy <= 2 * Iy + 1; ++iy) {
-      if (iy < 0 || iy >= Nfy) continue;
-      const double wy = w1d(iy, Iy, Nfy);
-      for (int ix = 2 * Ix - 1; ix <= 2 * Ix + 1; ++ix) {
-        if (ix < 0 || ix >= Nfx) continue;
-        const double w = w1d(ix, Ix, Nfx) * wy * wz;
-        const int64_t f = ix + (int64_t)Nfx * (iy + (int64_t)Nfy * iz);
+  int ix[3], iy[3], iz[3];
+  double wx[3], wy[3], wz[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) acc[a] += w * (double)rf[3 * f + a];
-      }
-    }
+  for (int d = 0; d < 3; ++d) {
+    const int fx = 2 * Ix - 1 + d, fy = 2 * Iy - 1 + d, fz = 2 * Iz - 1 + d;
+    const bool okx = fx >= 0 && fx < Nfx, oky = fy >= 0 && fy < Nfy, okz = fz >= 0 && fz < Nfz;
+    ix[d] = okx ? fx : 2 * Ix;
+    iy[d] = oky ? fy : 2 * Iy;
+    iz[d] = okz ? fz : 2 * Iz;
+    wx[d] = okx ? w1d(fx, Ix, Nfx) : 0.0;
+    wy[d] = oky ? w1d(fy, Iy, Nfy) : 0.0;
+    wz[d] = okz ? w1d(fz, Iz, Nfz) : 0.0;
   }
+  double acc[3] = {0, 0, 0};
 #pragma unroll
-  for (int a = 0; a < 3; ++a) bc[3 * I + a] = (T)acc[a];
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double w = wx[a] * wy[b] * wz[c];
+        const int64_t f = ix[a] + (int64_t)Nfx * (iy[b] + (int64_t)Nfy * iz[c]);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q] += w * (double)rf[3 * f + q];
+      }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) bc[3 * I + q] = (T)acc[q];
 }
 
 // x_f(i) += sum over coarse parents J of w(i,J) x_c(J)  (P x_c)
@@ -109,29 +123,35 @@ __global__ __launch_bounds__(NT) void k_prolong_add(const T *__restrict__ xc, in
   const int64_t nfn = (int64_t)Nfx * Nfy * Nfz;
   const int64_t f = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (f >= nfn) return;
-  const int ix = (int)(f % Nfx), iy = (int)((f / Nfx) % Nfy), iz = (int)(f / ((int64_t)Nfx * Nfy));
+  const int fx = (int)(f % Nfx), fy = (int)((f / Nfx) % Nfy), fz = (int)(f / ((int64_t)Nfx * Nfy));
+  // parents (i - 1) >> 1 and (i + 1) >> 1 (the same one for even i), weights w1d
+  int Jx[2], Jy[2], Jz[2];
+  double wx[2], wy[2], wz[2];
+  auto parents = [](int i, int Nf, int Nc, int (&J)[2], double (&w)[2]) {
+    const int j0 = (i - 1) >> 1, j1 = (i + 1) >> 1;
+    const bool ok0 = j0 >= 0 && j0 < Nc, ok1 = j1 != j0 && j1 >= 0 && j1 < Nc;
+    J[0] = ok0 ? j0 : j1;
+    J[1] = ok1 ? j1 : J[0];
+    w[0] = ok0 ? w1d(i, j0, Nf) : 0.0;
+    w[1] = ok1 ? w1d(i, j1, Nf) : 0.0;
+  };
+  parents(fx, Nfx, Ncx, Jx, wx);
+  parents(fy, Nfy, Ncy, Jy, wy);
+  parents(fz, Nfz, Ncz, Jz, wz);
   double acc[3] = {0, 0, 0};
-  for (int Jz = (iz - 1) >> 1; Jz <= (iz + 1) >> 1; ++Jz) {
-    if (Jz < 0 || Jz >= Ncz) continue;
-    const double wz = w1d(iz, Jz, Nfz);
-    if (wz == 0.0) continue;
-    for (int Jy = (iy - 1) >> 1; Jy <= (iy + 1) >> 1; ++Jy) {
-      if (Jy < 0 || Jy >= Ncy) continue;
-      const double wy = w1d(iy, Jy, Nfy);
-      if (wy == 0.0) continue;
-      for (int Jx = (ix - 1) >> 1; Jx <= (ix + 1) >> 1; ++Jx) {
-        if (Jx < 0 || Jx >= Ncx) continue;
-        const double wx = w1d(ix, Jx, Nfx);
-        if (wx == 0.0) continue;
-        const double w = wx * wy * wz;
-        const int64_t c = Jx + (int64_t)Ncx * (Jy + (int64_t)Ncy * Jz);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) acc[a] += w * (double)xc[3 * c + a];
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const double w = wx[a] * wy[b] * wz[c];
+        const int64_t J = Jx[a] + (int64_t)Ncx * (Jy[b] + (int64_t)Ncy * Jz[c]);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q] += w * (double)xc[3 * J + q];
       }
-    }
-  }
 #pragma unroll
-  for (int a = 0; a < 3; ++a) xf[3 * f + a] = (T)((double)xf[3 * f + a] + acc[a]);
+  for (int q = 0; q < 3; ++q) xf[3 * f + q] = (T)((double)xf[3 * f + q] + acc[q]);
 }
 
 template <typename Ta, typename Tb>
@@ -233,7 +253,7 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ x, double *_
                                                const double *__restrict__ q, int64_t n,
                                                const double *__restrict__ pq_part,
                                                const double *__restrict__ pr_part, int nparts,
-                                               double *scal, double *partial)
+                                               double *scal, double *partial, FirstCheb fc)
 {
   __shared__ double sred[NT / 64];
   const double pr = all_sum(pr_part, nparts, sred);
@@ -250,6 +270,13 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ x, double *_
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     s += ri * ri;
+    if (fc.b32) {  // the V-cycle's first step on the new residual (as k_cheb_zero_from64)
+      const float b = (float)ri;
+      fc.b32[i] = b;
+      const float v = fc.c2 * (fc.dinv[i] * b);
+      fc.d[i] = v;
+      fc.x[i] = v;
+    }
   }
   const double t = block_sum<NT>(s, sred);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -423,10 +450,10 @@ void record_scalar(hipStream_t s, const double *scal, int slot, double *dst)
 
 void fcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
                    int64_t n, const double *pq_part, const double *pr_part, int nparts,
-                   double *scal, double *partial)
+                   double *scal, double *partial, const FirstCheb &fc)
 {
   k_fcg_xr<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(x, r, p, q, n, pq_part, pr_part, nparts, scal,
-                                                 partial);
+                                                 partial, fc);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -304,8 +304,27 @@ struct VcycleIO {
   double *x64 = nullptr;
   const double *q64 = nullptr;
   double *partial = nullptr;
-  bool fused_out = false;  // set when the last step wrote x64 and the partials
+  bool first_done = false;  // b0, d and the first iterate already hold the first step
+  bool fused_out = false;   // set when the last step wrote x64 and the partials
 };
+
+// Where the mixed V-cycle's first fine-level Chebyshev step goes (b0, d, the iterate it
+// starts from), for kernels that form that step themselves; x.b32 == nullptr when the
+// V-cycle has no such step.
+FirstCheb vcycle_first_step(tofe_ctx *c)
+{
+  FirstCheb f;
+  const int nu1 = c->desc.nu_pre, nu2 = c->desc.nu_post;
+  if (c->built_precision != TOFE_PREC_MIXED || c->L == 1 || nu1 == 0) return f;
+  Level<float> &lv = *c->lv32[0];
+  const int flips = std::max(nu1 - 1, 0) + nu2;
+  f.b32 = c->vr32.p;
+  f.d = lv.d.p;
+  f.x = (flips & 1) ? lv.x2.p : c->vz32.p;
+  f.dinv = lv.dinv.p;
+  f.c2 = (float)c->c2[0];
+  return f;
+}
 
 // Multigrid V-cycle: x0 = M^-1 b0 (PCMG V-cycle, Chebyshev-Jacobi smoothers,
 // Galerkin coarse operators, direct coarse solve).
@@ -335,7 +354,9 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
     if (from64 && nu1 == 0) convert<double, T>(s, io->b64, const_cast<T *>(b0), nd);
     if (nu1 > 0) {
       if constexpr (sizeof(T) == 4) {
-        if (from64)
+        if (from64 && io->first_done) {
+          // formed by the kernel that produced b64
+        } else if (from64)
           cheb_zero_from64(s, io->b64, const_cast<float *>(b0), lv.dinv.p, (float)c2[0], lv.d.p,
                            xc, nd);
         else
@@ -391,12 +412,14 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
 
 // z = M^-1 r.  With q given (flexible CG), also the partial sums of z.q into c->partial;
 // returns how many there are.
-int precondition(tofe_ctx *c, const double *r, double *z, const double *q = nullptr)
+int precondition(tofe_ctx *c, const double *r, double *z, const double *q = nullptr,
+                 bool first_done = false)
 {
   const hipStream_t s = c->stream;
   if (c->built_precision == TOFE_PREC_MIXED) {
     VcycleIO io;
     io.b64 = r;
+    io.first_done = first_done;
     if (q && c->L > 1 && !c->fine_mfma) {
       io.x64 = z; io.q64 = q; io.partial = c->partial.p;
     }
@@ -475,9 +498,11 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
   // whenever the step is predicted (from the last contraction rate) not to converge: the
   // device then never waits for the host's test.  A converged step leaves them unused.
   double rn_prev = rn, rate = 0.5;
+  // mixed: the FCG update also forms the next V-cycle's first fine-level step
+  const FirstCheb first = flex ? vcycle_first_step(c) : FirstCheb();
   auto next_direction = [&] {
     if (flex) {
-      const int nparts = precondition(c, r, z, q);
+      const int nparts = precondition(c, r, z, q, first.b32 != nullptr);
       fcg_update_p(s, p, z, n, c->partial.p, nparts, c->scal.p);
     } else {
       precondition(c, r, z);
@@ -493,7 +518,8 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
       ep.b = r;
       ep.partial2 = c->partial2.p;
       level_apply<double>(c, 0, p, EPI_APPLY_DOT2, ep);
-      fcg_update_xr(s, U, r, p, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p);
+      fcg_update_xr(s, U, r, p, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p,
+                    first);
       reduce_final(s, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p);
     } else {
       level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);

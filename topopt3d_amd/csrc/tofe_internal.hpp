@@ -163,9 +163,16 @@ void record_scalar(hipStream_t s, const double *scal, int slot, double *dst);
 // alpha = sum(pr_part) / sum(pq_part) (stored with both sums in scal), x += alpha p,
 // r -= alpha q, partials of r.r; fcg_update_p forms beta = -sum(zq_part) / scal[PQ] and
 // p = z + beta p.
+// Optional output of fcg_update_xr: the mixed V-cycle's first Chebyshev step on the new
+// residual (b32 = (float) r, d = x = c2 dinv b32), when b32 is set.
+struct FirstCheb {
+  float *b32 = nullptr, *d = nullptr, *x = nullptr;
+  const float *dinv = nullptr;
+  float c2 = 0;
+};
 void fcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
                    int64_t n, const double *pq_part, const double *pr_part, int nparts,
-                   double *scal, double *partial);
+                   double *scal, double *partial, const FirstCheb &fc = FirstCheb());
 void fcg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *zq_part,
                   int nparts, double *scal);
 // Fused Jacobi-CG steps of the eigenvalue estimate (no separate reduction launches):
