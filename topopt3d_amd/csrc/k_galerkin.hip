@@ -61,6 +61,73 @@ __device__ inline int elem_form(int ex, int nel, int Nf)
   return 0;
 }
 
+// Block (I, I + D) of the level-1 operator for the compile-time offset O: the sum over
+// the fine elements whose coarse window holds both nodes, in ascending element order.
+// Per direction the candidate elements are ex = 2 I - 2 + d, d = 0..3 (window start
+// I - 1 + d / 2); both nodes lie in the window for d = 0..3 (D = 0), 2..3 (D = +1) or
+// 0..1 (D = -1), so positions, local node indices and the interpolation form of a regular
+// element are compile-time constants and its Q^T KE Q block comes through uniform
+// (scalar) loads.  Elements with a Dirichlet DOF (their own masked matrix) and the last
+// element of a grid with an even node count (single-parent form) take a general path.
+template <int O>
+__device__ inline void galerkin_block(const double *__restrict__ E, const int *__restrict__ emap,
+                                      const double *__restrict__ KEC,
+                                      const double *__restrict__ KCm, const int (&Ic)[3],
+                                      const int (&nel)[3], const int (&Nf)[3], double (&acc)[9])
+{
+  constexpr int DX = O % 3 - 1, DY = (O / 3) % 3 - 1, DZ = O / 9 - 1;
+  constexpr int X0 = DX > 0 ? 2 : 0, X1 = DX < 0 ? 2 : 4;
+  constexpr int Y0 = DY > 0 ? 2 : 0, Y1 = DY < 0 ? 2 : 4;
+  constexpr int Z0 = DZ > 0 ? 2 : 0, Z1 = DZ < 0 ? 2 : 4;
+#pragma unroll
+  for (int dz = Z0; dz < Z1; ++dz) {
+    const int ez = 2 * Ic[2] - 2 + dz;
+    if (ez < 0 || ez >= nel[2]) continue;
+#pragma unroll
+    for (int dy = Y0; dy < Y1; ++dy) {
+      const int ey = 2 * Ic[1] - 2 + dy;
+      if (ey < 0 || ey >= nel[1]) continue;
+#pragma unroll
+      for (int dx = X0; dx < X1; ++dx) {
+        const int ex = 2 * Ic[0] - 2 + dx;
+        if (ex < 0 || ex >= nel[0]) continue;
+        const int LI = lidx(1 - (dx >> 1), 1 - (dy >> 1), 1 - (dz >> 1));
+        const int LJ = lidx(1 - (dx >> 1) + DX, 1 - (dy >> 1) + DY, 1 - (dz >> 1) + DZ);
+        const int64_t e = ex + (int64_t)nel[0] * (ey + (int64_t)nel[1] * ez);
+        const int m = emap[e];
+        // single-parent form: even position (even ex) that is the last element of a grid
+        // with an even node count
+        const bool sx = !(dx & 1) && (Nf[0] & 1) == 0 && ex == Nf[0] - 2;
+        const bool sy = !(dy & 1) && (Nf[1] & 1) == 0 && ey == Nf[1] - 2;
+        const bool sz = !(dz & 1) && (Nf[2] & 1) == 0 && ez == Nf[2] - 2;
+        if (m < 0 && !(sx || sy || sz)) {
+          const int F = (dx & 1) + 3 * (dy & 1) + 9 * (dz & 1);
+          const double Ee = E[e];
+          const double *K = KEC + F * 576 + 3 * LI * 24 + 3 * LJ;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += Ee * K[a * 24 + b];
+        } else if (m < 0) {
+          const int F = (sx ? 2 : (dx & 1)) + 3 * (sy ? 2 : (dy & 1)) + 9 * (sz ? 2 : (dz & 1));
+          const double Ee = E[e];
+          const double *K = KEC + (int64_t)F * 576 + 3 * LI * 24 + 3 * LJ;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += Ee * K[a * 24 + b];
+        } else {
+          const double *K = KCm + (int64_t)m * 576 + 3 * LI * 24 + 3 * LJ;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += K[a * 24 + b];
+        }
+      }
+    }
+  }
+}
+
 // One thread per (coarse node I, coarse offset o >= 13); the transposed block goes to
 // (J, 26 - o) so the level-1 operator is exactly symmetric.
 __global__ __launch_bounds__(NT) void k_galerkin_fine(const double *__restrict__ E,
@@ -86,48 +153,25 @@ __global__ __launch_bounds__(NT) void k_galerkin_fine(const double *__restrict__
     Jc[d] = Ic[d] + D[d];
     if (Jc[d] < 0 || Jc[d] >= Nc[d]) return;
   }
-  // per direction: fine elements whose coarse window holds both I and J
-  int ex_l[3][4], lI_l[3][4], lJ_l[3][4], f_l[3][4], cnt[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    int c = 0;
-    const int lo = max(max(Ic[d], Jc[d]) - 1, 0), hi = min(min(Ic[d], Jc[d]), Nc[d] - 1);
-    for (int Jlo = lo; Jlo <= hi; ++Jlo)
-      for (int ex = 2 * Jlo; ex <= 2 * Jlo + 1; ++ex) {
-        if (ex >= nel[d]) continue;
-        ex_l[d][c] = ex;
-        lI_l[d][c] = Ic[d] - Jlo;
-        lJ_l[d][c] = Jc[d] - Jlo;
-        f_l[d][c] = elem_form(ex, nel[d], Nf[d]);
-        ++c;
-      }
-    cnt[d] = c;
-  }
   double acc[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) acc[q] = 0.0;
-  for (int cz = 0; cz < cnt[2]; ++cz)
-    for (int cy = 0; cy < cnt[1]; ++cy)
-      for (int cx = 0; cx < cnt[0]; ++cx) {
-        const int64_t e = ex_l[0][cx] + (int64_t)nx * (ex_l[1][cy] + (int64_t)ny * ex_l[2][cz]);
-        const int LI = lidx(lI_l[0][cx], lI_l[1][cy], lI_l[2][cz]);
-        const int LJ = lidx(lJ_l[0][cx], lJ_l[1][cy], lJ_l[2][cz]);
-        const int m = emap[e];
-        if (m < 0) {
-          const double Ee = E[e];
-          const double *K = KEC + (int64_t)(f_l[0][cx] + 3 * f_l[1][cy] + 9 * f_l[2][cz]) * 576;
-#pragma unroll
-          for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += Ee * K[(3 * LI + a) * 24 + 3 * LJ + b];
-        } else {
-          const double *K = KCm + (int64_t)m * 576;
-#pragma unroll
-          for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += K[(3 * LI + a) * 24 + 3 * LJ + b];
-        }
-      }
+  switch (o) {
+    case 13: galerkin_block<13>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 14: galerkin_block<14>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 15: galerkin_block<15>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 16: galerkin_block<16>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 17: galerkin_block<17>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 18: galerkin_block<18>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 19: galerkin_block<19>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 20: galerkin_block<20>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 21: galerkin_block<21>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 22: galerkin_block<22>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 23: galerkin_block<23>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 24: galerkin_block<24>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    case 25: galerkin_block<25>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+    default: galerkin_block<26>(E, emap, KEC, KCm, Ic, nel, Nf, acc); break;
+  }
   // P^T M P: unit diagonal of the fixed fine DOFs
   if (fix) {
     int il[3][3], cn[3];
