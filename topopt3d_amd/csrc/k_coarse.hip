@@ -45,7 +45,8 @@ __global__ __launch_bounds__(NT) void k_stencil_op(StencilOp<T> op, const T *__r
   double dsum = 0, dsum2 = 0;
   if (n < nn) {
     const int Nx = op.Nx, Ny = op.Ny, Nz = op.Nz;
-    const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+    int i, j, k;
+    node_ijk(n, Nx, Ny, i, j, k);
     constexpr int NA = SPLIT ? 1 : 3;  // components summed by this thread
     T acc[NA];
 #pragma unroll
@@ -106,15 +107,14 @@ __global__ __launch_bounds__(NT) void k_galerkin_axis(const double *__restrict__
   const int Nc = (Nf + 1) / 2;
   m[axis] = Nc;
   const int64_t nin = (int64_t)n0 * n1 * n2, nout = (int64_t)m[0] * m[1] * m[2];
-  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
-  if (t >= 243 * nout) return;
-  const int oq = (int)(t / nout);
-  const int64_t node = t - (int64_t)oq * nout;
+  // grid: x over the output nodes, y over the 243 (offset, component pair) values
+  const int64_t node = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (node >= nout) return;
+  const int oq = blockIdx.y;
+  const int64_t t = (int64_t)oq * nout + node;
   const int o = oq / 9, q = oq % 9;
   int c[3];
-  c[0] = (int)(node % m[0]);
-  c[1] = (int)((node / m[0]) % m[1]);
-  c[2] = (int)(node / ((int64_t)m[0] * m[1]));
+  node_ijk(node, m[0], m[1], c[0], c[1], c[2]);
   const int off[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
   // the coupled output node must exist
   bool inside = true;
@@ -135,18 +135,22 @@ __global__ __launch_bounds__(NT) void k_galerkin_axis(const double *__restrict__
   cin[axis] = 0;
   const int64_t base_node = cin[0] + (int64_t)n0 * (cin[1] + (int64_t)n1 * cin[2]);
   const int o_base = o - (D + 1) * so;
+  // The nine (i, d) terms are loaded unconditionally (an i outside the grid reads node 2I,
+  // a missing term has weight 0), so no branch separates the loads; the sum runs in the
+  // same (i, d) order as with the missing terms skipped, which they leave unchanged.
   double acc = 0.0;
-  for (int i = 2 * I - 1; i <= 2 * I + 1; ++i) {
-    if (i < 0 || i >= Nf) continue;
-    const double wi = w1d(i, I, Nf);
-    if (wi == 0.0) continue;
+#pragma unroll
+  for (int di = -1; di <= 1; ++di) {
+    const int i = 2 * I + di;
+    const bool iok = i >= 0 && i < Nf;
+    const int ic = iok ? i : 2 * I;
+    const double wi = iok ? w1d(i, I, Nf) : 0.0;
+#pragma unroll
     for (int d = -1; d <= 1; ++d) {
       const int j = i + d;
-      if (j < 0 || j >= Nf) continue;
-      const double wj = w1d(j, I + D, Nf);
-      if (wj == 0.0) continue;
+      const double wj = (iok && j >= 0 && j < Nf) ? w1d(j, I + D, Nf) : 0.0;
       const int oin = o_base + (d + 1) * so;
-      acc += wi * wj * in[(int64_t)(oin * 9 + q) * nin + base_node + (int64_t)i * sa];
+      acc += wi * wj * in[(int64_t)(oin * 9 + q) * nin + base_node + (int64_t)ic * sa];
     }
   }
   out[t] = acc;
@@ -174,7 +178,8 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
   double dsum = 0, dsum2 = 0;
   if (n < nn) {
     const int Nx = op.Nx, Ny = op.Ny, Nz = op.Nz;
-    const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+    int i, j, k;
+    node_ijk(n, Nx, Ny, i, j, k);
     constexpr int NA = SPLIT ? 1 : 3;
     T acc[NA];
 #pragma unroll
@@ -244,10 +249,12 @@ __global__ void k_stencil_dense(const double *__restrict__ S, int Nx, int Ny, in
   const int64_t nn = (int64_t)Nx * Ny * Nz, n = 3 * nn;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= 27 * nn) return;
-  const int o = (int)(t / nn);
-  const int64_t I = t - (int64_t)o * nn;
+  int o;
+  int64_t I;
+  udivmod(t, nn, o, I);
   const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
-  const int i = (int)(I % Nx), j = (int)((I / Nx) % Ny), k = (int)(I / ((int64_t)Nx * Ny));
+  int i, j, k;
+  node_ijk(I, Nx, Ny, i, j, k);
   if (i + dx < 0 || i + dx >= Nx || j + dy < 0 || j + dy >= Ny || k + dz < 0 || k + dz >= Nz) return;
   const int64_t J = I + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz);
 #pragma unroll
@@ -790,14 +797,14 @@ void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double
   const int Ncx = (Nfx + 1) / 2, Ncy = (Nfy + 1) / 2, Ncz = (Nfz + 1) / 2;
   const int64_t n1 = (int64_t)Ncx * Nfy * Nfz, n2 = (int64_t)Ncx * Ncy * Nfz,
                 n3 = (int64_t)Ncx * Ncy * Ncz;
-  k_galerkin_axis<<<dim3((unsigned)((243 * n1 + NT - 1) / NT)), dim3(NT), 0, s>>>(Sf, Nfx, Nfy, Nfz,
-                                                                                  0, T1, 0);
+  k_galerkin_axis<<<dim3((unsigned)((n1 + NT - 1) / NT), 243), dim3(NT), 0, s>>>(Sf, Nfx, Nfy, Nfz, 0,
+                                                                                T1, 0);
   TOFE_LAUNCH_CHECK();
-  k_galerkin_axis<<<dim3((unsigned)((243 * n2 + NT - 1) / NT)), dim3(NT), 0, s>>>(T1, Ncx, Nfy, Nfz,
-                                                                                  1, T2, 0);
+  k_galerkin_axis<<<dim3((unsigned)((n2 + NT - 1) / NT), 243), dim3(NT), 0, s>>>(T1, Ncx, Nfy, Nfz, 1,
+                                                                                T2, 0);
   TOFE_LAUNCH_CHECK();
-  k_galerkin_axis<<<dim3((unsigned)((243 * n3 + NT - 1) / NT)), dim3(NT), 0, s>>>(T2, Ncx, Ncy, Nfz,
-                                                                                  2, Sc, 1);
+  k_galerkin_axis<<<dim3((unsigned)((n3 + NT - 1) / NT), 243), dim3(NT), 0, s>>>(T2, Ncx, Ncy, Nfz, 2,
+                                                                                Sc, 1);
   TOFE_LAUNCH_CHECK();
 }
 

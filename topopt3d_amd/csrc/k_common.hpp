@@ -23,6 +23,25 @@ __device__ inline int xcd_remap(int bid, int nb)
   return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
 
+// (i, j, k) of node n of an Nx x Ny grid (x fastest), in 32-bit unsigned arithmetic: node
+// counts stay below 2^31 (checked when a context is created), and a 64-bit division costs
+// an order of magnitude more instructions than the rest of a light kernel's index work.
+__device__ inline void node_ijk(int64_t n, int Nx, int Ny, int &i, int &j, int &k)
+{
+  const unsigned u = (unsigned)n, t = u / (unsigned)Nx;
+  i = (int)(u - t * (unsigned)Nx);
+  k = (int)(t / (unsigned)Ny);
+  j = (int)(t - (unsigned)k * (unsigned)Ny);
+}
+
+// q = t / d, r = t - q d for 0 <= t < 2^32
+__device__ inline void udivmod(int64_t t, int64_t d, int &q, int64_t &r)
+{
+  const unsigned qq = (unsigned)t / (unsigned)d;
+  q = (int)qq;
+  r = (int64_t)((unsigned)t - qq * (unsigned)d);
+}
+
 __device__ inline double wave_sum(double v)
 {
 #pragma unroll

@@ -288,7 +288,8 @@ __global__ __launch_bounds__(NT) void k_mass_diag(const double *__restrict__ V, 
   const int64_t nn = (int64_t)Nx * Ny * Nz;
   const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (n >= nn) return;
-  const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+  int i, j, k;
+  node_ijk(n, Nx, Ny, i, j, k);
   double s = 0.0;
   for (int dz = -1; dz <= 0; ++dz)
     for (int dy = -1; dy <= 0; ++dy)

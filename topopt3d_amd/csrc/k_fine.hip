@@ -250,11 +250,13 @@ __global__ __launch_bounds__(256) void k_fine_stencil(FineOp<double> op, double 
   __syncthreads();
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= 27 * nn) return;
-  const int o = (int)(t / nn);
-  const int64_t n = t - (int64_t)o * nn;
+  int o;
+  int64_t n;
+  udivmod(t, nn, o, n);
   const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
   const int Nx = op.nx + 1, Ny = op.ny + 1, Nz = op.nz + 1;
-  const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+  int i, j, k;
+  node_ijk(n, Nx, Ny, i, j, k);
   const int mi = i + dx, mj = j + dy, mk = k + dz;
   double blk[9];
 #pragma unroll
@@ -336,6 +338,8 @@ void fine_stencil_build(hipStream_t s, const FineOp<double> &op, double *S)
 {
   const int64_t nn = (int64_t)(op.nx + 1) * (op.ny + 1) * (op.nz + 1);
   const int64_t nthreads = 27 * nn;
+  if (nthreads >= (int64_t(1) << 32))
+    throw Error(TOFE_ERR_ARG, "single-level fine stencil too large");
   dim3 grid((unsigned)((nthreads + 255) / 256)), block(256);
   k_fine_stencil<<<grid, block, 0, s>>>(op, S, nn);
   TOFE_LAUNCH_CHECK();

@@ -141,12 +141,13 @@ __global__ __launch_bounds__(NT) void k_galerkin_fine(const double *__restrict__
   const int nel[3] = {nx, ny, nz};
   const int Nc[3] = {(Nf[0] + 1) / 2, (Nf[1] + 1) / 2, (Nf[2] + 1) / 2};
   const int64_t ncn = (int64_t)Nc[0] * Nc[1] * Nc[2];
-  const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
-  if (t >= 14 * ncn) return;
-  const int o = 13 + (int)(t / ncn);
-  const int64_t I = t - (int64_t)(o - 13) * ncn;
+  // grid: x over the coarse nodes, y over the 14 upper offsets
+  const int64_t I = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (I >= ncn) return;
+  const int o = 13 + (int)blockIdx.y;
   const int D[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
-  const int Ic[3] = {(int)(I % Nc[0]), (int)((I / Nc[0]) % Nc[1]), (int)(I / ((int64_t)Nc[0] * Nc[1]))};
+  int Ic[3];
+  node_ijk(I, Nc[0], Nc[1], Ic[0], Ic[1], Ic[2]);
   int Jc[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -226,7 +227,8 @@ __global__ void k_fine_diag(const double *__restrict__ E, const double *__restri
   if (t >= 3 * nn) return;
   const int64_t n = t / 3;
   const int a = (int)(t - 3 * n);
-  const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+  int i, j, k;
+  node_ijk(n, Nx, Ny, i, j, k);
   double s = 0.0;
   for (int ez = k - 1; ez <= k; ++ez) {
     if (ez < 0 || ez >= nz) continue;
@@ -257,9 +259,8 @@ void galerkin_fine(hipStream_t s, const double *E, const int *emap, const double
   }
   const int64_t ncn = (int64_t)((nx + 2) / 2) * ((ny + 2) / 2) * ((nz + 2) / 2);
   TOFE_HIP(hipMemsetAsync(Sc, 0, sizeof(double) * 243 * ncn, s));
-  const int64_t nt = 14 * ncn;
-  k_galerkin_fine<<<dim3((unsigned)((nt + NT - 1) / NT)), dim3(NT), 0, s>>>(E, emap, KEC, KCm, fix,
-                                                                           nx, ny, nz, Sc);
+  k_galerkin_fine<<<dim3((unsigned)((ncn + NT - 1) / NT), 14), dim3(NT), 0, s>>>(E, emap, KEC, KCm,
+                                                                                 fix, nx, ny, nz, Sc);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -102,7 +102,8 @@ __global__ __launch_bounds__(NT) void k_apply_ks(const double *__restrict__ H,
   const int64_t nn = (int64_t)Nx * Ny * Nz, ne = (int64_t)nx * ny * nz;
   const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (n >= nn) return;
-  const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+  int i, j, k;
+  node_ijk(n, Nx, Ny, i, j, k);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int dz = -1; dz <= 0; ++dz)
     for (int dy = -1; dy <= 0; ++dy)
@@ -174,7 +175,8 @@ __global__ __launch_bounds__(NT) void k_stab_rhs(const double *__restrict__ phi,
   const int64_t nn = (int64_t)Nx * Ny * Nz;
   const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (n >= nn) return;
-  const int i = (int)(n % Nx), j = (int)((n / Nx) % Ny), k = (int)(n / ((int64_t)Nx * Ny));
+  int i, j, k;
+  node_ijk(n, Nx, Ny, i, j, k);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int dz = -1; dz <= 0; ++dz)
     for (int dy = -1; dy <= 0; ++dy)

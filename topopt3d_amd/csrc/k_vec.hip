@@ -84,7 +84,8 @@ __global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int N
   const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
   const int64_t I = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (I >= ncn) return;
-  const int Ix = (int)(I % Ncx), Iy = (int)((I / Ncx) % Ncy), Iz = (int)(I / ((int64_t)Ncx * Ncy));
+  int Ix, Iy, Iz;
+  node_ijk(I, Ncx, Ncy, Ix, Iy, Iz);
   int ix[3], iy[3], iz[3];
   double wx[3], wy[3], wz[3];
 #pragma unroll
@@ -123,7 +124,8 @@ __global__ __launch_bounds__(NT) void k_prolong_add(const T *__restrict__ xc, in
   const int64_t nfn = (int64_t)Nfx * Nfy * Nfz;
   const int64_t f = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (f >= nfn) return;
-  const int fx = (int)(f % Nfx), fy = (int)((f / Nfx) % Nfy), fz = (int)(f / ((int64_t)Nfx * Nfy));
+  int fx, fy, fz;
+  node_ijk(f, Nfx, Nfy, fx, fy, fz);
   // parents (i - 1) >> 1 and (i + 1) >> 1 (the same one for even i), weights w1d
   int Jx[2], Jy[2], Jz[2];
   double wx[2], wy[2], wz[2];

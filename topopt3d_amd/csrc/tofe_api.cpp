@@ -929,6 +929,9 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     c->ne = (int64_t)d.nx * d.ny * d.nz;
     c->nn = c->grids[0].nn();
     c->ndof = 3 * c->nn;
+    // kernels index nodes in 32-bit arithmetic
+    if (c->nn >= (int64_t(1) << 31))
+      throw Error(TOFE_ERR_ARG, "mesh has 2^31 or more nodes");
     // dense direct solve on the coarsest level: n^2 doubles, half-bandwidth below 31 tiles
     const Grid &gc = c->grids[c->L - 1];
     if (gc.ndof() > 20000 || 3 * (gc.Nx * gc.Ny + gc.Nx + 1) + 2 > 31 * kCholNB - 63)
