@@ -77,6 +77,24 @@ def test_operator_matches_assembled_K(oracle, tofe, nel, L):
         ctx.close()
 
 
+@pytest.mark.parametrize("tile", ["0:3", "1:8", "2:5", "3:16", "1:1"])
+@pytest.mark.parametrize("nel", [(30, 17, 9), (7, 5, 3)])
+def test_operator_tile_shapes(oracle, tofe, nel, tile, monkeypatch):
+    """Every compiled column-tile shape and z-chunk length of the fine operator (picked with
+    TOFE_FINE_TILE) gives the oracle's K u, in FP64 and inside the mixed solve."""
+    monkeypatch.setenv("TOFE_FINE_TILE", tile)
+    print(f"tile {tile}", flush=True)
+    pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, 2)
+    X = np.random.default_rng(5).standard_normal(pb.ndof)
+    assert rel(ctx.apply_K(X), K.matvec(X)) <= 1e-14
+    ctx.set_precision(tofe.PREC_MIXED)
+    ctx.assemble()
+    U, its, reason = ctx.solve(pb.F, rtol=1e-10, maxit=200)
+    assert reason == tofe.CONVERGED_RTOL
+    assert np.linalg.norm(pb.F - K.matvec(U)) <= 1e-10 * np.linalg.norm(pb.F) * 1.01
+    ctx.close()
+
+
 @pytest.mark.parametrize("nel,L", CASES)
 def test_galerkin_levels_match_ptap(oracle, tofe, nel, L):
     pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, L)

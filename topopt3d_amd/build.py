@@ -1,8 +1,7 @@
 """Build libtofe.so (HIP kernels + C ABI) in-tree for gfx950.
 
 Every translation unit under csrc/ is compiled with hipcc in parallel into
-topopt3d_amd/_build/, then linked into topopt3d_amd/libtofe.so against rocBLAS
-(dgemm/dsyrk/dtrtri inside the coarsest-level dense Cholesky factorisation and inverse).
+topopt3d_amd/_build/, then linked into topopt3d_amd/libtofe.so against the HIP runtime only.
 """
 from __future__ import annotations
 
@@ -41,7 +40,7 @@ def _compile(src: str) -> str:
     if src.endswith(".hip"):
         cmd += DEVICE
     cmd += ["-c", src, "-o", obj]
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=1200)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return obj

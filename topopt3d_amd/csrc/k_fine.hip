@@ -4,8 +4,9 @@
 // columns of MatZeroRowsColumns(K, fixed, 1.0) (FEAnalysis.cpp:473): fixed entries of u
 // are zeroed on load and fixed rows return u itself.
 //
-// Node-centric, deterministic, no atomics: a 256-thread block owns a 32x8 column tile of
-// nodes and marches up a chunk of z.  Each step stages one node plane of u (with a
+// Node-centric, deterministic, no atomics: a 256-thread block owns a TX x TY column tile of
+// nodes (one of a few compiled shapes, chosen per mesh so the tiles waste few lanes on the
+// N = 16k + 1 node counts of the usual meshes) and marches up a chunk of z.  Each step stages one node plane of u (with a
 // one-node halo) and one element layer of E in LDS; every thread keeps the 3x3 node
 // neighbourhood of the two planes around the current element layer in registers and
 // adds the layer's four elements to its lower node (which is then complete and written
@@ -17,10 +18,10 @@
 namespace tofe {
 namespace {
 
-constexpr int TX = 32, TY = 8, NT = TX * TY;
-constexpr int UW = TX + 2, UH = TY + 2, UPL = UW * UH * 3;
-constexpr int EW = TX + 1, EH = TY + 1, EPL = EW * EH;
-constexpr int ZLEN = 8;
+constexpr int NT = 256;  // threads per block (4 wavefronts)
+
+// Compiled column-tile shapes (TX x TY nodes, at most 256); the index is FineOp::tile.
+constexpr int kTileShape[kFineTiles][2] = {{32, 8}, {28, 9}, {25, 10}, {22, 11}};
 
 typedef double v8d __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
@@ -98,28 +99,53 @@ __device__ inline void element_rows(const T *base, const T (&ue)[24], T Ee, T (&
   }
 }
 
+// Buffer resource over one staged plane (byte size below 2^31, checked on the host): the
+// staging loads then take one 32-bit offset register each instead of a 64-bit address.
+__device__ inline __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, int bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
+}
 template <typename T>
+__device__ inline T plane_load(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
 struct FineGrid {
   int tiles_x, tiles_y, zchunks, nblocks;
 };
 
-template <typename T>
-FineGrid<T> fine_grid(const FineOp<T> &op)
+FineGrid fine_grid_of(int nx, int ny, int nz, int tile, int zlen)
 {
-  FineGrid<T> g;
-  g.tiles_x = (op.nx + 1 + TX - 1) / TX;
-  g.tiles_y = (op.ny + 1 + TY - 1) / TY;
-  g.zchunks = (op.nz + 1 + ZLEN - 1) / ZLEN;
+  FineGrid g;
+  const int TX = kTileShape[tile][0], TY = kTileShape[tile][1];
+  g.tiles_x = (nx + 1 + TX - 1) / TX;
+  g.tiles_y = (ny + 1 + TY - 1) / TY;
+  g.zchunks = (nz + 1 + zlen - 1) / zlen;
   g.nblocks = g.tiles_x * g.tiles_y * g.zchunks;
   return g;
 }
 
-template <typename T, int EPI>
+template <typename T>
+FineGrid fine_grid(const FineOp<T> &op)
+{
+  if (op.tile < 0 || op.tile >= kFineTiles || op.zlen < 1)
+    throw Error(TOFE_ERR_ARG, "fine operator: invalid tile shape or z-chunk length");
+  if ((int64_t)3 * (op.nx + 1) * (op.ny + 1) * (int64_t)sizeof(T) >= (int64_t(1) << 31))
+    throw Error(TOFE_ERR_ARG, "fine operator: node plane of 2 GiB or more");
+  return fine_grid_of(op.nx, op.ny, op.nz, op.tile, op.zlen);
+}
+
+template <typename T, int EPI, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restrict__ u,
                                                 EpiArgs<T> ep, int tiles_x, int tiles_y,
-                                                int nblocks)
+                                                int nblocks, int zlen)
 {
-  constexpr int PLN = UW * UH;  // nodes per staged plane
+  // threads beyond TX * TY (at most one partial row) mirror row TY - 1 and own no node
+  static_assert(TX * TY <= NT && NT - TX * TY < TX, "tile shape");
+  constexpr int UW = TX + 2, UH = TY + 2, PLN = UW * UH, UPL = 3 * PLN;  // staged node plane
+  constexpr int EW = TX + 1, EH = TY + 1, EPL = EW * EH;                 // staged element layer
   __shared__ T su[3][3][PLN];   // ring of three node planes, one array per component
   __shared__ T sE[2][EPL];
   __shared__ double sred[NT / 64];
@@ -132,41 +158,81 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
   const int rem = bid - bz * txy;
   const int by = rem / tiles_x, bx = rem - by * tiles_x;
   const int x0 = bx * TX, y0 = by * TY;
-  const int k0 = bz * ZLEN, k1 = min(k0 + ZLEN, Nz);
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int k0 = bz * zlen, k1 = min(k0 + zlen, Nz);
+  const int trow = threadIdx.x / TX;
+  const int tx = threadIdx.x - trow * TX, ty = min(trow, TY - 1);
   const int i = x0 + tx, j = y0 + ty;
-  const bool own = (i < Nx) && (j < Ny);
+  const bool own = (trow < TY) && (i < Nx) && (j < Ny);
   const T *KEbase = op.KE;
   const T *__restrict__ E = op.E;
   const uint8_t *__restrict__ fix = op.fix;
 
-  // plane gz of u into ring slot `slot` (Dirichlet entries zeroed); global reads in node
-  // order (coalesced), LDS writes per component
-  auto stage_u = [&](int slot, int gz) {
+  // Staging map, fixed for the block: node slot q of this thread is node nl = t + q * NT of
+  // the staged (TX + 2) x (TY + 2) window (its three DOFs are loaded together; lanes read
+  // consecutive nodes, so each load covers one contiguous span), element slot q is entry
+  // t + q * NT of the staged (TX + 1) x (TY + 1) element window.  Stored: the in-plane node
+  // / element index, or kNone outside the mesh or the window.
+  constexpr int NU = (PLN + NT - 1) / NT, NE = (EPL + NT - 1) / NT;
+  constexpr unsigned kNone = 0xffffffffu;
+  const int NxNy = Nx * Ny, nxny = nx * ny;
+  unsigned u_node[NU], e_idx[NE];
+#pragma unroll
+  for (int q = 0; q < NU; ++q) {
+    const int nl = threadIdx.x + q * NT;
+    const int ly = nl / UW, lx = nl - ly * UW;
+    const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+    u_node[q] = (nl < PLN && gx >= 0 && gx < Nx && gy >= 0 && gy < Ny) ? unsigned(gx + Nx * gy) : kNone;
+  }
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int v = threadIdx.x + q * NT;
+    const int ly = v / EW, lx = v - ly * EW;
+    const int ex = x0 - 1 + lx, ey = y0 - 1 + ly;
+    e_idx[q] = (v < EPL && ex >= 0 && ex < nx && ey >= 0 && ey < ny) ? unsigned(ex + nx * ey) : kNone;
+  }
+  // Node plane gz of u (Dirichlet entries zeroed, zero outside the mesh) into registers, and
+  // from registers into ring slot `slot`; likewise element layer ez of E.  The loads of the
+  // next step's plane and layer are issued before this step's FMAs and stored after them,
+  // so their latency hides behind the arithmetic; the fixed-DOF mask is applied at the
+  // store (a select at the load would wait for the data there).
+  auto load_u = [&](T (&pu)[NU][3], unsigned (&pf)[NU], int gz) {
     const bool zin = gz >= 0 && gz < Nz;
-    for (int v = threadIdx.x; v < UPL; v += NT) {
-      const int nl = v / 3, c = v - 3 * nl;
-      const int ly = nl / UW, lx = nl - ly * UW;
-      const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-      T val = 0;
-      if (zin && gx >= 0 && gx < Nx && gy >= 0 && gy < Ny) {
-        const int64_t n = gx + (int64_t)Nx * (gy + (int64_t)Ny * gz);
-        val = u[3 * n + c];
-        if (fix && ((fix[n] >> c) & 1)) val = 0;
+    const int64_t z = zin ? gz : 0;
+    const __amdgpu_buffer_rsrc_t ur = plane_rsrc(u + 3 * NxNy * z, 3 * NxNy * (int)sizeof(T));
+    const __amdgpu_buffer_rsrc_t fr = plane_rsrc(fix ? fix + NxNy * z : fix, NxNy);
+#pragma unroll
+    for (int q = 0; q < NU; ++q) {
+      pu[q][0] = pu[q][1] = pu[q][2] = 0;
+      pf[q] = 0;
+      if (zin && u_node[q] != kNone) {
+        const unsigned off = u_node[q] * (3 * sizeof(T));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pu[q][c] = plane_load<T>(ur, off + c * sizeof(T));
+        if (fix) pf[q] = __builtin_amdgcn_raw_buffer_load_b8(fr, u_node[q], 0, 0);
       }
-      su[slot][c][nl] = val;
     }
   };
-  auto stage_E = [&](int buf, int ez) {
-    const bool zin = ez >= 0 && ez < nz;
-    for (int v = threadIdx.x; v < EPL; v += NT) {
-      const int ly = v / EW, lx = v - ly * EW;
-      const int ex = x0 - 1 + lx, ey = y0 - 1 + ly;
-      T val = 0;
-      if (zin && ex >= 0 && ex < nx && ey >= 0 && ey < ny)
-        val = E[ex + (int64_t)nx * (ey + (int64_t)ny * ez)];
-      sE[buf][v] = val;
+  auto store_u = [&](const T (&pu)[NU][3], const unsigned (&pf)[NU], int slot) {
+#pragma unroll
+    for (int q = 0; q < NU; ++q) {
+      const int nl = threadIdx.x + q * NT;
+      if (nl < PLN) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) su[slot][c][nl] = ((pf[q] >> c) & 1u) ? T(0) : pu[q][c];
+      }
     }
+  };
+  auto load_E = [&](T (&pe)[NE], int ez) {
+    const bool zin = ez >= 0 && ez < nz;
+    const __amdgpu_buffer_rsrc_t er = plane_rsrc(E + (int64_t)nxny * (zin ? ez : 0), nxny * (int)sizeof(T));
+#pragma unroll
+    for (int q = 0; q < NE; ++q)
+      pe[q] = (zin && e_idx[q] != kNone) ? plane_load<T>(er, e_idx[q] * (unsigned)sizeof(T)) : T(0);
+  };
+  auto store_E = [&](const T (&pe)[NE], int buf) {
+#pragma unroll
+    for (int q = 0; q < NE; ++q)
+      if (threadIdx.x + q * NT < EPL) sE[buf][threadIdx.x + q * NT] = pe[q];
   };
 
   T alo[3], ahi[3];
@@ -174,16 +240,31 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
   for (int c = 0; c < 3; ++c) ahi[c] = 0;
   double dsum = 0, dsum2 = 0;
 
-  stage_u(0, k0 - 1);
+  {
+    T pu[NU][3], pe[NE];
+    unsigned pf[NU];
+    load_u(pu, pf, k0 - 1);
+    store_u(pu, pf, 0);
+    load_u(pu, pf, k0);
+    store_u(pu, pf, 1);
+    load_E(pe, k0 - 1);
+    store_E(pe, 0);
+  }
+  __syncthreads();
   // Step st (element layer ez = k0 - 1 + st) reads node planes ez (slot st % 3) and ez + 1
-  // (slot (st + 1) % 3); the slot written in step st + 1 was last read in step st - 1, so
-  // the single barrier of step st orders it.
+  // (slot (st + 1) % 3) and layer ez (buffer st % 2), and stages plane ez + 2 into slot
+  // (st + 2) % 3 and layer ez + 1 into buffer (st + 1) % 2: both were last read in step
+  // st - 1, which every wavefront has left at the barrier that ends it.
   int st = 0;
   for (int ez = k0 - 1; ez < k1; ++ez, ++st) {
     const int slo = st % 3, shi = (st + 1) % 3, eb = st & 1;
-    stage_u(shi, ez + 1);
-    stage_E(eb, ez);
-    __syncthreads();
+    const bool more = ez + 1 < k1;
+    T pu[NU][3], pe[NE];
+    unsigned pf[NU];
+    if (more) {
+      load_u(pu, pf, ez + 2);
+      load_E(pe, ez + 1);
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       alo[c] = ahi[c];
@@ -222,6 +303,11 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
         const T Au = ((fm >> c) & 1) ? uval : alo[c];
         apply_epilogue<T, EPI>(ep, dof, Au, uval, dsum, dsum2);
       }
+    }
+    if (more) {
+      store_u(pu, pf, (st + 2) % 3);
+      store_E(pe, (st + 1) & 1);
+      __syncthreads();
     }
   }
   if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2 || EPI == EPI_CHEB_OUT) {
@@ -300,6 +386,48 @@ __global__ __launch_bounds__(256) void k_fine_stencil(FineOp<double> op, double 
 
 }  // namespace
 
+template <typename T, int EPI>
+void launch_fine(hipStream_t s, const FineOp<T> &op, const T *u, const EpiArgs<T> &ep, const FineGrid &g)
+{
+  dim3 grid(g.nblocks), block(NT);
+  switch (op.tile) {
+    case 0:
+      k_fine_op<T, EPI, 32, 8><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks, op.zlen);
+      break;
+    case 1:
+      k_fine_op<T, EPI, 28, 9><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks, op.zlen);
+      break;
+    case 2:
+      k_fine_op<T, EPI, 25, 10><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks, op.zlen);
+      break;
+    case 3:
+      k_fine_op<T, EPI, 22, 11><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks, op.zlen);
+      break;
+  }
+}
+
+FineTile fine_tile_choose(int nx, int ny, int nz, int ncu)
+{
+  // Every block runs (zlen + 1) z-steps, the first and last with half the rows; blocks are
+  // spread evenly over the CUs and run together, so a launch lasts about ceil(blocks / CUs)
+  // block-times, and a CU needs about three resident blocks (wavefronts per SIMD) before its
+  // latency is hidden.  Among equal costs the larger z-chunk (less restaging) wins.
+  FineTile best{0, 8};
+  double best_cost = 1e300;
+  ncu = std::max(ncu, 1);
+  for (int t = 0; t < kFineTiles; ++t)
+    for (int zl = 4; zl <= 16; ++zl) {
+      const FineGrid g = fine_grid_of(nx, ny, nz, t, zl);
+      const int per_cu = (g.nblocks + ncu - 1) / ncu;
+      const double cost = (zl + 0.5) * std::max(per_cu, 3);
+      if (cost < best_cost - 1e-9 || (cost < best_cost + 1e-9 && zl > best.zlen)) {
+        best_cost = cost;
+        best = FineTile{t, zl};
+      }
+    }
+  return best;
+}
+
 template <typename T>
 int fine_op_blocks(const FineOp<T> &op)
 {
@@ -309,27 +437,14 @@ int fine_op_blocks(const FineOp<T> &op)
 template <typename T>
 void fine_op(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep)
 {
-  const FineGrid<T> g = fine_grid(op);
-  dim3 grid(g.nblocks), block(NT);
+  const FineGrid g = fine_grid(op);
   switch (epi) {
-    case EPI_APPLY:
-      k_fine_op<T, EPI_APPLY><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-    case EPI_APPLY_DOT:
-      k_fine_op<T, EPI_APPLY_DOT><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-    case EPI_APPLY_DOT2:
-      k_fine_op<T, EPI_APPLY_DOT2><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-    case EPI_RESID:
-      k_fine_op<T, EPI_RESID><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-    case EPI_CHEB:
-      k_fine_op<T, EPI_CHEB><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
-    case EPI_CHEB_OUT:
-      k_fine_op<T, EPI_CHEB_OUT><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);
-      break;
+    case EPI_APPLY: launch_fine<T, EPI_APPLY>(s, op, u, ep, g); break;
+    case EPI_APPLY_DOT: launch_fine<T, EPI_APPLY_DOT>(s, op, u, ep, g); break;
+    case EPI_APPLY_DOT2: launch_fine<T, EPI_APPLY_DOT2>(s, op, u, ep, g); break;
+    case EPI_RESID: launch_fine<T, EPI_RESID>(s, op, u, ep, g); break;
+    case EPI_CHEB: launch_fine<T, EPI_CHEB>(s, op, u, ep, g); break;
+    case EPI_CHEB_OUT: launch_fine<T, EPI_CHEB_OUT>(s, op, u, ep, g); break;
   }
   TOFE_LAUNCH_CHECK();
 }

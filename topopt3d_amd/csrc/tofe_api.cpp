@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -150,6 +151,7 @@ struct tofe_ctx {
 
   bool assembled = false;
   bool fine_mfma = false;  // TOFE_FINE_MFMA=1: element-centric MFMA fine operator
+  FineTile fine_tile{0, 8};  // k_fine_op launch shape (TOFE_FINE_TILE="<shape>:<zlen>" overrides)
   int precision = TOFE_PREC_FP64;
   int built_precision = -1;
   tofe_stats stats{};
@@ -219,6 +221,7 @@ FineOp<double> fine_of<double>(tofe_ctx *c)
   f.fix = c->has_fix ? c->dfix.p : nullptr;
   f.KE = c->dKE.p;
   f.nx = c->desc.nx; f.ny = c->desc.ny; f.nz = c->desc.nz;
+  f.tile = c->fine_tile.tile; f.zlen = c->fine_tile.zlen;
   return f;
 }
 template <>
@@ -229,6 +232,7 @@ FineOp<float> fine_of<float>(tofe_ctx *c)
   f.fix = c->has_fix ? c->dfix.p : nullptr;
   f.KE = c->dKE32.p;
   f.nx = c->desc.nx; f.ny = c->desc.ny; f.nz = c->desc.nz;
+  f.tile = c->fine_tile.tile; f.zlen = c->fine_tile.zlen;
   return f;
 }
 
@@ -945,6 +949,14 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     {
       int ncu = 0;
       TOFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+      c->fine_tile = fine_tile_choose(c->desc.nx, c->desc.ny, c->desc.nz, ncu);
+      if (const char *ft = std::getenv("TOFE_FINE_TILE")) {
+        int t = -1, z = 0;
+        if (std::sscanf(ft, "%d:%d", &t, &z) != 2 || t < 0 || t >= kFineTiles || z < 1 || z > 64)
+          throw Error(TOFE_ERR_ARG, std::string("TOFE_FINE_TILE must be <shape 0..") +
+                                        std::to_string(kFineTiles - 1) + ">:<zlen 1..64>, got " + ft);
+        c->fine_tile = FineTile{t, z};
+      }
       if (ncu >= 4 * kCoarseCUs) {
         std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
         for (int cu = kCoarseCUs; cu < ncu; ++cu) mask[cu / 32] |= 1u << (cu % 32);

@@ -65,7 +65,16 @@ struct FineOp {
   const uint8_t *fix = nullptr;// nnode fixed-DOF bitmasks (may be null)
   const T *KE = nullptr;       // 576, row-major
   int nx = 0, ny = 0, nz = 0;  // elements
+  int tile = 0, zlen = 8;      // column-tile shape index and z-chunk length of k_fine_op
 };
+
+// Launch shape of the fine operator: one of kFineTiles compiled column tiles and the number of
+// node planes per block, chosen from the mesh size and the number of CUs.
+constexpr int kFineTiles = 4;
+struct FineTile {
+  int tile, zlen;
+};
+FineTile fine_tile_choose(int nx, int ny, int nz, int ncu);
 
 // Stored 27-point block stencil operator of a coarse level.
 template <typename T>
