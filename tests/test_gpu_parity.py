@@ -77,6 +77,34 @@ def test_operator_matches_assembled_K(oracle, tofe, nel, L):
         ctx.close()
 
 
+@pytest.mark.parametrize("h,E0,nu,sym", [((1.0, 1.0, 1.0), 1.0, 0.3, "0"),
+                                          ((0.5, 0.5, 0.5), 2.5, 0.27, "1"),
+                                          ((0.5, 0.75, 1.25), 1.0, 0.3, "1"),
+                                          ((1.0, 1.0, 2.0), 1.7, 0.35, "1")])
+def test_operator_element_shapes(oracle, tofe, h, E0, nu, sym, monkeypatch):
+    """Cubic elements apply KE from its 12 cube-symmetry orbit values (TOFE_FINE_SYM=0 turns
+    that off), box elements from KE's rows: both give the oracle's K u with the element
+    matrix of (E0, nu, h)."""
+    monkeypatch.setenv("TOFE_FINE_SYM", sym)
+    nel = (13, 9, 7)
+    pb = configs.cantilever(nel, seed=4, lo=0.05, hi=1.0, mg_levels=2)
+    print(f"h={h} E0={E0} nu={nu} sym={sym}", flush=True)
+    ctx = tofe.Context(nel, h=h, E0=E0, nu=nu, mg_levels=2, fixed_dof=pb.fixed_dof)
+    ctx.set_design(pb.x)
+    E = ctx.field(tofe.FIELD_E)
+    K = oracle.assemble_K(*nel, oracle.ke(E0, nu, h), E, pb.fixed_dof)
+    X = np.random.default_rng(8).standard_normal((2, pb.ndof))
+    Y = ctx.apply_K(X.ravel()).reshape(2, -1)
+    for k in range(2):
+        assert rel(Y[k], K.matvec(X[k])) <= 1e-14
+    ctx.set_precision(tofe.PREC_MIXED)
+    ctx.assemble()
+    U, its, reason = ctx.solve(pb.F, rtol=1e-10, maxit=200)
+    assert reason == tofe.CONVERGED_RTOL
+    assert np.linalg.norm(pb.F - K.matvec(U)) <= 1e-10 * np.linalg.norm(pb.F) * 1.01
+    ctx.close()
+
+
 @pytest.mark.parametrize("tile", ["0:3", "1:8", "2:5", "3:16", "1:1"])
 @pytest.mark.parametrize("nel", [(30, 17, 9), (7, 5, 3)])
 def test_operator_tile_shapes(oracle, tofe, nel, tile, monkeypatch):

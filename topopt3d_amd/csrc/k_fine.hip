@@ -14,6 +14,9 @@
 // 576 FMAs.  KE is indexed with compile-time constants only, so it is read through
 // scalar loads, uniform across the wavefront.
 #include "k_common.hpp"
+#include "ke_orbits.hpp"
+
+#include <utility>
 
 namespace tofe {
 namespace {
@@ -80,22 +83,48 @@ __device__ inline T ke_row_dot(const T *base, const T (&ue)[24])
   return t;
 }
 
+// Cubic elements: KE(x, y) = +-v[orbit(x, y)] with the 12 orbit values v held in scalar
+// registers for the whole kernel (loaded once), so no KE row is fetched inside the z loop.
+template <int ROW, int Q>
+struct KeTerm {
+  static constexpr int idx = kKeCube.idx[ROW][Q];
+  static constexpr bool neg = kKeCube.neg[ROW][Q];
+};
+template <typename T, int ROW, int... Q>
+__device__ inline T ke_row_dot_sym(const T (&kv)[kKeOrbitsMax], const T (&ue)[24],
+                                   std::integer_sequence<int, Q...>)
+{
+  T t = 0;
+  ((t += (KeTerm<ROW, Q>::neg ? -kv[KeTerm<ROW, Q>::idx] : kv[KeTerm<ROW, Q>::idx]) * ue[Q]), ...);
+  return t;
+}
+
+// Row 3L + R of KE applied to ue, from scalar loads (SYM false) or the orbit values.
+template <typename T, bool SYM, int L, int R>
+__device__ inline T row_dot(const T *base, const T (&kv)[kKeOrbitsMax], const T (&ue)[24])
+{
+  if constexpr (SYM)
+    return ke_row_dot_sym<T, 3 * L + R>(kv, ue, std::make_integer_sequence<int, 24>());
+  else
+    return ke_row_dot<T, L, R>(base, ue);
+}
+
 // The rows of element (A, B) of the layer for the thread's lower node (want_lo) and upper
 // node (want_hi); the flags are uniform across the block.
-template <typename T, int A, int B>
-__device__ inline void element_rows(const T *base, const T (&ue)[24], T Ee, T (&alo)[3], T (&ahi)[3],
-                                    bool want_lo, bool want_hi)
+template <typename T, bool SYM, int A, int B>
+__device__ inline void element_rows(const T *base, const T (&kv)[kKeOrbitsMax], const T (&ue)[24], T Ee,
+                                    T (&alo)[3], T (&ahi)[3], bool want_lo, bool want_hi)
 {
   constexpr int Llo = lidx(1 - A, 1 - B, 0), Lhi = lidx(1 - A, 1 - B, 1);
   if (want_lo) {
-    alo[0] += Ee * ke_row_dot<T, Llo, 0>(base, ue);
-    alo[1] += Ee * ke_row_dot<T, Llo, 1>(base, ue);
-    alo[2] += Ee * ke_row_dot<T, Llo, 2>(base, ue);
+    alo[0] += Ee * row_dot<T, SYM, Llo, 0>(base, kv, ue);
+    alo[1] += Ee * row_dot<T, SYM, Llo, 1>(base, kv, ue);
+    alo[2] += Ee * row_dot<T, SYM, Llo, 2>(base, kv, ue);
   }
   if (want_hi) {
-    ahi[0] += Ee * ke_row_dot<T, Lhi, 0>(base, ue);
-    ahi[1] += Ee * ke_row_dot<T, Lhi, 1>(base, ue);
-    ahi[2] += Ee * ke_row_dot<T, Lhi, 2>(base, ue);
+    ahi[0] += Ee * row_dot<T, SYM, Lhi, 0>(base, kv, ue);
+    ahi[1] += Ee * row_dot<T, SYM, Lhi, 1>(base, kv, ue);
+    ahi[2] += Ee * row_dot<T, SYM, Lhi, 2>(base, kv, ue);
   }
 }
 
@@ -137,7 +166,7 @@ FineGrid fine_grid(const FineOp<T> &op)
   return fine_grid_of(op.nx, op.ny, op.nz, op.tile, op.zlen);
 }
 
-template <typename T, int EPI, int TX, int TY>
+template <typename T, int EPI, int TX, int TY, bool SYM>
 __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restrict__ u,
                                                 EpiArgs<T> ep, int tiles_x, int tiles_y,
                                                 int nblocks, int zlen)
@@ -164,6 +193,11 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
   const int i = x0 + tx, j = y0 + ty;
   const bool own = (trow < TY) && (i < Nx) && (j < Ny);
   const T *KEbase = op.KE;
+  T kv[kKeOrbitsMax];
+  if constexpr (SYM) {
+#pragma unroll
+    for (int k = 0; k < kKeOrbitsMax; ++k) kv[k] = op.KE[576 + k];
+  }
   const T *__restrict__ E = op.E;
   const uint8_t *__restrict__ fix = op.fix;
 
@@ -287,7 +321,8 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
 #pragma unroll
         for (int c = 0; c < 3; ++c) ue[3 * l + c] = su[lnz(l) ? shi : slo][c][nl];
       }
-      element_rows<T, A, B>(KEbase, ue, sE[eb][(ty + B) * EW + tx + A], alo, ahi, want_lo, want_hi);
+      element_rows<T, SYM, A, B>(KEbase, kv, ue, sE[eb][(ty + B) * EW + tx + A], alo, ahi, want_lo,
+                                 want_hi);
     };
     element(std::integral_constant<int, 0>(), std::integral_constant<int, 0>());
     element(std::integral_constant<int, 1>(), std::integral_constant<int, 0>());
@@ -386,32 +421,34 @@ __global__ __launch_bounds__(256) void k_fine_stencil(FineOp<double> op, double 
 
 }  // namespace
 
+template <typename T, int EPI, int TX, int TY>
+void launch_tile(hipStream_t s, const FineOp<T> &op, const T *u, const EpiArgs<T> &ep, const FineGrid &g)
+{
+  if (op.sym)
+    k_fine_op<T, EPI, TX, TY, true><<<dim3(g.nblocks), dim3(NT), 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y,
+                                                                         g.nblocks, op.zlen);
+  else
+    k_fine_op<T, EPI, TX, TY, false><<<dim3(g.nblocks), dim3(NT), 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y,
+                                                                          g.nblocks, op.zlen);
+}
+
 template <typename T, int EPI>
 void launch_fine(hipStream_t s, const FineOp<T> &op, const T *u, const EpiArgs<T> &ep, const FineGrid &g)
 {
-  dim3 grid(g.nblocks), block(NT);
   switch (op.tile) {
-    case 0:
-      k_fine_op<T, EPI, 32, 8><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks, op.zlen);
-      break;
-    case 1:
-      k_fine_op<T, EPI, 28, 9><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks, op.zlen);
-      break;
-    case 2:
-      k_fine_op<T, EPI, 25, 10><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks, op.zlen);
-      break;
-    case 3:
-      k_fine_op<T, EPI, 22, 11><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks, op.zlen);
-      break;
+    case 0: launch_tile<T, EPI, 32, 8>(s, op, u, ep, g); break;
+    case 1: launch_tile<T, EPI, 28, 9>(s, op, u, ep, g); break;
+    case 2: launch_tile<T, EPI, 25, 10>(s, op, u, ep, g); break;
+    case 3: launch_tile<T, EPI, 22, 11>(s, op, u, ep, g); break;
   }
 }
 
-FineTile fine_tile_choose(int nx, int ny, int nz, int ncu)
+FineTile fine_tile_choose(int nx, int ny, int nz, int ncu, int hide)
 {
   // Every block runs (zlen + 1) z-steps, the first and last with half the rows; blocks are
   // spread evenly over the CUs and run together, so a launch lasts about ceil(blocks / CUs)
-  // block-times, and a CU needs about three resident blocks (wavefronts per SIMD) before its
-  // latency is hidden.  Among equal costs the larger z-chunk (less restaging) wins.
+  // block-times, and a CU needs about `hide` resident blocks (wavefronts per SIMD) before
+  // its latency is hidden.  Among equal costs the larger z-chunk (less restaging) wins.
   FineTile best{0, 8};
   double best_cost = 1e300;
   ncu = std::max(ncu, 1);
@@ -419,7 +456,7 @@ FineTile fine_tile_choose(int nx, int ny, int nz, int ncu)
     for (int zl = 4; zl <= 16; ++zl) {
       const FineGrid g = fine_grid_of(nx, ny, nz, t, zl);
       const int per_cu = (g.nblocks + ncu - 1) / ncu;
-      const double cost = (zl + 0.5) * std::max(per_cu, 3);
+      const double cost = (zl + 0.5) * std::max(per_cu, hide);
       if (cost < best_cost - 1e-9 || (cost < best_cost + 1e-9 && zl > best.zlen)) {
         best_cost = cost;
         best = FineTile{t, zl};

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "host_setup.hpp"
+#include "ke_orbits.hpp"
 #include "tofe_internal.hpp"
 
 using namespace tofe;
@@ -151,7 +152,9 @@ struct tofe_ctx {
 
   bool assembled = false;
   bool fine_mfma = false;  // TOFE_FINE_MFMA=1: element-centric MFMA fine operator
-  FineTile fine_tile{0, 8};  // k_fine_op launch shape (TOFE_FINE_TILE="<shape>:<zlen>" overrides)
+  FineTile fine_tile{0, 8}, fine_tile32{0, 8};  // k_fine_op launch shapes, FP64 and FP32
+                                                // (TOFE_FINE_TILE="<shape>:<zlen>" sets both)
+  bool ke_sym = false;  // KE matches the cube-orbit table: fine operator keeps KE in registers
   int precision = TOFE_PREC_FP64;
   int built_precision = -1;
   tofe_stats stats{};
@@ -222,6 +225,7 @@ FineOp<double> fine_of<double>(tofe_ctx *c)
   f.KE = c->dKE.p;
   f.nx = c->desc.nx; f.ny = c->desc.ny; f.nz = c->desc.nz;
   f.tile = c->fine_tile.tile; f.zlen = c->fine_tile.zlen;
+  f.sym = c->ke_sym;
   return f;
 }
 template <>
@@ -232,7 +236,8 @@ FineOp<float> fine_of<float>(tofe_ctx *c)
   f.fix = c->has_fix ? c->dfix.p : nullptr;
   f.KE = c->dKE32.p;
   f.nx = c->desc.nx; f.ny = c->desc.ny; f.nz = c->desc.nz;
-  f.tile = c->fine_tile.tile; f.zlen = c->fine_tile.zlen;
+  f.tile = c->fine_tile32.tile; f.zlen = c->fine_tile32.zlen;
+  f.sym = c->ke_sym;
   return f;
 }
 
@@ -949,13 +954,14 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     {
       int ncu = 0;
       TOFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
-      c->fine_tile = fine_tile_choose(c->desc.nx, c->desc.ny, c->desc.nz, ncu);
+      c->fine_tile = fine_tile_choose(c->desc.nx, c->desc.ny, c->desc.nz, ncu, 3);
+      c->fine_tile32 = fine_tile_choose(c->desc.nx, c->desc.ny, c->desc.nz, ncu, 6);
       if (const char *ft = std::getenv("TOFE_FINE_TILE")) {
         int t = -1, z = 0;
         if (std::sscanf(ft, "%d:%d", &t, &z) != 2 || t < 0 || t >= kFineTiles || z < 1 || z > 64)
           throw Error(TOFE_ERR_ARG, std::string("TOFE_FINE_TILE must be <shape 0..") +
                                         std::to_string(kFineTiles - 1) + ">:<zlen 1..64>, got " + ft);
-        c->fine_tile = FineTile{t, z};
+        c->fine_tile = c->fine_tile32 = FineTile{t, z};
       }
       if (ncu >= 4 * kCoarseCUs) {
         std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
@@ -974,12 +980,30 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     compute_ke(d.E0, d.nu, d.hx, d.hy, d.hz, c->KE);
     dmatrix(d.E0, d.nu, c->shape.D);
     shape_gradients(d.hx, d.hy, d.hz, c->shape.dN, c->shape.detJ);
-    float ke32[576];
-    for (int i = 0; i < 576; ++i) ke32[i] = (float)c->KE[i];
-    c->dKE.alloc(576);
-    c->dKE32.alloc(576);
-    h2d(c.get(), c->dKE.p, c->KE, 576);
+    // KE, then (cubic elements) the values of its 12 orbits under the symmetries of the
+    // cube, used by the fine operator when every entry matches its orbit value up to the
+    // rounding of the quadrature.
+    double ke64[576 + kKeOrbitsMax] = {};
+    std::copy(c->KE, c->KE + 576, ke64);
+    {
+      double kmax = 0, dev = 0;
+      for (int k = 0; k < kKeCube.n; ++k) ke64[576 + k] = c->KE[24 * kKeCube.rep[k][0] + kKeCube.rep[k][1]];
+      for (int x = 0; x < 24; ++x)
+        for (int y = 0; y < 24; ++y) {
+          const double v = ke64[576 + kKeCube.idx[x][y]];
+          kmax = std::max(kmax, std::fabs(c->KE[24 * x + y]));
+          dev = std::max(dev, std::fabs(c->KE[24 * x + y] - (kKeCube.neg[x][y] ? -v : v)));
+        }
+      c->ke_sym = dev <= 1e-14 * kmax;
+      if (const char *fs = std::getenv("TOFE_FINE_SYM")) c->ke_sym = c->ke_sym && fs[0] != '0';
+    }
+    float ke32[576 + kKeOrbitsMax];
+    for (int i = 0; i < 576 + kKeOrbitsMax; ++i) ke32[i] = (float)ke64[i];
+    c->dKE.alloc(576 + kKeOrbitsMax);
+    c->dKE32.alloc(576 + kKeOrbitsMax);
+    h2d(c.get(), c->dKE.p, ke64, 576 + kKeOrbitsMax);
     TOFE_HIP(hipMemcpyAsync(c->dKE32.p, ke32, sizeof(ke32), hipMemcpyHostToDevice, c->stream));
+    TOFE_HIP(hipStreamSynchronize(c->stream));  // the host copies are stack arrays
 
     if (filter_template(d.hx, d.hy, d.hz, d.rmin, c->tpl_off, c->tpl_w) < 1 || c->tpl_w.size() > 1024)
       throw Error(TOFE_ERR_ARG, "filter radius gives an unsupported neighbourhood");

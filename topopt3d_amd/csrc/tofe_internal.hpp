@@ -66,15 +66,18 @@ struct FineOp {
   const T *KE = nullptr;       // 576, row-major
   int nx = 0, ny = 0, nz = 0;  // elements
   int tile = 0, zlen = 8;      // column-tile shape index and z-chunk length of k_fine_op
+  bool sym = false;            // cubic element: KE[576 + k] holds the cube-orbit values of KE
 };
 
 // Launch shape of the fine operator: one of kFineTiles compiled column tiles and the number of
-// node planes per block, chosen from the mesh size and the number of CUs.
+// node planes per block, chosen from the mesh size, the number of CUs and the number of
+// resident blocks per CU that hide the kernel's latency (measured on MI355X at 5.4M DOF:
+// 3 for FP64, 6 for FP32).
 constexpr int kFineTiles = 4;
 struct FineTile {
   int tile, zlen;
 };
-FineTile fine_tile_choose(int nx, int ny, int nz, int ncu);
+FineTile fine_tile_choose(int nx, int ny, int nz, int ncu, int hide);
 
 // Stored 27-point block stencil operator of a coarse level.
 template <typename T>
