@@ -107,6 +107,9 @@ typedef struct tofe_stats {
   double bnorm;            /* ||F|| of the last solve */
   double setup_seconds;    /* last tofe_assemble (Galerkin, diagonals, eigen bounds, coarse factor) */
   double solve_seconds;    /* last tofe_solve wall time */
+  int fine_form;           /* fine operator kernel: 0 node-centric with KE rows, 1 node-centric
+                              with KE's cube-orbit values, 2 element-centric Walsh-Hadamard
+                              (box element), 3 the same with the cubic element's 9 values */
 } tofe_stats;
 
 /* Create the context: mesh numbering, KE, density filter, transfer operators and

@@ -77,19 +77,27 @@ def test_operator_matches_assembled_K(oracle, tofe, nel, L):
         ctx.close()
 
 
-@pytest.mark.parametrize("h,E0,nu,sym", [((1.0, 1.0, 1.0), 1.0, 0.3, "0"),
-                                          ((0.5, 0.5, 0.5), 2.5, 0.27, "1"),
-                                          ((0.5, 0.75, 1.25), 1.0, 0.3, "1"),
-                                          ((1.0, 1.0, 2.0), 1.7, 0.35, "1")])
-def test_operator_element_shapes(oracle, tofe, h, E0, nu, sym, monkeypatch):
-    """Cubic elements apply KE from its 12 cube-symmetry orbit values (TOFE_FINE_SYM=0 turns
-    that off), box elements from KE's rows: both give the oracle's K u with the element
-    matrix of (E0, nu, h)."""
-    monkeypatch.setenv("TOFE_FINE_SYM", sym)
+@pytest.mark.parametrize("mode", ["wht", "whtbox", "sym", "rows"])
+@pytest.mark.parametrize("h,E0,nu", [((1.0, 1.0, 1.0), 1.0, 0.3),
+                                     ((0.5, 0.5, 0.5), 2.5, 0.27),
+                                     ((0.5, 0.75, 1.25), 1.0, 0.3),
+                                     ((1.0, 1.0, 2.0), 1.7, 0.35)])
+def test_operator_element_shapes(oracle, tofe, h, E0, nu, mode, monkeypatch):
+    """The three forms of the fine operator give the oracle's K u with the element matrix of
+    (E0, nu, h): the element-centric Walsh-Hadamard form (default; 9 block values for cubic
+    elements, 33 for box elements or with TOFE_FINE_WHT_CUBE=0), the node-centric form with
+    KE from its 12 cube-symmetry orbit values (cubic elements; TOFE_FINE_WHT=0) and the
+    node-centric form reading KE's rows (TOFE_FINE_WHT=0 TOFE_FINE_SYM=0, and box elements)."""
+    monkeypatch.setenv("TOFE_FINE_WHT", "1" if mode.startswith("wht") else "0")
+    monkeypatch.setenv("TOFE_FINE_WHT_CUBE", "0" if mode == "whtbox" else "1")
+    monkeypatch.setenv("TOFE_FINE_SYM", "1" if mode == "sym" else "0")
     nel = (13, 9, 7)
     pb = configs.cantilever(nel, seed=4, lo=0.05, hi=1.0, mg_levels=2)
-    print(f"h={h} E0={E0} nu={nu} sym={sym}", flush=True)
+    print(f"h={h} E0={E0} nu={nu} mode={mode}", flush=True)
     ctx = tofe.Context(nel, h=h, E0=E0, nu=nu, mg_levels=2, fixed_dof=pb.fixed_dof)
+    cube = h[0] == h[1] == h[2]
+    want = {"wht": 3 if cube else 2, "whtbox": 2, "sym": 1 if cube else 0, "rows": 0}[mode]
+    assert ctx.stats()["fine_form"] == want
     ctx.set_design(pb.x)
     E = ctx.field(tofe.FIELD_E)
     K = oracle.assemble_K(*nel, oracle.ke(E0, nu, h), E, pb.fixed_dof)
@@ -105,12 +113,17 @@ def test_operator_element_shapes(oracle, tofe, h, E0, nu, sym, monkeypatch):
     ctx.close()
 
 
-@pytest.mark.parametrize("tile", ["0:3", "1:8", "2:5", "3:16", "1:1"])
+@pytest.mark.parametrize("tile", ["0:3", "1:8", "2:5", "3:16", "1:1", "wht1:2", "wht5:16", "wht24:3"])
 @pytest.mark.parametrize("nel", [(30, 17, 9), (7, 5, 3)])
 def test_operator_tile_shapes(oracle, tofe, nel, tile, monkeypatch):
-    """Every compiled column-tile shape and z-chunk length of the fine operator (picked with
-    TOFE_FINE_TILE) gives the oracle's K u, in FP64 and inside the mixed solve."""
-    monkeypatch.setenv("TOFE_FINE_TILE", tile)
+    """Every compiled column-tile shape and z-chunk length of the node-centric fine operator
+    (picked with TOFE_FINE_TILE), and z-chunk lengths of the element-centric one
+    (TOFE_FINE_WHT_Z), give the oracle's K u, in FP64 and inside the mixed solve."""
+    if tile.startswith("wht"):
+        monkeypatch.setenv("TOFE_FINE_WHT_Z", tile[3:])
+    else:
+        monkeypatch.setenv("TOFE_FINE_WHT", "0")
+        monkeypatch.setenv("TOFE_FINE_TILE", tile)
     print(f"tile {tile}", flush=True)
     pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, 2)
     X = np.random.default_rng(5).standard_normal(pb.ndof)

@@ -106,4 +106,81 @@ __device__ inline void apply_epilogue(const EpiArgs<T> &ep, int64_t dof, T Au, T
   }
 }
 
+// Epilogue of a node's three DOFs in two halves, so the inputs can be loaded a step ahead
+// of the arithmetic that produces A u: epi_load issues every load the epilogue needs
+// (u, the fixed-DOF mask, and per EPI b, D^-1, d, q), epi_store forms the outputs exactly as
+// apply_epilogue does for each component.
+template <typename T, int EPI>
+struct EpiIn {
+  T u[3], b[3], dinv[3], d[3];
+  double q[3];
+  unsigned fm;
+};
+
+template <typename T, int EPI>
+__device__ inline void epi_load(const EpiArgs<T> &ep, const T *__restrict__ u, const uint8_t *__restrict__ fix,
+                                int64_t n, EpiIn<T, EPI> &in)
+{
+  const int64_t d0 = 3 * n;
+  in.fm = fix ? fix[n] : 0u;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) in.u[c] = u[d0 + c];
+  if constexpr (EPI == EPI_APPLY_DOT2 || EPI == EPI_RESID || EPI == EPI_CHEB || EPI == EPI_CHEB_OUT) {
+    const T *__restrict__ b = ep.b;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) in.b[c] = b[d0 + c];
+  }
+  if constexpr (EPI == EPI_CHEB || EPI == EPI_CHEB_OUT) {
+    const T *__restrict__ dinv = ep.dinv;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) in.dinv[c] = dinv[d0 + c];
+    if (ep.c1 != T(0)) {
+      const T *__restrict__ d = ep.d;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) in.d[c] = d[d0 + c];
+    }
+  }
+  if constexpr (EPI == EPI_CHEB_OUT) {
+    const double *__restrict__ q = ep.q64;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) in.q[c] = q[d0 + c];
+  }
+}
+
+template <typename T, int EPI>
+__device__ inline void epi_store(const EpiArgs<T> &ep, int64_t n, const EpiIn<T, EPI> &in, const T (&Ku)[3],
+                                 double &dsum, double &dsum2)
+{
+  const int64_t d0 = 3 * n;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t dof = d0 + c;
+    const T uval = in.u[c];
+    const T Au = ((in.fm >> c) & 1) ? uval : Ku[c];
+    if constexpr (EPI == EPI_APPLY) {
+      ep.y[dof] = Au;
+    } else if constexpr (EPI == EPI_APPLY_DOT) {
+      ep.y[dof] = Au;
+      dsum += (double)uval * (double)Au;
+    } else if constexpr (EPI == EPI_APPLY_DOT2) {
+      ep.y[dof] = Au;
+      dsum += (double)uval * (double)Au;
+      dsum2 += (double)uval * (double)in.b[c];
+    } else if constexpr (EPI == EPI_RESID) {
+      ep.y[dof] = in.b[c] - Au;
+    } else if constexpr (EPI == EPI_CHEB) {
+      const T r = in.dinv[c] * (in.b[c] - Au);
+      const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * in.d[c] + ep.c2 * r;
+      ep.d[dof] = dn;
+      ep.xout[dof] = uval + dn;
+    } else if constexpr (EPI == EPI_CHEB_OUT) {
+      const T r = in.dinv[c] * (in.b[c] - Au);
+      const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * in.d[c] + ep.c2 * r;
+      const double x = (double)(uval + dn);
+      ep.xout64[dof] = x;
+      dsum += x * in.q[c];
+    }
+  }
+}
+
 }  // namespace tofe

@@ -19,6 +19,7 @@
 
 #include "host_setup.hpp"
 #include "ke_orbits.hpp"
+#include "ke_wht.hpp"
 #include "tofe_internal.hpp"
 
 using namespace tofe;
@@ -155,6 +156,9 @@ struct tofe_ctx {
   FineTile fine_tile{0, 8}, fine_tile32{0, 8};  // k_fine_op launch shapes, FP64 and FP32
                                                 // (TOFE_FINE_TILE="<shape>:<zlen>" sets both)
   bool ke_sym = false;  // KE matches the cube-orbit table: fine operator keeps KE in registers
+  bool ke_wht = false;  // KE is block diagonal in the Walsh-Hadamard basis: k_fine_wht
+  bool ke_wht_cube = false;  // ... with the 9-value block structure of a cubic element
+  int wht_zlen = 8, wht_zlen32 = 8;  // its z-chunk lengths, FP64 and FP32
   int precision = TOFE_PREC_FP64;
   int built_precision = -1;
   tofe_stats stats{};
@@ -226,6 +230,9 @@ FineOp<double> fine_of<double>(tofe_ctx *c)
   f.nx = c->desc.nx; f.ny = c->desc.ny; f.nz = c->desc.nz;
   f.tile = c->fine_tile.tile; f.zlen = c->fine_tile.zlen;
   f.sym = c->ke_sym;
+  f.wht = c->ke_wht;
+  f.wht_cube = c->ke_wht_cube;
+  if (f.wht) f.zlen = c->wht_zlen;
   return f;
 }
 template <>
@@ -238,6 +245,9 @@ FineOp<float> fine_of<float>(tofe_ctx *c)
   f.nx = c->desc.nx; f.ny = c->desc.ny; f.nz = c->desc.nz;
   f.tile = c->fine_tile32.tile; f.zlen = c->fine_tile32.zlen;
   f.sym = c->ke_sym;
+  f.wht = c->ke_wht;
+  f.wht_cube = c->ke_wht_cube;
+  if (f.wht) f.zlen = c->wht_zlen32;
   return f;
 }
 
@@ -247,8 +257,10 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
 {
   if (!s) s = c->stream;
   if (l == 0) {
-    if (c->fine_mfma) fine_mfma<T>(s, fine_of<T>(c), u, epi, ep);
-    else fine_op<T>(s, fine_of<T>(c), u, epi, ep);
+    const FineOp<T> f = fine_of<T>(c);
+    if (c->fine_mfma) fine_mfma<T>(s, f, u, epi, ep);
+    else if (f.wht) fine_wht<T>(s, f, u, epi, ep);
+    else fine_op<T>(s, f, u, epi, ep);
   } else {
     StencilOp<T> op;
     auto &lv = *levels<T>(c)[l];
@@ -263,7 +275,10 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
 template <typename T>
 int level_blocks(tofe_ctx *c, int l)
 {
-  if (l == 0) return c->fine_mfma ? fine_mfma_blocks<T>(fine_of<T>(c)) : fine_op_blocks<T>(fine_of<T>(c));
+  if (l == 0) {
+    const FineOp<T> f = fine_of<T>(c);
+    return c->fine_mfma ? fine_mfma_blocks<T>(f) : f.wht ? fine_wht_blocks<T>(f) : fine_op_blocks<T>(f);
+  }
   StencilOp<T> op;
   op.Nx = c->grids[l].Nx; op.Ny = c->grids[l].Ny; op.Nz = c->grids[l].Nz;
   return stencil_op_blocks<T>(op);
@@ -956,6 +971,15 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
       TOFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
       c->fine_tile = fine_tile_choose(c->desc.nx, c->desc.ny, c->desc.nz, ncu, 3);
       c->fine_tile32 = fine_tile_choose(c->desc.nx, c->desc.ny, c->desc.nz, ncu, 6);
+      c->wht_zlen = fine_wht_zlen(c->desc.nx, c->desc.ny, c->desc.nz, ncu, 3);
+      c->wht_zlen32 = fine_wht_zlen(c->desc.nx, c->desc.ny, c->desc.nz, ncu, 5);
+      if (const char *wz = std::getenv("TOFE_FINE_WHT_Z")) {
+        int z64 = 0, z32 = 0;
+        if (std::sscanf(wz, "%d:%d", &z64, &z32) != 2 || z64 < 1 || z64 > 64 || z32 < 1 || z32 > 64)
+          throw Error(TOFE_ERR_ARG, std::string("TOFE_FINE_WHT_Z must be <zlen fp64>:<zlen fp32>, got ") + wz);
+        c->wht_zlen = z64;
+        c->wht_zlen32 = z32;
+      }
       if (const char *ft = std::getenv("TOFE_FINE_TILE")) {
         int t = -1, z = 0;
         if (std::sscanf(ft, "%d:%d", &t, &z) != 2 || t < 0 || t >= kFineTiles || z < 1 || z > 64)
@@ -983,7 +1007,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     // KE, then (cubic elements) the values of its 12 orbits under the symmetries of the
     // cube, used by the fine operator when every entry matches its orbit value up to the
     // rounding of the quadrature.
-    double ke64[576 + kKeOrbitsMax] = {};
+    double ke64[kKeBufLen] = {};
     std::copy(c->KE, c->KE + 576, ke64);
     {
       double kmax = 0, dev = 0;
@@ -997,11 +1021,62 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
       c->ke_sym = dev <= 1e-14 * kmax;
       if (const char *fs = std::getenv("TOFE_FINE_SYM")) c->ke_sym = c->ke_sym && fs[0] != '0';
     }
-    float ke32[576 + kKeOrbitsMax];
-    for (int i = 0; i < 576 + kKeOrbitsMax; ++i) ke32[i] = (float)ke64[i];
-    c->dKE.alloc(576 + kKeOrbitsMax);
-    c->dKE32.alloc(576 + kKeOrbitsMax);
-    h2d(c.get(), c->dKE.p, ke64, 576 + kKeOrbitsMax);
+    // Walsh-Hadamard blocks B_t[i][j] = (H KE H)_((t ^ e_i, i), (t ^ e_j, j)) / 64 over the node
+    // coordinate index m = x + 2y + 4z; k_fine_wht is used when every entry of H KE H / 64
+    // outside the stored blocks vanishes and the stored ones are symmetric, to rounding.
+    static_assert(kWhtOffset == 576 + kKeOrbitsMax, "KE buffer layout");
+    {
+      int mo[8];
+      for (int l = 0; l < 8; ++l) mo[l] = ko_coord(l, 0) + 2 * ko_coord(l, 1) + 4 * ko_coord(l, 2);
+      auto h = [](int s, int m) { return (__builtin_popcount(s & m) & 1) ? -1.0 : 1.0; };
+      double M[24][24];
+      double mmax = 0;
+      for (int s1 = 0; s1 < 8; ++s1)
+        for (int i = 0; i < 3; ++i)
+          for (int s2 = 0; s2 < 8; ++s2)
+            for (int j = 0; j < 3; ++j) {
+              double acc = 0;
+              for (int l1 = 0; l1 < 8; ++l1)
+                for (int l2 = 0; l2 < 8; ++l2)
+                  acc += h(s1, mo[l1]) * c->KE[24 * (3 * l1 + i) + 3 * l2 + j] * h(s2, mo[l2]);
+              M[3 * s1 + i][3 * s2 + j] = acc / 64;
+              mmax = std::max(mmax, std::fabs(acc / 64));
+            }
+      for (int t = 0; t < 8; ++t)
+        for (int i = 0; i < 3; ++i)
+          for (int j = i; j < 3; ++j)
+            if (wht_index(t, i, j) >= 0)
+              ke64[kWhtOffset + wht_index(t, i, j)] = M[3 * (t ^ (1 << i)) + i][3 * (t ^ (1 << j)) + j];
+      double dev = 0;
+      for (int s1 = 0; s1 < 8; ++s1)
+        for (int i = 0; i < 3; ++i)
+          for (int s2 = 0; s2 < 8; ++s2)
+            for (int j = 0; j < 3; ++j) {
+              const int t = s1 ^ (1 << i);
+              const int k = (t == (s2 ^ (1 << j))) ? wht_index(t, i, j) : -1;
+              const double want = k >= 0 ? ke64[kWhtOffset + k] : 0.0;
+              dev = std::max(dev, std::fabs(M[3 * s1 + i][3 * s2 + j] - want));
+            }
+      c->ke_wht = dev <= 1e-13 * mmax;
+      if (const char *fw = std::getenv("TOFE_FINE_WHT")) c->ke_wht = c->ke_wht && fw[0] != '0';
+      // cubic element: the values of each group agree to rounding; the group value is the
+      // first member's
+      double gdev = 0;
+      for (int k = 0; k < kWhtVals; ++k) {
+        int first = 0;
+        while (kWhtCubeGroup[first] != kWhtCubeGroup[k]) ++first;
+        ke64[kWhtCubeOffset + kWhtCubeGroup[k]] = ke64[kWhtOffset + first];
+        gdev = std::max(gdev, std::fabs(ke64[kWhtOffset + k] - ke64[kWhtOffset + first]));
+      }
+      c->ke_wht_cube = c->ke_wht && gdev <= 1e-13 * mmax;
+      if (const char *fc = std::getenv("TOFE_FINE_WHT_CUBE")) c->ke_wht_cube = c->ke_wht_cube && fc[0] != '0';
+    }
+    constexpr int kKeBuf = kKeBufLen;
+    float ke32[kKeBuf];
+    for (int i = 0; i < kKeBuf; ++i) ke32[i] = (float)ke64[i];
+    c->dKE.alloc(kKeBuf);
+    c->dKE32.alloc(kKeBuf);
+    h2d(c.get(), c->dKE.p, ke64, kKeBuf);
     TOFE_HIP(hipMemcpyAsync(c->dKE32.p, ke32, sizeof(ke32), hipMemcpyHostToDevice, c->stream));
     TOFE_HIP(hipStreamSynchronize(c->stream));  // the host copies are stack arrays
 
@@ -1702,6 +1777,7 @@ int tofe_get_stats(const tofe_ctx *c, tofe_stats *st)
 {
   if (!c || !st) return TOFE_ERR_ARG;
   *st = c->stats;
+  st->fine_form = c->ke_wht ? (c->ke_wht_cube ? 3 : 2) : c->ke_sym ? 1 : 0;
   return TOFE_OK;
 }
 

@@ -67,6 +67,8 @@ struct FineOp {
   int nx = 0, ny = 0, nz = 0;  // elements
   int tile = 0, zlen = 8;      // column-tile shape index and z-chunk length of k_fine_op
   bool sym = false;            // cubic element: KE[576 + k] holds the cube-orbit values of KE
+  bool wht = false;            // KE[592 + k] holds its Walsh-Hadamard blocks: use k_fine_wht
+  bool wht_cube = false;       // cubic element: KE[625 + k] holds the 9 cube block values
 };
 
 // Launch shape of the fine operator: one of kFineTiles compiled column tiles and the number of
@@ -78,6 +80,12 @@ struct FineTile {
   int tile, zlen;
 };
 FineTile fine_tile_choose(int nx, int ny, int nz, int ncu, int hide);
+// Element-centric fine operator in the Walsh-Hadamard basis of KE (k_fine_wht).
+int fine_wht_zlen(int nx, int ny, int nz, int ncu, int hide);
+template <typename T>
+int fine_wht_blocks(const FineOp<T> &op);
+template <typename T>
+void fine_wht(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
 
 // Stored 27-point block stencil operator of a coarse level.
 template <typename T>
