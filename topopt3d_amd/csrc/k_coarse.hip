@@ -263,51 +263,57 @@ __global__ void k_stencil_dense(const double *__restrict__ S, int Nx, int Ny, in
     for (int b = 0; b < 3; ++b) A[(3 * I + a) * n + 3 * J + b] = S[(int64_t)(o * 9 + a * 3 + b) * nn + I];
 }
 
-// y = A x for a symmetric dense A: one wavefront per row, fixed-order reduction.
-template <typename Tin, typename Tout>
-__global__ __launch_bounds__(NT) void k_gemv(const double *__restrict__ A, int64_t n,
-                                             const Tin *__restrict__ x, Tout *__restrict__ y)
-{
-  const int64_t row = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;
-  const double *a = A + row * n;
-  double s = 0;
-  for (int64_t c = lane; c < n; c += 64) s += a[c] * (double)x[c];
-  s = wave_sum(s);
-  if (lane == 0) y[row] = (Tout)s;
-}
-
-// FP32 copy of the symmetric inverse, rows padded to ld floats (multiple of 256)
-__global__ void k_pad32(const double *__restrict__ A, int n, int ld, float *__restrict__ B)
+// X = L^-1 (column-major, lower triangular; entries above the diagonal are not stored)
+// -> row-major copies of X and of X^T in T, rows padded to ld with zeros outside the
+// triangle, for the two triangular GEMVs of the coarse solve.
+template <typename T>
+__global__ void k_tri_pack(const double *__restrict__ X, int n, int ld, T *__restrict__ Xr,
+                           T *__restrict__ XTr)
 {
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (t >= (int64_t)n * ld) return;
-  const int64_t row = t / ld, col = t - row * ld;
-  B[t] = col < n ? (float)A[col + row * (int64_t)n] : 0.0f;
+  const int row = (int)(t / ld), col = (int)(t - (int64_t)row * ld);
+  // Xr(row, col) = X(row, col) for col <= row; XTr(row, col) = X(col, row) for col >= row
+  Xr[t] = col <= row ? (T)X[row + (int64_t)col * n] : T(0);
+  XTr[t] = (col >= row && col < n) ? (T)X[col + (int64_t)row * n] : T(0);
 }
 
-// y = A x with the padded FP32 inverse: four rows per block (one per wavefront), x staged
-// in LDS, 16-byte loads along the row, products summed in FP64 in a fixed order.
-__global__ __launch_bounds__(NT) void k_gemv32(const float *__restrict__ A, int n, int ld,
-                                               const float *__restrict__ x, float *__restrict__ y)
+// y = A x for a row-major triangular A (padded rows of ld, zeros outside the triangle):
+// LOWER rows use columns [0, row], upper rows [row, n).  Four rows per block (one per
+// wavefront), x staged in LDS, 16-byte loads along the row, products summed in FP64 in a
+// fixed order.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(NT) void k_trigemv(const T *__restrict__ A, int n, int ld,
+                                                const T *__restrict__ x, T *__restrict__ y)
 {
-  extern __shared__ float xs[];
-  for (int i = threadIdx.x; i < ld; i += NT) xs[i] = i < n ? x[i] : 0.0f;
+  extern __shared__ unsigned char xs_raw[];
+  T *xs = reinterpret_cast<T *>(xs_raw);
+  for (int i = threadIdx.x; i < ld; i += NT) xs[i] = i < n ? x[i] : T(0);
   __syncthreads();
-  const int64_t row = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const int row = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const float4 *a = reinterpret_cast<const float4 *>(A + row * ld);
-  const float4 *xv = reinterpret_cast<const float4 *>(xs);
+  constexpr int V = 16 / (int)sizeof(T);  // values per 16-byte load
+  const int c_lo = LOWER ? 0 : row / V, c_hi = LOWER ? row / V + 1 : ld / V;
+  const T *a = A + (int64_t)row * ld;
   double s = 0;
-#pragma unroll 8
-  for (int c = lane; c < ld / 4; c += 64) {
-    const float4 v = a[c], w = xv[c];
-    s += (double)(v.x * w.x) + (double)(v.y * w.y) + (double)(v.z * w.z) + (double)(v.w * w.w);
+  if constexpr (sizeof(T) == 4) {
+    const float4 *a4 = reinterpret_cast<const float4 *>(a);
+    const float4 *x4 = reinterpret_cast<const float4 *>(xs);
+    for (int c = c_lo + lane; c < c_hi; c += 64) {
+      const float4 v = a4[c], w = x4[c];
+      s += (double)(v.x * w.x) + (double)(v.y * w.y) + (double)(v.z * w.z) + (double)(v.w * w.w);
+    }
+  } else {
+    const double2 *a2 = reinterpret_cast<const double2 *>(a);
+    const double2 *x2 = reinterpret_cast<const double2 *>(xs);
+    for (int c = c_lo + lane; c < c_hi; c += 64) {
+      const double2 v = a2[c], w = x2[c];
+      s += v.x * w.x + v.y * w.y;
+    }
   }
   s = wave_sum(s);
-  if (lane == 0) y[row] = (float)s;
+  if (lane == 0) y[row] = (T)s;
 }
 
 // ---- dense coarse solve: banded blocked Cholesky, X = L^-1, A^-1 = X^T X ------------
@@ -612,74 +618,6 @@ __device__ void x_part(const double *Lb, double *X, int n, int jb, int ix, int k
     if (rok && c0 + v < n) X[grow + (int64_t)(c0 + v) * n] = T[v];
 }
 
-// Contribution of X row block K to tile (I, J) of A^-1 (J <= I <= K): the first one
-// (K == I) initialises the tile, the last (K == nbk - 1) also writes its transpose.
-__device__ void xtx_part(const double *X, double *Ai, int n, int nbk, int K, int I, int J,
-                         double *lds)
-{
-  double *XI = lds, *XJ = lds + CB * (CB + 1);  // [row][column], padded rows
-  const int t = threadIdx.x;
-  constexpr int PT = CB * CB / 256;
-  double vi[PT], vj[PT];  // all 32 loads in flight at once
-#pragma unroll
-  for (int k = 0; k < PT; ++k) {
-    const int e = 256 * k + t, r = e & 63, cc = e >> 6;
-    const int64_t gr = (int64_t)CB * K + r;
-    const int64_t gi = (int64_t)CB * I + cc, gj = (int64_t)CB * J + cc;
-    const bool oki = gr < n && gi < n, okj = gr < n && gj < n;
-    const double a = X[(oki ? gr : 0) + (oki ? gi : 0) * n];
-    const double b = X[(okj ? gr : 0) + (okj ? gj : 0) * n];
-    vi[k] = oki ? a : 0.0;
-    vj[k] = okj ? b : 0.0;
-  }
-#pragma unroll
-  for (int k = 0; k < PT; ++k) {
-    const int e = 256 * k + t, r = e & 63, cc = e >> 6;
-    XI[r * (CB + 1) + cc] = vi[k];
-    XJ[r * (CB + 1) + cc] = vj[k];
-  }
-  __syncthreads();
-  const int ra = (t & 15) * 4, cb = (t >> 4) * 4;
-  double acc[4][4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-  for (int r0 = 0; r0 < CB; r0 += 8) {
-    double a[8][4], b[8][4];  // 8 rows read together (pinned), then 128 FMAs
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a[r][u] = XI[(r0 + r) * (CB + 1) + ra + u];
-        b[r][u] = XJ[(r0 + r) * (CB + 1) + cb + u];
-      }
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-#pragma unroll
-      for (int u = 0; u < 4; ++u) asm volatile("" : "+v"(a[r][u]), "+v"(b[r][u]));
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] += a[r][u] * b[r][v];
-  }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int64_t row = (int64_t)CB * I + ra + u;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int64_t col = (int64_t)CB * J + cb + v;
-      if (row < n && col < n) {
-        const double val = (K == I) ? acc[u][v] : Ai[row + col * n] + acc[u][v];
-        Ai[row + col * n] = val;
-        if (K == nbk - 1 && I != J) Ai[col + row * n] = val;
-      }
-    }
-  }
-}
-
 template <bool FULL>
 __global__ __launch_bounds__(256) void k_coarse_launch(double *A, double *Lb, double *X, int n,
                                                        int nbk, int jb, int k, int kb, int m,
@@ -689,25 +627,20 @@ __global__ __launch_bounds__(256) void k_coarse_launch(double *A, double *Lb, do
   const int b = blockIdx.x;
   if (b < nchol) {
     chol_part<FULL>(A, Lb, n, CB * k, kb, m, info, b, lds);
-  } else if (b < nchol + nx) {
+  } else {
     const int xb = b - nchol;
     x_part(Lb, X, n, jb, k - 1, xb / XPARTS, xb % XPARTS, lds);
-  } else {
-    int I, J;
-    tile_ij(b - nchol - nx, I, J);
-    xtx_part(X, A, n, nbk, k - 2, I, J, lds);
   }
 }
 
 }  // namespace
 
-void coarse_factor_invert(hipStream_t s, double *A, double *Lb, double *X, int n, int bw,
-                          int *info)
+void coarse_inverse_factor(hipStream_t s, double *A, double *Lb, double *X, int n, int bw, int *info)
 {
   const int nbk = (n + CB - 1) / CB;
   const int jb = std::min((bw + CB - 1) / CB, nbk);
   TOFE_HIP(hipMemsetAsync(Lb, 0, sizeof(double) * (size_t)n * n, s));
-  for (int k = 0; k <= nbk + 1; ++k) {
+  for (int k = 0; k <= nbk; ++k) {
     int kb = 0, m = 0, nchol = 0;
     if (k < nbk) {
       kb = std::min(CB, n - CB * k);
@@ -715,10 +648,8 @@ void coarse_factor_invert(hipStream_t s, double *A, double *Lb, double *X, int n
       const int nI = (m + CB - 1) / CB;
       nchol = m > 0 ? nI * (nI + 1) / 2 : 1;
     }
-    const int nx = (k >= 1 && k <= nbk) ? k * XPARTS : 0;
-    const int K = k - 2;
-    const int nxtx = (K >= 0) ? (K + 1) * (K + 2) / 2 : 0;
-    const dim3 grid((unsigned)(nchol + nx + nxtx)), block(256);
+    const int nx = k >= 1 ? k * XPARTS : 0;
+    const dim3 grid((unsigned)(nchol + nx)), block(256);
     if (kb == CB || k >= nbk)
       k_coarse_launch<true><<<grid, block, 0, s>>>(A, Lb, X, n, nbk, jb, k, kb, m, nchol, nx, info);
     else
@@ -829,22 +760,24 @@ void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, do
   TOFE_LAUNCH_CHECK();
 }
 
-template <typename Tin, typename Tout>
-void dense_gemv(hipStream_t s, const double *A, int64_t n, const Tin *x, Tout *y)
+
+template <typename T>
+void tri_pack(hipStream_t s, const double *X, int n, int ld, T *Xr, T *XTr)
 {
-  k_gemv<Tin, Tout><<<dim3((unsigned)((n + 3) / 4)), dim3(NT), 0, s>>>(A, n, x, y);
+  k_tri_pack<T><<<dim3((unsigned)(((int64_t)n * ld + NT - 1) / NT)), dim3(NT), 0, s>>>(X, n, ld, Xr, XTr);
   TOFE_LAUNCH_CHECK();
 }
 
-void pad32(hipStream_t s, const double *A, int n, int ld, float *B)
+template <typename T>
+void coarse_solve_tri(hipStream_t s, const T *Xr, const T *XTr, int n, int ld, const T *b, T *tmp, T *x)
 {
-  k_pad32<<<dim3((unsigned)(((int64_t)n * ld + NT - 1) / NT)), dim3(NT), 0, s>>>(A, n, ld, B);
+  if (ld % (16 / (int)sizeof(T)) != 0 || ld < n)
+    throw Error(TOFE_ERR_ARG, "coarse solve: bad padded leading dimension");
+  const dim3 grid((unsigned)((n + 3) / 4)), block(NT);
+  const size_t lds = sizeof(T) * (size_t)ld;
+  k_trigemv<T, true><<<grid, block, lds, s>>>(Xr, n, ld, b, tmp);
   TOFE_LAUNCH_CHECK();
-}
-
-void dense_gemv32(hipStream_t s, const float *A, int n, int ld, const float *x, float *y)
-{
-  k_gemv32<<<dim3((unsigned)((n + 3) / 4)), dim3(NT), sizeof(float) * ld, s>>>(A, n, ld, x, y);
+  k_trigemv<T, false><<<grid, block, lds, s>>>(XTr, n, ld, tmp, x);
   TOFE_LAUNCH_CHECK();
 }
 
@@ -854,8 +787,12 @@ template void stencil_op<double>(hipStream_t, const StencilOp<double> &, const d
                                  const EpiArgs<double> &);
 template void stencil_op<float>(hipStream_t, const StencilOp<float> &, const float *, Epi,
                                 const EpiArgs<float> &);
-template void dense_gemv<double, double>(hipStream_t, const double *, int64_t, const double *, double *);
-template void dense_gemv<float, float>(hipStream_t, const double *, int64_t, const float *, float *);
+template void tri_pack<double>(hipStream_t, const double *, int, int, double *, double *);
+template void tri_pack<float>(hipStream_t, const double *, int, int, float *, float *);
+template void coarse_solve_tri<double>(hipStream_t, const double *, const double *, int, int, const double *,
+                                       double *, double *);
+template void coarse_solve_tri<float>(hipStream_t, const float *, const float *, int, int, const float *,
+                                      float *, float *);
 
 template void stencil_op_sym<float>(hipStream_t, const StencilOp<float> &, const float *, Epi,
                                     const EpiArgs<float> &);

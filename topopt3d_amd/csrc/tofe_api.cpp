@@ -136,8 +136,11 @@ struct tofe_ctx {
   int n_masked = 0;
   DBuf<double> gal_t1, gal_t2;  // axis-by-axis Galerkin scratch
   DBuf<double> Ainv;
-  DBuf<float> Ainv32;  // FP32 copy for the mixed-precision V-cycle, rows padded to ld32
-  int ld32 = 0;
+  // X = L^-1 of the coarse operator as padded row-major triangles X and X^T, in the V-cycle's
+  // precision, and an n-vector for the solve's intermediate X b
+  DBuf<double> X64r, XT64r, ctmp64;
+  DBuf<float> X32r, XT32r, ctmp32;
+  int ldc = 0;  // padded row length of the packed coarse triangles
   DBuf<int> dinfo;
   DBuf<double> chol_l, chol_x;
   int64_t nc = 0;
@@ -312,12 +315,15 @@ void alloc_level32(tofe_ctx *c, Level<float> &lv, int l)
   lv.d.zero(c->stream);
 }
 
-// Exact solve with the coarsest operator: its explicit inverse, in the V-cycle's precision.
+// Exact solve with the coarsest operator, x = A^-1 b = X^T (X b) with X = L^-1, in the
+// V-cycle's precision.
 template <typename T>
 void coarse_apply(tofe_ctx *c, const T *b, T *x)
 {
-  if constexpr (sizeof(T) == 4) dense_gemv32(c->stream, c->Ainv32.p, (int)c->nc, c->ld32, b, x);
-  else dense_gemv<double, double>(c->stream, c->Ainv.p, c->nc, b, x);
+  if constexpr (sizeof(T) == 4)
+    coarse_solve_tri<float>(c->stream, c->X32r.p, c->XT32r.p, (int)c->nc, c->ldc, b, c->ctmp32.p, x);
+  else
+    coarse_solve_tri<double>(c->stream, c->X64r.p, c->XT64r.p, (int)c->nc, c->ldc, b, c->ctmp64.p, x);
 }
 
 // FP64 ends of the mixed-precision V-cycle, fused into its first and last fine-level
@@ -703,10 +709,11 @@ void chebyshev_coefficients(tofe_ctx *c)
   }
 }
 
-// A (dense SPD, column-major, in c->Ainv) -> A^-1 in place on the second stream: banded
-// blocked Cholesky (one launch per 64-column step), X = L^-1, A^-1 = X^T X, stored full
-// for the per-V-cycle GEMV.  The coarse operator couples DOFs at most bw apart (27-point
-// stencil, x-fastest nodes) and the factor's fill stays inside that band.
+// A (dense SPD, column-major, in c->Ainv) -> X = L^-1 on the second stream: banded blocked
+// Cholesky (one launch per 64-column step, each also forming the previous row block of X),
+// then X and X^T packed as padded row-major triangles for the per-V-cycle solve
+// X^T (X b).  The coarse operator couples DOFs at most bw apart (27-point stencil,
+// x-fastest nodes) and the factor's fill stays inside that band.
 void coarse_inverse_enqueue(tofe_ctx *c)
 {
   const hipStream_t s = c->stream2;
@@ -717,11 +724,18 @@ void coarse_inverse_enqueue(tofe_ctx *c)
   const Grid &gL = c->grids[c->L - 1];
   const int bw = 3 * (gL.Nx * gL.Ny + gL.Nx + 1) + 2;
   // a failed factorisation is reported by coarse_inverse_finish; the inverse is then unused
-  coarse_factor_invert(s, c->Ainv.p, c->chol_l.p, c->chol_x.p, n, bw, c->dinfo.p);
+  coarse_inverse_factor(s, c->Ainv.p, c->chol_l.p, c->chol_x.p, n, bw, c->dinfo.p);
+  c->ldc = (n + 255) / 256 * 256;
   if (c->precision == TOFE_PREC_MIXED) {
-    c->ld32 = (n + 255) / 256 * 256;
-    c->Ainv32.alloc((size_t)n * c->ld32);
-    pad32(s, c->Ainv.p, n, c->ld32, c->Ainv32.p);
+    c->X32r.alloc((size_t)n * c->ldc);
+    c->XT32r.alloc((size_t)n * c->ldc);
+    c->ctmp32.alloc(n);
+    tri_pack<float>(s, c->chol_x.p, n, c->ldc, c->X32r.p, c->XT32r.p);
+  } else {
+    c->X64r.alloc((size_t)n * c->ldc);
+    c->XT64r.alloc((size_t)n * c->ldc);
+    c->ctmp64.alloc(n);
+    tri_pack<double>(s, c->chol_x.p, n, c->ldc, c->X64r.p, c->XT64r.p);
   }
 }
 

@@ -126,17 +126,17 @@ int64_t galerkin_scratch(int Nfx, int Nfy, int Nfz, int64_t *n2);
 void stencil_diag(hipStream_t s, const double *S, int64_t nn, double *diag, double *dinv);
 void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A);
 constexpr int kCholNB = 64;  // tile of the coarse Cholesky factorisation
-// A (n x n SPD, column-major, half-bandwidth bw) -> A^-1 in place: banded blocked
-// Cholesky into Lb, X = L^-1 into X, then A^-1 = X^T X.  info[0] = first non-positive
-// pivot index + 1 (0 if none).  Lb and X are n x n scratch.
-void coarse_factor_invert(hipStream_t s, double *A, double *Lb, double *X, int n, int bw,
-                          int *info);
-template <typename Tin, typename Tout>
-void dense_gemv(hipStream_t s, const double *A, int64_t n, const Tin *x, Tout *y);
-// FP32 copy of the n x n symmetric coarse inverse with rows padded to ld (multiple of 256)
-// floats, and y = A x with it.
-void pad32(hipStream_t s, const double *A, int n, int ld, float *B);
-void dense_gemv32(hipStream_t s, const float *A, int n, int ld, const float *x, float *y);
+// A (n x n SPD, column-major, half-bandwidth bw): banded blocked Cholesky A = L L^T into
+// Lb and X = L^-1 into X (column-major, lower triangle), so A^-1 = X^T X.  info[0] = first
+// non-positive pivot index + 1 (0 if none).  Lb and X are n x n scratch.
+void coarse_inverse_factor(hipStream_t s, double *A, double *Lb, double *X, int n, int bw, int *info);
+// Row-major copies of X and X^T in T, rows padded to ld (a multiple of 16 bytes / sizeof(T)),
+// zero outside the triangle.
+template <typename T>
+void tri_pack(hipStream_t s, const double *X, int n, int ld, T *Xr, T *XTr);
+// x = A^-1 b = X^T (X b) with the packed triangles; tmp holds n values.
+template <typename T>
+void coarse_solve_tri(hipStream_t s, const T *Xr, const T *XTr, int n, int ld, const T *b, T *tmp, T *x);
 
 // ---- level-1 Galerkin operator from E and KE (k_galerkin.hip) -------------
 struct MaskedElems {
