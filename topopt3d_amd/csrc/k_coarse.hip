@@ -166,6 +166,14 @@ __global__ __launch_bounds__(NT) void k_galerkin_axis(const double *__restrict__
 // the block of a lower offset o < 13 is the transpose of the neighbour's block 26 - o.  Each
 // stored block is read by its node and by one neighbour, so the HBM traffic is about half of
 // the full stencil's.
+// Node-blocked layout of the FP32 upper-half stencil: value v (0..125) of node n at
+// (n / 64) * 126 * 64 + v * 64 + n % 64, so the 126 values of a wavefront's 64 nodes form one
+// contiguous 32 KB range (each value still one coalesced 256-byte read).
+__host__ __device__ inline int64_t su_index(int v, int64_t n)
+{
+  return (n >> 6) * (126 * 64) + (int64_t)v * 64 + (n & 63);
+}
+
 template <typename T, int EPI, bool SPLIT>
 __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T *__restrict__ u,
                                                        EpiArgs<T> ep, int64_t nn)
@@ -197,11 +205,11 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
         const int a = SPLIT ? a0 : c;
         T t;
         if (o >= 13) {
-          const T *B = S + (int64_t)((o - 13) * 9) * nn + n;
-          t = B[(a * 3 + 0) * nn] * u0 + B[(a * 3 + 1) * nn] * u1 + B[(a * 3 + 2) * nn] * u2;
+          const T *B = S + su_index((o - 13) * 9, n);
+          t = B[(a * 3 + 0) * 64] * u0 + B[(a * 3 + 1) * 64] * u1 + B[(a * 3 + 2) * 64] * u2;
         } else {
-          const T *B = S + (int64_t)((13 - o) * 9) * nn + m;  // block (m, 26 - o), transposed
-          t = B[(0 * 3 + a) * nn] * u0 + B[(1 * 3 + a) * nn] * u1 + B[(2 * 3 + a) * nn] * u2;
+          const T *B = S + su_index((13 - o) * 9, m);  // block (m, 26 - o), transposed
+          t = B[(0 * 3 + a) * 64] * u0 + B[(1 * 3 + a) * 64] * u1 + B[(2 * 3 + a) * 64] * u2;
         }
         acc[c] = in ? acc[c] + t : acc[c];
       }
@@ -223,12 +231,14 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
   }
 }
 
-// FP64 full stencil -> FP32 upper half (offsets 13..26).
+// FP64 full stencil -> FP32 upper half (offsets 13..26) in the node-blocked layout.
 __global__ void k_stencil_upper32(const double *__restrict__ S, int64_t nn, float *Su)
 {
   const int64_t tot = 126 * nn;
-  for (int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x; t < tot; t += (int64_t)gridDim.x * NT)
-    Su[t] = (float)S[13 * 9 * nn + t];
+  for (int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x; t < tot; t += (int64_t)gridDim.x * NT) {
+    const int v = (int)(t / nn);
+    Su[su_index(v, t - v * nn)] = (float)S[13 * 9 * nn + t];
+  }
 }
 
 __global__ void k_stencil_diag(const double *__restrict__ S, int64_t nn, double *diag,

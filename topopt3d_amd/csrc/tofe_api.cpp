@@ -305,7 +305,8 @@ void alloc_level32(tofe_ctx *c, Level<float> &lv, int l)
 {
   lv.g = c->grids[l];
   const int64_t nd = lv.g.ndof();
-  if (l > 0) lv.S.alloc((size_t)126 * lv.g.nn());  // upper half of the symmetric stencil
+  // upper half of the symmetric stencil, node-blocked by 64 (k_stencil_op_sym)
+  if (l > 0) lv.S.alloc((size_t)126 * ((lv.g.nn() + 63) / 64 * 64));
   lv.dinv.alloc(nd);
   lv.b.alloc(nd);
   lv.x.alloc(nd);
