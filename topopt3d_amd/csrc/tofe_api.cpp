@@ -146,6 +146,7 @@ struct tofe_ctx {
   int64_t nc = 0;
   DBuf<double> partial, partial2, partial3, scal, eig_ab;
   DBuf<double> partial_b, partial2_b, scal_b;  // reductions of the third stream
+  DBuf<double> partial_c, partial2_c, scal_c;  // set-up reductions of the main stream
   int npart = 0;
   std::vector<double> lam;
   std::vector<double> c1, c2;  // Chebyshev coefficients, per level * kmax
@@ -612,15 +613,18 @@ void lanczos_coefficients(tofe_ctx *c, int l, hipStream_t s, T *r, T *p, T *q, d
   }
 }
 
-// The smoothed levels' estimates are independent: levels 1.. run in order on the third
-// stream and the fine level on the fourth, each on its level's V-cycle work vectors, once
-// the operators are built on the main stream.  In the mixed-precision build they run on
+// The smoothed levels' estimates are independent: level 1 runs on the third stream, the fine
+// level on the fourth and levels 2.. in order on the main stream, each on its level's V-cycle
+// work vectors and its stream's reduction buffers, once the operators are built on the main
+// stream.  In the mixed-precision build they run on
 // the FP32 operators of the V-cycle (scalars and recorded coefficients stay FP64), whose
 // copies are made on the same streams first.  ev_lz / ev_lz0 mark their ends.
 void lanczos_enqueue(tofe_ctx *c)
 {
   const int L = c->L;
-  const hipStream_t s3 = c->stream3, s4 = c->stream4;
+  // level 1 on the third stream, the fine level on the fourth, levels 2.. on the main
+  // stream (idle until it joins the others)
+  const hipStream_t s0 = c->stream, s3 = c->stream3, s4 = c->stream4;
   c->eig_ab.alloc((size_t)2 * L * kMaxEig);
   TOFE_HIP(hipEventRecord(c->ev_ops, c->stream));
   TOFE_HIP(hipStreamWaitEvent(s3, c->ev_ops, 0));
@@ -638,13 +642,18 @@ void lanczos_enqueue(tofe_ctx *c)
     convert<double, float>(s4, c->lv64[0]->dinv.p, c->lv32[0]->dinv.p, c->ndof);
     convert<double, float>(s4, c->E.p, c->E32.p, c->ne);
     for (int l = 1; l < L; ++l) {
-      convert<double, float>(s3, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, c->grids[l].ndof());
-      stencil_upper32(s3, c->lv64[l]->S.p, c->grids[l].nn(), c->lv32[l]->S.p);
+      const hipStream_t sl = l == 1 ? s3 : s0;
+      convert<double, float>(sl, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, c->grids[l].ndof());
+      stencil_upper32(sl, c->lv64[l]->S.p, c->grids[l].nn(), c->lv32[l]->S.p);
     }
     for (int l = 1; l + 1 < L; ++l) {
       Level<float> &lv = *c->lv32[l];
-      lanczos_coefficients<float>(c, l, s3, lv.r.p, lv.x.p, lv.x2.p, c->partial_b.p,
-                                  c->partial2_b.p, c->scal_b.p);
+      if (l == 1)
+        lanczos_coefficients<float>(c, l, s3, lv.r.p, lv.x.p, lv.x2.p, c->partial_b.p,
+                                    c->partial2_b.p, c->scal_b.p);
+      else
+        lanczos_coefficients<float>(c, l, s0, lv.r.p, lv.x.p, lv.x2.p, c->partial_c.p,
+                                    c->partial2_c.p, c->scal_c.p);
     }
     if (L > 1) {
       Level<float> &lv = *c->lv32[0];
@@ -654,8 +663,12 @@ void lanczos_enqueue(tofe_ctx *c)
   } else {
     for (int l = 1; l + 1 < L; ++l) {
       Level<double> &lv = *c->lv64[l];
-      lanczos_coefficients<double>(c, l, s3, lv.r.p, lv.x.p, lv.x2.p, c->partial_b.p,
-                                   c->partial2_b.p, c->scal_b.p);
+      if (l == 1)
+        lanczos_coefficients<double>(c, l, s3, lv.r.p, lv.x.p, lv.x2.p, c->partial_b.p,
+                                     c->partial2_b.p, c->scal_b.p);
+      else
+        lanczos_coefficients<double>(c, l, s0, lv.r.p, lv.x.p, lv.x2.p, c->partial_c.p,
+                                     c->partial2_c.p, c->scal_c.p);
     }
     if (L > 1)
       lanczos_coefficients<double>(c, 0, s4, c->vr.p, c->vp.p, c->vq.p, c->partial.p,
@@ -791,9 +804,9 @@ void assemble(tofe_ctx *c)
   c->dinfo.alloc(2);
   stencil_to_dense(s, SL, gL.Nx, gL.Ny, gL.Nz, c->Ainv.p);
   TOFE_HIP(hipEventRecord(c->ev_dense, s));
-  // Four streams from here: the coarsest level's factorisation and inverse on the second,
-  // the Lanczos estimates of levels 1.. on the third and of the fine level on the fourth;
-  // the main stream then joins the other three.
+  // Four streams from here: the coarsest level's factorisation on the second, the Lanczos
+  // estimates of level 1 on the third, of the fine level on the fourth and of levels 2.. on
+  // the main stream, which then joins the other three.
   TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
   coarse_inverse_enqueue(c);
   TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
@@ -1218,6 +1231,10 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     c->partial2_b.alloc(np);
     c->scal_b.alloc(SL_COUNT);
     c->scal_b.zero(c->stream);
+    c->partial_c.alloc(np);
+    c->partial2_c.alloc(np);
+    c->scal_c.alloc(SL_COUNT);
+    c->scal_c.zero(c->stream);
     TOFE_HIP(hipStreamSynchronize(c->stream));
     c->stats.levels = c->L;
     c->stats.coarse_size = c->grids[c->L - 1].ndof();
