@@ -383,11 +383,22 @@ __device__ inline int chol_diag(const double *A, int64_t lda, int k0, int kb, do
   // latency overlaps that work.  A non-positive pivot is replaced by 1 (and reported).
   int bad = 0;
   double d, rd;
+  // rd = 1/sqrt(p) from the hardware estimate and two Newton steps, d = sqrt(p) from rd with
+  // one correction: about a dozen dependent operations per pivot instead of the twenty-five
+  // of a correctly rounded square root and division (both results within about 1 ulp).
   auto pivot = [&](double p, int j) {
     const bool ok = p > 0;
     bad = (bad == 0 && !ok) ? k0 + j + 1 : bad;
-    d = sqrt(ok ? p : 1.0);
-    rd = 1.0 / d;
+    const double q = ok ? p : 1.0;
+    double y = __builtin_amdgcn_rsq(q);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const double e = fma(-q * y, y, 1.0);
+      y = fma(0.5 * y, e, y);
+    }
+    const double s0 = q * y;
+    d = fma(fma(-s0, s0, q), 0.5 * y, s0);
+    rd = y;
   };
   pivot(lane_value(r[0], 0), 0);
 #pragma unroll
@@ -445,7 +456,7 @@ __device__ inline void tile_ij(int b, int &I, int &J)
 // factorisation has finished with, and the upper triangle is written at the last
 // contribution, after the last factorisation step.
 constexpr int XCPW = 4, XCW = 4 * XCPW, XPARTS = CB / XCW;
-constexpr int kCoarseLds = CB * CB + CB + 2 * CB * CB;  // doubles
+constexpr int kCoarseLds = CB * CB + CB + 3 * CB * CB;  // doubles
 
 // Factorisation step: diagonal block (every workgroup, redundantly, so none waits for
 // another), the panel rows P = A21 L11^-T of its row blocks I and J of the window, and
@@ -456,9 +467,33 @@ __device__ void chol_part(double *A, double *Lb, int n, int k0, int kb, int m, i
                           double *lds)
 {
   double *Ls = lds, *dinv = lds + CB * CB, *Pt0 = lds + CB * CB + CB, *Pt1 = Pt0 + CB * CB;
+  double *Ct = Pt1 + CB * CB;  // tile C(I,J) of the trailing window, column-major
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   int I, J;
   tile_ij(b, I, J);
+  // While wavefront 0 factors the diagonal block, wavefronts 1 and 2 load the panel rows of
+  // row blocks I and J and wavefront 3 stages tile C(I,J), so no global load waits on the
+  // steps after the factorisation.
+  const int w0 = k0 + kb;  // first row and column of the trailing window
+  const bool panel_wave = m > 0 && (w == 1 || (w == 2 && J != I));
+  const int prow = 64 * (w == 2 ? J : I) + lane;  // this lane's row of the window
+  const bool plive = panel_wave && prow < m;
+  double x[CB];
+  if (panel_wave) {
+    const int rowc = plive ? prow : 0;
+#pragma unroll
+    for (int q = 0; q < CB; ++q) {
+      const double a = A[(w0 + rowc) + (int64_t)(k0 + q) * n];
+      x[q] = plive ? a : 0.0;
+    }
+  }
+  if (w == 3 && m > 0) {
+    const int r = 64 * I + lane, rc = r < m ? r : 0;
+    for (int c = 0; c < CB; ++c) {
+      const int col = 64 * J + c, cc = col < m ? col : 0;
+      Ct[c * CB + lane] = A[(w0 + rc) + (int64_t)(w0 + cc) * n];
+    }
+  }
   if (w == 0) {
     const int bad = chol_diag<FULL>(A, n, k0, kb, Ls, dinv);
     if (b == 0 && lane == 0 && bad != 0 && info[0] == 0) info[0] = bad;
@@ -472,19 +507,10 @@ __device__ void chol_part(double *A, double *Lb, int n, int k0, int kb, int m, i
   }
   if (m <= 0) return;
   // m > 0 means a full diagonal block (kb == 64) with rows below it
-  const int w0 = k0 + kb;  // first row and column of the trailing window
-  if (t < 128 && (t < 64 || J != I)) {
-    const int s = t >> 6;
-    double *Pt = s ? Pt1 : Pt0;
-    const int row = 64 * (s ? J : I) + lane;  // row of the window
-    const bool live = row < m;
-    const int rowc = live ? row : 0;
-    double x[CB];
-#pragma unroll
-    for (int q = 0; q < CB; ++q) {
-      const double a = A[(w0 + rowc) + (int64_t)(k0 + q) * n];
-      x[q] = live ? a : 0.0;
-    }
+  if (panel_wave) {
+    double *Pt = (w == 2) ? Pt1 : Pt0;
+    const int row = prow;
+    const bool live = plive;
     // x L11^T = a: column by column, each finished x[q] updates the later entries
 #pragma unroll
     for (int q = 0; q < CB; ++q) {
@@ -507,36 +533,33 @@ __device__ void chol_part(double *A, double *Lb, int n, int k0, int kb, int m, i
     }
   }
   __syncthreads();
+  // C(I,J) -= P_I P_J^T on the FP64 matrix cores: wavefront w forms rows 16w..16w+15 of the
+  // tile as four 16x16 blocks, 16 k-steps of v_mfma_f64_16x16x4 each.  Operand layouts (lane
+  // l): A(16x4) element (l % 16, l / 16), B(4x16) element (l / 16, l % 16); output item j of
+  // lane l is D(l / 16 + 4 j, l % 16).
   const double *PI = Pt0, *PJ = (I == J) ? Pt0 : Pt1;
-  const int ra = (t & 15) * 4, cb = (t >> 4) * 4;
-  double acc[4][4];
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  const int mr = lane & 15, kq = lane >> 4;
+  v4d acc[4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int cbk = 0; cbk < 4; ++cbk) acc[cbk] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int q0 = 0; q0 < CB; q0 += 4) {
+    const double a = PI[(q0 + kq) * CB + 16 * w + mr];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-#pragma unroll 8
-  for (int q = 0; q < CB; ++q) {
-    double a[4], bb[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      a[u] = PI[q * CB + ra + u];
-      bb[u] = PJ[q * CB + cb + u];
+    for (int cbk = 0; cbk < 4; ++cbk) {
+      const double bv = PJ[(q0 + kq) * CB + 16 * cbk + mr];
+      acc[cbk] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[cbk], 0, 0, 0);
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * bb[v];
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int row = 64 * I + ra + u;
+  for (int cbk = 0; cbk < 4; ++cbk)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int col = 64 * J + cb + v;
+    for (int jj = 0; jj < 4; ++jj) {
+      const int row = 64 * I + 16 * w + kq + 4 * jj, col = 64 * J + 16 * cbk + mr;
       if (row < m && col < m && (I != J || row >= col))
-        A[(w0 + row) + (int64_t)(w0 + col) * n] -= acc[u][v];
+        A[(w0 + row) + (int64_t)(w0 + col) * n] = Ct[(16 * cbk + mr) * CB + 16 * w + kq + 4 * jj] - acc[cbk][jj];
     }
-  }
 }
 
 // Row block ix of X = L^-1, columns [CB kc + XCW part, + XCW): T = E - sum_j L(ix,j) X(j,kc)
