@@ -161,6 +161,79 @@ __global__ __launch_bounds__(NT) void k_galerkin_axis(const double *__restrict__
   }
 }
 
+// The same 1-D Galerkin product for an intermediate step (no mirroring), three outputs per
+// thread: for a fixed output node, (b, c)-offset and component pair, the outputs for the
+// three offsets D along the axis read the same nine input values (input nodes 2I - 1 .. 2I + 1
+// along the axis, input offsets -1 .. 1 along it), so each is loaded once instead of three
+// times.  Each output is summed in the same (i, d) order as in k_galerkin_axis.
+__global__ __launch_bounds__(NT) void k_galerkin_axis3(const double *__restrict__ in, int n0, int n1,
+                                                       int n2, int axis, double *__restrict__ out)
+{
+  int m[3] = {n0, n1, n2};
+  const int Nf = m[axis];
+  const int Nc = (Nf + 1) / 2;
+  m[axis] = Nc;
+  const int64_t nin = (int64_t)n0 * n1 * n2, nout = (int64_t)m[0] * m[1] * m[2];
+  // grid: x over the output nodes, y over the 81 (other-axes offset, component pair) values
+  const int64_t node = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (node >= nout) return;
+  const int ob = blockIdx.y / 9, q = blockIdx.y % 9;
+  int c[3];
+  node_ijk(node, m[0], m[1], c[0], c[1], c[2]);
+  // offsets along the other two axes (increasing axis order) from ob
+  const int ax1 = axis == 0 ? 1 : 0, ax2 = axis == 2 ? 1 : 2;
+  int off[3];
+  off[ax1] = ob % 3 - 1;
+  off[ax2] = ob / 3 - 1;
+  off[axis] = 0;
+  bool other_in = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (k == axis) continue;
+    const int cc = c[k] + off[k];
+    if (cc < 0 || cc >= m[k]) other_in = false;
+  }
+  const int I = c[axis];
+  const int sa = axis == 0 ? 1 : (axis == 1 ? n0 : n0 * n1);  // input node stride along axis
+  const int so = axis == 0 ? 1 : (axis == 1 ? 3 : 9);         // offset-index stride of axis
+  const int o_mid = (off[0] + 1) + 3 * (off[1] + 1) + 9 * (off[2] + 1);  // axis offset 0
+  int cin[3] = {c[0], c[1], c[2]};
+  cin[axis] = 0;
+  const int64_t base_node = cin[0] + (int64_t)n0 * (cin[1] + (int64_t)n1 * cin[2]);
+  double v[3][3];  // v[di + 1][d + 1] = in(node 2I + di, axis offset d)
+#pragma unroll
+  for (int di = -1; di <= 1; ++di) {
+    const int i = 2 * I + di;
+    const int ic = (i >= 0 && i < Nf) ? i : 2 * I;
+#pragma unroll
+    for (int d = -1; d <= 1; ++d)
+      v[di + 1][d + 1] = in[(int64_t)((o_mid + d * so) * 9 + q) * nin + base_node + (int64_t)ic * sa];
+  }
+#pragma unroll
+  for (int D = -1; D <= 1; ++D) {
+    const int o = o_mid + D * so;
+    const int64_t t = (int64_t)(o * 9 + q) * nout + node;
+    if (!other_in || I + D < 0 || I + D >= Nc) {
+      out[t] = 0.0;  // the coupled output node does not exist
+      continue;
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int di = -1; di <= 1; ++di) {
+      const int i = 2 * I + di;
+      const bool iok = i >= 0 && i < Nf;
+      const double wi = iok ? w1d(i, I, Nf) : 0.0;
+#pragma unroll
+      for (int d = -1; d <= 1; ++d) {
+        const int j = i + d;
+        const double wj = (iok && j >= 0 && j < Nf) ? w1d(j, I + D, Nf) : 0.0;
+        acc += wi * wj * v[di + 1][d + 1];
+      }
+    }
+    out[t] = acc;
+  }
+}
+
 // The same operator from the upper half of an exactly symmetric stencil: offsets
 // o >= 13 are stored at the node (Su[((o-13)*9 + a*3 + b)*nn + n], 126 values per node) and
 // the block of a lower offset o < 13 is the transpose of the neighbour's block 26 - o.  Each
@@ -761,11 +834,9 @@ void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double
   const int Ncx = (Nfx + 1) / 2, Ncy = (Nfy + 1) / 2, Ncz = (Nfz + 1) / 2;
   const int64_t n1 = (int64_t)Ncx * Nfy * Nfz, n2 = (int64_t)Ncx * Ncy * Nfz,
                 n3 = (int64_t)Ncx * Ncy * Ncz;
-  k_galerkin_axis<<<dim3((unsigned)((n1 + NT - 1) / NT), 243), dim3(NT), 0, s>>>(Sf, Nfx, Nfy, Nfz, 0,
-                                                                                T1, 0);
+  k_galerkin_axis3<<<dim3((unsigned)((n1 + NT - 1) / NT), 81), dim3(NT), 0, s>>>(Sf, Nfx, Nfy, Nfz, 0, T1);
   TOFE_LAUNCH_CHECK();
-  k_galerkin_axis<<<dim3((unsigned)((n2 + NT - 1) / NT), 243), dim3(NT), 0, s>>>(T1, Ncx, Nfy, Nfz, 1,
-                                                                                T2, 0);
+  k_galerkin_axis3<<<dim3((unsigned)((n2 + NT - 1) / NT), 81), dim3(NT), 0, s>>>(T1, Ncx, Nfy, Nfz, 1, T2);
   TOFE_LAUNCH_CHECK();
   k_galerkin_axis<<<dim3((unsigned)((n3 + NT - 1) / NT), 243), dim3(NT), 0, s>>>(T2, Ncx, Ncy, Nfz, 2,
                                                                                 Sc, 1);

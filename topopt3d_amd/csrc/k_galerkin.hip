@@ -130,6 +130,28 @@ __device__ inline void galerkin_block(const double *__restrict__ E, const int *_
 
 // One thread per (coarse node I, coarse offset o >= 13); the transposed block goes to
 // (J, 26 - o) so the level-1 operator is exactly symmetric.
+// The couplings of level-1 node I to a neighbour I + D outside the grid are zero; no thread
+// of k_galerkin_fine writes them (it returns for such offsets), so they are cleared here, the
+// only entries of Sc that k_galerkin_fine leaves alone.
+__global__ __launch_bounds__(NT) void k_galerkin_zero_outside(int nx, int ny, int nz, double *__restrict__ Sc)
+{
+  const int Nc[3] = {(nx + 2) / 2, (ny + 2) / 2, (nz + 2) / 2};
+  const int64_t ncn = (int64_t)Nc[0] * Nc[1] * Nc[2];
+  const int64_t I = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (I >= ncn) return;
+  const int o = (int)blockIdx.y;
+  const int D[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+  int Ic[3];
+  node_ijk(I, Nc[0], Nc[1], Ic[0], Ic[1], Ic[2]);
+  bool out = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (Ic[d] + D[d] < 0 || Ic[d] + D[d] >= Nc[d]) out = true;
+  if (!out) return;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) Sc[(int64_t)(o * 9 + q) * ncn + I] = 0.0;
+}
+
 __global__ __launch_bounds__(NT) void k_galerkin_fine(const double *__restrict__ E,
                                                       const int *__restrict__ emap,
                                                       const double *__restrict__ KEC,
@@ -258,7 +280,8 @@ void galerkin_fine(hipStream_t s, const double *E, const int *emap, const double
     TOFE_LAUNCH_CHECK();
   }
   const int64_t ncn = (int64_t)((nx + 2) / 2) * ((ny + 2) / 2) * ((nz + 2) / 2);
-  TOFE_HIP(hipMemsetAsync(Sc, 0, sizeof(double) * 243 * ncn, s));
+  k_galerkin_zero_outside<<<dim3((unsigned)((ncn + NT - 1) / NT), 27), dim3(NT), 0, s>>>(nx, ny, nz, Sc);
+  TOFE_LAUNCH_CHECK();
   k_galerkin_fine<<<dim3((unsigned)((ncn + NT - 1) / NT), 14), dim3(NT), 0, s>>>(E, emap, KEC, KCm,
                                                                                  fix, nx, ny, nz, Sc);
   TOFE_LAUNCH_CHECK();
