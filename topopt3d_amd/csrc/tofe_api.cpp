@@ -70,7 +70,7 @@ struct Level {
 };
 
 constexpr int kMaxEig = 64;
-constexpr int kCoarseCUs = 32;  // compute units left to the coarse factorisation during set-up
+constexpr int kCoarseCUs = 64;  // compute units left to the coarse factorisation during set-up
 
 double now_s()
 {
