@@ -271,6 +271,13 @@ int tofe_get_level_lambda(const tofe_ctx *ctx, int level, double *lambda);
 int tofe_time_operator(tofe_ctx *ctx, int level, int precision, int epilogue, int nrep,
                        double *ms);
 
+/* Average duration (ms) of the mixed-precision solves' dominant kernel, the FP32 fine
+ * operator fused with a Chebyshev step, as it runs inside the solves: start != 0 begins
+ * recording (HIP events around each such launch on the context's stream, up to 512
+ * launches); start == 0 stops and returns the average and the number of launches timed.
+ * No reference counterpart: a measurement aid for the same timing line as above. */
+int tofe_time_solve_kernel(tofe_ctx *ctx, int start, double *ms_avg, int *count);
+
 /* ---- on-disk formats ------------------------------------------------------------
  * PETSc binary files as PetscViewerBinaryOpen + VecView / MatView write them (big-endian;
  * Vec: classid 1211214, n, n doubles; sequential AIJ: classid 1211216, rows, cols, nnz,

@@ -161,6 +161,10 @@ struct tofe_ctx {
                                                 // (TOFE_FINE_TILE="<shape>:<zlen>" sets both)
   bool ke_sym = false;  // KE matches the cube-orbit table: fine operator keeps KE in registers
   bool ke_wht = false;  // KE is block diagonal in the Walsh-Hadamard basis: k_fine_wht
+  // tofe_time_solve_kernel: event pairs around the FP32 fine Chebyshev launches of solves
+  bool kt_on = false;
+  size_t kt_n = 0;
+  std::vector<hipEvent_t> kt_ev;
   bool ke_wht_cube = false;  // ... with the 9-value block structure of a cubic element
   int wht_zlen = 8, wht_zlen32 = 8;  // its z-chunk lengths, FP64 and FP32
   int precision = TOFE_PREC_FP64;
@@ -180,6 +184,8 @@ struct tofe_ctx {
     if (stream3) (void)hipStreamSynchronize(stream3);
     if (stream4) (void)hipStreamSynchronize(stream4);
     for (hipEvent_t e : {ev_dense, ev_inv, ev_ops, ev_lz, ev_scal, ev_lz0})
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : kt_ev)
       if (e) (void)hipEventDestroy(e);
     if (hscal) (void)hipHostFree(hscal);
     if (stream) (void)hipStreamDestroy(stream);
@@ -262,9 +268,17 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
   if (!s) s = c->stream;
   if (l == 0) {
     const FineOp<T> f = fine_of<T>(c);
+    // tofe_time_solve_kernel: bracket the FP32 Chebyshev launches on the main stream
+    const bool timed = sizeof(T) == 4 && epi == EPI_CHEB && c->kt_on && s == c->stream &&
+                       2 * c->kt_n + 1 < c->kt_ev.size();
+    if (timed) TOFE_HIP(hipEventRecord(c->kt_ev[2 * c->kt_n], s));
     if (c->fine_mfma) fine_mfma<T>(s, f, u, epi, ep);
     else if (f.wht) fine_wht<T>(s, f, u, epi, ep);
     else fine_op<T>(s, f, u, epi, ep);
+    if (timed) {
+      TOFE_HIP(hipEventRecord(c->kt_ev[2 * c->kt_n + 1], s));
+      ++c->kt_n;
+    }
   } else {
     StencilOp<T> op;
     auto &lv = *levels<T>(c)[l];
@@ -1871,6 +1885,33 @@ int tofe_time_operator(tofe_ctx *c, int level, int precision, int epilogue, int 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms = (double)t / nrep;
+  });
+}
+
+int tofe_time_solve_kernel(tofe_ctx *c, int start, double *ms_avg, int *count)
+{
+  return guarded(c, [&] {
+    if (start) {
+      constexpr size_t kPairs = 512;
+      if (c->kt_ev.empty()) {
+        c->kt_ev.assign(2 * kPairs, nullptr);
+        for (auto &e : c->kt_ev) TOFE_HIP(hipEventCreate(&e));
+      }
+      c->kt_n = 0;
+      c->kt_on = true;
+      return;
+    }
+    if (!ms_avg || !count) throw Error(TOFE_ERR_ARG, "bad arguments");
+    c->kt_on = false;
+    double tot = 0;
+    for (size_t k = 0; k < c->kt_n; ++k) {
+      TOFE_HIP(hipEventSynchronize(c->kt_ev[2 * k + 1]));
+      float t = 0;
+      TOFE_HIP(hipEventElapsedTime(&t, c->kt_ev[2 * k], c->kt_ev[2 * k + 1]));
+      tot += t;
+    }
+    *count = (int)c->kt_n;
+    *ms_avg = c->kt_n ? tot / (double)c->kt_n : 0.0;
   });
 }
 
