@@ -3,7 +3,7 @@
 Bars: bit-exact for the filter pattern/weights, rho = P x, P^T g, the SIMP fields
 (p = 3) and element sensitivities for a given U; <= 1e-14 relative for the matrix-free
 operator and the Galerkin coarse operators; solves driven to 1e-12 agree to 1e-10
-relative in U, compliance and sensitivities (FP64 tolerance of the north star).
+relative in U, compliance and sensitivities (the FP64 tolerance of the workload).
 """
 import numpy as np
 import pytest
