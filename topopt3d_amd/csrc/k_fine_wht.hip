@@ -107,8 +107,12 @@ template <typename T, int EPI, bool CUBE>
 __global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *__restrict__ u, EpiArgs<T> ep,
                                                  int tiles_x, int tiles_y, int nblocks, int zlen)
 {
+  // partial sums handed to the row below (lower, upper node): double-buffered in FP32; one
+  // buffer and a second barrier per step in FP64, which keeps the block at 33 KB of LDS (four
+  // resident blocks per CU instead of three)
+  constexpr int NPB = sizeof(T) == 8 ? 1 : 2;
   __shared__ T su[3][3][WPL];     // ring of three node planes, one array per component
-  __shared__ T sp[2][6][NW];      // partial sums handed to the row below (lower, upper node)
+  __shared__ T sp[NPB][6][NW];
   __shared__ double sred[NW / 64];
 
   const int nx = op.nx, ny = op.ny, nz = op.nz;
@@ -197,11 +201,12 @@ __global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *__restri
   __syncthreads();
   // Step st (element layer ez = k0 - 1 + st) reads node planes ez and ez + 1 from ring slots
   // st % 3 and (st + 1) % 3, writes partial sums to sp[st % 2] and node plane ez + 2 to slot
-  // (st + 2) % 3, then passes the step's single barrier.  Those two were last read before
-  // the barrier of step st - 1.
+  // (st + 2) % 3, then passes the step's barrier.  Those two were last read before the
+  // barrier of step st - 1 (FP32); with the single sp buffer of FP64 a second barrier ahead
+  // of the sp writes orders them after the previous step's reads.
   int st = 0;
   for (int ez = k0 - 1; ez < k1; ++ez, ++st) {
-    const int slo = st % 3, shi = (st + 1) % 3, pb = st & 1;
+    const int slo = st % 3, shi = (st + 1) % 3, pb = NPB == 2 ? (st & 1) : 0;
     const bool more = ez + 1 < k1;
     T pu[NU][3];
     unsigned pf[NU];
@@ -227,6 +232,7 @@ __global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *__restri
     // corner (0, 1) [m = 2, 6]; the row above contributes corners (1, 0) and (0, 0)
     // [m = 1, 5 and 0, 4] of its two elements, summed there and passed through LDS
     T plo[3], phi[3];
+    if constexpr (NPB == 1) __syncthreads();  // the previous step's readers of sp are done
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       plo[c] = v[c][3] + from_next_lane(v[c][2]);
