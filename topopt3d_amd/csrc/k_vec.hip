@@ -180,7 +180,6 @@ __global__ __launch_bounds__(NT) void k_cg_xr(double *__restrict__ x, double *__
   const double alpha = scal[alpha_slot];
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-    x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     s += ri * ri;
@@ -250,9 +249,8 @@ __device__ inline double all_sum(const double *__restrict__ partial, int nparts,
 // Flexible-CG step without separate reduction launches, first half: alpha = p.r / p.Ap
 // from the operator's two sets of partials (block 0 stores p.r, p.Ap and alpha in scal);
 // x += alpha p, r -= alpha q, partials of r.r.
-__global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ x, double *__restrict__ r,
-                                               const double *__restrict__ p,
-                                               const double *__restrict__ q, int64_t n,
+__global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const double *__restrict__ q,
+                                               int64_t n,
                                                const double *__restrict__ pq_part,
                                                const double *__restrict__ pr_part, int nparts,
                                                double *scal, double *partial, FirstCheb fc)
@@ -287,18 +285,32 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ x, double *_
 // Second half: beta = -z.Ap / p.Ap from the z.q partials (block 0 stores z.Ap and beta);
 // p = z + beta p.
 __global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const double *__restrict__ z,
-                                              int64_t n, const double *__restrict__ zq_part,
-                                              int nparts, double *scal)
+                                              double *__restrict__ x, int64_t n,
+                                              const double *__restrict__ zq_part, int nparts,
+                                              double *scal)
 {
   __shared__ double sred[NT / 64];
   const double zq = all_sum(zq_part, nparts, sred);
   const double beta = -zq / scal[SL_PQ];
+  const double alpha = scal[SL_ALPHA];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     scal[SL_ZQ] = zq;
     scal[SL_BETA] = beta;
   }
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const double pi = p[i];
+    x[i] += alpha * pi;  // the iterate's update of the step that formed alpha
+    p[i] = z[i] + beta * pi;
+  }
+}
+
+// x += alpha p with alpha = scal[SL_ALPHA]: the iterate's update when no k_fcg_p follows.
+__global__ __launch_bounds__(NT) void k_fcg_x(double *__restrict__ x, const double *__restrict__ p,
+                                              int64_t n, const double *scal)
+{
+  const double alpha = scal[SL_ALPHA];
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
-    p[i] = z[i] + beta * p[i];
+    x[i] += alpha * p[i];
 }
 
 // Jacobi-CG step of the eigenvalue estimate, first half: alpha = rz_old / p.Ap from the
@@ -450,19 +462,23 @@ void record_scalar(hipStream_t s, const double *scal, int slot, double *dst)
   TOFE_LAUNCH_CHECK();
 }
 
-void fcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
-                   int64_t n, const double *pq_part, const double *pr_part, int nparts,
-                   double *scal, double *partial, const FirstCheb &fc)
+void fcg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *pq_part,
+                  const double *pr_part, int nparts, double *scal, double *partial, const FirstCheb &fc)
 {
-  k_fcg_xr<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(x, r, p, q, n, pq_part, pr_part, nparts, scal,
-                                                 partial, fc);
+  k_fcg_xr<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, q, n, pq_part, pr_part, nparts, scal, partial, fc);
   TOFE_LAUNCH_CHECK();
 }
 
-void fcg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *zq_part,
-                  int nparts, double *scal)
+void fcg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n,
+                   const double *zq_part, int nparts, double *scal)
 {
-  k_fcg_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, n, zq_part, nparts, scal);
+  k_fcg_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, x, n, zq_part, nparts, scal);
+  TOFE_LAUNCH_CHECK();
+}
+
+void fcg_update_x(hipStream_t s, double *x, const double *p, int64_t n, const double *scal)
+{
+  k_fcg_x<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(x, p, n, scal);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -546,10 +546,14 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
   double rn_prev = rn, rate = 0.5;
   // mixed: the FCG update also forms the next V-cycle's first fine-level step
   const FirstCheb first = flex ? vcycle_first_step(c) : FirstCheb();
+  // flexible CG applies the iterate's update x += alpha p in the next search-direction
+  // kernel, which reads p anyway; x_pending marks a step whose update is still owed
+  bool x_pending = false;
   auto next_direction = [&] {
     if (flex) {
       const int nparts = precondition(c, r, z, q, first.b32 != nullptr);
-      fcg_update_p(s, p, z, n, c->partial.p, nparts, c->scal.p);
+      fcg_update_xp(s, p, z, U, n, c->partial.p, nparts, c->scal.p);
+      x_pending = false;
     } else {
       precondition(c, r, z);
       dot_into(c, r, z, n, SL_TMP, POST_BETA_CG);
@@ -564,8 +568,8 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
       ep.b = r;
       ep.partial2 = c->partial2.p;
       level_apply<double>(c, 0, p, EPI_APPLY_DOT2, ep);
-      fcg_update_xr(s, U, r, p, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p,
-                    first);
+      fcg_update_r(s, r, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p, first);
+      x_pending = true;
       reduce_final(s, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p);
     } else {
       level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
@@ -591,6 +595,7 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
     rn_prev = rn;
     if (!ahead) next_direction();
   }
+  if (x_pending) fcg_update_x(s, U, p, n, c->scal.p);
   *its_out = k;
   *rn_out = rn;
   return reason;

@@ -179,22 +179,24 @@ void power_start(hipStream_t s, T *v, int64_t n);
 template <typename T>
 void jacobi_dot(hipStream_t s, const T *r, const T *dinv, T *z, int64_t n, double *partial);
 void record_scalar(hipStream_t s, const double *scal, int slot, double *dst);
-// Flexible-CG updates that sum the producers' partials themselves: fcg_update_xr forms
-// alpha = sum(pr_part) / sum(pq_part) (stored with both sums in scal), x += alpha p,
-// r -= alpha q, partials of r.r; fcg_update_p forms beta = -sum(zq_part) / scal[PQ] and
-// p = z + beta p.
-// Optional output of fcg_update_xr: the mixed V-cycle's first Chebyshev step on the new
+// Flexible-CG updates that sum the producers' partials themselves: fcg_update_r forms
+// alpha = sum(pr_part) / sum(pq_part) (stored with both sums in scal), r -= alpha q, partials
+// of r.r; fcg_update_xp forms beta = -sum(zq_part) / scal[PQ], applies the iterate's update
+// x += scal[ALPHA] p of the same step (deferred to where p is read anyway) and p = z + beta p;
+// fcg_update_x applies x += scal[ALPHA] p alone (a step that ends the solve).
+// Optional output of fcg_update_r: the mixed V-cycle's first Chebyshev step on the new
 // residual (b32 = (float) r, d = x = c2 dinv b32), when b32 is set.
 struct FirstCheb {
   float *b32 = nullptr, *d = nullptr, *x = nullptr;
   const float *dinv = nullptr;
   float c2 = 0;
 };
-void fcg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
-                   int64_t n, const double *pq_part, const double *pr_part, int nparts,
-                   double *scal, double *partial, const FirstCheb &fc = FirstCheb());
-void fcg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *zq_part,
-                  int nparts, double *scal);
+void fcg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *pq_part,
+                  const double *pr_part, int nparts, double *scal, double *partial,
+                  const FirstCheb &fc = FirstCheb());
+void fcg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n,
+                   const double *zq_part, int nparts, double *scal);
+void fcg_update_x(hipStream_t s, double *x, const double *p, int64_t n, const double *scal);
 // Fused Jacobi-CG steps of the eigenvalue estimate (no separate reduction launches):
 // lanczos_r: alpha = scal[rz_slot] / sum(pq_part[0..npq)), recorded; r -= alpha q; partials
 // of r.(D^-1 r).  lanczos_p: beta = sum(rz_part) / scal[rz_old], recorded; scal[rz_new] =
