@@ -79,7 +79,7 @@ __global__ void k_cheb_zero_from64(const double *__restrict__ b64, float *__rest
 template <typename T>
 __global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int Nfx, int Nfy,
                                                  int Nfz, T *__restrict__ bc, int Ncx, int Ncy,
-                                                 int Ncz)
+                                                 int Ncz, FirstStep<T> fs)
 {
   const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
   const int64_t I = (int64_t)blockIdx.x * NT + threadIdx.x;
@@ -113,6 +113,14 @@ __global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int N
       }
 #pragma unroll
   for (int q = 0; q < 3; ++q) bc[3 * I + q] = (T)acc[q];
+  if (fs.d) {  // the coarse level's first Chebyshev step from x = 0 (as k_cheb_zero)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T v = fs.c2 * (fs.dinv[3 * I + q] * (T)acc[q]);
+      fs.d[3 * I + q] = v;
+      fs.x[3 * I + q] = v;
+    }
+  }
 }
 
 // x_f(i) += sum over coarse parents J of w(i,J) x_c(J)  (P x_c)
@@ -397,11 +405,11 @@ void cheb_zero_from64(hipStream_t s, const double *b64, float *b32, const float 
 
 template <typename T>
 void restrict_op(hipStream_t s, const T *rf, int Nfx, int Nfy, int Nfz, T *bc, int Ncx, int Ncy,
-                 int Ncz)
+                 int Ncz, const FirstStep<T> &fs)
 {
   const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
   k_restrict<T><<<dim3((unsigned)((ncn + NT - 1) / NT)), dim3(NT), 0, s>>>(rf, Nfx, Nfy, Nfz, bc,
-                                                                           Ncx, Ncy, Ncz);
+                                                                           Ncx, Ncy, Ncz, fs);
   TOFE_LAUNCH_CHECK();
 }
 
@@ -503,7 +511,8 @@ void lanczos_p(hipStream_t s, T *p, const T *r, const T *dinv, int64_t n, const 
 #define TOFE_INST_T(T)                                                                      \
   template void dot_partial<T>(hipStream_t, const T *, const T *, int64_t, double *);       \
   template void cheb_zero<T>(hipStream_t, const T *, const T *, T, T *, T *, int64_t);      \
-  template void restrict_op<T>(hipStream_t, const T *, int, int, int, T *, int, int, int);  \
+  template void restrict_op<T>(hipStream_t, const T *, int, int, int, T *, int, int, int,        \
+                               const FirstStep<T> &);                                         \
   template void prolong_add<T>(hipStream_t, const T *, int, int, int, T *, int, int, int);  \
 
 TOFE_INST_T(double)

@@ -399,16 +399,17 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
     const bool from64 = l == 0 && mixed_io && io->b64;
     if (from64 && nu1 == 0) convert<double, T>(s, io->b64, const_cast<T *>(b0), nd);
     if (nu1 > 0) {
+      // the first step from x = 0 on levels l >= 1 was formed by the restriction into l
       if constexpr (sizeof(T) == 4) {
         if (from64 && io->first_done) {
           // formed by the kernel that produced b64
         } else if (from64)
           cheb_zero_from64(s, io->b64, const_cast<float *>(b0), lv.dinv.p, (float)c2[0], lv.d.p,
                            xc, nd);
-        else
+        else if (l == 0)
           cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd);
       } else {
-        cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd);
+        if (l == 0) cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd);
       }
       for (int k = 1; k < nu1; ++k) {
         EpiArgs<T> ep;
@@ -424,7 +425,15 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
     er.y = lv.r.p; er.b = b;
     level_apply<T>(c, l, xc, EPI_RESID, er);
     const Grid &gf = lv.g, &gc = LV[l + 1]->g;
-    restrict_op<T>(s, lv.r.p, gf.Nx, gf.Ny, gf.Nz, LV[l + 1]->b.p, gc.Nx, gc.Ny, gc.Nz);
+    FirstStep<T> fs;
+    if (nu1 > 0 && l + 1 < L - 1) {  // level l + 1 is smoothed: its first step here
+      Level<T> &ln = *LV[l + 1];
+      fs.dinv = ln.dinv.p;
+      fs.c2 = (T)c->c2[(size_t)(l + 1) * km];
+      fs.d = ln.d.p;
+      fs.x = (flips & 1) ? ln.x2.p : ln.x.p;  // the level's xc (same flip count on every level)
+    }
+    restrict_op<T>(s, lv.r.p, gf.Nx, gf.Ny, gf.Nz, LV[l + 1]->b.p, gc.Nx, gc.Ny, gc.Nz, fs);
     cur[l] = xc;
     oth[l] = xo;
   }

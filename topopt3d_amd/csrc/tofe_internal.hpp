@@ -161,9 +161,17 @@ void cheb_zero(hipStream_t s, const T *b, const T *dinv, T c2, T *d, T *x, int64
 // first Chebyshev step from an FP64 right-hand side: b32 = (float) b64, d = x = c2 dinv b32
 void cheb_zero_from64(hipStream_t s, const double *b64, float *b32, const float *dinv, float c2,
                       float *d, float *x, int64_t n);
+// Optional output of restrict_op: the coarse level's first Chebyshev step from x = 0,
+// d = x = c2 dinv b (k_cheb_zero), when d is set.
+template <typename T>
+struct FirstStep {
+  const T *dinv = nullptr;
+  T c2 = 0;
+  T *d = nullptr, *x = nullptr;
+};
 template <typename T>
 void restrict_op(hipStream_t s, const T *rf, int Nfx, int Nfy, int Nfz, T *bc, int Ncx, int Ncy,
-                 int Ncz);
+                 int Ncz, const FirstStep<T> &fs = FirstStep<T>());
 template <typename T>
 void prolong_add(hipStream_t s, const T *xc, int Ncx, int Ncy, int Ncz, T *xf, int Nfx, int Nfy,
                  int Nfz);
