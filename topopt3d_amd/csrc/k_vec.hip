@@ -188,6 +188,7 @@ __global__ __launch_bounds__(NT) void k_cg_xr(double *__restrict__ x, double *__
   const double alpha = scal[alpha_slot];
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     s += ri * ri;
@@ -256,7 +257,7 @@ __device__ inline double all_sum(const double *__restrict__ partial, int nparts,
 
 // Flexible-CG step without separate reduction launches, first half: alpha = p.r / p.Ap
 // from the operator's two sets of partials (block 0 stores p.r, p.Ap and alpha in scal);
-// x += alpha p, r -= alpha q, partials of r.r.
+// r -= alpha q, partials of r.r (the iterate's update x += alpha p follows in k_fcg_p).
 __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const double *__restrict__ q,
                                                int64_t n,
                                                const double *__restrict__ pq_part,
@@ -274,7 +275,6 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const dou
   }
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-    x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     s += ri * ri;
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const dou
 }
 
 // Second half: beta = -z.Ap / p.Ap from the z.q partials (block 0 stores z.Ap and beta);
-// p = z + beta p.
+// x += alpha p with the first half's alpha, then p = z + beta p.
 __global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const double *__restrict__ z,
                                               double *__restrict__ x, int64_t n,
                                               const double *__restrict__ zq_part, int nparts,

@@ -647,15 +647,14 @@ void lanczos_coefficients(tofe_ctx *c, int l, hipStream_t s, T *r, T *p, T *q, d
 // stream.  In the mixed-precision build they run on
 // the FP32 operators of the V-cycle (scalars and recorded coefficients stay FP64), whose
 // copies are made on the same streams first.  ev_lz / ev_lz0 mark their ends.
-void lanczos_enqueue(tofe_ctx *c)
+// The fine level's estimate on the fourth stream, once the fine diagonal (main stream) is
+// formed: it needs nothing of the Galerkin hierarchy.  ev_lz0 marks its end.
+void lanczos_fine_enqueue(tofe_ctx *c)
 {
   const int L = c->L;
-  // level 1 on the third stream, the fine level on the fourth, levels 2.. on the main
-  // stream (idle until it joins the others)
-  const hipStream_t s0 = c->stream, s3 = c->stream3, s4 = c->stream4;
+  const hipStream_t s4 = c->stream4;
   c->eig_ab.alloc((size_t)2 * L * kMaxEig);
   TOFE_HIP(hipEventRecord(c->ev_ops, c->stream));
-  TOFE_HIP(hipStreamWaitEvent(s3, c->ev_ops, 0));
   TOFE_HIP(hipStreamWaitEvent(s4, c->ev_ops, 0));
   if (c->precision == TOFE_PREC_MIXED) {
     if (c->lv32.empty()) {
@@ -669,6 +668,29 @@ void lanczos_enqueue(tofe_ctx *c)
     }
     convert<double, float>(s4, c->lv64[0]->dinv.p, c->lv32[0]->dinv.p, c->ndof);
     convert<double, float>(s4, c->E.p, c->E32.p, c->ne);
+    if (L > 1) {
+      Level<float> &lv = *c->lv32[0];
+      lanczos_coefficients<float>(c, 0, s4, lv.r.p, lv.x.p, lv.x2.p, c->partial.p,
+                                  c->partial2.p, c->scal.p);
+    }
+  } else if (L > 1) {
+    lanczos_coefficients<double>(c, 0, s4, c->vr.p, c->vp.p, c->vq.p, c->partial.p,
+                                 c->partial2.p, c->scal.p);
+  }
+  TOFE_HIP(hipEventRecord(c->ev_lz0, s4));
+}
+
+void lanczos_enqueue(tofe_ctx *c, bool fine_done)
+{
+  const int L = c->L;
+  // level 1 on the third stream, the fine level on the fourth (unless enqueued already),
+  // levels 2.. on the main stream (idle until it joins the others)
+  const hipStream_t s0 = c->stream, s3 = c->stream3, s4 = c->stream4;
+  if (!fine_done) lanczos_fine_enqueue(c);
+  TOFE_HIP(hipEventRecord(c->ev_ops, c->stream));
+  TOFE_HIP(hipStreamWaitEvent(s3, c->ev_ops, 0));
+  (void)s4;
+  if (c->precision == TOFE_PREC_MIXED) {
     for (int l = 1; l < L; ++l) {
       const hipStream_t sl = l == 1 ? s3 : s0;
       convert<double, float>(sl, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, c->grids[l].ndof());
@@ -683,11 +705,6 @@ void lanczos_enqueue(tofe_ctx *c)
         lanczos_coefficients<float>(c, l, s0, lv.r.p, lv.x.p, lv.x2.p, c->partial_c.p,
                                     c->partial2_c.p, c->scal_c.p);
     }
-    if (L > 1) {
-      Level<float> &lv = *c->lv32[0];
-      lanczos_coefficients<float>(c, 0, s4, lv.r.p, lv.x.p, lv.x2.p, c->partial.p,
-                                  c->partial2.p, c->scal.p);
-    }
   } else {
     for (int l = 1; l + 1 < L; ++l) {
       Level<double> &lv = *c->lv64[l];
@@ -698,12 +715,8 @@ void lanczos_enqueue(tofe_ctx *c)
         lanczos_coefficients<double>(c, l, s0, lv.r.p, lv.x.p, lv.x2.p, c->partial_c.p,
                                      c->partial2_c.p, c->scal_c.p);
     }
-    if (L > 1)
-      lanczos_coefficients<double>(c, 0, s4, c->vr.p, c->vp.p, c->vq.p, c->partial.p,
-                                   c->partial2.p, c->scal.p);
   }
   TOFE_HIP(hipEventRecord(c->ev_lz, s3));
-  TOFE_HIP(hipEventRecord(c->ev_lz0, s4));
 }
 
 void lanczos_lambdas(tofe_ctx *c)
@@ -801,6 +814,8 @@ void assemble(tofe_ctx *c)
   const uint8_t *fix = c->has_fix ? c->dfix.p : nullptr;
   fine_diag(s, c->E.p, c->dKE.p, fix, c->desc.nx, c->desc.ny, c->desc.nz, c->diag0.p,
             c->lv64[0]->dinv.p);
+  const bool early_fine = std::getenv("TOFE_EARLY_FINE_LZ") != nullptr;
+  if (early_fine) lanczos_fine_enqueue(c);
   if (L > 1) {
     MaskedElems me;
     me.elem = c->dme_elem.p;
@@ -838,7 +853,7 @@ void assemble(tofe_ctx *c)
   TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
   coarse_inverse_enqueue(c);
   TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
-  lanczos_enqueue(c);
+  lanczos_enqueue(c, early_fine);
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz, 0));
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz0, 0));
   lanczos_lambdas(c);
