@@ -814,8 +814,6 @@ void assemble(tofe_ctx *c)
   const uint8_t *fix = c->has_fix ? c->dfix.p : nullptr;
   fine_diag(s, c->E.p, c->dKE.p, fix, c->desc.nx, c->desc.ny, c->desc.nz, c->diag0.p,
             c->lv64[0]->dinv.p);
-  const bool early_fine = std::getenv("TOFE_EARLY_FINE_LZ") != nullptr;
-  if (early_fine) lanczos_fine_enqueue(c);
   if (L > 1) {
     MaskedElems me;
     me.elem = c->dme_elem.p;
@@ -853,7 +851,7 @@ void assemble(tofe_ctx *c)
   TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
   coarse_inverse_enqueue(c);
   TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
-  lanczos_enqueue(c, early_fine);
+  lanczos_enqueue(c, false);
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz, 0));
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz0, 0));
   lanczos_lambdas(c);
