@@ -177,18 +177,15 @@ __global__ void k_fill(double *a, double v, int64_t n)
     a[i] = v;
 }
 
-// x += alpha p; r -= alpha q; partial ||r||^2
-__global__ __launch_bounds__(NT) void k_cg_xr(double *__restrict__ x, double *__restrict__ r,
-                                              const double *__restrict__ p,
-                                              const double *__restrict__ q, int64_t n,
-                                              const double *__restrict__ scal, int alpha_slot,
-                                              double *partial)
+// r -= alpha q; partial ||r||^2 (the iterate's update x += alpha p follows in k_cg_xp)
+__global__ __launch_bounds__(NT) void k_cg_r(double *__restrict__ r, const double *__restrict__ q,
+                                             int64_t n, const double *__restrict__ scal, int alpha_slot,
+                                             double *partial)
 {
   __shared__ double sred[NT / 64];
   const double alpha = scal[alpha_slot];
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-    x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     s += ri * ri;
@@ -197,13 +194,16 @@ __global__ __launch_bounds__(NT) void k_cg_xr(double *__restrict__ x, double *__
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// p = z + beta p
-__global__ void k_cg_p(double *__restrict__ p, const double *__restrict__ z, int64_t n,
-                       const double *__restrict__ scal, int beta_slot)
+// x += alpha p (the iterate's update of the step that formed alpha), then p = z + beta p
+__global__ void k_cg_xp(double *__restrict__ p, const double *__restrict__ z, double *__restrict__ x,
+                        int64_t n, const double *__restrict__ scal, int beta_slot, int alpha_slot)
 {
-  const double beta = scal[beta_slot];
-  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
-    p[i] = z[i] + beta * p[i];
+  const double beta = scal[beta_slot], alpha = scal[alpha_slot];
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const double pi = p[i];
+    x[i] += alpha * pi;
+    p[i] = z[i] + beta * pi;
+  }
 }
 
 // v_i = 0.5 + frac(i * 2654435761 / 2^32): deterministic start vector of the estimate
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const doub
   }
 }
 
-// x += alpha p with alpha = scal[SL_ALPHA]: the iterate's update when no k_fcg_p follows.
+// x += alpha p with alpha = scal[SL_ALPHA]: the iterate's update when no k_fcg_p / k_cg_xp follows.
 __global__ __launch_bounds__(NT) void k_fcg_x(double *__restrict__ x, const double *__restrict__ p,
                                               int64_t n, const double *scal)
 {
@@ -436,17 +436,17 @@ void fill_d(hipStream_t s, double *a, double v, int64_t n)
   TOFE_LAUNCH_CHECK();
 }
 
-void cg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
-                  int64_t n, const double *scal, int alpha_slot, double *partial)
+void cg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *scal,
+                 int alpha_slot, double *partial)
 {
-  k_cg_xr<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(x, r, p, q, n, scal, alpha_slot, partial);
+  k_cg_r<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, q, n, scal, alpha_slot, partial);
   TOFE_LAUNCH_CHECK();
 }
 
-void cg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *scal,
-                 int beta_slot)
+void cg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n, const double *scal,
+                  int beta_slot, int alpha_slot)
 {
-  k_cg_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, n, scal, beta_slot);
+  k_cg_xp<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, x, n, scal, beta_slot, alpha_slot);
   TOFE_LAUNCH_CHECK();
 }
 

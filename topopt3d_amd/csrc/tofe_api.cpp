@@ -555,7 +555,7 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
   double rn_prev = rn, rate = 0.5;
   // mixed: the FCG update also forms the next V-cycle's first fine-level step
   const FirstCheb first = flex ? vcycle_first_step(c) : FirstCheb();
-  // flexible CG applies the iterate's update x += alpha p in the next search-direction
+  // CG and flexible CG apply the iterate's update x += alpha p in the next search-direction
   // kernel, which reads p anyway; x_pending marks a step whose update is still owed
   bool x_pending = false;
   auto next_direction = [&] {
@@ -566,7 +566,8 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
     } else {
       precondition(c, r, z);
       dot_into(c, r, z, n, SL_TMP, POST_BETA_CG);
-      cg_update_p(s, p, z, n, c->scal.p, SL_BETA);
+      cg_update_xp(s, p, z, U, n, c->scal.p, SL_BETA, SL_ALPHA);
+      x_pending = false;
     }
   };
   while (!reason) {
@@ -583,7 +584,8 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
     } else {
       level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
       reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p);
-      cg_update_xr(s, U, r, p, q, n, c->scal.p, SL_ALPHA, c->partial.p);
+      cg_update_r(s, r, q, n, c->scal.p, SL_ALPHA, c->partial.p);
+      x_pending = true;
       reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p);
     }
     scal_to_host_async(c);

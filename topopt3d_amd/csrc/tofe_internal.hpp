@@ -178,10 +178,11 @@ void prolong_add(hipStream_t s, const T *xc, int Ncx, int Ncy, int Ncz, T *xf, i
 template <typename Ta, typename Tb>
 void convert(hipStream_t s, const Ta *a, Tb *b, int64_t n);
 void fill_d(hipStream_t s, double *a, double v, int64_t n);
-void cg_update_xr(hipStream_t s, double *x, double *r, const double *p, const double *q,
-                  int64_t n, const double *scal, int alpha_slot, double *partial);
-void cg_update_p(hipStream_t s, double *p, const double *z, int64_t n, const double *scal,
-                 int beta_slot);
+void cg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *scal,
+                 int alpha_slot, double *partial);
+// x += scal[alpha_slot] p (the iterate's update, deferred to where p is read), p = z + beta p
+void cg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n, const double *scal,
+                  int beta_slot, int alpha_slot);
 template <typename T>
 void power_start(hipStream_t s, T *v, int64_t n);
 template <typename T>
