@@ -265,27 +265,63 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
     T acc[NA];
 #pragma unroll
     for (int c = 0; c < NA; ++c) acc[c] = 0;
+    if constexpr (sizeof(T) == 4) {
+      // FP32: fully unrolled; the compiler issues all 27 neighbours' loads up front (one wave
+      // per SIMD, 256 VGPRs), which measured faster than the rolled form below
 #pragma unroll
-    for (int o = 0; o < 27; ++o) {
-      const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
-      const int mi = i + dx, mj = j + dy, mk = k + dz;
-      // unconditional loads, selected contribution (as in k_stencil_op)
-      const bool in = mi >= 0 && mi < Nx && mj >= 0 && mj < Ny && mk >= 0 && mk < Nz;
-      const int64_t m = in ? n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz) : n;
-      const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+      for (int o = 0; o < 27; ++o) {
+        const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+        const int mi = i + dx, mj = j + dy, mk = k + dz;
+        // unconditional loads, selected contribution (as in k_stencil_op)
+        const bool in = mi >= 0 && mi < Nx && mj >= 0 && mj < Ny && mk >= 0 && mk < Nz;
+        const int64_t m = in ? n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz) : n;
+        const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
 #pragma unroll
-      for (int c = 0; c < NA; ++c) {
-        const int a = SPLIT ? a0 : c;
-        T t;
-        if (o >= 13) {
-          const T *B = S + su_index((o - 13) * 9, n);
-          t = B[(a * 3 + 0) * 64] * u0 + B[(a * 3 + 1) * 64] * u1 + B[(a * 3 + 2) * 64] * u2;
-        } else {
-          const T *B = S + su_index((13 - o) * 9, m);  // block (m, 26 - o), transposed
-          t = B[(0 * 3 + a) * 64] * u0 + B[(1 * 3 + a) * 64] * u1 + B[(2 * 3 + a) * 64] * u2;
+        for (int c = 0; c < NA; ++c) {
+          const int a = SPLIT ? a0 : c;
+          T t;
+          if (o >= 13) {
+            const T *B = S + su_index((o - 13) * 9, n);
+            t = B[(a * 3 + 0) * 64] * u0 + B[(a * 3 + 1) * 64] * u1 + B[(a * 3 + 2) * 64] * u2;
+          } else {
+            const T *B = S + su_index((13 - o) * 9, m);  // block (m, 26 - o), transposed
+            t = B[(0 * 3 + a) * 64] * u0 + B[(1 * 3 + a) * 64] * u1 + B[(2 * 3 + a) * 64] * u2;
+          }
+          acc[c] = in ? acc[c] + t : acc[c];
         }
-        acc[c] = in ? acc[c] + t : acc[c];
       }
+    } else {
+    // FP64: one (dy, dz) row of three neighbours per iteration, the loop kept rolled so that
+    // one row's loads (30) are in flight per thread; fully unrolled, the FP64 instance spills.
+#pragma unroll 1
+    for (int g = 0; g < 9; ++g) {
+      const int dy = g % 3 - 1, dz = g / 3 - 1;
+#pragma unroll
+      for (int ix = 0; ix < 3; ++ix) {
+        const int o = 3 * g + ix, dx = ix - 1;
+        const int mi = i + dx, mj = j + dy, mk = k + dz;
+        // Unconditional loads (a missing neighbour reads node n) weighted by w = 0 or 1, with
+        // no branch: fma(w, t, acc) is acc + t or acc exactly.  (A select of acc + t, or a
+        // short-circuit bounds test, lets the compiler put each neighbour's loads behind a
+        // branch of their own.)
+        const bool in = ((unsigned)mi < (unsigned)Nx) & ((unsigned)mj < (unsigned)Ny) &
+                        ((unsigned)mk < (unsigned)Nz);
+        const T w = in ? T(1) : T(0);
+        const int64_t m = in ? n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz) : n;
+        const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+        // o >= 13: stored block (n, o); o < 13: stored block (m, 26 - o), transposed
+        const bool up = o >= 13;
+        const T *B = S + (up ? su_index((o - 13) * 9, n) : su_index((13 - o) * 9, m));
+        const int sr = up ? 3 : 1, sc = up ? 1 : 3;  // strides of (row a, column b)
+#pragma unroll
+        for (int c = 0; c < NA; ++c) {
+          const int a = SPLIT ? a0 : c;
+          const T t = B[(a * sr + 0 * sc) * 64] * u0 + B[(a * sr + 1 * sc) * 64] * u1 +
+                      B[(a * sr + 2 * sc) * 64] * u2;
+          acc[c] = __builtin_fma(w, t, acc[c]);
+        }
+      }
+    }
     }
 #pragma unroll
     for (int c = 0; c < NA; ++c) {
@@ -304,14 +340,15 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
   }
 }
 
-// FP64 full stencil -> FP32 upper half (offsets 13..26) in the node-blocked layout.
-__global__ void k_stencil_upper32(const double *__restrict__ S, int64_t nn, float *Su)
+// FP64 full stencil -> upper half (offsets 13..26) in the node-blocked layout, as To;
+// blockIdx.y is the stored value v (0..125), so no thread divides by nn.
+template <typename To>
+__global__ void k_stencil_upper(const double *__restrict__ S, int64_t nn, To *__restrict__ Su)
 {
-  const int64_t tot = 126 * nn;
-  for (int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x; t < tot; t += (int64_t)gridDim.x * NT) {
-    const int v = (int)(t / nn);
-    Su[su_index(v, t - v * nn)] = (float)S[13 * 9 * nn + t];
-  }
+  const int v = blockIdx.y;
+  const double *__restrict__ src = S + (int64_t)(13 * 9 + v) * nn;
+  for (int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x; n < nn; n += (int64_t)gridDim.x * NT)
+    Su[su_index(v, n)] = (To)src[n];
 }
 
 __global__ void k_stencil_diag(const double *__restrict__ S, int64_t nn, double *diag,
@@ -819,11 +856,12 @@ void stencil_op_sym(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, 
   TOFE_STENCIL_SWITCH(k_stencil_op_sym);
 }
 
-void stencil_upper32(hipStream_t s, const double *S, int64_t nn, float *Su)
+template <typename To>
+void stencil_upper(hipStream_t s, const double *S, int64_t nn, To *Su)
 {
-  unsigned nb = (unsigned)((126 * nn + NT - 1) / NT);
-  if (nb > 4096) nb = 4096;
-  k_stencil_upper32<<<dim3(nb), dim3(NT), 0, s>>>(S, nn, Su);
+  if (nn <= 0) return;
+  const unsigned nb = (unsigned)std::min<int64_t>((nn + NT - 1) / NT, 1024);
+  k_stencil_upper<To><<<dim3(nb, 126), dim3(NT), 0, s>>>(S, nn, Su);
   TOFE_LAUNCH_CHECK();
 }
 
@@ -898,6 +936,10 @@ template void coarse_solve_tri<double>(hipStream_t, const double *, const double
 template void coarse_solve_tri<float>(hipStream_t, const float *, const float *, int, int, const float *,
                                       float *, float *);
 
+template void stencil_upper<float>(hipStream_t, const double *, int64_t, float *);
+template void stencil_upper<double>(hipStream_t, const double *, int64_t, double *);
+template void stencil_op_sym<double>(hipStream_t, const StencilOp<double> &, const double *, Epi,
+                                     const EpiArgs<double> &);
 template void stencil_op_sym<float>(hipStream_t, const StencilOp<float> &, const float *, Epi,
                                     const EpiArgs<float> &);
 }  // namespace tofe

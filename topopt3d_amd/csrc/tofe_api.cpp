@@ -65,6 +65,8 @@ template <typename T>
 struct Level {
   Grid g;
   DBuf<T> S;     // 27-point block stencil (levels >= 1)
+  DBuf<T> Su;    // FP64 solve: upper half of S in k_stencil_op_sym's layout, valid while su_ok
+  bool su_ok = false;
   DBuf<T> dinv;  // inverse Jacobi diagonal
   DBuf<T> b, x, x2, r, d;
 };
@@ -284,9 +286,16 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
     auto &lv = *levels<T>(c)[l];
     op.S = lv.S.p;
     op.Nx = lv.g.Nx; op.Ny = lv.g.Ny; op.Nz = lv.g.Nz;
-    // FP32 levels keep the upper half of the (exactly symmetric) Galerkin stencil
-    if constexpr (sizeof(T) == 4) stencil_op_sym<T>(s, op, u, epi, ep);
-    else stencil_op<T>(s, op, u, epi, ep);
+    // FP32 levels keep the upper half of the (exactly symmetric) Galerkin stencil; FP64 levels
+    // read their upper-half copy once the FP64 set-up has formed it (half the stencil traffic)
+    if constexpr (sizeof(T) == 4) {
+      stencil_op_sym<T>(s, op, u, epi, ep);
+    } else if (lv.su_ok) {
+      op.S = lv.Su.p;
+      stencil_op_sym<T>(s, op, u, epi, ep);
+    } else {
+      stencil_op<T>(s, op, u, epi, ep);
+    }
   }
 }
 
@@ -648,7 +657,8 @@ void lanczos_coefficients(tofe_ctx *c, int l, hipStream_t s, T *r, T *p, T *q, d
 // work vectors and its stream's reduction buffers, once the operators are built on the main
 // stream.  In the mixed-precision build they run on
 // the FP32 operators of the V-cycle (scalars and recorded coefficients stay FP64), whose
-// copies are made on the same streams first.  ev_lz / ev_lz0 mark their ends.
+// copies are made on the same streams first; the FP64 build first copies each level's
+// stencil into its upper-half layout there.  ev_lz / ev_lz0 mark their ends.
 // The fine level's estimate on the fourth stream, once the fine diagonal (main stream) is
 // formed: it needs nothing of the Galerkin hierarchy.  ev_lz0 marks its end.
 void lanczos_fine_enqueue(tofe_ctx *c)
@@ -696,7 +706,7 @@ void lanczos_enqueue(tofe_ctx *c, bool fine_done)
     for (int l = 1; l < L; ++l) {
       const hipStream_t sl = l == 1 ? s3 : s0;
       convert<double, float>(sl, c->lv64[l]->dinv.p, c->lv32[l]->dinv.p, c->grids[l].ndof());
-      stencil_upper32(sl, c->lv64[l]->S.p, c->grids[l].nn(), c->lv32[l]->S.p);
+      stencil_upper<float>(sl, c->lv64[l]->S.p, c->grids[l].nn(), c->lv32[l]->S.p);
     }
     for (int l = 1; l + 1 < L; ++l) {
       Level<float> &lv = *c->lv32[l];
@@ -708,6 +718,13 @@ void lanczos_enqueue(tofe_ctx *c, bool fine_done)
                                     c->partial2_c.p, c->scal_c.p);
     }
   } else {
+    // FP64: upper-half copies of the level stencils for the Lanczos steps and the solve
+    for (int l = 1; l < L; ++l) {
+      Level<double> &lv = *c->lv64[l];
+      lv.Su.alloc((size_t)126 * ((lv.g.nn() + 63) / 64 * 64));
+      stencil_upper<double>(l == 1 ? s3 : s0, lv.S.p, lv.g.nn(), lv.Su.p);
+      lv.su_ok = true;
+    }
     for (int l = 1; l + 1 < L; ++l) {
       Level<double> &lv = *c->lv64[l];
       if (l == 1)
@@ -813,6 +830,7 @@ void assemble(tofe_ctx *c)
   const int L = c->L;
   // fine level: Jacobi diagonal of K; level 1 = P^T K P element by element from E
   c->diag0.alloc(c->ndof);
+  for (auto &lv : c->lv64) lv->su_ok = false;  // the stencils are formed anew below
   const uint8_t *fix = c->has_fix ? c->dfix.p : nullptr;
   fine_diag(s, c->E.p, c->dKE.p, fix, c->desc.nx, c->desc.ny, c->desc.nz, c->diag0.p,
             c->lv64[0]->dinv.p);

@@ -119,7 +119,9 @@ void stencil_op(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, cons
 // the FP64 full -> FP32 upper-half conversion.
 template <typename T>
 void stencil_op_sym(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
-void stencil_upper32(hipStream_t s, const double *S, int64_t nn, float *Su);
+// FP64 full stencil -> its upper half (offsets 13..26) in k_stencil_op_sym's layout
+template <typename To>
+void stencil_upper(hipStream_t s, const double *S, int64_t nn, To *Su);
 void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double *Sc, double *T1,
               double *T2);
 int64_t galerkin_scratch(int Nfx, int Nfy, int Nfz, int64_t *n2);
