@@ -247,6 +247,16 @@ __host__ __device__ inline int64_t su_index(int v, int64_t n)
   return (n >> 6) * (126 * 64) + (int64_t)v * 64 + (n & 63);
 }
 
+// Neighbours per iteration of the rolled neighbour loop (3, 9 or 27).  27 for FP32 selects
+// the fully unrolled form; FP64 keeps one dz plane (9) per iteration, 220 VGPRs and no spills.
+// Level-1 Chebyshev step on the 192x96x96 cantilever: FP32 31 us (27) / 34 us (9 or 3); FP64
+// 62 us (9) / 75 us (3); the fully unrolled FP64 instance spills.
+#ifndef TOFE_SYM32_G
+#define TOFE_SYM32_G 27
+#endif
+#ifndef TOFE_SYM64_G
+#define TOFE_SYM64_G 9
+#endif
 template <typename T, int EPI, bool SPLIT>
 __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T *__restrict__ u,
                                                        EpiArgs<T> ep, int64_t nn)
@@ -265,7 +275,7 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
     T acc[NA];
 #pragma unroll
     for (int c = 0; c < NA; ++c) acc[c] = 0;
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && TOFE_SYM32_G == 27) {
       // FP32: fully unrolled; the compiler issues all 27 neighbours' loads up front (one wave
       // per SIMD, 256 VGPRs), which measured faster than the rolled form below
 #pragma unroll
@@ -291,14 +301,15 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
         }
       }
     } else {
-    // FP64: one (dy, dz) row of three neighbours per iteration, the loop kept rolled so that
-    // one row's loads (30) are in flight per thread; fully unrolled, the FP64 instance spills.
+    // GN neighbours per iteration, the loop kept rolled so that one group's loads are in
+    // flight per thread
+    constexpr int GN = sizeof(T) == 4 ? TOFE_SYM32_G : TOFE_SYM64_G;
 #pragma unroll 1
-    for (int g = 0; g < 9; ++g) {
-      const int dy = g % 3 - 1, dz = g / 3 - 1;
+    for (int g = 0; g < 27 / GN; ++g) {
 #pragma unroll
-      for (int ix = 0; ix < 3; ++ix) {
-        const int o = 3 * g + ix, dx = ix - 1;
+      for (int ix = 0; ix < GN; ++ix) {
+        const int o = GN * g + ix;
+        const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
         const int mi = i + dx, mj = j + dy, mk = k + dz;
         // Unconditional loads (a missing neighbour reads node n) weighted by w = 0 or 1, with
         // no branch: fma(w, t, acc) is acc + t or acc exactly.  (A select of acc + t, or a
