@@ -305,6 +305,35 @@ def test_pc_apply_matches_oracle_vcycle(oracle, tofe, nel, L):
     ctx.close()
 
 
+def test_fp64_level_copies_follow_the_design(oracle, tofe):
+    """The FP64 levels read an upper-half copy of their Galerkin stencil that every FP64
+    set-up forms anew: after a new design and a mixed-precision set-up in between, one FP64
+    V-cycle equals the oracle's for the new design and a fresh context's."""
+    nel, L = (16, 8, 8), 3
+    pbA, ctx = setup_pair(oracle, tofe, nel, L, seed=7)[:2]
+    pbB, fresh, Pf, rho, E, dEdz, Es, dEsdz, KB = setup_pair(oracle, tofe, nel, L, seed=8)
+    r = np.random.default_rng(5).standard_normal(pbA.ndof)
+    r[pbA.fixed_dof.astype(bool)] = 0
+    ctx.set_precision(tofe.PREC_FP64)
+    ctx.assemble()
+    zA = ctx.pc_apply(r)
+    ctx.set_design(pbB.x)
+    ctx.set_precision(tofe.PREC_MIXED)
+    ctx.assemble()
+    ctx.set_precision(tofe.PREC_FP64)
+    ctx.assemble()
+    z = ctx.pc_apply(r)
+    fresh.set_precision(tofe.PREC_FP64)
+    fresh.assemble()
+    zf = fresh.pc_apply(r)
+    zo = oracle.MG(KB, tuple(n + 1 for n in nel), L).vcycle(r)
+    assert rel(z, zo) <= 1e-11
+    assert rel(z, zf) <= 1e-14
+    assert rel(zA, zo) > 1e-6  # the first design's V-cycle is a different operator
+    ctx.close()
+    fresh.close()
+
+
 @pytest.mark.parametrize("nel", [(8, 4, 4), (10, 6, 5)])
 def test_single_level_is_exact_coarse_solve(oracle, tofe, nel):
     """With one level the preconditioner is the blocked Cholesky inverse of K (several
