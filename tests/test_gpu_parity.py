@@ -305,6 +305,35 @@ def test_pc_apply_matches_oracle_vcycle(oracle, tofe, nel, L):
     ctx.close()
 
 
+def test_stencil_thread_modes(oracle, tofe, monkeypatch):
+    """The coarse-level stencil kernels run one thread per node, or one per DOF on grids below
+    TOFE_STENCIL_SPLIT_NODES nodes (the 192x96x96 cantilever's level 1 runs per node, these small
+    grids per DOF by default).  Both modes sum each DOF in the same order; only the block
+    partial sums of the dot products (Lanczos estimate) are grouped differently.  FP64 and
+    mixed V-cycles in either mode against the oracle's V-cycle."""
+    nel, L = (16, 8, 8), 3
+    out = {}
+    for mode, threshold in [("node", "0"), ("dof", "1000000000")]:
+        monkeypatch.setenv("TOFE_STENCIL_SPLIT_NODES", threshold)
+        pb, ctx, Pf, rho, E, dEdz, Es, dEsdz, K = setup_pair(oracle, tofe, nel, L)
+        r = np.random.default_rng(4).standard_normal(pb.ndof)
+        r[pb.fixed_dof.astype(bool)] = 0
+        ctx.set_precision(tofe.PREC_FP64)
+        ctx.assemble()
+        z64 = ctx.pc_apply(r)
+        ctx.set_precision(tofe.PREC_MIXED)
+        ctx.assemble()
+        z32 = ctx.pc_apply(r)
+        out[mode] = (z64, z32)
+        ctx.close()
+    zo = oracle.MG(K, tuple(n + 1 for n in nel), L).vcycle(r)
+    for mode in out:
+        assert rel(out[mode][0], zo) <= 1e-11, mode
+        assert rel(out[mode][1], zo) <= 1e-4, mode
+    assert rel(out["node"][0], out["dof"][0]) <= 1e-12
+    assert rel(out["node"][1], out["dof"][1]) <= 1e-5
+
+
 def test_fp64_level_copies_follow_the_design(oracle, tofe):
     """The FP64 levels read an upper-half copy of their Galerkin stencil that every FP64
     set-up forms anew: after a new design and a mixed-precision set-up in between, one FP64

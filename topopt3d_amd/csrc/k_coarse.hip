@@ -812,21 +812,18 @@ void coarse_inverse_factor(hipStream_t s, double *A, double *Lb, double *X, int 
   }
 }
 
-// grids below this many nodes run one thread per DOF (see split_index)
-constexpr int64_t kSplitNodes = 131072;
-
 template <typename T>
 int stencil_op_blocks(const StencilOp<T> &op)
 {
   const int64_t nn = (int64_t)op.Nx * op.Ny * op.Nz;
   const int nb = (int)((nn + NT - 1) / NT);
-  return nn < kSplitNodes ? 3 * nb : nb;
+  return nn < op.split_nodes ? 3 * nb : nb;
 }
 
 #define TOFE_STENCIL_SWITCH(KERNEL)                                                         \
   do {                                                                                      \
     const int64_t nn = (int64_t)op.Nx * op.Ny * op.Nz;                                      \
-    const bool split = nn < kSplitNodes;                                                    \
+    const bool split = nn < op.split_nodes;                                                 \
     dim3 grid((unsigned)stencil_op_blocks<T>(op)), block(NT);                               \
     switch (epi) {                                                                          \
       case EPI_APPLY:                                                                       \

@@ -159,6 +159,7 @@ struct tofe_ctx {
 
   bool assembled = false;
   bool fine_mfma = false;  // TOFE_FINE_MFMA=1: element-centric MFMA fine operator
+  int64_t split_nodes = kStencilSplitNodes;  // TOFE_STENCIL_SPLIT_NODES: one thread per DOF below
   FineTile fine_tile{0, 8}, fine_tile32{0, 8};  // k_fine_op launch shapes, FP64 and FP32
                                                 // (TOFE_FINE_TILE="<shape>:<zlen>" sets both)
   bool ke_sym = false;  // KE matches the cube-orbit table: fine operator keeps KE in registers
@@ -286,6 +287,7 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
     auto &lv = *levels<T>(c)[l];
     op.S = lv.S.p;
     op.Nx = lv.g.Nx; op.Ny = lv.g.Ny; op.Nz = lv.g.Nz;
+    op.split_nodes = c->split_nodes;
     // FP32 levels keep the upper half of the (exactly symmetric) Galerkin stencil; FP64 levels
     // read their upper-half copy once the FP64 set-up has formed it (half the stencil traffic)
     if constexpr (sizeof(T) == 4) {
@@ -308,6 +310,7 @@ int level_blocks(tofe_ctx *c, int l)
   }
   StencilOp<T> op;
   op.Nx = c->grids[l].Nx; op.Ny = c->grids[l].Ny; op.Nz = c->grids[l].Nz;
+  op.split_nodes = c->split_nodes;
   return stencil_op_blocks<T>(op);
 }
 
@@ -1021,6 +1024,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
   try {
     TOFE_HIP(hipGetDevice(&c->device));
     if (const char *fm = std::getenv("TOFE_FINE_MFMA")) c->fine_mfma = fm[0] == '1';
+    if (const char *sn = std::getenv("TOFE_STENCIL_SPLIT_NODES")) c->split_nodes = std::atoll(sn);
     // level count: floor(log2(min nel)) unless given (RecMesh.cpp:552-554), truncated
     // once a coarse grid has <= 10 DOFs (Interpolation.cpp:70-71)
     int L = d.mg_levels;

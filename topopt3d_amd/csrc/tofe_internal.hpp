@@ -88,10 +88,16 @@ template <typename T>
 void fine_wht(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep);
 
 // Stored 27-point block stencil operator of a coarse level.
+// Stencil grids with fewer nodes than this run one thread per DOF instead of one per node
+// (level 2 of the 192x96x96 cantilever, 30k nodes, is faster per node; level 1, 233k, per DOF
+// 44 us against 31 us).  TOFE_STENCIL_SPLIT_NODES overrides it per context.
+constexpr int64_t kStencilSplitNodes = 16384;
+
 template <typename T>
 struct StencilOp {
   const T *S = nullptr;        // 243 * nnode
   int Nx = 0, Ny = 0, Nz = 0;  // nodes
+  int64_t split_nodes = kStencilSplitNodes;  // grids below this run one thread per DOF
 };
 
 constexpr int kRedBlocks = 1024;   // grid of the grid-stride reduction kernels
