@@ -235,13 +235,13 @@ __global__ __launch_bounds__(NT) void k_galerkin_axis3(const double *__restrict_
 }
 
 // The same operator from the upper half of an exactly symmetric stencil: offsets
-// o >= 13 are stored at the node (Su[((o-13)*9 + a*3 + b)*nn + n], 126 values per node) and
+// o >= 13 are stored at the node (value (o-13)*9 + a*3 + b, 126 values per node) and
 // the block of a lower offset o < 13 is the transpose of the neighbour's block 26 - o.  Each
 // stored block is read by its node and by one neighbour, so the HBM traffic is about half of
 // the full stencil's.
-// Node-blocked layout of the FP32 upper-half stencil: value v (0..125) of node n at
+// Node-blocked layout of the upper-half stencil (FP32 and FP64): value v (0..125) of node n at
 // (n / 64) * 126 * 64 + v * 64 + n % 64, so the 126 values of a wavefront's 64 nodes form one
-// contiguous 32 KB range (each value still one coalesced 256-byte read).
+// contiguous range (each value still one coalesced read of 64 consecutive values).
 __host__ __device__ inline int64_t su_index(int v, int64_t n)
 {
   return (n >> 6) * (126 * 64) + (int64_t)v * 64 + (n & 63);
@@ -276,8 +276,8 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
 #pragma unroll
     for (int c = 0; c < NA; ++c) acc[c] = 0;
     if constexpr (sizeof(T) == 4 && TOFE_SYM32_G == 27) {
-      // FP32: fully unrolled; the compiler issues all 27 neighbours' loads up front (one wave
-      // per SIMD, 256 VGPRs), which measured faster than the rolled form below
+      // FP32: fully unrolled; the compiler issues all 27 neighbours' loads up front (323
+      // registers, one wave per SIMD), which measured faster than the rolled form below
 #pragma unroll
       for (int o = 0; o < 27; ++o) {
         const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
@@ -301,38 +301,38 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
         }
       }
     } else {
-    // GN neighbours per iteration, the loop kept rolled so that one group's loads are in
-    // flight per thread
-    constexpr int GN = sizeof(T) == 4 ? TOFE_SYM32_G : TOFE_SYM64_G;
+      // GN neighbours per iteration, the loop kept rolled so that one group's loads are in
+      // flight per thread
+      constexpr int GN = sizeof(T) == 4 ? TOFE_SYM32_G : TOFE_SYM64_G;
 #pragma unroll 1
-    for (int g = 0; g < 27 / GN; ++g) {
+      for (int g = 0; g < 27 / GN; ++g) {
 #pragma unroll
-      for (int ix = 0; ix < GN; ++ix) {
-        const int o = GN * g + ix;
-        const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
-        const int mi = i + dx, mj = j + dy, mk = k + dz;
-        // Unconditional loads (a missing neighbour reads node n) weighted by w = 0 or 1, with
-        // no branch: fma(w, t, acc) is acc + t or acc exactly.  (A select of acc + t, or a
-        // short-circuit bounds test, lets the compiler put each neighbour's loads behind a
-        // branch of their own.)
-        const bool in = ((unsigned)mi < (unsigned)Nx) & ((unsigned)mj < (unsigned)Ny) &
-                        ((unsigned)mk < (unsigned)Nz);
-        const T w = in ? T(1) : T(0);
-        const int64_t m = in ? n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz) : n;
-        const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
-        // o >= 13: stored block (n, o); o < 13: stored block (m, 26 - o), transposed
-        const bool up = o >= 13;
-        const T *B = S + (up ? su_index((o - 13) * 9, n) : su_index((13 - o) * 9, m));
-        const int sr = up ? 3 : 1, sc = up ? 1 : 3;  // strides of (row a, column b)
+        for (int ix = 0; ix < GN; ++ix) {
+          const int o = GN * g + ix;
+          const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+          const int mi = i + dx, mj = j + dy, mk = k + dz;
+          // Unconditional loads (a missing neighbour reads node n) weighted by w = 0 or 1, with
+          // no branch: fma(w, t, acc) is acc + t or acc exactly.  (A select of acc + t, or a
+          // short-circuit bounds test, lets the compiler put each neighbour's loads behind a
+          // branch of their own.)
+          const bool in = ((unsigned)mi < (unsigned)Nx) & ((unsigned)mj < (unsigned)Ny) &
+                          ((unsigned)mk < (unsigned)Nz);
+          const T w = in ? T(1) : T(0);
+          const int64_t m = in ? n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz) : n;
+          const T u0 = u[3 * m], u1 = u[3 * m + 1], u2 = u[3 * m + 2];
+          // o >= 13: stored block (n, o); o < 13: stored block (m, 26 - o), transposed
+          const bool up = o >= 13;
+          const T *B = S + (up ? su_index((o - 13) * 9, n) : su_index((13 - o) * 9, m));
+          const int sr = up ? 3 : 1, sc = up ? 1 : 3;  // strides of (row a, column b)
 #pragma unroll
-        for (int c = 0; c < NA; ++c) {
-          const int a = SPLIT ? a0 : c;
-          const T t = B[(a * sr + 0 * sc) * 64] * u0 + B[(a * sr + 1 * sc) * 64] * u1 +
-                      B[(a * sr + 2 * sc) * 64] * u2;
-          acc[c] = __builtin_fma(w, t, acc[c]);
+          for (int c = 0; c < NA; ++c) {
+            const int a = SPLIT ? a0 : c;
+            const T t = B[(a * sr + 0 * sc) * 64] * u0 + B[(a * sr + 1 * sc) * 64] * u1 +
+                        B[(a * sr + 2 * sc) * 64] * u2;
+            acc[c] = __builtin_fma(w, t, acc[c]);
+          }
         }
       }
-    }
     }
 #pragma unroll
     for (int c = 0; c < NA; ++c) {
