@@ -250,7 +250,10 @@ __host__ __device__ inline int64_t su_index(int v, int64_t n)
 // Neighbours per iteration of the rolled neighbour loop (3, 9 or 27).  27 for FP32 selects
 // the fully unrolled form; FP64 keeps one dz plane (9) per iteration, 220 VGPRs and no spills.
 // Level-1 Chebyshev step on the 192x96x96 cantilever: FP32 31 us (27) / 34 us (9 or 3); FP64
-// 62 us (9) / 75 us (3); the fully unrolled FP64 instance spills.
+// 62 us (9) / 75 us (3); the fully unrolled FP64 instance spills.  The FP32 one-thread-per-DOF
+// instances take the rolled form's branch-free body with all 27 neighbours unrolled: 5.3 us
+// against 5.8 us per Chebyshev step on level 3 (4,225 nodes); per node that body is slower
+// (33.6 against 31.2 us on level 1).
 #ifndef TOFE_SYM32_G
 #define TOFE_SYM32_G 27
 #endif
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
     T acc[NA];
 #pragma unroll
     for (int c = 0; c < NA; ++c) acc[c] = 0;
-    if constexpr (sizeof(T) == 4 && TOFE_SYM32_G == 27) {
+    if constexpr (sizeof(T) == 4 && TOFE_SYM32_G == 27 && !SPLIT) {
       // FP32: fully unrolled; the compiler issues all 27 neighbours' loads up front (323
       // registers, one wave per SIMD), which measured faster than the rolled form below
 #pragma unroll
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
     } else {
       // GN neighbours per iteration, the loop kept rolled so that one group's loads are in
       // flight per thread
-      constexpr int GN = sizeof(T) == 4 ? TOFE_SYM32_G : TOFE_SYM64_G;
+      constexpr int GN = sizeof(T) == 4 ? (SPLIT ? 27 : TOFE_SYM32_G) : TOFE_SYM64_G;
 #pragma unroll 1
       for (int g = 0; g < 27 / GN; ++g) {
 #pragma unroll
