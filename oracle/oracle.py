@@ -102,6 +102,12 @@ def lib():
     L.orc_sensitivity.argtypes = [C.c_int] * 3 + [_f64p, _f64p, _f64p, _f64p]
     L.orc_power_start.argtypes = [C.c_int64, _f64p]
     L.orc_num_threads.restype = C.c_int
+    L.orc_lanczos_lambda.restype = C.c_double
+    L.orc_lanczos_lambda.argtypes = [vp, _f64p, C.c_int]
+    L.orc_tridiag_max_eig.restype = C.c_double
+    L.orc_tridiag_max_eig.argtypes = [C.c_int, _f64p, _f64p]
+    L.orc_smooth.argtypes = [vp, _f64p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                             C.c_int, _f64p, _f64p, _f64p, _f64p]
     _lib = L
     return L
 
@@ -291,6 +297,27 @@ def stability_rhs(nx, ny, nz, Es, phi, fixed_dof=None, E0=1.0, nu=0.3, h=(1.0, 1
     lib().orc_stability_rhs(nx, ny, nz, E0, nu, h[0], h[1], h[2], np.ascontiguousarray(Es, np.float64),
                             phi, nvec, None if fx is None else fx.ctypes.data_as(C.c_void_p), out)
     return out.reshape(nvec, ndof)
+
+
+def lanczos_lambda(A: CSR, dinv, its):
+    """lambda_max(D^-1 A) estimate from `its` Jacobi-CG (Lanczos) steps."""
+    return lib().orc_lanczos_lambda(A.h, np.ascontiguousarray(dinv, np.float64), its)
+
+
+def tridiag_max_eig(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return lib().orc_tridiag_max_eig(a.size, a, b)
+
+
+def smooth(A: CSR, dinv, lam, b, x0=None, steps=2, lo=0.1, hi=1.1, smoother=0):
+    """`steps` level-smoother steps (Chebyshev 0 / Jacobi 1) from x0 (None: zero guess)."""
+    n = A.shape[0]
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64, copy=True)
+    d, r = np.zeros(n), np.zeros(n)
+    lib().orc_smooth(A.h, np.ascontiguousarray(dinv, np.float64), lam, lo, hi, smoother, steps,
+                     1 if x0 is None else 0, np.ascontiguousarray(b, np.float64), x, d, r)
+    return x
 
 
 def power_start(n):

@@ -906,19 +906,22 @@ void orc_mg_free(orc_mg *mg)
   free(mg);
 }
 
-/* Chebyshev iteration on D^-1 A over [lo*lam, hi*lam]; step 0 from x = 0 needs no
- * operator apply.  Jacobi smoother (smoother == 1) uses omega = 2/(emin+emax). */
-static void smooth(orc_mg *mg, int l, int steps, int zero_guess)
+/* `steps` steps of the Chebyshev iteration on D^-1 A over [lo*lam, hi*lam] (smoother 0) or
+ * of damped Jacobi with omega = 2/(emin+emax) (smoother 1), updating x in place; step 0 from
+ * x = 0 (zero_guess) needs no operator apply.  d and r are n-vectors of scratch.  This is the
+ * level smoother of the PCMG V-cycle (FEAnalysis.cpp:397-416 selects PCMG; PETSc's
+ * KSPChebyshev with the eigenvalue transform 0.1/1.1 is restated from its documentation). */
+void orc_smooth(const orc_csr *A, const double *dinv, double lam, double lo, double hi,
+                int smoother, int steps, int zero_guess, const double *b, double *x, double *d,
+                double *r)
 {
-  const orc_csr *A = mg->A[l];
   int64_t n = A->n;
-  double *b = mg->b[l], *x = mg->x[l], *r = mg->r[l], *d = mg->d[l], *dinv = mg->dinv[l];
-  double emax = mg->hi * mg->lam[l], emin = mg->lo * mg->lam[l];
+  double emax = hi * lam, emin = lo * lam;
   double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin);
   double sigma = theta / delta, rho_old = 1.0 / sigma;
   for (int k = 0; k < steps; k++) {
     double c1, c2;
-    if (mg->smoother == 1) { c1 = 0.0; c2 = 1.0 / theta; }
+    if (smoother == 1) { c1 = 0.0; c2 = 1.0 / theta; }
     else if (k == 0) { c1 = 0.0; c2 = 1.0 / theta; }
     else {
       double rho = 1.0 / (2 * sigma - rho_old);
@@ -938,6 +941,12 @@ static void smooth(orc_mg *mg, int l, int steps, int zero_guess)
       }
     }
   }
+}
+
+static void smooth(orc_mg *mg, int l, int steps, int zero_guess)
+{
+  orc_smooth(mg->A[l], mg->dinv[l], mg->lam[l], mg->lo, mg->hi, mg->smoother, steps, zero_guess,
+             mg->b[l], mg->x[l], mg->d[l], mg->r[l]);
 }
 
 static void vcycle(orc_mg *mg, int l)

@@ -111,6 +111,10 @@ void orc_sensitivity(int nx, int ny, int nz, const double KE[576], const double 
 void orc_power_start(int64_t n, double *v);
 double orc_tridiag_max_eig(int k, const double *a, const double *b);
 double orc_lanczos_lambda(const orc_csr *A, const double *dinv, int its);
+/* the PCMG level smoother: Chebyshev (smoother 0) or damped Jacobi (1) on D^-1 A */
+void orc_smooth(const orc_csr *A, const double *dinv, double lam, double lo, double hi,
+                int smoother, int steps, int zero_guess, const double *b, double *x, double *d,
+                double *r);
 
 int orc_num_threads(void);
 
