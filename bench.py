@@ -9,17 +9,26 @@ relative residual.
 
 One step = one FESolve: Galerkin set-up for the current design (tofe_assemble) plus the
 solve from a zero initial guess (tofe_solve_dev), with F and U resident in HBM.  The
-headline value is the mixed-precision path (FP32 V-cycle inside FP64 flexible CG);
-the FP64 path, the operator rate and the per-kernel roofline are reported beside it,
-with the oracle (a C restatement of the reference's PETSc path) timed on host cores.
+headline value is the mixed-precision path (FP32 V-cycle inside FP64 flexible CG); the
+FP64 path is measured beside it.  The timed steps carry no instrumentation: the kernel
+durations come from one extra, untimed step of each precision with HIP events around every
+operator launch (tofe_time_kernels), from which the dominant solve kernel's roofline, the
+three largest solve kernels and the whole-solve roofline fraction are formed.  The design
+iteration (MatIntFnc -> FEAssemble -> FESolve -> Compliance + Chain_Filter) is timed as
+its own loop.  The oracle (a C restatement of the reference's PETSc path) is timed on the
+host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --summarize-trace <rocprofv3 output dir>   (phase-split kernel table)
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -32,6 +41,8 @@ METRIC = "s/solve to 1e-8 res at 5.45M DOF, mixed vs FP64 on GPU; operator MDOF/
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+EPI_NAMES = {0: "apply", 1: "apply+dot", 2: "residual", 3: "Chebyshev step",
+             4: "apply+2 dots", 5: "last Chebyshev step (FP64 out)"}
 
 
 def log(*a):
@@ -53,35 +64,209 @@ def read_traffic(name):
     return None
 
 
-def cpu_baseline(pb, its_ref, n_iter=3):
-    """Oracle (C + OpenMP restatement of the reference PETSc path) on the same workload:
-    full set-up (filter, SIMP, CSR assembly, Galerkin PtAP hierarchy, dense Cholesky)
-    plus n_iter timed PCG iterations, extrapolated to its_ref iterations."""
+# ---- algorithmic work per kernel (bytes and FLOP that the operation needs at minimum) ----
+
+class Levels:
+    """Node grids of the multigrid hierarchy: level 0 is the mesh, each coarser level keeps
+    every second node ((N + 1) // 2 per axis)."""
+
+    def __init__(self, nel, L):
+        self.nodes = [tuple(n + 1 for n in nel)]
+        for _ in range(L - 1):
+            self.nodes.append(tuple((n + 1) // 2 for n in self.nodes[-1]))
+        self.N = [3 * a * b * c for a, b, c in self.nodes]          # DOF
+        self.Ne = [(a - 1) * (b - 1) * (c - 1) for a, b, c in self.nodes]  # cells
+        self.nn = [a * b * c for a, b, c in self.nodes]
+
+
+def op_work(lv, level, s, epi):
+    """(algorithmic bytes, algorithmic FLOP, stored bytes) of one level-operator launch:
+    K-apply s(2N + Ne) and 1152 Ne FLOP (cells of the level), plus the epilogue's vectors --
+    Chebyshev step +5sN (the K-apply + s*5N of the reference accounting), residual +sN (b),
+    apply with a second dot +sN (b), last step +3sN and its FP64 output and q (16N).  Stored
+    bytes replace the element scales by what the kernel actually reads as operator: E for the
+    matrix-free fine level, the 126-value upper-half stencil per node on coarse levels."""
+    N, Ne = lv.N[level], lv.Ne[level]
+    extra = {0: 0, 1: 0, 2: s * N, 3: 5 * s * N, 4: s * N, 5: 3 * s * N + 16 * N}[epi]
+    alg = s * (2 * N + Ne) + extra
+    flops = 1152.0 * Ne
+    if level == 0:
+        stored = alg
+    else:
+        stored = alg - s * Ne + s * 126 * lv.nn[level]
+    return alg, flops, stored
+
+
+def op_time_bound(alg, flops, s):
+    peak = (PEAK_FP32_TFLOPS if s == 4 else PEAK_FP64_TFLOPS) * 1e12
+    return max(alg / (PEAK_HBM_GBS * 1e9), flops / peak)
+
+
+def vector_bytes(lv, prec, n_outer, n_vcycle):
+    """Algorithmic bytes of the solve's non-operator kernels: CG / flexible-CG updates, the
+    transfers, the coarse triangular solves and the dot products, per the iteration counts."""
+    s = 4 if prec == "mixed" else 8
+    L = len(lv.N)
+    N0 = lv.N[0]
+    v = 0.0
+    for l in range(L - 1):
+        Nf, Nc = lv.N[l], lv.N[l + 1]
+        v += s * (Nf + Nc)                    # restriction
+        if l + 1 < L - 1:
+            v += 3 * s * Nc                   # its fused first smoothing step (dinv in, d, x out)
+        v += s * (2 * Nf + Nc)                # prolongation-add
+    nc = lv.N[-1]
+    v += s * nc * nc                          # X and X^T triangles of the coarse solve
+    per_vcycle = v
+    if prec == "mixed":
+        per_vcycle += 0                       # the fine first step is fused into the FCG update
+        per_outer = 40 * N0 + 40 * N0         # fcg_update_r (+ first step), fcg_update_xp
+    else:
+        per_vcycle += 32 * N0                 # fine first Chebyshev step from zero
+        per_outer = 24 * N0 + 16 * N0 + 40 * N0  # r update, r.z, p / x update
+    start = 16 * N0 * 3                       # F.F, r.r, p = z copy
+    return n_vcycle * per_vcycle + n_outer * per_outer + start
+
+
+def summarize_kernel_times(kt, lv, prec):
+    """Per-kind rows (dicts) of the traced operator launches of one solve."""
+    rows = []
+    for k in kt:
+        s = 4 if k["precision"] == 1 else 8
+        alg, flops, stored = op_work(lv, k["level"], s, k["epilogue"])
+        avg_ms = k["total_ms"] / max(k["count"], 1)
+        name = ("k_fine_wht" if k["level"] == 0 else "k_stencil_op_sym") + \
+            f"<{'float' if s == 4 else 'double'}, {k['epilogue']}>"
+        rows.append({
+            "kernel": name, "level": k["level"], "precision": "fp32" if s == 4 else "fp64",
+            "epilogue": EPI_NAMES[k["epilogue"]], "count": k["count"],
+            "total_ms": k["total_ms"], "ms_per_launch": avg_ms,
+            "algorithmic_bytes": alg, "stored_bytes": stored, "algorithmic_flops": flops,
+            "achieved_GBs": alg / (avg_ms * 1e-3) / 1e9,
+            "frac_hbm": alg / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "frac_roofline": op_time_bound(alg, flops, s) / (avg_ms * 1e-3),
+        })
+    rows.sort(key=lambda r: -r["total_ms"])
+    return rows
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+CPU_SAMPLE = r'''
+import json, sys, time
+sys.path.insert(0, {root!r})
+from oracle import oracle as O
+from topopt3d_amd import configs
+pb = configs.config2()
+t0 = time.perf_counter()
+rho = O.filter_csr(*pb.nel, pb.rmin).matvec(pb.x)
+E = O.simp(rho, pb.penal, pb.Emin)[0]
+K = O.assemble_K(*pb.nel, O.ke(), E, pb.fixed_dof)
+mg = O.MG(K, tuple(n + 1 for n in pb.nel), pb.mg_levels)
+t1 = time.perf_counter()
+x, its, reason, rn = mg.solve(pb.F, rtol={rtol}, maxit={maxit})
+t2 = time.perf_counter()
+print(json.dumps({{"setup": t1 - t0, "solve": t2 - t1, "its": its, "reason": reason,
+                  "threads": O.num_threads()}}))
+'''
+
+
+def cpu_run(threads, rtol, maxit):
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    code = CPU_SAMPLE.format(root=ROOT, rtol=rtol, maxit=maxit)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-2000:])
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(its_gpu):
+    """The oracle (C + OpenMP restatement of the reference PETSc path: assembled CSR, Galerkin
+    PtAP hierarchy, CG + V(2,2) Chebyshev-Jacobi, dense Cholesky coarse solve) on the same
+    workload: a full measured solve to 1e-8 on all host threads, and one thread on a bounded
+    sample (set-up + one PCG iteration, measured; the solve extrapolated per iteration)."""
     from oracle import oracle as O
 
     O.build()
-    t0 = time.perf_counter()
-    rho = O.filter_csr(*pb.nel, pb.rmin).matvec(pb.x)
-    E = O.simp(rho, pb.penal, pb.Emin)[0]
-    K = O.assemble_K(*pb.nel, O.ke(), E, pb.fixed_dof)
-    mg = O.MG(K, tuple(n + 1 for n in pb.nel), pb.mg_levels)
-    t1 = time.perf_counter()
-    _, its, reason, _ = mg.solve(pb.F, rtol=1e-30, maxit=n_iter)
-    t2 = time.perf_counter()
-    setup = t1 - t0
-    per_it = (t2 - t1) / max(its, 1)
+    full = cpu_run(None, 1e-8, 1000)
+    one = cpu_run(1, 1e-30, 1)
+    per_it_1 = one["solve"] / max(one["its"], 1)
     return {
-        "value": setup + per_it * its_ref,
+        "value": full["setup"] + full["solve"],
         "unit": "s/solve",
-        "cores": O.num_threads(),
+        "cores": full["threads"],
         "kind": "port",
-        "sample": (f"same 5.45M-DOF workload on host cores: full CPU set-up measured "
-                   f"({setup:.1f} s: filter, SIMP, CSR assembly, Galerkin PtAP, dense "
-                   f"Cholesky) + {its} PCG iterations measured ({per_it:.2f} s/it), "
-                   f"extrapolated to the {its_ref} iterations of the FP64 GPU solve"),
-        "setup_s": setup,
-        "s_per_iteration": per_it,
+        "sample": (f"full oracle solve of the same 5.45M-DOF workload to 1e-8 on {full['threads']} "
+                   f"threads, measured: set-up {full['setup']:.2f} s (filter, SIMP, CSR assembly, "
+                   f"Galerkin PtAP, dense Cholesky) + {full['its']} PCG iterations "
+                   f"{full['solve']:.2f} s"),
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "setup_s": full["setup"],
+        "solve_s": full["solve"],
+        "its": full["its"],
+        "one_thread": {
+            "setup_s": one["setup"],
+            "s_per_iteration": per_it_1,
+            "s_per_solve_extrapolated": one["setup"] + per_it_1 * full["its"],
+            "sample": "1 thread: set-up and one PCG iteration measured; the solve = set-up + "
+                      "iterations x the measured per-iteration time",
+        },
     }
+
+
+# ---- rocprofv3 trace summary split by phase ----------------------------------------------
+
+def summarize_trace(d):
+    """Kernel table per roctx phase range of the library ('assemble:mixed', 'solve:mixed',
+    'assemble:fp64', 'solve:fp64', other entry points) from a rocprofv3
+    --kernel-trace --marker-trace --output-format csv directory."""
+    kfile = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
+    mfile = glob.glob(os.path.join(d, "**", "*marker_api_trace.csv"), recursive=True)
+    if not kfile or not mfile:
+        sys.exit(f"no kernel_trace.csv / marker_api_trace.csv under {d}")
+    ranges = []
+    for row in csv.DictReader(open(mfile[0])):
+        ranges.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]),
+                       row.get("Function") or row.get("Message") or row.get("Operation", "")))
+    # innermost range containing the kernel
+    ranges.sort(key=lambda r: r[1] - r[0])
+    phases = {}
+    for row in csv.DictReader(open(kfile[0])):
+        t0, t1 = int(row["Start_Timestamp"]), int(row["End_Timestamp"])
+        name = row["Kernel_Name"]
+        ph = "outside any entry point"
+        for a, b, nm in ranges:
+            if a <= t0 and t1 <= b:
+                ph = nm
+                break
+        phases.setdefault(ph, {}).setdefault(name, []).append((t1 - t0) / 1e3)
+    out = []
+    order = ["solve:mixed", "assemble:mixed", "solve:fp64", "assemble:fp64"]
+    for ph in order + sorted(p for p in phases if p not in order):
+        if ph not in phases:
+            continue
+        ks = phases[ph]
+        tot = sum(sum(v) for v in ks.values())
+        out.append(f"\n### {ph}: {sum(len(v) for v in ks.values())} kernels, {tot:.1f} us of kernel time\n")
+        out.append("| kernel | calls | total (us) | average (us) | % of phase |")
+        out.append("|---|---:|---:|---:|---:|")
+        for name, v in sorted(ks.items(), key=lambda kv: -sum(kv[1])):
+            short = name.split("(")[0].replace("void ", "").replace("tofe::(anonymous namespace)::", "")
+            out.append(f"| `{short}` | {len(v)} | {sum(v):.1f} | {np.mean(v):.2f} | "
+                       f"{100 * sum(v) / tot:.2f} |")
+    print("\n".join(out))
 
 
 def main():
@@ -91,10 +276,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="48x24x24 instead of 192x96x96")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print the phase-split kernel table of a rocprofv3 trace and exit")
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="only launch the dominant kernel 5 times and a 512 MiB device copy, "
-                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs")
+                    help="only launch the fine FP32 Chebyshev kernel and the level-1 FP32 stencil "
+                         "Chebyshev kernel 6 times each and a 512 MiB device copy, for rocprofv3 "
+                         "--pmc runs")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box in this "
               "setting has a single MI355X and the path is not sharded here", file=sys.stderr)
@@ -111,9 +302,10 @@ def main():
     log(f"bench: {pb.name} ndof={pb.ndof} nelem={pb.nelem}")
     ctx = _lib.Context(pb.nel, Emin=pb.Emin, penal=pb.penal, rmin=pb.rmin,
                        mg_levels=pb.mg_levels, fixed_dof=pb.fixed_dof)
-    ctx.set_design(pb.x)
+    lv = Levels(pb.nel, ctx.levels)
+    xd = torch.from_numpy(pb.x).to(dev)
+    ctx.set_design_dev(xd.data_ptr())
     if args.pmc_probe:
-        # calibration copy: 512 MiB read + 512 MiB written by one copy kernel
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
@@ -121,10 +313,10 @@ def main():
         torch.cuda.synchronize()
         ctx.set_precision(_lib.PREC_MIXED)
         ctx.assemble()
-        ms = ctx.time_operator(0, _lib.PREC_MIXED, 1, 5)
-        log(f"bench: pmc probe: copy 536870912 B read + written; fine operator fp32 cheb "
-            f"x5 {ms * 1e3:.1f} us each, algorithmic bytes "
-            f"{4.0 * (2 * pb.ndof + pb.nelem) + 16.0 * pb.ndof:.0f}")
+        ms0 = ctx.time_operator(0, _lib.PREC_MIXED, 1, 5)
+        ms1 = ctx.time_operator(1, _lib.PREC_MIXED, 1, 5)
+        log(f"bench: pmc probe: copy 536870912 B read + written; fine fp32 cheb x6 "
+            f"{ms0 * 1e3:.1f} us, level-1 fp32 cheb x6 {ms1 * 1e3:.1f} us")
         ctx.close()
         return
     F = torch.from_numpy(pb.F).to(dev)
@@ -132,7 +324,7 @@ def main():
     r = torch.zeros_like(F)
     torch.cuda.synchronize()
 
-    def step(prec):
+    def step():
         U.zero_()
         torch.cuda.synchronize()
         ctx.assemble()
@@ -140,21 +332,17 @@ def main():
         st = ctx.stats()
         return its, reason, st["setup_seconds"], st["solve_seconds"]
 
-    results = {}
+    results, traces, sens = {}, {}, {}
     for prec, name in [(_lib.PREC_FP64, "fp64"), (_lib.PREC_MIXED, "mixed")]:
         ctx.set_precision(prec)
         for _ in range(args.warmup):
-            step(prec)
+            step()
         torch.cuda.synchronize()
-        if name == "mixed":  # HIP events around the dominant kernel's launches in these steps
-            ctx.time_solve_kernel(True)
         t0 = time.perf_counter()
-        rec = [step(prec) for _ in range(args.steps)]
+        rec = [step() for _ in range(args.steps)]
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
-        if name == "mixed":
-            kt_ms, kt_n = ctx.time_solve_kernel(False)
-        # independent FP64 residual check of the last solve
+        # independent FP64 residual check of the last timed solve
         ctx.apply_K_dev(U.data_ptr(), r.data_ptr())
         res = (torch.linalg.vector_norm(F - r) / torch.linalg.vector_norm(F)).item()
         its = rec[-1][0]
@@ -169,66 +357,111 @@ def main():
         log(f"bench: {name} {wall:.4f} s/solve its={its} relres={res:.2e} "
             f"setup={results[name]['setup_s']:.4f} solve={results[name]['solve_s']:.4f}")
         assert rec[-1][1] == 2 and res <= 1.01e-8, "solve did not reach 1e-8"
-        if name == "fp64":
-            U64 = U.clone()
-    err_u = (torch.linalg.vector_norm(U - U64) / torch.linalg.vector_norm(U64)).item()
-    c64 = ctx.compliance_dev(U64.data_ptr(), F.data_ptr())
-    cmx = ctx.compliance_dev(U.data_ptr(), F.data_ptr())
+        g = torch.empty(pb.nelem, dtype=torch.float64, device=dev)
+        c = ctx.compliance_dev(U.data_ptr(), F.data_ptr(), g.data_ptr())
+        sens[name] = (U.clone(), c, g)
+        # one extra, untimed step with HIP events around every operator launch of the solve
+        U.zero_()
+        torch.cuda.synchronize()
+        ctx.assemble()
+        ctx.time_kernels(True)
+        ctx.solve_dev(F.data_ptr(), U.data_ptr(), rtol=1e-8, maxit=500)
+        kt = ctx.time_kernels(False)
+        traces[name] = (kt, ctx.stats()["solve_seconds"])
 
-    # operator rate and the dominant kernel's roofline (HIP events on the library stream)
-    ndof, ne = pb.ndof, pb.nelem
-    # isolated back-to-back launches (operands largely cache-resident) for the operator rates;
-    # the dominant kernel's roofline uses its launches inside the timed solves (t32c)
+    U64, c64, g64 = sens["fp64"]
+    Umx, cmx, gmx = sens["mixed"]
+    err = {
+        "u_rel_err": (torch.linalg.vector_norm(Umx - U64) / torch.linalg.vector_norm(U64)).item(),
+        "compliance_fp64": c64, "compliance_mixed": cmx,
+        "compliance_rel_err": abs(cmx - c64) / abs(c64),
+        "sensitivity_rel_err_max": (torch.max(torch.abs(gmx - g64)) / torch.max(torch.abs(g64))).item(),
+        "sensitivity_rel_err_l2": (torch.linalg.vector_norm(gmx - g64) / torch.linalg.vector_norm(g64)).item(),
+    }
+
+    # design iteration: MatIntFnc -> FEAssemble -> FESolve -> Compliance + Chain_Filter (mixed)
+    ctx.set_precision(_lib.PREC_MIXED)
+    g = torch.empty(pb.nelem, dtype=torch.float64, device=dev)
+
+    def design_iteration():
+        ctx.set_design_dev(xd.data_ptr())
+        ctx.assemble()
+        ctx.solve_dev(F.data_ptr(), U.data_ptr(), rtol=1e-8, maxit=500)
+        ctx.compliance_dev(U.data_ptr(), F.data_ptr(), g.data_ptr())
+
+    U.zero_()
+    design_iteration()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        U.zero_()
+        design_iteration()
+    torch.cuda.synchronize()
+    t_design = (time.perf_counter() - t0) / args.steps
+
+    # kernel rows of the instrumented solves; the dominant solve kernel by total time
+    rows = {name: summarize_kernel_times(traces[name][0], lv, name) for name in traces}
+    top = rows["mixed"][0]
+    roof = {
+        "kernel": f"{top['kernel']} (level {top['level']}, {top['precision']}, {top['epilogue']})",
+        "bound": "hbm",
+        "achieved": top["achieved_GBs"],
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": top["frac_hbm"],
+        "traffic": read_traffic(f"level{top['level']}_{top['precision']}_cheb")
+        if top["epilogue"] == EPI_NAMES[3] else None,
+        "algorithmic_bytes_per_launch": top["algorithmic_bytes"],
+        "stored_bytes_per_launch": top["stored_bytes"],
+        "ms_per_launch": top["ms_per_launch"],
+        "launches_timed": top["count"],
+        "timing": "HIP events around each launch of the kernel in one untimed mixed solve "
+                  "identical to the timed ones",
+        "top3_solve_kernels": rows["mixed"][:3],
+    }
+    # whole-solve roofline fraction: sum over the solve's kernels of max(bytes/BW, FLOP/peak)
+    whole = {}
+    for name in ("mixed", "fp64"):
+        kt, t_solve = traces[name]
+        s = 4 if name == "mixed" else 8
+        t_ops = sum(op_time_bound(op_work(lv, k["level"], 4 if k["precision"] == 1 else 8,
+                                          k["epilogue"])[0],
+                                  1152.0 * lv.Ne[k["level"]], 4 if k["precision"] == 1 else 8)
+                    * k["count"] for k in kt)
+        cnt = {(k["level"], k["precision"], k["epilogue"]): k["count"] for k in kt}
+        if name == "mixed":
+            n_outer = cnt.get((0, 0, 4), 0)
+            n_vc = cnt.get((0, 1, 2), 0)
+        else:
+            n_outer = cnt.get((0, 0, 1), 0)
+            n_vc = max(cnt.get((0, 0, 2), 0) - 1, 0)
+        t_vec = vector_bytes(lv, name, n_outer, n_vc) / (PEAK_HBM_GBS * 1e9)
+        whole[name] = {
+            "algorithmic_s": t_ops + t_vec, "measured_solve_s": t_solve,
+            "frac": (t_ops + t_vec) / t_solve, "operator_part_s": t_ops, "vector_part_s": t_vec,
+            "outer_iterations": n_outer, "vcycles": n_vc,
+            "note": "sum over the solve's kernels of max(algorithmic bytes / 8 TB/s, FLOP / peak), "
+                    "over the measured solve time (set-up excluded)",
+        }
+    roof["whole_solve"] = whole
+
+    # isolated operator rates (back-to-back launches, operands largely cache-resident)
+    ctx.set_precision(_lib.PREC_FP64)
+    ctx.assemble()
     t64 = ctx.time_operator(0, _lib.PREC_FP64, 0, 30)
+    ctx.set_precision(_lib.PREC_MIXED)
+    ctx.assemble()
     t32 = ctx.time_operator(0, _lib.PREC_MIXED, 0, 30)
-    t1_32 = ctx.time_operator(1, _lib.PREC_MIXED, 1, 30)
-    t32c = kt_ms
-    log(f"bench: fine op fp64 {t64 * 1e3:.1f} us, fp32 {t32 * 1e3:.1f} us (isolated); "
-        f"fp32 + Chebyshev in the timed solves {t32c * 1e3:.1f} us ({kt_n} launches); "
-        f"level-1 stencil fp32 cheb {t1_32 * 1e3:.1f} us (isolated)")
-    # Dominant solve kernel: the FP32 fine operator fused with a Chebyshev step.  Its
-    # algorithmic HBM bytes per launch are u, b, D^-1, d read and d, x written (4 B per DOF
-    # each) plus E (4 B per element).  The element-centric Walsh-Hadamard form needs about
-    # 260 FLOP per element (two 8-point transforms per component and 45 block terms), under
-    # 4 FLOP per HBM byte, far below the FP32 ridge point: the launch is bounded by HBM.  The
-    # node-centric forms apply KE densely (1152 FLOP per element) and are compute-bound.
-    form = ctx.stats()["fine_form"]
-    abytes = 4.0 * (2 * ndof + ne) + 4.0 * 4 * ndof
-    if form >= 2:
-        achieved = abytes / (t32c * 1e-3) / 1e9
-        roof = {
-            "kernel": "k_fine_wht<float, EPI_CHEB> (element-centric fine operator + Chebyshev step)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS,
-            "traffic": read_traffic("k_fine_wht_f32_cheb"),
-        }
-    else:
-        achieved = 1152.0 * ne / (t32c * 1e-3) / 1e12
-        roof = {
-            "kernel": "k_fine_op<float, EPI_CHEB> (node-centric fine operator + Chebyshev step)",
-            "bound": "mfma",
-            "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": read_traffic("k_fine_op_f32_cheb"),
-        }
-    roof.update({
-        "algorithmic_flops_per_launch_dense_ke": 1152.0 * ne,
-        "algorithmic_bytes_per_launch": abytes,
-        "ms_per_launch": t32c,
-        "launches_timed": kt_n,
-        "timing": "HIP events around each launch of the kernel inside the timed mixed solves",
-    })
+    log(f"bench: fine op fp64 {t64 * 1e3:.1f} us, fp32 {t32 * 1e3:.1f} us (isolated); dominant "
+        f"solve kernel {roof['kernel']} {top['ms_per_launch'] * 1e3:.1f} us x {top['count']}; "
+        f"whole-solve roofline frac mixed {whole['mixed']['frac']:.3f} fp64 {whole['fp64']['frac']:.3f}")
 
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")
-        cpu = cpu_baseline(pb, results["fp64"]["its"])
-        log(f"bench: CPU {cpu['value']:.1f} s/solve on {cpu['cores']} threads")
+        cpu = cpu_baseline(results["fp64"]["its"])
+        log(f"bench: CPU {cpu['value']:.2f} s/solve on {cpu['cores']} threads; 1 thread "
+            f"{cpu['one_thread']['s_per_solve_extrapolated']:.1f} s")
 
     mixed = results["mixed"]["s_per_solve"]
     out = {
@@ -255,12 +488,11 @@ def main():
         "mixed_speedup_vs_fp64": results["fp64"]["s_per_solve"] / mixed,
         "fp64": results["fp64"],
         "mixed": results["mixed"],
-        "mixed_vs_fp64": {"u_rel_err": err_u, "compliance_fp64": c64, "compliance_mixed": cmx,
-                          "compliance_rel_err": abs(cmx - c64) / abs(c64)},
-        "operator_mdof_s": {"fp64": ndof / (t64 * 1e-3) / 1e6, "fp32": ndof / (t32 * 1e-3) / 1e6},
-        "operator_us": {"fp64_isolated": t64 * 1e3, "fp32_isolated": t32 * 1e3,
-                        "fp32_cheb_in_solve": t32c * 1e3,
-                        "level1_stencil_fp32_cheb_isolated": t1_32 * 1e3},
+        "mixed_vs_fp64": err,
+        "design_iteration_s": t_design,
+        "operator_mdof_s": {"fp64": pb.ndof / (t64 * 1e-3) / 1e6, "fp32": pb.ndof / (t32 * 1e-3) / 1e6},
+        "operator_us_isolated": {"fp64": t64 * 1e3, "fp32": t32 * 1e3},
+        "solve_kernels": {k: v for k, v in rows.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (cpu["value"] / mixed) if cpu else None,

@@ -271,12 +271,23 @@ int tofe_get_level_lambda(const tofe_ctx *ctx, int level, double *lambda);
 int tofe_time_operator(tofe_ctx *ctx, int level, int precision, int epilogue, int nrep,
                        double *ms);
 
-/* Average duration (ms) of the mixed-precision solves' dominant kernel, the FP32 fine
- * operator fused with a Chebyshev step, as it runs inside the solves: start != 0 begins
- * recording (HIP events around each such launch on the context's stream, up to 512
- * launches); start == 0 stops and returns the average and the number of launches timed.
- * No reference counterpart: a measurement aid for the same timing line as above. */
-int tofe_time_solve_kernel(tofe_ctx *ctx, int start, double *ms_avg, int *count);
+/* Durations of the level operator kernels as they run inside the calls that follow.
+ * start != 0 begins timing: an HIP event pair is recorded around every operator launch on the
+ * context's stream (up to 4096 launches; the fine operator is level 0, a Galerkin stencil
+ * level >= 1).  start == 0 stops and aggregates them by kind: *nkinds receives the number of
+ * kinds, and up to max_kinds of them are written to out.  No reference counterpart: it
+ * instruments the kernels that FESolve's timing line (FEAnalysis.cpp:687-711) only sees in
+ * aggregate. */
+typedef struct tofe_kernel_time {
+  int level;        /* 0 fine operator, l >= 1 Galerkin stencil of level l */
+  int precision;    /* TOFE_PREC_FP64, or TOFE_PREC_MIXED for an FP32 kernel */
+  int epilogue;     /* 0 y = Au; 1 with u.Au; 2 residual b - Au; 3 Chebyshev step;
+                       4 with u.Au and u.b; 5 last Chebyshev step with FP64 output */
+  int count;        /* launches timed */
+  double total_ms;  /* their summed durations */
+} tofe_kernel_time;
+int tofe_time_kernels(tofe_ctx *ctx, int start, tofe_kernel_time *out, int max_kinds,
+                      int *nkinds);
 
 /* ---- on-disk formats ------------------------------------------------------------
  * PETSc binary files as PetscViewerBinaryOpen + VecView / MatView write them (big-endian;

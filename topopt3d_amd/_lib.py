@@ -40,7 +40,7 @@ EXPORTS = (
     "tofe_pc_apply_dev", "tofe_compliance", "tofe_compliance_dev", "tofe_sensitivity",
     "tofe_filter", "tofe_filter_T", "tofe_filter_pattern", "tofe_set_precision",
     "tofe_get_stats", "tofe_get_level_stencil", "tofe_get_level_lambda", "tofe_time_operator",
-    "tofe_time_solve_kernel",
+    "tofe_time_kernels",
     "tofe_set_penalty", "tofe_chain_filter", "tofe_assemble_mass", "tofe_apply_M",
     "tofe_apply_M_dev", "tofe_element_quadratic", "tofe_petsc_vec_write", "tofe_petsc_vec_read",
     "tofe_petsc_aij_write", "tofe_petsc_aij_read", "tofe_mesh_out", "tofe_print_vals",
@@ -70,6 +70,11 @@ class TofeStats(C.Structure):
         ("rnorm", C.c_double), ("bnorm", C.c_double),
         ("setup_seconds", C.c_double), ("solve_seconds", C.c_double), ("fine_form", C.c_int),
     ]
+
+
+class TofeKernelTime(C.Structure):
+    _fields_ = [("level", C.c_int), ("precision", C.c_int), ("epilogue", C.c_int),
+                ("count", C.c_int), ("total_ms", C.c_double)]
 
 
 class TofeError(RuntimeError):
@@ -121,7 +126,7 @@ def lib():
         "tofe_get_level_stencil": ([vp, C.c_int, dp, i64p], C.c_int),
         "tofe_get_level_lambda": ([vp, C.c_int, dp], C.c_int),
         "tofe_time_operator": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, dp], C.c_int),
-        "tofe_time_solve_kernel": ([vp, C.c_int, dp, C.POINTER(C.c_int)], C.c_int),
+        "tofe_time_kernels": ([vp, C.c_int, C.POINTER(TofeKernelTime), C.c_int, ip], C.c_int),
         "tofe_set_penalty": ([vp, C.c_double, C.c_double], C.c_int),
         "tofe_chain_filter": ([vp, dp, dp], C.c_int),
         "tofe_assemble_mass": ([vp, C.c_double, dp], C.c_int),
@@ -443,12 +448,18 @@ class Context:
         self._check(lib().tofe_time_operator(self.h, level, precision, epilogue, nrep, C.byref(ms)))
         return ms.value
 
-    def time_solve_kernel(self, start):
-        """start=True: begin timing the FP32 fine Chebyshev launches of the following solves;
-        start=False: stop and return (average ms per launch, number of launches)."""
-        ms, n = C.c_double(), C.c_int()
-        self._check(lib().tofe_time_solve_kernel(self.h, 1 if start else 0, C.byref(ms), C.byref(n)))
-        return ms.value, n.value
+    def time_kernels(self, start):
+        """start=True: begin timing every operator launch of the following calls;
+        start=False: stop and return a list of dicts (level, precision, epilogue, count,
+        total_ms), one per kernel kind."""
+        n = C.c_int()
+        if start:
+            self._check(lib().tofe_time_kernels(self.h, 1, None, 0, C.byref(n)))
+            return None
+        out = (TofeKernelTime * 256)()
+        self._check(lib().tofe_time_kernels(self.h, 0, out, 256, C.byref(n)))
+        return [{k: getattr(out[j], k) for k, _ in TofeKernelTime._fields_}
+                for j in range(min(n.value, 256))]
 
     def level_lambda(self, level):
         lam = C.c_double()

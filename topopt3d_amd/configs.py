@@ -106,3 +106,44 @@ def config2(Emin=1e-9) -> Problem:
     5-level V(2,2) Chebyshev GMG.  The densities are filtered inside tofe_set_design."""
     return cantilever((192, 96, 96), seed=2, lo=0.01, hi=1.0, rmin=1.5, penal=3.0, Emin=Emin,
                       mg_levels=5, name="cantilever-192x96x96")
+
+
+def pressure_load(nel, axis: int, side: int, value) -> np.ndarray:
+    """Load vector of a uniform pressure on the face coord[axis] = side * nel[axis] with the
+    reference's per-face-node semantics (Inputs.cpp:126-147, 174-199): every node of every
+    selected element face receives the full value vector, so a face node is loaded once per
+    face it belongs to (4 inside the face, 2 on its edges, 1 at its corners)."""
+    N = tuple(n + 1 for n in nel)
+    value = np.asarray(value, np.float64)
+    F = np.zeros((N[2], N[1], N[0], 3))
+    others = [a for a in range(3) if a != axis]
+    cnt = np.ones((N[2], N[1], N[0]))
+    for a in others:
+        m = np.full(N[a], 2.0)
+        m[0] = m[-1] = 1.0
+        shape = [1, 1, 1]
+        shape[2 - a] = N[a]
+        cnt = cnt * m.reshape(shape)
+    sl = [slice(None)] * 3
+    sl[2 - axis] = side * nel[axis]
+    F[tuple(sl)] = cnt[tuple(sl)][..., None] * value
+    return F.reshape(-1)
+
+
+def column(nel, seed=3, lo=0.1, hi=1.0, rmin=2.0, penal=3.0, Emin=1e-9, mg_levels=0,
+           name="column") -> Problem:
+    """Linear-buckling column: the z = 0 face clamped, a uniform unit pressure (0, 0, -1)
+    on the top face z = nz in the reference's per-face-node semantics, densities
+    U[lo, hi) from SplitMix64(seed)."""
+    nx, ny, nz = nel
+    fixed = face_fixed_dof(nel, 2, 0)
+    F = pressure_load(nel, 2, 1, (0.0, 0.0, -1.0))
+    x = uniform(seed, nx * ny * nz, lo, hi)
+    return Problem(name, tuple(nel), fixed, F, x, rmin, penal, Emin, mg_levels)
+
+
+def config3(Emin=1e-9) -> Problem:
+    """128x64x64 column (1,635,075 DOF), bottom clamped, unit pressure on the top face,
+    densities U[0.1,1] seed 3, filter R = 2.0, p = 3, 5-level GMG; 6 buckling modes."""
+    return column((128, 64, 64), seed=3, lo=0.1, hi=1.0, rmin=2.0, penal=3.0, Emin=Emin,
+                  mg_levels=5, name="column-128x64x64")

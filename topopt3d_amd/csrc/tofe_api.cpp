@@ -17,6 +17,8 @@
 #include <memory>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "host_setup.hpp"
 #include "ke_orbits.hpp"
 #include "ke_wht.hpp"
@@ -25,6 +27,15 @@
 using namespace tofe;
 
 namespace {
+
+// roctx range around one C-ABI call (or one phase of it): rocprofv3 --marker-trace shows
+// which kernels each entry point launched
+struct Range {
+  explicit Range(const char *name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range &) = delete;
+  Range &operator=(const Range &) = delete;
+};
 
 template <typename T>
 struct DBuf {
@@ -164,10 +175,12 @@ struct tofe_ctx {
                                                 // (TOFE_FINE_TILE="<shape>:<zlen>" sets both)
   bool ke_sym = false;  // KE matches the cube-orbit table: fine operator keeps KE in registers
   bool ke_wht = false;  // KE is block diagonal in the Walsh-Hadamard basis: k_fine_wht
-  // tofe_time_solve_kernel: event pairs around the FP32 fine Chebyshev launches of solves
+  // tofe_time_kernels: an event pair around each operator launch on the main stream, with
+  // its kind (level, precision, epilogue)
   bool kt_on = false;
   size_t kt_n = 0;
   std::vector<hipEvent_t> kt_ev;
+  std::vector<int> kt_kind;
   bool ke_wht_cube = false;  // ... with the 9-value block structure of a cubic element
   int wht_zlen = 8, wht_zlen32 = 8;  // its z-chunk lengths, FP64 and FP32
   int precision = TOFE_PREC_FP64;
@@ -264,6 +277,21 @@ FineOp<float> fine_of<float>(tofe_ctx *c)
   return f;
 }
 
+// tofe_time_kernels: record the start event of an operator launch on the main stream
+bool kernel_timer_begin(tofe_ctx *c, hipStream_t s, int level, bool fp32, Epi epi)
+{
+  if (!c->kt_on || s != c->stream || 2 * c->kt_n + 1 >= c->kt_ev.size()) return false;
+  TOFE_HIP(hipEventRecord(c->kt_ev[2 * c->kt_n], s));
+  c->kt_kind[c->kt_n] = (level * 2 + (fp32 ? 1 : 0)) * 8 + (int)epi;
+  return true;
+}
+
+void kernel_timer_end(tofe_ctx *c, hipStream_t s)
+{
+  TOFE_HIP(hipEventRecord(c->kt_ev[2 * c->kt_n + 1], s));
+  ++c->kt_n;
+}
+
 template <typename T>
 void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
                  hipStream_t s = nullptr)
@@ -271,23 +299,18 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
   if (!s) s = c->stream;
   if (l == 0) {
     const FineOp<T> f = fine_of<T>(c);
-    // tofe_time_solve_kernel: bracket the FP32 Chebyshev launches on the main stream
-    const bool timed = sizeof(T) == 4 && epi == EPI_CHEB && c->kt_on && s == c->stream &&
-                       2 * c->kt_n + 1 < c->kt_ev.size();
-    if (timed) TOFE_HIP(hipEventRecord(c->kt_ev[2 * c->kt_n], s));
+    const bool timed = kernel_timer_begin(c, s, l, sizeof(T) == 4, epi);
     if (c->fine_mfma) fine_mfma<T>(s, f, u, epi, ep);
     else if (f.wht) fine_wht<T>(s, f, u, epi, ep);
     else fine_op<T>(s, f, u, epi, ep);
-    if (timed) {
-      TOFE_HIP(hipEventRecord(c->kt_ev[2 * c->kt_n + 1], s));
-      ++c->kt_n;
-    }
+    if (timed) kernel_timer_end(c, s);
   } else {
     StencilOp<T> op;
     auto &lv = *levels<T>(c)[l];
     op.S = lv.S.p;
     op.Nx = lv.g.Nx; op.Ny = lv.g.Ny; op.Nz = lv.g.Nz;
     op.split_nodes = c->split_nodes;
+    const bool timed = kernel_timer_begin(c, s, l, sizeof(T) == 4, epi);
     // FP32 levels keep the upper half of the (exactly symmetric) Galerkin stencil; FP64 levels
     // read their upper-half copy once the FP64 set-up has formed it (half the stencil traffic)
     if constexpr (sizeof(T) == 4) {
@@ -298,6 +321,7 @@ void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
     } else {
       stencil_op<T>(s, op, u, epi, ep);
     }
+    if (timed) kernel_timer_end(c, s);
   }
 }
 
@@ -828,6 +852,7 @@ void coarse_inverse_finish(tofe_ctx *c)
 
 void assemble(tofe_ctx *c)
 {
+  Range range(c->precision == TOFE_PREC_MIXED ? "assemble:mixed" : "assemble:fp64");
   const double t0 = now_s();
   const hipStream_t s = c->stream;
   const int L = c->L;
@@ -894,9 +919,10 @@ void check_ctx(const tofe_ctx *c)
 }
 
 template <typename F>
-int guarded(tofe_ctx *c, F &&f)
+int guarded(tofe_ctx *c, const char *name, F &&f)
 {
   if (!c) return TOFE_ERR_ARG;
+  Range range(name);
   try {
     check_ctx(c);
     f();
@@ -932,19 +958,24 @@ void upload_design(tofe_ctx *c, const double *dx)
   c->assembled = false;
 }
 
-// Chain_Filter for dfdE, device to device (Material_Interpolation.cpp:164-183);
-// g is overwritten when the maximum length scale is on.
-void chain_dev(tofe_ctx *c, double *g, double *out)
+// Chain_Filter for dfdE, device to device (Material_Interpolation.cpp:164-183), with the
+// element buffers t1, t2 as scratch of the maximum-length-scale terms.
+void chain_dev(tofe_ctx *c, const double *g, double *out, double *t1, double *t2)
 {
   const hipStream_t s = c->stream;
   if (c->has_vd) {
-    mul(s, g, c->rho.p, c->ebuf3.p, c->ne);
-    rfilter_T(s, rfilter_tpl(c), c->ebuf3.p, c->ebuf2.p);
-    chain_mid(s, g, c->y.p, c->rhoq.p, c->ebuf2.p, c->ebuf3.p, c->ne);
-    filter_apply_T(s, filter_tpl(c), c->filt_inv.p, c->ebuf3.p, out);
+    mul(s, g, c->rho.p, t1, c->ne);
+    rfilter_T(s, rfilter_tpl(c), t1, t2);
+    chain_mid(s, g, c->y.p, c->rhoq.p, t2, t1, c->ne);
+    filter_apply_T(s, filter_tpl(c), c->filt_inv.p, t1, out);
   } else {
     filter_apply_T(s, filter_tpl(c), c->filt_inv.p, g, out);
   }
+}
+
+void chain_dev(tofe_ctx *c, const double *g, double *out)
+{
+  chain_dev(c, g, out, c->ebuf3.p, c->ebuf2.p);
 }
 
 void need_assembled(tofe_ctx *c)
@@ -959,6 +990,7 @@ void solve_dev(tofe_ctx *c, const double *dF, double *dU, int nrhs, double rtol,
 {
   need_assembled(c);
   if (nrhs < 1 || !dF || !dU) throw Error(TOFE_ERR_ARG, "bad solve arguments");
+  Range range(c->built_precision == TOFE_PREC_MIXED ? "solve:mixed" : "solve:fp64");
   const double t0 = now_s();
   int k = 0, rs = 0;
   double rn = 0, bn = 0;
@@ -991,6 +1023,7 @@ extern "C" {
 int tofe_create(const tofe_desc *desc, tofe_ctx **out)
 {
   if (!desc || !out) return TOFE_ERR_ARG;
+  Range range("tofe_create");
   *out = nullptr;
   std::unique_ptr<tofe_ctx> c(new tofe_ctx());
   c->desc = *desc;
@@ -1315,6 +1348,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
 int tofe_destroy(tofe_ctx *c)
 {
   if (!c) return TOFE_OK;
+  Range range("tofe_destroy");
   (void)hipSetDevice(c->device);
   delete c;  // ~tofe_ctx synchronises and releases streams, events and handles
   return TOFE_OK;
@@ -1341,7 +1375,7 @@ int tofe_get_ke(const tofe_ctx *c, double *ke)
 
 int tofe_set_design(tofe_ctx *c, const double *x)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_set_design", [&] {
     if (!x) throw Error(TOFE_ERR_ARG, "null design");
     h2d(c, c->x.p, x, c->ne);
     upload_design(c, c->x.p);
@@ -1350,7 +1384,7 @@ int tofe_set_design(tofe_ctx *c, const double *x)
 
 int tofe_set_design_dev(tofe_ctx *c, const double *dx)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_set_design_dev", [&] {
     if (!dx) throw Error(TOFE_ERR_ARG, "null design");
     upload_design(c, dx);
   });
@@ -1358,7 +1392,7 @@ int tofe_set_design_dev(tofe_ctx *c, const double *dx)
 
 int tofe_set_E(tofe_ctx *c, const double *E)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_set_E", [&] {
     if (!E) throw Error(TOFE_ERR_ARG, "null E");
     h2d(c, c->E.p, E, c->ne);
     convert<double, float>(c->stream, c->E.p, c->E32.p, c->ne);
@@ -1371,7 +1405,7 @@ int tofe_set_E(tofe_ctx *c, const double *E)
 int tofe_get_field(const tofe_ctx *cc, int field, double *out)
 {
   tofe_ctx *c = const_cast<tofe_ctx *>(cc);
-  return guarded(c, [&] {
+  return guarded(c, "tofe_get_field", [&] {
     if (!out) throw Error(TOFE_ERR_ARG, "null output");
     const double *src = nullptr;
     int64_t n = c->ne;
@@ -1407,7 +1441,7 @@ int tofe_get_field(const tofe_ctx *cc, int field, double *out)
 
 int tofe_assemble(tofe_ctx *c)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_assemble", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     assemble(c);
   });
@@ -1416,13 +1450,13 @@ int tofe_assemble(tofe_ctx *c)
 int tofe_solve_dev(tofe_ctx *c, const double *dF, double *dU, int nrhs, double rtol, int maxit,
                    int *its, int *reason)
 {
-  return guarded(c, [&] { solve_dev(c, dF, dU, nrhs, rtol, maxit, its, reason); });
+  return guarded(c, "tofe_solve_dev", [&] { solve_dev(c, dF, dU, nrhs, rtol, maxit, its, reason); });
 }
 
 int tofe_solve(tofe_ctx *c, const double *F, double *U, int nrhs, double rtol, int maxit,
                int *its, int *reason)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_solve", [&] {
     if (!F || !U || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad solve arguments");
     int k = 0, rs = 0;
     for (int i = 0; i < nrhs; ++i) {
@@ -1438,7 +1472,7 @@ int tofe_solve(tofe_ctx *c, const double *F, double *U, int nrhs, double rtol, i
 
 int tofe_apply_K_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_apply_K_dev", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!dX || !dY || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
     for (int i = 0; i < nrhs; ++i) {
@@ -1452,7 +1486,7 @@ int tofe_apply_K_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
 
 int tofe_apply_K(tofe_ctx *c, const double *X, double *Y, int nrhs)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_apply_K", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!X || !Y || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
     for (int i = 0; i < nrhs; ++i) {
@@ -1467,7 +1501,7 @@ int tofe_apply_K(tofe_ctx *c, const double *X, double *Y, int nrhs)
 
 int tofe_assemble_mass(tofe_ctx *c, double density, const double *mlump)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_assemble_mass", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!(density > 0)) throw Error(TOFE_ERR_ARG, "density must be positive");
     // elemSize = 1 * dx * dy * dz (RecMesh.cpp:529-533); m0 = 1/2^3/3 * elemSize * density
@@ -1491,7 +1525,7 @@ int tofe_assemble_mass(tofe_ctx *c, double density, const double *mlump)
 
 int tofe_apply_M_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_apply_M_dev", [&] {
     if (!c->has_mass) throw Error(TOFE_ERR_STATE, "no mass matrix (call tofe_assemble_mass)");
     if (!dX || !dY || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
     diag_mul(c->stream, c->Mdiag.p, dX, dY, c->ndof, nrhs);
@@ -1501,7 +1535,7 @@ int tofe_apply_M_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
 
 int tofe_apply_M(tofe_ctx *c, const double *X, double *Y, int nrhs)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_apply_M", [&] {
     if (!c->has_mass) throw Error(TOFE_ERR_STATE, "no mass matrix (call tofe_assemble_mass)");
     if (!X || !Y || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
     for (int i = 0; i < nrhs; ++i) {
@@ -1514,7 +1548,7 @@ int tofe_apply_M(tofe_ctx *c, const double *X, double *Y, int nrhs)
 
 int tofe_element_quadratic(tofe_ctx *c, const double *phi, int nvec, double *kq, double *mq)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_element_quadratic", [&] {
     if (!phi || !kq || nvec < 1) throw Error(TOFE_ERR_ARG, "bad arguments");
     const size_t nd = (size_t)c->ndof * nvec, ne = (size_t)c->ne * nvec;
     DBuf<double> dphi, dk, dm;
@@ -1531,7 +1565,7 @@ int tofe_element_quadratic(tofe_ctx *c, const double *phi, int nvec, double *kq,
 
 int tofe_set_stress_dev(tofe_ctx *c, const double *dU)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_set_stress_dev", [&] {
     if (!dU) throw Error(TOFE_ERR_ARG, "null displacement");
     if (!c->Hs.p) c->Hs.alloc((size_t)64 * c->ne);
     stress_h(c->stream, dU, c->desc.nx, c->desc.ny, c->desc.nz, c->shape, c->Hs.p);
@@ -1542,7 +1576,7 @@ int tofe_set_stress_dev(tofe_ctx *c, const double *dU)
 
 int tofe_set_stress(tofe_ctx *c, const double *U)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_set_stress", [&] {
     if (!U) throw Error(TOFE_ERR_ARG, "null displacement");
     h2d(c, c->vp.p, U, c->ndof);
     if (!c->Hs.p) c->Hs.alloc((size_t)64 * c->ne);
@@ -1554,7 +1588,7 @@ int tofe_set_stress(tofe_ctx *c, const double *U)
 
 int tofe_apply_Ks_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_apply_Ks_dev", [&] {
     if (!c->has_stress) throw Error(TOFE_ERR_STATE, "no stress state (call tofe_set_stress)");
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!dX || !dY || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
@@ -1567,7 +1601,7 @@ int tofe_apply_Ks_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
 
 int tofe_apply_Ks(tofe_ctx *c, const double *X, double *Y, int nrhs)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_apply_Ks", [&] {
     if (!c->has_stress) throw Error(TOFE_ERR_STATE, "no stress state (call tofe_set_stress)");
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!X || !Y || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
@@ -1582,7 +1616,7 @@ int tofe_apply_Ks(tofe_ctx *c, const double *X, double *Y, int nrhs)
 
 int tofe_stability_terms(tofe_ctx *c, const double *phi, int nvec, double *pksp, double *dksdu)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_stability_terms", [&] {
     if (!phi || nvec < 1) throw Error(TOFE_ERR_ARG, "bad arguments");
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     const size_t nd = (size_t)c->ndof * nvec, ne = (size_t)c->ne * nvec;
@@ -1609,7 +1643,7 @@ int tofe_stability_terms(tofe_ctx *c, const double *phi, int nvec, double *pksp,
 
 int tofe_element_bilinear(tofe_ctx *c, const double *A, int nvec, const double *B, double *out)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_element_bilinear", [&] {
     if (!A || !B || !out || nvec < 1) throw Error(TOFE_ERR_ARG, "bad arguments");
     const size_t nd = (size_t)c->ndof * nvec, ne = (size_t)c->ne * nvec;
     DBuf<double> da, dout;
@@ -1625,7 +1659,7 @@ int tofe_element_bilinear(tofe_ctx *c, const double *A, int nvec, const double *
 
 int tofe_pc_apply_dev(tofe_ctx *c, const double *dR, double *dZ, int nrhs)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_pc_apply_dev", [&] {
     need_assembled(c);
     if (!dR || !dZ || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad pc arguments");
     for (int i = 0; i < nrhs; ++i) precondition(c, dR + (size_t)i * c->ndof, dZ + (size_t)i * c->ndof);
@@ -1635,7 +1669,7 @@ int tofe_pc_apply_dev(tofe_ctx *c, const double *dR, double *dZ, int nrhs)
 
 int tofe_pc_apply(tofe_ctx *c, const double *R, double *Z, int nrhs)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_pc_apply", [&] {
     need_assembled(c);
     if (!R || !Z || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad pc arguments");
     for (int i = 0; i < nrhs; ++i) {
@@ -1649,7 +1683,7 @@ int tofe_pc_apply(tofe_ctx *c, const double *R, double *Z, int nrhs)
 int tofe_compliance_dev(tofe_ctx *c, const double *dU, const double *dF, double *cval,
                         double *d_dcdx)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_compliance_dev", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!dU || !dF) throw Error(TOFE_ERR_ARG, "bad compliance arguments");
     if (cval) {
@@ -1668,7 +1702,7 @@ int tofe_compliance_dev(tofe_ctx *c, const double *dU, const double *dF, double 
 
 int tofe_compliance(tofe_ctx *c, const double *U, const double *F, double *cval, double *dcdx)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_compliance", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!U || !F) throw Error(TOFE_ERR_ARG, "bad compliance arguments");
     h2d(c, c->vp.p, U, c->ndof);
@@ -1690,7 +1724,7 @@ int tofe_compliance(tofe_ctx *c, const double *U, const double *F, double *cval,
 
 int tofe_sensitivity(tofe_ctx *c, const double *U, double *g)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_sensitivity", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!U || !g) throw Error(TOFE_ERR_ARG, "bad arguments");
     h2d(c, c->vp.p, U, c->ndof);
@@ -1701,7 +1735,7 @@ int tofe_sensitivity(tofe_ctx *c, const double *U, double *g)
 
 int tofe_set_penalty(tofe_ctx *c, double penal, double vd_penal)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_set_penalty", [&] {
     if (!(penal > 0) || !(vd_penal > 0)) throw Error(TOFE_ERR_ARG, "penalties must be positive");
     c->desc.penal = c->sp.penal = penal;
     c->desc.vd_penal = c->sp.vd_penal = vd_penal;
@@ -1710,7 +1744,7 @@ int tofe_set_penalty(tofe_ctx *c, double penal, double vd_penal)
 
 int tofe_chain_filter(tofe_ctx *c, double *dfdV, double *dfdE)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_chain_filter", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (dfdV) {
       h2d(c, c->ebuf.p, dfdV, c->ne);
@@ -1727,7 +1761,7 @@ int tofe_chain_filter(tofe_ctx *c, double *dfdV, double *dfdE)
 
 int tofe_filter(tofe_ctx *c, const double *x, double *rho)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_filter", [&] {
     if (!x || !rho) throw Error(TOFE_ERR_ARG, "bad arguments");
     h2d(c, c->ebuf.p, x, c->ne);
     filter_apply(c->stream, filter_tpl(c), c->filt_inv.p, c->ebuf.p, c->ebuf2.p);
@@ -1737,7 +1771,7 @@ int tofe_filter(tofe_ctx *c, const double *x, double *rho)
 
 int tofe_filter_T(tofe_ctx *c, const double *g, double *gx)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_filter_T", [&] {
     if (!g || !gx) throw Error(TOFE_ERR_ARG, "bad arguments");
     h2d(c, c->ebuf.p, g, c->ne);
     filter_apply_T(c->stream, filter_tpl(c), c->filt_inv.p, c->ebuf.p, c->ebuf2.p);
@@ -1806,7 +1840,7 @@ void write_vec(tofe_ctx *c, const std::string &path, const double *dsrc, int64_t
 
 int tofe_filter_pattern(tofe_ctx *c, int64_t *rowptr, int32_t *col, double *w)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_filter_pattern", [&] {
     if (!rowptr) throw Error(TOFE_ERR_ARG, "null rowptr");
     template_pattern(c, filter_tpl(c), c->filt_inv.p, rowptr, col, w);
   });
@@ -1814,7 +1848,7 @@ int tofe_filter_pattern(tofe_ctx *c, int64_t *rowptr, int32_t *col, double *w)
 
 int tofe_mesh_out(tofe_ctx *c, const char *dir, const double origin[3], int index_bytes)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_mesh_out", [&] {
     if (!dir || (index_bytes != 4 && index_bytes != 8)) throw Error(TOFE_ERR_ARG, "bad arguments");
     const std::string d(dir);
     const double zero[3] = {0, 0, 0};
@@ -1844,7 +1878,7 @@ int tofe_mesh_out(tofe_ctx *c, const char *dir, const double origin[3], int inde
 int tofe_print_vals(tofe_ctx *c, const char *dir, const char *suffix, const double *U,
                     int index_bytes)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_print_vals", [&] {
     if (!dir || (index_bytes != 4 && index_bytes != 8)) throw Error(TOFE_ERR_ARG, "bad arguments");
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     const std::string d(dir), sfx(suffix ? suffix : "");
@@ -1880,7 +1914,7 @@ int tofe_get_stats(const tofe_ctx *c, tofe_stats *st)
 
 int tofe_get_level_stencil(tofe_ctx *c, int level, double *out, int64_t *nnode_l)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_get_level_stencil", [&] {
     if (level < 0 || level >= c->L) throw Error(TOFE_ERR_ARG, "bad level");
     if (!c->assembled) throw Error(TOFE_ERR_STATE, "not assembled");
     const int64_t nn = c->grids[level].nn();
@@ -1895,7 +1929,7 @@ int tofe_get_level_stencil(tofe_ctx *c, int level, double *out, int64_t *nnode_l
 
 int tofe_time_operator(tofe_ctx *c, int level, int precision, int epilogue, int nrep, double *ms)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_time_operator", [&] {
     if (level < 0 || level >= c->L || nrep < 1 || !ms) throw Error(TOFE_ERR_ARG, "bad arguments");
     need_assembled(c);
     const bool f32 = precision == TOFE_PREC_MIXED;
@@ -1939,30 +1973,48 @@ int tofe_time_operator(tofe_ctx *c, int level, int precision, int epilogue, int 
   });
 }
 
-int tofe_time_solve_kernel(tofe_ctx *c, int start, double *ms_avg, int *count)
+int tofe_time_kernels(tofe_ctx *c, int start, tofe_kernel_time *out, int max_kinds, int *nkinds)
 {
-  return guarded(c, [&] {
+  return guarded(c, "tofe_time_kernels", [&] {
     if (start) {
-      constexpr size_t kPairs = 512;
+      constexpr size_t kLaunches = 4096;
       if (c->kt_ev.empty()) {
-        c->kt_ev.assign(2 * kPairs, nullptr);
+        c->kt_ev.assign(2 * kLaunches, nullptr);
+        c->kt_kind.assign(kLaunches, 0);
         for (auto &e : c->kt_ev) TOFE_HIP(hipEventCreate(&e));
       }
       c->kt_n = 0;
       c->kt_on = true;
       return;
     }
-    if (!ms_avg || !count) throw Error(TOFE_ERR_ARG, "bad arguments");
+    if (!nkinds || (max_kinds > 0 && !out)) throw Error(TOFE_ERR_ARG, "bad arguments");
     c->kt_on = false;
-    double tot = 0;
+    std::vector<int> kinds;
+    std::vector<int> cnt;
+    std::vector<double> tot;
     for (size_t k = 0; k < c->kt_n; ++k) {
       TOFE_HIP(hipEventSynchronize(c->kt_ev[2 * k + 1]));
       float t = 0;
       TOFE_HIP(hipEventElapsedTime(&t, c->kt_ev[2 * k], c->kt_ev[2 * k + 1]));
-      tot += t;
+      size_t j = 0;
+      while (j < kinds.size() && kinds[j] != c->kt_kind[k]) ++j;
+      if (j == kinds.size()) {
+        kinds.push_back(c->kt_kind[k]);
+        cnt.push_back(0);
+        tot.push_back(0.0);
+      }
+      ++cnt[j];
+      tot[j] += t;
     }
-    *count = (int)c->kt_n;
-    *ms_avg = c->kt_n ? tot / (double)c->kt_n : 0.0;
+    *nkinds = (int)kinds.size();
+    for (int j = 0; j < (int)kinds.size() && j < max_kinds; ++j) {
+      tofe_kernel_time &o = out[j];
+      o.level = kinds[j] / 16;
+      o.precision = (kinds[j] / 8) % 2 ? TOFE_PREC_MIXED : TOFE_PREC_FP64;
+      o.epilogue = kinds[j] % 8;
+      o.count = cnt[j];
+      o.total_ms = tot[j];
+    }
   });
 }
 

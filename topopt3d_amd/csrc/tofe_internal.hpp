@@ -102,6 +102,7 @@ struct StencilOp {
 
 constexpr int kRedBlocks = 1024;   // grid of the grid-stride reduction kernels
 
+
 // ---- fine operator (k_fine.hip) -------------------------------------------
 template <typename T>
 int fine_op_blocks(const FineOp<T> &op);

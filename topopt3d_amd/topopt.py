@@ -18,8 +18,19 @@ from ._lib import Context
 
 
 class BCType(IntEnum):
+    """Boundary-condition kinds of the input file's [BC] section (Inputs.cpp:148-217)."""
     SUPPORT = 0
     LOAD = 1
+    PRESSURE = 2
+    MASS = 3
+    SPRING = 4
+    EIGEN = 5
+
+
+# Node lists of the six faces of a hexahedral element in its local node order
+# (Get_Faces, Inputs.cpp:62-91).
+FACES = np.array([[0, 1, 2, 3], [4, 5, 6, 7], [0, 1, 5, 4], [3, 2, 6, 7], [0, 4, 7, 3],
+                  [1, 5, 6, 2]])
 
 
 class TopOpt:
@@ -40,6 +51,12 @@ class TopOpt:
         self.supports = np.zeros((0, 3), bool)
         self.loadNode = np.zeros(0, np.int64)
         self.loads = np.zeros((0, 3))
+        self.massNode = np.zeros(0, np.int64)
+        self.masses = np.zeros((0, 3))
+        self.springNode = np.zeros(0, np.int64)
+        self.springs = np.zeros((0, 3))
+        self.eigenSuppNode = np.zeros(0, np.int64)
+        self.eigenSupports = np.zeros((0, 3), bool)
         self.mg_levels = 0
         self.smoother = _lib.SMOOTHER_CHEBYSHEV
         self.precision = _lib.PREC_FP64
@@ -73,28 +90,57 @@ class TopOpt:
         self.Rmin = float(Rmin)
         self.Rmax = float(Rmax)
 
-    # -- boundary conditions (Inputs.cpp:105-223, SUPPORT and LOAD) -------------
+    # -- boundary conditions (Inputs.cpp:105-223) ----------------------------------
     def Set_BC(self, center, radius, limits, values, TYPE):
-        """Select nodes with sum(((x_i - c_i)/r_i)^2) <= 1 inside the box limits."""
+        """Select nodes with sum(((x_i - c_i)/r_i)^2) <= 1 inside the box limits, or, for
+        PRESSURE, element faces whose centre (mean of the face's four node coordinates)
+        satisfies the same test; every node of every selected face then receives the full
+        value vector as a load, so a node shared by several selected faces is loaded once per
+        face (Inputs.cpp:114-147, 174-199).  Returns the selected nodes (PRESSURE: with
+        repetitions, in face order)."""
         center = np.asarray(center, np.float64)
         radius = np.asarray(radius, np.float64)
         limits = np.asarray(limits, np.float64).reshape(3, 2)
         values = np.asarray(values, np.float64)
-        d = np.zeros(self.nNode)
-        valid = np.ones(self.nNode, bool)
+        TYPE = BCType(TYPE)
+        if TYPE != BCType.PRESSURE:
+            pts = self.node
+        else:
+            # face centres, element-major then face (index el * 6 + face)
+            pts = np.zeros((self.nElem * 6, 3))
+            for nd in range(4):
+                pts += self.node[self.element[:, FACES[:, nd]]].reshape(-1, 3)
+            pts /= 4
+        d = np.zeros(pts.shape[0])
+        valid = np.ones(pts.shape[0], bool)
         for i in range(3):
-            d += ((self.node[:, i] - center[i]) / radius[i]) ** 2
-            valid &= (self.node[:, i] >= limits[i, 0]) & (self.node[:, i] <= limits[i, 1])
+            d += ((pts[:, i] - center[i]) / radius[i]) ** 2
+            valid &= (pts[:, i] >= limits[i, 0]) & (pts[:, i] <= limits[i, 1])
         valid &= d <= 1
-        newNode = np.flatnonzero(valid)
+        sel = np.flatnonzero(valid)
+        if TYPE == BCType.PRESSURE:
+            el, face = sel // 6, sel % 6
+            newNode = self.element[el[:, None], FACES[face]].reshape(-1)
+            self.loads = np.vstack([self.loads, np.tile(values, (newNode.size, 1))])
+            self.loadNode = np.concatenate([self.loadNode, newNode])
+            return newNode
+        newNode = sel
         if TYPE == BCType.SUPPORT:
             self.supports = np.vstack([self.supports, np.tile(values.astype(bool), (newNode.size, 1))])
             self.suppNode = np.concatenate([self.suppNode, newNode])
+        elif TYPE == BCType.EIGEN:
+            self.eigenSupports = np.vstack([self.eigenSupports,
+                                            np.tile(values.astype(bool), (newNode.size, 1))])
+            self.eigenSuppNode = np.concatenate([self.eigenSuppNode, newNode])
         elif TYPE == BCType.LOAD:
             self.loads = np.vstack([self.loads, np.tile(values, (newNode.size, 1))])
             self.loadNode = np.concatenate([self.loadNode, newNode])
-        else:
-            raise ValueError("unsupported BC type")
+        elif TYPE == BCType.MASS:
+            self.masses = np.vstack([self.masses, np.tile(values, (newNode.size, 1))])
+            self.massNode = np.concatenate([self.massNode, newNode])
+        elif TYPE == BCType.SPRING:
+            self.springs = np.vstack([self.springs, np.tile(values, (newNode.size, 1))])
+            self.springNode = np.concatenate([self.springNode, newNode])
         return newNode
 
     # -- FE set-up (FEAnalysis.cpp:215-424) -----------------------------------
@@ -111,6 +157,20 @@ class TopOpt:
             for j in range(3):
                 self.F[3 * n + j] += l[j]   # F[3 loadNode + j] += loads (349-356)
         self.U = np.zeros(ndof)
+        # lumped point masses MLump (FEAnalysis.cpp:374-384), spring stiffnesses spKVec
+        # (360-368), extra eigenproblem supports eigenFixedDof (249-255)
+        self.MLump = np.zeros(ndof)
+        for n, m in zip(self.massNode, self.masses):
+            self.MLump[3 * n:3 * n + 3] += m
+        self.spKVec = np.zeros(ndof)
+        for n, k in zip(self.springNode, self.springs):
+            self.spKVec[3 * n:3 * n + 3] += k
+        self.eigen_fixed_dof = np.zeros(ndof, np.uint8)
+        for n, s in zip(self.eigenSuppNode, self.eigenSupports):
+            for j in range(3):
+                if s[j]:
+                    self.eigen_fixed_dof[3 * n + j] = 1
+        self.eigenFixedDof = np.flatnonzero(self.eigen_fixed_dof)
         if self.ctx is not None:
             self.ctx.close()
         self.ctx = Context(self.Nel, h=self.dx, E0=self.E0, nu=self.Nu0, Emin=self.eps,
