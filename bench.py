@@ -242,6 +242,9 @@ def summarize_trace(d):
                        row.get("Function") or row.get("Message") or row.get("Operation", "")))
     # innermost range containing the kernel
     ranges.sort(key=lambda r: r[1] - r[0])
+    ncalls = {}
+    for _, _, nm in ranges:
+        ncalls[nm] = ncalls.get(nm, 0) + 1
     phases = {}
     for row in csv.DictReader(open(kfile[0])):
         t0, t1 = int(row["Start_Timestamp"]), int(row["End_Timestamp"])
@@ -259,12 +262,15 @@ def summarize_trace(d):
             continue
         ks = phases[ph]
         tot = sum(sum(v) for v in ks.values())
-        out.append(f"\n### {ph}: {sum(len(v) for v in ks.values())} kernels, {tot:.1f} us of kernel time\n")
-        out.append("| kernel | calls | total (us) | average (us) | % of phase |")
+        nc = max(ncalls.get(ph, 1), 1)
+        out.append(f"\n### {ph}: {nc} call(s), {sum(len(v) for v in ks.values())} kernels, "
+                   f"{tot:.1f} us of kernel time ({tot / nc:.1f} us per call)\n")
+        out.append("| kernel | launches per call | us per call | average (us) | % of phase |")
         out.append("|---|---:|---:|---:|---:|")
         for name, v in sorted(ks.items(), key=lambda kv: -sum(kv[1])):
-            short = name.split("(")[0].replace("void ", "").replace("tofe::(anonymous namespace)::", "")
-            out.append(f"| `{short}` | {len(v)} | {sum(v):.1f} | {np.mean(v):.2f} | "
+            short = name.replace("void ", "").replace("tofe::(anonymous namespace)::", "")
+            short = short.replace("tofe::", "").split("(")[0]
+            out.append(f"| `{short}` | {len(v) / nc:.1f} | {sum(v) / nc:.1f} | {np.mean(v):.2f} | "
                        f"{100 * sum(v) / tot:.2f} |")
     print("\n".join(out))
 
