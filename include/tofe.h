@@ -229,6 +229,20 @@ int tofe_compliance(tofe_ctx *ctx, const double *U, const double *F, double *c,
 int tofe_compliance_dev(tofe_ctx *ctx, const double *d_U, const double *d_F, double *c,
                         double *d_dcdx);
 
+/* Context-free compliance c = U.F and element sensitivities g_e = -dEdz_e u_e^T KE u_e (KE of
+ * an nx x ny x nz mesh of hx x hy x hz elements with E0, nu) on device pointers, allocating no
+ * device memory.  No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops that forbid hipMalloc).  It gives the c of
+ * tofe_compliance_dev and the g of tofe_sensitivity bit for bit (the filter chain rule is left
+ * to the caller).  c or d_g may be NULL (not both).  Scratch: a fixed 16 KiB device array per
+ * GPU, whatever the mesh size; meshes above 2^31 - 1 elements are rejected with
+ * TOFE_ERR_ARG.  Work runs on the null stream and the call returns synchronised.  Calls
+ * overlapping on one GPU share that scratch and race.  The message of a failure is
+ * tofe_last_error(NULL). */
+int tofe_compliance_noalloc(int nx, int ny, int nz, double hx, double hy, double hz, double E0,
+                            double nu, const double *d_U, const double *d_F,
+                            const double *d_dEdz, double *c, double *d_g);
+
 /* Element sensitivities g_e = -dEdz_e u_e^T KE u_e before the filter chain rule
  * (Compliance.cpp:37-46). */
 int tofe_sensitivity(tofe_ctx *ctx, const double *U, double *g);

@@ -45,7 +45,7 @@ EXPORTS = (
     "tofe_apply_M_dev", "tofe_element_quadratic", "tofe_petsc_vec_write", "tofe_petsc_vec_read",
     "tofe_petsc_aij_write", "tofe_petsc_aij_read", "tofe_mesh_out", "tofe_print_vals",
     "tofe_set_stress", "tofe_set_stress_dev", "tofe_apply_Ks", "tofe_apply_Ks_dev",
-    "tofe_stability_terms", "tofe_element_bilinear",
+    "tofe_stability_terms", "tofe_element_bilinear", "tofe_compliance_noalloc",
 )
 
 
@@ -146,6 +146,9 @@ def lib():
         "tofe_apply_Ks_dev": ([vp, vp, vp, C.c_int], C.c_int),
         "tofe_stability_terms": ([vp, dp, C.c_int, dp, dp], C.c_int),
         "tofe_element_bilinear": ([vp, dp, C.c_int, dp, dp], C.c_int),
+        "tofe_compliance_noalloc": ([C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, vp, vp, vp, dp, vp],
+                                    C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -1354,7 +1354,15 @@ int tofe_destroy(tofe_ctx *c)
   return TOFE_OK;
 }
 
-const char *tofe_last_error(const tofe_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *tofe_last_error(const tofe_ctx *c)
+{
+  // without a context: the message of this thread's last failing context-free call
+  if (!c) {
+    const char *m = noalloc_last_error();
+    return (m && *m) ? m : "null context";
+  }
+  return c->err.c_str();
+}
 
 int tofe_sizes(const tofe_ctx *c, int64_t *nelem, int64_t *nnode, int64_t *ndof, int *levels)
 {

@@ -293,4 +293,8 @@ void elem_quad(hipStream_t s, const double *phi, const double *KE, int nx, int n
 void sensitivity(hipStream_t s, const double *U, const double *dEdz, const double *KE, int nx,
                  int ny, int nz, double *g);
 
+// ---- context-free entry point (k_noalloc.hip) -------------------------------
+// message of the calling thread's last failing tofe_compliance_noalloc
+const char *noalloc_last_error();
+
 }  // namespace tofe
