@@ -58,6 +58,8 @@ enum { TOFE_SMOOTHER_CHEBYSHEV = 0, TOFE_SMOOTHER_JACOBI = 1 };
 /* Material interpolation (TopOpt::interpolation, TopOpt.h:27; Inputs.cpp:339-358). */
 enum { TOFE_INTERP_SIMP = 0, TOFE_INTERP_SIMP_CUT = 1, TOFE_INTERP_SIMP_LOGISTIC = 2 };
 enum { TOFE_PREC_FP64 = 0, TOFE_PREC_MIXED = 1 };
+/* Mass matrix forms (tofe_set_mass_form). */
+enum { TOFE_MASS_LUMPED = 0, TOFE_MASS_CONSISTENT = 1 };
 
 /* Fields readable with tofe_get_field (the TopOpt Vecs of the same names,
  * TopOpt.h:180-184). */
@@ -179,6 +181,15 @@ int tofe_apply_K_dev(tofe_ctx *ctx, const double *d_X, double *d_Y, int nrhs);
  * Dirichlet rows and columns of MatZeroRowsColumns(M, fixed, 1e-8) (Dynamic.cpp:44-45). */
 int tofe_assemble_mass(tofe_ctx *ctx, double density, const double *mlump);
 
+/* Mass form used by the next tofe_assemble_mass: TOFE_MASS_LUMPED (default) is the
+ * reference's DiagMassFnc diagonal above; TOFE_MASS_CONSISTENT has no counterpart in the
+ * reference: M = sum_e V_e m0 Q (x) I_3 + diag(mlump) with the same m0 and Q_ab =
+ * 24 prod_d (a_d == b_d ? 1/3 : 1/6), the exact integral of N_a N_b over the box element
+ * divided by m0 / density (each row of Q sums to 3: the consistent matrix holds three times
+ * the reference's lumped element mass, elemSize * density per displacement component).  Dirichlet
+ * rows and columns as MatZeroRowsColumns(M, fixed, 1e-8).  Applied matrix-free. */
+int tofe_set_mass_form(tofe_ctx *ctx, int form);
+
 /* Y = M X for nrhs vectors: MatMult(M) of the block eigensolvers (LOPGMRES/PRINVIT
  * operators set at Dynamic.cpp:67). */
 int tofe_apply_M(tofe_ctx *ctx, const double *X, double *Y, int nrhs);
@@ -187,7 +198,8 @@ int tofe_apply_M_dev(tofe_ctx *ctx, const double *d_X, double *d_Y, int nrhs);
 /* Element quadratic forms of nvec vectors phi (ndof each): kq[j*nelem+e] = phi_e^T KE phi_e
  * and, if mq is not NULL, mq[j*nelem+e] = sum of phi^2 over the element's 24 DOFs -- the
  * per-element terms of the eigenvalue sensitivities (Dynamic.cpp:113-154: dKdy = dEdz KE,
- * dMdy = dVdrho m0 I). */
+ * dMdy = dVdrho m0 I) -- or, with the consistent mass form, phi_e^T (Q (x) I_3) phi_e
+ * (dMdy = dVdrho m0 Q (x) I_3). */
 int tofe_element_quadratic(tofe_ctx *ctx, const double *phi, int nvec, double *kq, double *mq);
 
 /* Linearised buckling (Stability::Function, Buckling.cpp:15-414).

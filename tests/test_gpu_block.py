@@ -1,0 +1,96 @@
+"""Block passes over several vectors (the eigensolvers' blocks of search vectors, several load
+cases): one launch per kernel serves up to 16 vectors, each with its own partial sums,
+scalars and convergence test.  Every column must equal the single-vector result bit for
+bit -- the block path runs the same operations per column -- and the block solve must meet
+the oracle like the single one.  The reference applies these operators one vector at a time
+(MatMult in PRINVIT.cpp:353-354, PCApply in LOPGMRES.cpp:119-129, KSPSolve per load in
+FEAnalysis.cpp:673)."""
+import numpy as np
+import pytest
+
+from topopt3d_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(tofe, nel, L, precision, seed=11):
+    pb = configs.cantilever(nel, seed=seed, lo=0.05, hi=1.0, mg_levels=L)
+    ctx = tofe.Context(nel, mg_levels=L, fixed_dof=pb.fixed_dof, precision=precision)
+    ctx.set_design(pb.x)
+    ctx.assemble()
+    return pb, ctx
+
+
+def _rhs(pb, k, seed=5):
+    rng = np.random.default_rng(seed)
+    F = rng.standard_normal((k, pb.ndof))
+    F[:, pb.fixed_dof.astype(bool)] = 0
+    return F
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("nel,L", [((12, 6, 6), 2), ((16, 8, 8), 3)])
+def test_block_operator_and_vcycle_match_single(tofe, nel, L, precision):
+    print(f"case block apply/pc nel={nel} L={L} precision={precision}", flush=True)
+    pb, ctx = _ctx(tofe, nel, L, precision)
+    X = _rhs(pb, 5)
+    KX = ctx.apply_K(X)
+    ZX = ctx.pc_apply(X)
+    for j in range(5):
+        np.testing.assert_array_equal(KX[j], ctx.apply_K(X[j]))
+        np.testing.assert_array_equal(ZX[j], ctx.pc_apply(X[j]))
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_block_solve_matches_single_and_oracle(oracle, tofe, precision):
+    nel, L = (16, 8, 8), 3
+    print(f"case block solve nel={nel} precision={precision}", flush=True)
+    pb, ctx = _ctx(tofe, nel, L, precision)
+    F = _rhs(pb, 4)
+    F[2] = 0.0          # a zero right-hand side: converged at the start, stays zero
+    F[3] *= 1e-3        # a scaled copy converges on its own schedule
+    U, its, reason = ctx.solve(F, rtol=1e-10)
+    assert reason == tofe.CONVERGED_RTOL or reason == tofe.CONVERGED_ATOL
+    for j in range(4):
+        Uj, itsj, rj = ctx.solve(F[j], rtol=1e-10)
+        np.testing.assert_array_equal(U[j], Uj)
+        assert itsj <= its
+    assert not U[2].any()
+    E = oracle.simp(oracle.filter_csr(*nel, 1.5).matvec(pb.x))[0]
+    K = oracle.assemble_K(*nel, oracle.ke(), E, pb.fixed_dof).to_scipy()
+    for j in (0, 1, 3):
+        res = np.linalg.norm(F[j] - K @ U[j]) / np.linalg.norm(F[j])
+        assert res <= 1.01e-10
+    ctx.close()
+
+
+def test_block_solve_more_than_one_chunk(tofe):
+    """17 right-hand sides: a 16-vector pass and a 1-vector pass."""
+    nel, L = (8, 4, 4), 2
+    pb, ctx = _ctx(tofe, nel, L, 1)
+    F = _rhs(pb, 17, seed=8)
+    U, its, reason = ctx.solve(F, rtol=1e-9)
+    assert reason == tofe.CONVERGED_RTOL
+    for j in (0, 15, 16):
+        np.testing.assert_array_equal(U[j], ctx.solve(F[j], rtol=1e-9)[0])
+    ctx.close()
+
+
+def test_block_stress_and_mass_match_single(tofe):
+    nel = (9, 5, 4)
+    pb = configs.cantilever(nel, seed=6, lo=0.05, hi=1.0)
+    ctx = tofe.Context(nel, mg_levels=2, fixed_dof=pb.fixed_dof)
+    ctx.set_design(pb.x)
+    rng = np.random.default_rng(4)
+    U = rng.standard_normal(pb.ndof)
+    U[pb.fixed_dof.astype(bool)] = 0
+    ctx.set_stress(U)
+    ctx.assemble_mass(1.0)
+    X = rng.standard_normal((6, pb.ndof))
+    KsX = ctx.apply_Ks(X)
+    MX = ctx.apply_M(X)
+    for j in range(6):
+        np.testing.assert_array_equal(KsX[j], ctx.apply_Ks(X[j]))
+        np.testing.assert_array_equal(MX[j], ctx.apply_M(X[j]))
+    ctx.close()

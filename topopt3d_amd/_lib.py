@@ -26,6 +26,7 @@ DIVERGED_NANORINF = -9
 DIVERGED_INDEFINITE_MAT = -10
 
 SMOOTHER_CHEBYSHEV, SMOOTHER_JACOBI = 0, 1
+MASS_LUMPED, MASS_CONSISTENT = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
 (FIELD_X, FIELD_RHO, FIELD_E, FIELD_DEDZ, FIELD_ES, FIELD_DESDZ, FIELD_V, FIELD_DIAG, FIELD_Y,
@@ -46,6 +47,7 @@ EXPORTS = (
     "tofe_petsc_aij_write", "tofe_petsc_aij_read", "tofe_mesh_out", "tofe_print_vals",
     "tofe_set_stress", "tofe_set_stress_dev", "tofe_apply_Ks", "tofe_apply_Ks_dev",
     "tofe_stability_terms", "tofe_element_bilinear", "tofe_compliance_noalloc",
+    "tofe_set_mass_form",
 )
 
 
@@ -146,6 +148,7 @@ def lib():
         "tofe_apply_Ks_dev": ([vp, vp, vp, C.c_int], C.c_int),
         "tofe_stability_terms": ([vp, dp, C.c_int, dp, dp], C.c_int),
         "tofe_element_bilinear": ([vp, dp, C.c_int, dp, dp], C.c_int),
+        "tofe_set_mass_form": ([vp, C.c_int], C.c_int),
         "tofe_compliance_noalloc": ([C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double, vp, vp, vp, dp, vp],
                                     C.c_int),
@@ -326,6 +329,15 @@ class Context:
     def pc_apply_dev(self, R_ptr, Z_ptr, nrhs=1):
         self._check(lib().tofe_pc_apply_dev(self.h, C.c_void_p(R_ptr), C.c_void_p(Z_ptr), nrhs))
 
+    def apply_Ks_dev(self, X_ptr, Y_ptr, nrhs=1):
+        self._check(lib().tofe_apply_Ks_dev(self.h, C.c_void_p(X_ptr), C.c_void_p(Y_ptr), nrhs))
+
+    def apply_M_dev(self, X_ptr, Y_ptr, nrhs=1):
+        self._check(lib().tofe_apply_M_dev(self.h, C.c_void_p(X_ptr), C.c_void_p(Y_ptr), nrhs))
+
+    def set_stress_dev(self, U_ptr):
+        self._check(lib().tofe_set_stress_dev(self.h, C.c_void_p(U_ptr)))
+
     def compliance(self, U, F, gradient=True):
         U, F = _f64(U), _f64(F)
         c = C.c_double()
@@ -360,6 +372,9 @@ class Context:
     def assemble_mass(self, density, mlump=None):
         self._check(lib().tofe_assemble_mass(self.h, density,
                                              None if mlump is None else _dp(_f64(mlump))))
+
+    def set_mass_form(self, form):
+        self._check(lib().tofe_set_mass_form(self.h, form))
 
     def apply_M(self, X):
         X = _f64(X)

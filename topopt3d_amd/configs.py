@@ -147,3 +147,13 @@ def config3(Emin=1e-9) -> Problem:
     densities U[0.1,1] seed 3, filter R = 2.0, p = 3, 5-level GMG; 6 buckling modes."""
     return column((128, 64, 64), seed=3, lo=0.1, hi=1.0, rmin=2.0, penal=3.0, Emin=Emin,
                   mg_levels=5, name="column-128x64x64")
+
+
+def config5(Emin=1e-9) -> Problem:
+    """Free vibration of the 160x80x80 cantilever (3,168,963 DOF): x = 0 face clamped,
+    densities U[0.05,1] seed 5 through the R = 1.5 filter, SIMP p = 3 on the stiffness, the
+    mass scaled by V = rho (p = 1) with density 1, 5-level GMG; 12 lowest modes."""
+    pb = cantilever((160, 80, 80), seed=5, lo=0.05, hi=1.0, rmin=1.5, penal=3.0, Emin=Emin,
+                    mg_levels=5, name="cantilever-160x80x80")
+    pb.extra.update(density=1.0, nev=12)
+    return pb

@@ -34,10 +34,12 @@ __device__ inline void split_index(int64_t nn, int64_t &n, int &a0)
 }
 
 template <typename T, int EPI, bool SPLIT>
-__global__ __launch_bounds__(NT) void k_stencil_op(StencilOp<T> op, const T *__restrict__ u,
+__global__ __launch_bounds__(NT) void k_stencil_op(StencilOp<T> op, const T *u0,
                                                    EpiArgs<T> ep, int64_t nn)
 {
   __shared__ double sred[NT / 64];
+  epi_select_vector(ep, u0, 3 * nn);  // vector blockIdx.y
+  const T *__restrict__ u = u0;
   int64_t n;
   int a0;
   split_index<SPLIT>(nn, n, a0);
@@ -261,10 +263,12 @@ __host__ __device__ inline int64_t su_index(int v, int64_t n)
 #define TOFE_SYM64_G 9
 #endif
 template <typename T, int EPI, bool SPLIT>
-__global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T *__restrict__ u,
+__global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T *u0,
                                                        EpiArgs<T> ep, int64_t nn)
 {
   __shared__ double sred[NT / 64];
+  epi_select_vector(ep, u0, 3 * nn);  // vector blockIdx.y
+  const T *__restrict__ u = u0;
   int64_t n;
   int a0;
   split_index<SPLIT>(nn, n, a0);
@@ -420,6 +424,8 @@ template <typename T, bool LOWER>
 __global__ __launch_bounds__(NT) void k_trigemv(const T *__restrict__ A, int n, int ld,
                                                 const T *__restrict__ x, T *__restrict__ y)
 {
+  x += (int64_t)blockIdx.y * n;  // vector blockIdx.y of a block of right-hand sides
+  y += (int64_t)blockIdx.y * n;
   extern __shared__ unsigned char xs_raw[];
   T *xs = reinterpret_cast<T *>(xs_raw);
   for (int i = threadIdx.x; i < ld; i += NT) xs[i] = i < n ? x[i] : T(0);
@@ -827,7 +833,7 @@ int stencil_op_blocks(const StencilOp<T> &op)
   do {                                                                                      \
     const int64_t nn = (int64_t)op.Nx * op.Ny * op.Nz;                                      \
     const bool split = nn < op.split_nodes;                                                 \
-    dim3 grid((unsigned)stencil_op_blocks<T>(op)), block(NT);                               \
+    dim3 grid((unsigned)stencil_op_blocks<T>(op), ep.nvec), block(NT);                      \
     switch (epi) {                                                                          \
       case EPI_APPLY:                                                                       \
         if (split) KERNEL<T, EPI_APPLY, true><<<grid, block, 0, s>>>(op, u, ep, nn);        \
@@ -922,11 +928,12 @@ void tri_pack(hipStream_t s, const double *X, int n, int ld, T *Xr, T *XTr)
 }
 
 template <typename T>
-void coarse_solve_tri(hipStream_t s, const T *Xr, const T *XTr, int n, int ld, const T *b, T *tmp, T *x)
+void coarse_solve_tri(hipStream_t s, const T *Xr, const T *XTr, int n, int ld, const T *b, T *tmp, T *x,
+                      int nvec)
 {
   if (ld % (16 / (int)sizeof(T)) != 0 || ld < n)
     throw Error(TOFE_ERR_ARG, "coarse solve: bad padded leading dimension");
-  const dim3 grid((unsigned)((n + 3) / 4)), block(NT);
+  const dim3 grid((unsigned)((n + 3) / 4), nvec), block(NT);
   const size_t lds = sizeof(T) * (size_t)ld;
   k_trigemv<T, true><<<grid, block, lds, s>>>(Xr, n, ld, b, tmp);
   TOFE_LAUNCH_CHECK();
@@ -943,9 +950,9 @@ template void stencil_op<float>(hipStream_t, const StencilOp<float> &, const flo
 template void tri_pack<double>(hipStream_t, const double *, int, int, double *, double *);
 template void tri_pack<float>(hipStream_t, const double *, int, int, float *, float *);
 template void coarse_solve_tri<double>(hipStream_t, const double *, const double *, int, int, const double *,
-                                       double *, double *);
+                                       double *, double *, int);
 template void coarse_solve_tri<float>(hipStream_t, const float *, const float *, int, int, const float *,
-                                      float *, float *);
+                                      float *, float *, int);
 
 template void stencil_upper<float>(hipStream_t, const double *, int64_t, float *);
 template void stencil_upper<double>(hipStream_t, const double *, int64_t, double *);

@@ -77,6 +77,26 @@ __host__ __device__ inline double w1d(int i, int J, int Nf)
   return 0.5;
 }
 
+// Epilogue arguments and input of vector blockIdx.y of a launch over ep.nvec vectors stored
+// back to back (ndof values each); the per-block partial sums of vector v start at
+// v * gridDim.x.  Vector 0 keeps the pointers as given.
+template <typename T>
+__device__ inline void epi_select_vector(EpiArgs<T> &ep, const T *&u, int64_t ndof)
+{
+  const int64_t v = blockIdx.y;
+  if (v == 0) return;
+  const int64_t vo = v * ndof, po = v * (int64_t)gridDim.x;
+  u += vo;
+  if (ep.y) ep.y += vo;
+  if (ep.b) ep.b += vo;
+  if (ep.d) ep.d += vo;
+  if (ep.xout) ep.xout += vo;
+  if (ep.xout64) ep.xout64 += vo;
+  if (ep.q64) ep.q64 += vo;
+  if (ep.partial) ep.partial += po;
+  if (ep.partial2) ep.partial2 += po;
+}
+
 template <typename T, int EPI>
 __device__ inline void apply_epilogue(const EpiArgs<T> &ep, int64_t dof, T Au, T uval,
                                       double &dsum, double &dsum2)

@@ -318,10 +318,85 @@ __global__ void k_diag_mul(const double *__restrict__ d, const double *__restric
 // Element quadratic forms of nvec vectors (the element terms of the eigenvalue
 // sensitivities, Dynamic.cpp:113-154): kq = phi_e^T KE phi_e, mq = sum over the element's
 // 24 DOFs of phi^2.  One thread per (element, vector).
+// Consistent element mass over m0 = elemSize * density / 24 (the lumped matrix's per-DOF
+// element mass): Q (x) I_3 with Q_ab = 24 prod_d (a_d == b_d ? 1/3 : 1/6) for element-local
+// nodes a, b (the exact integral of N_a N_b over the box element, over elemSize / 24).
+__device__ inline double mass_q(int a, int b)
+{
+  const int same = (lnx(a) == lnx(b)) + (lny(a) == lny(b)) + (lnz(a) == lnz(b));
+  // 24 / 216 * 2^same
+  return same == 3 ? 8.0 / 9.0 : same == 2 ? 4.0 / 9.0 : same == 1 ? 2.0 / 9.0 : 1.0 / 9.0;
+}
+
+// y = M x with the consistent mass M = sum_e V_e m0 Q (x) I_3, plus mlump on the diagonal,
+// and the Dirichlet rows and columns of MatZeroRowsColumns(M, fixed, 1e-8): one thread per
+// node gathers its eight elements; V vectors per thread (block row blockIdx.y holds vectors
+// V * blockIdx.y ...).
+template <int V>
+__global__ __launch_bounds__(NT) void k_apply_mc(const double *__restrict__ Ve, double m0,
+                                                 const double *__restrict__ mlump,
+                                                 const uint8_t *__restrict__ fix, int nx, int ny,
+                                                 int nz, const double *__restrict__ x, double *y,
+                                                 int nvec)
+{
+  const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
+  const int64_t nn = (int64_t)Nx * Ny * Nz;
+  const int v0 = blockIdx.y * V, nv = min(V, nvec - v0);
+  x += (int64_t)v0 * 3 * nn;
+  y += (int64_t)v0 * 3 * nn;
+  const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (n >= nn) return;
+  int i, j, k;
+  node_ijk(n, Nx, Ny, i, j, k);
+  double acc[V][3];
+#pragma unroll
+  for (int v = 0; v < V; ++v) acc[v][0] = acc[v][1] = acc[v][2] = 0.0;
+  for (int dz = -1; dz <= 0; ++dz)
+    for (int dy = -1; dy <= 0; ++dy)
+      for (int dx = -1; dx <= 0; ++dx) {
+        const int ex = i + dx, ey = j + dy, ez = k + dz;
+        if (ex < 0 || ex >= nx || ey < 0 || ey >= ny || ez < 0 || ez >= nz) continue;
+        const int kl = lidx(-dx, -dy, -dz);
+        double t[V][3];
+#pragma unroll
+        for (int v = 0; v < V; ++v) t[v][0] = t[v][1] = t[v][2] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          const int64_t nm = (ex + lnx(m)) + (int64_t)Nx * ((ey + lny(m)) + (int64_t)Ny * (ez + lnz(m)));
+          const double w = mass_q(kl, m);
+          const uint8_t fm = fix ? fix[nm] : 0;
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if (v >= nv) break;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const double xv = ((fm >> c) & 1) ? 0.0 : x[v * 3 * nn + 3 * nm + c];
+              t[v][c] += w * xv;
+            }
+          }
+        }
+        const double se = Ve[ex + (int64_t)nx * (ey + (int64_t)ny * ez)] * m0;
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[v][c] += se * t[v][c];
+      }
+  const uint8_t fn = fix ? fix[n] : 0;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    if (v >= nv) break;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t d = v * 3 * nn + 3 * n + c;
+      y[d] = ((fn >> c) & 1) ? 1e-8 * x[d] : acc[v][c] + (mlump ? mlump[3 * n + c] * x[d] : 0.0);
+    }
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_elem_quad(const double *__restrict__ phi,
                                                   const double *__restrict__ KE, int nx, int ny,
                                                   int nz, int64_t ndof, int nvec, double *kq,
-                                                  double *mq)
+                                                  double *mq, int consistent)
 {
   const int64_t ne = (int64_t)nx * ny * nz;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
@@ -348,6 +423,17 @@ __global__ __launch_bounds__(NT) void k_elem_quad(const double *__restrict__ phi
     m = m + u[r] * u[r];
   }
   kq[t] = s;
+  if (mq && consistent) {  // phi_e^T (Q (x) I_3) phi_e
+    m = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const double w = mass_q(a, b);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) m += w * u[3 * a + c] * u[3 * b + c];
+      }
+  }
   if (mq) mq[t] = m;
 }
 
@@ -457,6 +543,19 @@ void mass_diag(hipStream_t s, const double *V, int nx, int ny, int nz, double m0
   TOFE_LAUNCH_CHECK();
 }
 
+void apply_mass_consistent(hipStream_t s, const double *V, double m0, const double *mlump,
+                           const uint8_t *fix, int nx, int ny, int nz, const double *x, double *y,
+                           int nvec)
+{
+  const int64_t nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  if (nvec == 1)
+    k_apply_mc<1><<<dim3(blocks_for(nn)), dim3(NT), 0, s>>>(V, m0, mlump, fix, nx, ny, nz, x, y, 1);
+  else
+    k_apply_mc<4><<<dim3(blocks_for(nn), (nvec + 3) / 4), dim3(NT), 0, s>>>(V, m0, mlump, fix, nx,
+                                                                           ny, nz, x, y, nvec);
+  TOFE_LAUNCH_CHECK();
+}
+
 void diag_mul(hipStream_t s, const double *d, const double *x, double *y, int64_t n, int nrhs)
 {
   k_diag_mul<<<dim3(stride_blocks(n * nrhs)), dim3(NT), 0, s>>>(d, x, y, n, nrhs);
@@ -464,11 +563,11 @@ void diag_mul(hipStream_t s, const double *d, const double *x, double *y, int64_
 }
 
 void elem_quad(hipStream_t s, const double *phi, const double *KE, int nx, int ny, int nz,
-               int64_t ndof, int nvec, double *kq, double *mq)
+               int64_t ndof, int nvec, double *kq, double *mq, bool consistent)
 {
   const int64_t ne = (int64_t)nx * ny * nz;
   k_elem_quad<<<dim3(blocks_for(ne * nvec)), dim3(NT), 0, s>>>(phi, KE, nx, ny, nz, ndof, nvec,
-                                                              kq, mq);
+                                                              kq, mq, consistent ? 1 : 0);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -167,7 +167,7 @@ FineGrid fine_grid(const FineOp<T> &op)
 }
 
 template <typename T, int EPI, int TX, int TY, bool SYM>
-__global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restrict__ u,
+__global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *u0,
                                                 EpiArgs<T> ep, int tiles_x, int tiles_y,
                                                 int nblocks, int zlen)
 {
@@ -181,6 +181,8 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *__restric
 
   const int nx = op.nx, ny = op.ny, nz = op.nz;
   const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
+  epi_select_vector(ep, u0, 3 * (int64_t)Nx * Ny * Nz);  // vector blockIdx.y
+  const T *__restrict__ u = u0;
   const int bid = xcd_remap(blockIdx.x, nblocks);
   const int txy = tiles_x * tiles_y;
   const int bz = bid / txy;
@@ -425,10 +427,10 @@ template <typename T, int EPI, int TX, int TY>
 void launch_tile(hipStream_t s, const FineOp<T> &op, const T *u, const EpiArgs<T> &ep, const FineGrid &g)
 {
   if (op.sym)
-    k_fine_op<T, EPI, TX, TY, true><<<dim3(g.nblocks), dim3(NT), 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y,
+    k_fine_op<T, EPI, TX, TY, true><<<dim3(g.nblocks, ep.nvec), dim3(NT), 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y,
                                                                          g.nblocks, op.zlen);
   else
-    k_fine_op<T, EPI, TX, TY, false><<<dim3(g.nblocks), dim3(NT), 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y,
+    k_fine_op<T, EPI, TX, TY, false><<<dim3(g.nblocks, ep.nvec), dim3(NT), 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y,
                                                                           g.nblocks, op.zlen);
 }
 

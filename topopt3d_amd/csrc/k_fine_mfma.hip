@@ -60,7 +60,7 @@ MfmaGrid mfma_grid(const FineOp<T> &op)
 }
 
 template <typename T, int EPI>
-__global__ __launch_bounds__(NTH) void k_fine_mfma(FineOp<T> op, const T *__restrict__ u,
+__global__ __launch_bounds__(NTH) void k_fine_mfma(FineOp<T> op, const T *u0,
                                                    EpiArgs<T> ep, int tiles_x, int tiles_y,
                                                    int nblocks)
 {
@@ -72,6 +72,8 @@ __global__ __launch_bounds__(NTH) void k_fine_mfma(FineOp<T> op, const T *__rest
 
   const int nx = op.nx, ny = op.ny, nz = op.nz;
   const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
+  epi_select_vector(ep, u0, 3 * (int64_t)Nx * Ny * Nz);  // vector blockIdx.y
+  const T *__restrict__ u = u0;
   const int bid = xcd_remap(blockIdx.x, nblocks);
   const int txy = tiles_x * tiles_y;
   const int bz = bid / txy;
@@ -220,7 +222,7 @@ template <typename T>
 void fine_mfma(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep)
 {
   const MfmaGrid g = mfma_grid(op);
-  dim3 grid(g.nblocks), block(NTH);
+  dim3 grid(g.nblocks, ep.nvec), block(NTH);
   switch (epi) {
     case EPI_APPLY:
       k_fine_mfma<T, EPI_APPLY><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks);

@@ -104,7 +104,7 @@ __device__ inline void element_product(T (&v)[3][8], const T (&kb)[NV], T Ee)
 }
 
 template <typename T, int EPI, bool CUBE>
-__global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *__restrict__ u, EpiArgs<T> ep,
+__global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *u0, EpiArgs<T> ep,
                                                  int tiles_x, int tiles_y, int nblocks, int zlen)
 {
   // partial sums handed to the row below (lower, upper node): double-buffered in FP32; one
@@ -117,6 +117,8 @@ __global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *__restri
 
   const int nx = op.nx, ny = op.ny, nz = op.nz;
   const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
+  epi_select_vector(ep, u0, 3 * (int64_t)Nx * Ny * Nz);  // vector blockIdx.y
+  const T *__restrict__ u = u0;
   const int bid = xcd_remap(blockIdx.x, nblocks);
   const int txy = tiles_x * tiles_y;
   const int bz = bid / txy;
@@ -311,7 +313,7 @@ int fine_wht_blocks(const FineOp<T> &op)
 template <typename T, int EPI>
 void launch_wht(hipStream_t s, const FineOp<T> &op, const T *u, const EpiArgs<T> &ep, const WhtGrid &g)
 {
-  dim3 grid(g.nblocks), block(NW);
+  dim3 grid(g.nblocks, ep.nvec), block(NW);
   if (op.wht_cube)
     k_fine_wht<T, EPI, true><<<grid, block, 0, s>>>(op, u, ep, g.tiles_x, g.tiles_y, g.nblocks, op.zlen);
   else

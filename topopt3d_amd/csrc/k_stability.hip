@@ -93,18 +93,27 @@ __global__ __launch_bounds__(NT) void k_stress_h(const double *__restrict__ U, i
 // y = Ks x with Ks = sum_e (-Es_e) H_e (x) I_3 and the Dirichlet rows and columns of
 // MatZeroRowsColumns(Ks, fixed, 0.0) (Buckling.cpp:28-29): fixed x entries are ignored
 // and fixed rows return 0.  One thread per node; elements in ascending order.
+// V vectors per thread (block row blockIdx.y holds vectors V*blockIdx.y ...), so each H value
+// read serves all of them; every vector's sums run in the same order as for one vector.
+template <int V>
 __global__ __launch_bounds__(NT) void k_apply_ks(const double *__restrict__ H,
                                                  const double *__restrict__ Es,
                                                  const uint8_t *__restrict__ fix, int nx, int ny,
-                                                 int nz, const double *__restrict__ x, double *y)
+                                                 int nz, const double *__restrict__ x, double *y,
+                                                 int nvec)
 {
   const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
   const int64_t nn = (int64_t)Nx * Ny * Nz, ne = (int64_t)nx * ny * nz;
+  const int v0 = blockIdx.y * V, nv = min(V, nvec - v0);
+  x += (int64_t)v0 * 3 * nn;
+  y += (int64_t)v0 * 3 * nn;
   const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (n >= nn) return;
   int i, j, k;
   node_ijk(n, Nx, Ny, i, j, k);
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[V][3];
+#pragma unroll
+  for (int v = 0; v < V; ++v) acc[v][0] = acc[v][1] = acc[v][2] = 0.0;
   for (int dz = -1; dz <= 0; ++dz)
     for (int dy = -1; dy <= 0; ++dy)
       for (int dx = -1; dx <= 0; ++dx) {
@@ -114,24 +123,36 @@ __global__ __launch_bounds__(NT) void k_apply_ks(const double *__restrict__ H,
         const int kl = lidx(-dx, -dy, -dz);
         int64_t node[8];
         element_nodes(e, nx, ny, node);
-        double t[3] = {0.0, 0.0, 0.0};
+        double t[V][3];
+#pragma unroll
+        for (int v = 0; v < V; ++v) t[v][0] = t[v][1] = t[v][2] = 0.0;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           const double hkm = H[(int64_t)(kl * 8 + m) * ne + e];
           const uint8_t fm = fix ? fix[node[m]] : 0;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const double xv = ((fm >> c) & 1) ? 0.0 : x[3 * node[m] + c];
-            t[c] += hkm * xv;
+          for (int v = 0; v < V; ++v) {
+            if (v >= nv) break;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const double xv = ((fm >> c) & 1) ? 0.0 : x[v * 3 * nn + 3 * node[m] + c];
+              t[v][c] += hkm * xv;
+            }
           }
         }
         const double s = -Es[e];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc[c] += s * t[c];
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[v][c] += s * t[v][c];
       }
   const uint8_t fn = fix ? fix[n] : 0;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) y[3 * n + c] = ((fn >> c) & 1) ? 0.0 : acc[c];
+  for (int v = 0; v < V; ++v) {
+    if (v >= nv) break;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[v * 3 * nn + 3 * n + c] = ((fn >> c) & 1) ? 0.0 : acc[v][c];
+  }
 }
 
 // q_e = sum_c phi_c^T H_e phi_c for each vector (the unscaled element term of
@@ -278,10 +299,14 @@ void stress_h(hipStream_t s, const double *U, int nx, int ny, int nz, const Shap
 }
 
 void apply_ks(hipStream_t s, const double *H, const double *Es, const uint8_t *fix, int nx,
-              int ny, int nz, const double *x, double *y)
+              int ny, int nz, const double *x, double *y, int nvec)
 {
   const int64_t nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
-  k_apply_ks<<<dim3(blocks_for(nn)), dim3(NT), 0, s>>>(H, Es, fix, nx, ny, nz, x, y);
+  if (nvec == 1)
+    k_apply_ks<1><<<dim3(blocks_for(nn)), dim3(NT), 0, s>>>(H, Es, fix, nx, ny, nz, x, y, 1);
+  else
+    k_apply_ks<4><<<dim3(blocks_for(nn), (nvec + 3) / 4), dim3(NT), 0, s>>>(H, Es, fix, nx, ny, nz,
+                                                                           x, y, nvec);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -16,6 +16,10 @@ __global__ __launch_bounds__(NT) void k_dot_partial(const T *__restrict__ a,
                                                     double *partial)
 {
   __shared__ double sred[NT / 64];
+  const int64_t vo = (int64_t)blockIdx.y * n;  // vector blockIdx.y
+  a += vo;
+  b += vo;
+  partial += (int64_t)blockIdx.y * gridDim.x;
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
     s += (double)a[i] * (double)b[i];
@@ -28,28 +32,46 @@ __global__ __launch_bounds__(NT) void k_reduce_final(const double *__restrict__ 
                                                      double *scal)
 {
   __shared__ double sred[NT / 64];
+  // vector blockIdx.y: its partials, and its scalars (out included) SL_COUNT further on
+  partial += (int64_t)blockIdx.y * nparts;
+  if (out) out += (int64_t)blockIdx.y * SL_COUNT;
+  if (scal) scal += (int64_t)blockIdx.y * SL_COUNT;
   double s = 0;
   for (int i = threadIdx.x; i < nparts; i += NT) s += partial[i];
   const double v = block_sum<NT>(s, sred);
   if (threadIdx.x == 0) {
     if (out) *out = v;
+    // a frozen right-hand side (converged in a block solve) takes no further steps
+    const bool frozen = scal && scal[SL_FROZEN] != 0.0;
     switch (post) {
-      case POST_ALPHA_CG: scal[SL_ALPHA] = scal[SL_RZ] / v; break;
+      case POST_ALPHA_CG: scal[SL_ALPHA] = frozen ? 0.0 : scal[SL_RZ] / v; break;
       case POST_BETA_CG:
-        scal[SL_BETA] = v / scal[SL_RZ];
+        scal[SL_BETA] = frozen ? 0.0 : v / scal[SL_RZ];
         scal[SL_RZ] = v;
         break;
-      case POST_ALPHA_FCG: scal[SL_ALPHA] = scal[SL_PR] / v; break;
-      case POST_BETA_FCG: scal[SL_BETA] = -v / scal[SL_PQ]; break;
+      case POST_ALPHA_FCG: scal[SL_ALPHA] = frozen ? 0.0 : scal[SL_PR] / v; break;
+      case POST_BETA_FCG: scal[SL_BETA] = frozen ? 0.0 : -v / scal[SL_PQ]; break;
       default: break;
     }
   }
+}
+
+// scal[v * SL_COUNT + SL_FROZEN] = bit v of mask, v < nvec (the right-hand sides of a block
+// solve that have converged and take no further steps)
+__global__ void k_set_frozen(double *scal, int nvec, unsigned long long mask)
+{
+  const int v = threadIdx.x;
+  if (v < nvec) scal[(int64_t)v * SL_COUNT + SL_FROZEN] = ((mask >> v) & 1ull) ? 1.0 : 0.0;
 }
 
 template <typename T>
 __global__ void k_cheb_zero(const T *__restrict__ b, const T *__restrict__ dinv, T c2, T *d,
                             T *x, int64_t n)
 {
+  const int64_t vo = (int64_t)blockIdx.y * n;  // vector blockIdx.y; dinv is shared
+  b += vo;
+  d += vo;
+  x += vo;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
     const T v = c2 * (dinv[i] * b[i]);
     d[i] = v;
@@ -61,6 +83,11 @@ __global__ void k_cheb_zero_from64(const double *__restrict__ b64, float *__rest
                                    const float *__restrict__ dinv, float c2, float *d, float *x,
                                    int64_t n)
 {
+  const int64_t vo = (int64_t)blockIdx.y * n;  // vector blockIdx.y; dinv is shared
+  b64 += vo;
+  b32 += vo;
+  d += vo;
+  x += vo;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
     const float b = (float)b64[i];
     b32[i] = b;
@@ -82,6 +109,15 @@ __global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int N
                                                  int Ncz, FirstStep<T> fs)
 {
   const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
+  {  // vector blockIdx.y of fine residuals and coarse right-hand sides; dinv is shared
+    const int64_t v = blockIdx.y;
+    rf += v * 3 * ((int64_t)Nfx * Nfy * Nfz);
+    bc += v * 3 * ncn;
+    if (fs.d) {
+      fs.d += v * 3 * ncn;
+      fs.x += v * 3 * ncn;
+    }
+  }
   const int64_t I = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (I >= ncn) return;
   int Ix, Iy, Iz;
@@ -130,6 +166,8 @@ __global__ __launch_bounds__(NT) void k_prolong_add(const T *__restrict__ xc, in
                                                     int Nfz)
 {
   const int64_t nfn = (int64_t)Nfx * Nfy * Nfz;
+  xc += (int64_t)blockIdx.y * 3 * ((int64_t)Ncx * Ncy * Ncz);  // vector blockIdx.y
+  xf += (int64_t)blockIdx.y * 3 * nfn;
   const int64_t f = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (f >= nfn) return;
   int fx, fy, fz;
@@ -183,6 +221,11 @@ __global__ __launch_bounds__(NT) void k_cg_r(double *__restrict__ r, const doubl
                                              double *partial)
 {
   __shared__ double sred[NT / 64];
+  const int64_t vo = (int64_t)blockIdx.y * n;  // vector blockIdx.y
+  r += vo;
+  q += vo;
+  scal += (int64_t)blockIdx.y * SL_COUNT;
+  partial += (int64_t)blockIdx.y * gridDim.x;
   const double alpha = scal[alpha_slot];
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
@@ -198,6 +241,11 @@ __global__ __launch_bounds__(NT) void k_cg_r(double *__restrict__ r, const doubl
 __global__ void k_cg_xp(double *__restrict__ p, const double *__restrict__ z, double *__restrict__ x,
                         int64_t n, const double *__restrict__ scal, int beta_slot, int alpha_slot)
 {
+  const int64_t vo = (int64_t)blockIdx.y * n;  // vector blockIdx.y
+  p += vo;
+  z += vo;
+  x += vo;
+  scal += (int64_t)blockIdx.y * SL_COUNT;
   const double beta = scal[beta_slot], alpha = scal[alpha_slot];
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
     const double pi = p[i];
@@ -265,9 +313,24 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const dou
                                                double *scal, double *partial, FirstCheb fc)
 {
   __shared__ double sred[NT / 64];
+  {  // vector blockIdx.y: its vectors, partials and scalars; dinv is shared
+    const int64_t v = blockIdx.y, vo = v * n;
+    r += vo;
+    q += vo;
+    pq_part += v * nparts;
+    pr_part += v * nparts;
+    scal += v * SL_COUNT;
+    partial += v * gridDim.x;
+    if (fc.b32) {
+      fc.b32 += vo;
+      fc.d += vo;
+      fc.x += vo;
+    }
+  }
   const double pr = all_sum(pr_part, nparts, sred);
   const double pq = all_sum(pq_part, nparts, sred);
-  const double alpha = pr / pq;
+  // a frozen right-hand side (converged in a block solve) takes no further steps
+  const double alpha = scal[SL_FROZEN] != 0.0 ? 0.0 : pr / pq;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     scal[SL_PR] = pr;
     scal[SL_PQ] = pq;
@@ -298,8 +361,16 @@ __global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const doub
                                               double *scal)
 {
   __shared__ double sred[NT / 64];
+  {  // vector blockIdx.y
+    const int64_t v = blockIdx.y, vo = v * n;
+    p += vo;
+    z += vo;
+    x += vo;
+    zq_part += v * nparts;
+    scal += v * SL_COUNT;
+  }
   const double zq = all_sum(zq_part, nparts, sred);
-  const double beta = -zq / scal[SL_PQ];
+  const double beta = scal[SL_FROZEN] != 0.0 ? 0.0 : -zq / scal[SL_PQ];
   const double alpha = scal[SL_ALPHA];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     scal[SL_ZQ] = zq;
@@ -316,6 +387,9 @@ __global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const doub
 __global__ __launch_bounds__(NT) void k_fcg_x(double *__restrict__ x, const double *__restrict__ p,
                                               int64_t n, const double *scal)
 {
+  x += (int64_t)blockIdx.y * n;  // vector blockIdx.y
+  p += (int64_t)blockIdx.y * n;
+  scal += (int64_t)blockIdx.y * SL_COUNT;
   const double alpha = scal[SL_ALPHA];
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
     x[i] += alpha * p[i];
@@ -376,49 +450,56 @@ inline unsigned stride_blocks(int64_t n)
 }  // namespace
 
 template <typename T>
-void dot_partial(hipStream_t s, const T *a, const T *b, int64_t n, double *partial)
+void dot_partial(hipStream_t s, const T *a, const T *b, int64_t n, double *partial, int nvec)
 {
-  k_dot_partial<T><<<dim3(kRedBlocks), dim3(NT), 0, s>>>(a, b, n, partial);
+  k_dot_partial<T><<<dim3(kRedBlocks, nvec), dim3(NT), 0, s>>>(a, b, n, partial);
   TOFE_LAUNCH_CHECK();
 }
 
 void reduce_final(hipStream_t s, const double *partial, int nparts, double *out, int post,
-                  double *scal)
+                  double *scal, int nvec)
 {
-  k_reduce_final<<<dim3(1), dim3(NT), 0, s>>>(partial, nparts, out, post, scal);
+  k_reduce_final<<<dim3(1, nvec), dim3(NT), 0, s>>>(partial, nparts, out, post, scal);
+  TOFE_LAUNCH_CHECK();
+}
+
+void set_frozen(hipStream_t s, double *scal, int nvec, unsigned long long mask)
+{
+  if (nvec < 1 || nvec > 64) throw Error(TOFE_ERR_ARG, "set_frozen: 1..64 vectors");
+  k_set_frozen<<<dim3(1), dim3(64), 0, s>>>(scal, nvec, mask);
   TOFE_LAUNCH_CHECK();
 }
 
 template <typename T>
-void cheb_zero(hipStream_t s, const T *b, const T *dinv, T c2, T *d, T *x, int64_t n)
+void cheb_zero(hipStream_t s, const T *b, const T *dinv, T c2, T *d, T *x, int64_t n, int nvec)
 {
-  k_cheb_zero<T><<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(b, dinv, c2, d, x, n);
+  k_cheb_zero<T><<<dim3(stride_blocks(n), nvec), dim3(NT), 0, s>>>(b, dinv, c2, d, x, n);
   TOFE_LAUNCH_CHECK();
 }
 
 void cheb_zero_from64(hipStream_t s, const double *b64, float *b32, const float *dinv, float c2,
-                      float *d, float *x, int64_t n)
+                      float *d, float *x, int64_t n, int nvec)
 {
-  k_cheb_zero_from64<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(b64, b32, dinv, c2, d, x, n);
+  k_cheb_zero_from64<<<dim3(stride_blocks(n), nvec), dim3(NT), 0, s>>>(b64, b32, dinv, c2, d, x, n);
   TOFE_LAUNCH_CHECK();
 }
 
 template <typename T>
 void restrict_op(hipStream_t s, const T *rf, int Nfx, int Nfy, int Nfz, T *bc, int Ncx, int Ncy,
-                 int Ncz, const FirstStep<T> &fs)
+                 int Ncz, const FirstStep<T> &fs, int nvec)
 {
   const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
-  k_restrict<T><<<dim3((unsigned)((ncn + NT - 1) / NT)), dim3(NT), 0, s>>>(rf, Nfx, Nfy, Nfz, bc,
+  k_restrict<T><<<dim3((unsigned)((ncn + NT - 1) / NT), nvec), dim3(NT), 0, s>>>(rf, Nfx, Nfy, Nfz, bc,
                                                                            Ncx, Ncy, Ncz, fs);
   TOFE_LAUNCH_CHECK();
 }
 
 template <typename T>
 void prolong_add(hipStream_t s, const T *xc, int Ncx, int Ncy, int Ncz, T *xf, int Nfx, int Nfy,
-                 int Nfz)
+                 int Nfz, int nvec)
 {
   const int64_t nfn = (int64_t)Nfx * Nfy * Nfz;
-  k_prolong_add<T><<<dim3((unsigned)((nfn + NT - 1) / NT)), dim3(NT), 0, s>>>(xc, Ncx, Ncy, Ncz,
+  k_prolong_add<T><<<dim3((unsigned)((nfn + NT - 1) / NT), nvec), dim3(NT), 0, s>>>(xc, Ncx, Ncy, Ncz,
                                                                               xf, Nfx, Nfy, Nfz);
   TOFE_LAUNCH_CHECK();
 }
@@ -437,16 +518,16 @@ void fill_d(hipStream_t s, double *a, double v, int64_t n)
 }
 
 void cg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *scal,
-                 int alpha_slot, double *partial)
+                 int alpha_slot, double *partial, int nvec)
 {
-  k_cg_r<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, q, n, scal, alpha_slot, partial);
+  k_cg_r<<<dim3(kRedBlocks, nvec), dim3(NT), 0, s>>>(r, q, n, scal, alpha_slot, partial);
   TOFE_LAUNCH_CHECK();
 }
 
 void cg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n, const double *scal,
-                  int beta_slot, int alpha_slot)
+                  int beta_slot, int alpha_slot, int nvec)
 {
-  k_cg_xp<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, x, n, scal, beta_slot, alpha_slot);
+  k_cg_xp<<<dim3(stride_blocks(n), nvec), dim3(NT), 0, s>>>(p, z, x, n, scal, beta_slot, alpha_slot);
   TOFE_LAUNCH_CHECK();
 }
 
@@ -471,22 +552,23 @@ void record_scalar(hipStream_t s, const double *scal, int slot, double *dst)
 }
 
 void fcg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *pq_part,
-                  const double *pr_part, int nparts, double *scal, double *partial, const FirstCheb &fc)
+                  const double *pr_part, int nparts, double *scal, double *partial, const FirstCheb &fc,
+                  int nvec)
 {
-  k_fcg_xr<<<dim3(kRedBlocks), dim3(NT), 0, s>>>(r, q, n, pq_part, pr_part, nparts, scal, partial, fc);
+  k_fcg_xr<<<dim3(kRedBlocks, nvec), dim3(NT), 0, s>>>(r, q, n, pq_part, pr_part, nparts, scal, partial, fc);
   TOFE_LAUNCH_CHECK();
 }
 
 void fcg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n,
-                   const double *zq_part, int nparts, double *scal)
+                   const double *zq_part, int nparts, double *scal, int nvec)
 {
-  k_fcg_p<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(p, z, x, n, zq_part, nparts, scal);
+  k_fcg_p<<<dim3(stride_blocks(n), nvec), dim3(NT), 0, s>>>(p, z, x, n, zq_part, nparts, scal);
   TOFE_LAUNCH_CHECK();
 }
 
-void fcg_update_x(hipStream_t s, double *x, const double *p, int64_t n, const double *scal)
+void fcg_update_x(hipStream_t s, double *x, const double *p, int64_t n, const double *scal, int nvec)
 {
-  k_fcg_x<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(x, p, n, scal);
+  k_fcg_x<<<dim3(stride_blocks(n), nvec), dim3(NT), 0, s>>>(x, p, n, scal);
   TOFE_LAUNCH_CHECK();
 }
 
@@ -509,11 +591,11 @@ void lanczos_p(hipStream_t s, T *p, const T *r, const T *dinv, int64_t n, const 
 }
 
 #define TOFE_INST_T(T)                                                                      \
-  template void dot_partial<T>(hipStream_t, const T *, const T *, int64_t, double *);       \
-  template void cheb_zero<T>(hipStream_t, const T *, const T *, T, T *, T *, int64_t);      \
+  template void dot_partial<T>(hipStream_t, const T *, const T *, int64_t, double *, int);  \
+  template void cheb_zero<T>(hipStream_t, const T *, const T *, T, T *, T *, int64_t, int); \
   template void restrict_op<T>(hipStream_t, const T *, int, int, int, T *, int, int, int,        \
-                               const FirstStep<T> &);                                         \
-  template void prolong_add<T>(hipStream_t, const T *, int, int, int, T *, int, int, int);  \
+                               const FirstStep<T> &, int);                                    \
+  template void prolong_add<T>(hipStream_t, const T *, int, int, int, T *, int, int, int, int); \
 
 TOFE_INST_T(double)
 TOFE_INST_T(float)

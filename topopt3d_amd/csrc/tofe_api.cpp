@@ -83,6 +83,9 @@ struct Level {
 };
 
 constexpr int kMaxEig = 64;
+// Right-hand sides per block pass: solves, V-cycles and operator applies with more vectors
+// run in chunks of this many (each chunk one launch per kernel).
+constexpr int kMaxBlockVec = 16;
 constexpr int kCoarseCUs = 64;  // compute units left to the coarse factorisation during set-up
 
 double now_s()
@@ -130,9 +133,12 @@ struct tofe_ctx {
   std::vector<int> rtpl_off;
   DBuf<int> drtpl_off;
   DBuf<double> drtpl_w, REdge, y, rhoq, ebuf3;
-  // lumped mass diagonal (Frequency::DiagMassFnc)
-  DBuf<double> Mdiag;
-  bool has_mass = false;
+  // mass matrix: the lumped diagonal (Frequency::DiagMassFnc), or the consistent element mass
+  // V_e m0 Q (x) I_3 applied matrix-free from a snapshot of V and the point masses
+  DBuf<double> Mdiag, mV, mlump;
+  int mass_form = TOFE_MASS_LUMPED;
+  double mass_m0 = 0;
+  bool has_mass = false, has_mlump = false;
   // stress state of the buckling analysis: H_e per element (Stability::StressFnc)
   ShapeData shape{};
   DBuf<double> Hs;
@@ -157,6 +163,10 @@ struct tofe_ctx {
   DBuf<int> dinfo;
   DBuf<double> chol_l, chol_x;
   int64_t nc = 0;
+  // vectors per block pass the work buffers hold (level vectors, CG vectors, partial sums,
+  // scalars); grows on demand up to kMaxBlockVec
+  int nvcap = 1;
+  bool frozen_dirty = false;  // some SL_FROZEN flag of scal may be set
   DBuf<double> partial, partial2, partial3, scal, eig_ab;
   DBuf<double> partial_b, partial2_b, scal_b;  // reductions of the third stream
   DBuf<double> partial_c, partial2_c, scal_c;  // set-up reductions of the main stream
@@ -341,41 +351,73 @@ int level_blocks(tofe_ctx *c, int l)
 void alloc_level(tofe_ctx *c, Level<double> &lv, int l)
 {
   lv.g = c->grids[l];
-  const int64_t nd = lv.g.ndof();
+  const int64_t nd = lv.g.ndof(), ndv = nd * c->nvcap;
   if (l > 0) lv.S.alloc((size_t)243 * lv.g.nn());
   lv.dinv.alloc(nd);
-  if (l > 0) lv.b.alloc(nd);
-  lv.x.alloc(nd);
-  lv.x2.alloc(nd);
-  lv.r.alloc(nd);
-  lv.d.alloc(nd);
+  if (l > 0) lv.b.alloc(ndv);
+  lv.x.alloc(ndv);
+  lv.x2.alloc(ndv);
+  lv.r.alloc(ndv);
+  lv.d.alloc(ndv);
   lv.d.zero(c->stream);
 }
 
 void alloc_level32(tofe_ctx *c, Level<float> &lv, int l)
 {
   lv.g = c->grids[l];
-  const int64_t nd = lv.g.ndof();
+  const int64_t nd = lv.g.ndof(), ndv = nd * c->nvcap;
   // upper half of the symmetric stencil, node-blocked by 64 (k_stencil_op_sym)
   if (l > 0) lv.S.alloc((size_t)126 * ((lv.g.nn() + 63) / 64 * 64));
   lv.dinv.alloc(nd);
-  lv.b.alloc(nd);
-  lv.x.alloc(nd);
-  lv.x2.alloc(nd);
-  lv.r.alloc(nd);
-  lv.d.alloc(nd);
+  lv.b.alloc(ndv);
+  lv.x.alloc(ndv);
+  lv.x2.alloc(ndv);
+  lv.r.alloc(ndv);
+  lv.d.alloc(ndv);
   lv.d.zero(c->stream);
+}
+
+// Grow the per-vector work buffers to hold nvec vectors (block solves, V-cycles and applies).
+// Every stream is idle on return; nothing a later call reads survives in them.
+void ensure_nvec(tofe_ctx *c, int nvec)
+{
+  if (nvec < 1 || nvec > kMaxBlockVec) throw Error(TOFE_ERR_ARG, "vectors per block pass out of range");
+  if (nvec <= c->nvcap) return;
+  for (hipStream_t st : {c->stream, c->stream2, c->stream3, c->stream4})
+    if (st) TOFE_HIP(hipStreamSynchronize(st));
+  c->nvcap = nvec;
+  for (size_t l = 0; l < c->lv64.size(); ++l) alloc_level(c, *c->lv64[l], (int)l);
+  for (size_t l = 0; l < c->lv32.size(); ++l) alloc_level32(c, *c->lv32[l], (int)l);
+  const size_t nd = (size_t)c->ndof * nvec;
+  for (auto *b : {&c->vr, &c->vz, &c->vp, &c->vq, &c->vF, &c->vU}) b->alloc(nd);
+  if (c->vr32.p) {
+    c->vr32.alloc(nd);
+    c->vz32.alloc(nd);
+  }
+  if (c->ctmp64.p) c->ctmp64.alloc((size_t)c->nc * nvec);
+  if (c->ctmp32.p) c->ctmp32.alloc((size_t)c->nc * nvec);
+  for (auto *b : {&c->partial, &c->partial2, &c->partial3}) b->alloc((size_t)c->npart * nvec);
+  c->scal.alloc((size_t)SL_COUNT * nvec);
+  c->scal.zero(c->stream);
+  c->frozen_dirty = false;
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  c->hscal = nullptr;
+  TOFE_HIP(hipHostMalloc((void **)&c->hscal, (size_t)SL_COUNT * nvec * sizeof(double),
+                         hipHostMallocDefault));
+  TOFE_HIP(hipStreamSynchronize(c->stream));
 }
 
 // Exact solve with the coarsest operator, x = A^-1 b = X^T (X b) with X = L^-1, in the
 // V-cycle's precision.
 template <typename T>
-void coarse_apply(tofe_ctx *c, const T *b, T *x)
+void coarse_apply(tofe_ctx *c, const T *b, T *x, int nvec = 1)
 {
   if constexpr (sizeof(T) == 4)
-    coarse_solve_tri<float>(c->stream, c->X32r.p, c->XT32r.p, (int)c->nc, c->ldc, b, c->ctmp32.p, x);
+    coarse_solve_tri<float>(c->stream, c->X32r.p, c->XT32r.p, (int)c->nc, c->ldc, b, c->ctmp32.p, x,
+                            nvec);
   else
-    coarse_solve_tri<double>(c->stream, c->X64r.p, c->XT64r.p, (int)c->nc, c->ldc, b, c->ctmp64.p, x);
+    coarse_solve_tri<double>(c->stream, c->X64r.p, c->XT64r.p, (int)c->nc, c->ldc, b, c->ctmp64.p, x,
+                             nvec);
 }
 
 // FP64 ends of the mixed-precision V-cycle, fused into its first and last fine-level
@@ -411,15 +453,15 @@ FirstCheb vcycle_first_step(tofe_ctx *c)
 // Multigrid V-cycle: x0 = M^-1 b0 (PCMG V-cycle, Chebyshev-Jacobi smoothers,
 // Galerkin coarse operators, direct coarse solve).
 template <typename T>
-void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
+void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 1)
 {
   auto &LV = levels<T>(c);
   const int L = c->L;
   const hipStream_t s = c->stream;
   const bool mixed_io = io && sizeof(T) == 4;
   if (L == 1) {
-    if (mixed_io && io->b64) convert<double, T>(s, io->b64, const_cast<T *>(b0), c->ndof);
-    coarse_apply<T>(c, b0, x0);
+    if (mixed_io && io->b64) convert<double, T>(s, io->b64, const_cast<T *>(b0), c->ndof * nvec);
+    coarse_apply<T>(c, b0, x0, nvec);
     return;
   }
   const int nu1 = c->desc.nu_pre, nu2 = c->desc.nu_post, km = c->kmax();
@@ -433,7 +475,7 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
     const int64_t nd = lv.g.ndof();
     const double *c1 = &c->c1[(size_t)l * km], *c2 = &c->c2[(size_t)l * km];
     const bool from64 = l == 0 && mixed_io && io->b64;
-    if (from64 && nu1 == 0) convert<double, T>(s, io->b64, const_cast<T *>(b0), nd);
+    if (from64 && nu1 == 0) convert<double, T>(s, io->b64, const_cast<T *>(b0), nd * nvec);
     if (nu1 > 0) {
       // the first step from x = 0 on levels l >= 1 was formed by the restriction into l
       if constexpr (sizeof(T) == 4) {
@@ -441,24 +483,26 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
           // formed by the kernel that produced b64
         } else if (from64)
           cheb_zero_from64(s, io->b64, const_cast<float *>(b0), lv.dinv.p, (float)c2[0], lv.d.p,
-                           xc, nd);
+                           xc, nd, nvec);
         else if (l == 0)
-          cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd);
+          cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd, nvec);
       } else {
-        if (l == 0) cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd);
+        if (l == 0) cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd, nvec);
       }
       for (int k = 1; k < nu1; ++k) {
         EpiArgs<T> ep;
         ep.b = b; ep.dinv = lv.dinv.p; ep.d = lv.d.p; ep.xout = xo;
         ep.c1 = (T)c1[k]; ep.c2 = (T)c2[k];
+        ep.nvec = nvec;
         level_apply<T>(c, l, xc, EPI_CHEB, ep);
         std::swap(xc, xo);
       }
     } else {
-      TOFE_HIP(hipMemsetAsync(xc, 0, nd * sizeof(T), s));
+      TOFE_HIP(hipMemsetAsync(xc, 0, nd * nvec * sizeof(T), s));
     }
     EpiArgs<T> er;
     er.y = lv.r.p; er.b = b;
+    er.nvec = nvec;
     level_apply<T>(c, l, xc, EPI_RESID, er);
     const Grid &gf = lv.g, &gc = LV[l + 1]->g;
     FirstStep<T> fs;
@@ -469,24 +513,25 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
       fs.d = ln.d.p;
       fs.x = (flips & 1) ? ln.x2.p : ln.x.p;  // the level's xc (same flip count on every level)
     }
-    restrict_op<T>(s, lv.r.p, gf.Nx, gf.Ny, gf.Nz, LV[l + 1]->b.p, gc.Nx, gc.Ny, gc.Nz, fs);
+    restrict_op<T>(s, lv.r.p, gf.Nx, gf.Ny, gf.Nz, LV[l + 1]->b.p, gc.Nx, gc.Ny, gc.Nz, fs, nvec);
     cur[l] = xc;
     oth[l] = xo;
   }
   Level<T> &lc = *LV[L - 1];
-  coarse_apply<T>(c, lc.b.p, lc.x.p);
+  coarse_apply<T>(c, lc.b.p, lc.x.p, nvec);
   cur[L - 1] = lc.x.p;
   for (int l = L - 2; l >= 0; --l) {
     Level<T> &lv = *LV[l];
     const T *b = (l == 0) ? b0 : lv.b.p;
     T *xc = cur[l], *xo = oth[l];
     const Grid &gf = lv.g, &gc = LV[l + 1]->g;
-    prolong_add<T>(s, cur[l + 1], gc.Nx, gc.Ny, gc.Nz, xc, gf.Nx, gf.Ny, gf.Nz);
+    prolong_add<T>(s, cur[l + 1], gc.Nx, gc.Ny, gc.Nz, xc, gf.Nx, gf.Ny, gf.Nz, nvec);
     const double *c1 = &c->c1[(size_t)l * km], *c2 = &c->c2[(size_t)l * km];
     for (int k = 0; k < nu2; ++k) {
       EpiArgs<T> ep;
       ep.b = b; ep.dinv = lv.dinv.p; ep.d = lv.d.p; ep.xout = xo;
       ep.c1 = (T)(k == 0 ? 0.0 : c1[k]); ep.c2 = (T)c2[k];
+      ep.nvec = nvec;
       if (l == 0 && k == nu2 - 1 && mixed_io && io->x64) {
         ep.xout64 = io->x64; ep.q64 = io->q64; ep.partial = io->partial;
         level_apply<T>(c, l, xc, EPI_CHEB_OUT, ep);
@@ -498,13 +543,14 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr)
     }
     cur[l] = xc;
   }
-  if (cur[0] != x0) TOFE_HIP(hipMemcpyAsync(x0, cur[0], c->ndof * sizeof(T), hipMemcpyDeviceToDevice, s));
+  if (cur[0] != x0)
+    TOFE_HIP(hipMemcpyAsync(x0, cur[0], c->ndof * nvec * sizeof(T), hipMemcpyDeviceToDevice, s));
 }
 
 // z = M^-1 r.  With q given (flexible CG), also the partial sums of z.q into c->partial;
 // returns how many there are.
 int precondition(tofe_ctx *c, const double *r, double *z, const double *q = nullptr,
-                 bool first_done = false)
+                 bool first_done = false, int nvec = 1)
 {
   const hipStream_t s = c->stream;
   if (c->built_precision == TOFE_PREC_MIXED) {
@@ -514,36 +560,39 @@ int precondition(tofe_ctx *c, const double *r, double *z, const double *q = null
     if (q && c->L > 1 && !c->fine_mfma) {
       io.x64 = z; io.q64 = q; io.partial = c->partial.p;
     }
-    vcycle<float>(c, c->vr32.p, c->vz32.p, &io);
+    vcycle<float>(c, c->vr32.p, c->vz32.p, &io, nvec);
     if (io.fused_out) return level_blocks<float>(c, 0);
-    convert<float, double>(s, c->vz32.p, z, c->ndof);
+    convert<float, double>(s, c->vz32.p, z, c->ndof * nvec);
   } else {
-    vcycle<double>(c, r, z);
+    vcycle<double>(c, r, z, nullptr, nvec);
   }
   if (!q) return 0;
-  dot_partial<double>(s, z, q, c->ndof, c->partial.p);
+  dot_partial<double>(s, z, q, c->ndof, c->partial.p, nvec);
   return kRedBlocks;
 }
 
-void dot_into(tofe_ctx *c, const double *a, const double *b, int64_t n, int slot, int post = POST_NONE)
+// scal[v * SL_COUNT + slot] = a_v . b_v for the nvec vectors of length n
+void dot_into(tofe_ctx *c, const double *a, const double *b, int64_t n, int slot, int post = POST_NONE,
+              int nvec = 1)
 {
-  dot_partial<double>(c->stream, a, b, n, c->partial.p);
-  reduce_final(c->stream, c->partial.p, kRedBlocks, c->scal.p + slot, post, c->scal.p);
+  dot_partial<double>(c->stream, a, b, n, c->partial.p, nvec);
+  reduce_final(c->stream, c->partial.p, kRedBlocks, c->scal.p + slot, post, c->scal.p, nvec);
 }
 
-// Device scalars to the host through the pinned buffer: queue the copy and an event ...
-void scal_to_host_async(tofe_ctx *c)
+// Device scalars (of nvec vectors) to the host through the pinned buffer: queue the copy and
+// an event ...
+void scal_to_host_async(tofe_ctx *c, int nvec = 1)
 {
-  TOFE_HIP(hipMemcpyAsync(c->hscal, c->scal.p, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost,
-                          c->stream));
+  TOFE_HIP(hipMemcpyAsync(c->hscal, c->scal.p, (size_t)SL_COUNT * nvec * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
   TOFE_HIP(hipEventRecord(c->ev_scal, c->stream));
 }
 
 // ... and wait for them.
-void scal_wait(tofe_ctx *c, double *h)
+void scal_wait(tofe_ctx *c, double *h, int nvec = 1)
 {
   TOFE_HIP(hipEventSynchronize(c->ev_scal));
-  std::memcpy(h, c->hscal, SL_COUNT * sizeof(double));
+  std::memcpy(h, c->hscal, (size_t)SL_COUNT * nvec * sizeof(double));
 }
 
 void read_scal(tofe_ctx *c, double *h)
@@ -553,42 +602,66 @@ void read_scal(tofe_ctx *c, double *h)
 }
 
 // CG / flexible CG on the fine level with the KSPConvergedDefault test
-// ||r|| <= max(rtol ||b||, 1e-50) (FEAnalysis.cpp:390-393).
-int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, int *its_out,
-              double *rn_out, double *bn_out)
+// ||r|| <= max(rtol ||b||, 1e-50) (FEAnalysis.cpp:390-393), for nvec right-hand sides at
+// once: every launch serves all of them (vector v of F, U and the work vectors at v * ndof,
+// its scalars at v * SL_COUNT), each with its own recurrence, test and reason.  A right-hand
+// side that has finished is frozen: its alpha and beta are zero from then on, so it takes
+// no further steps while the others go on.  its_out, reason_out, rn_out and bn_out hold nvec
+// values.
+void solve_block(tofe_ctx *c, const double *F, double *U, int nvec, double rtol, int maxit,
+                 int *its_out, int *reason_out, double *rn_out, double *bn_out)
 {
   const hipStream_t s = c->stream;
   const int64_t n = c->ndof;
   double *r = c->vr.p, *z = c->vz.p, *p = c->vp.p, *q = c->vq.p;
   const bool flex = c->built_precision == TOFE_PREC_MIXED;
   const int nfb = level_blocks<double>(c, 0);
-  double h[SL_COUNT];
+  std::vector<double> hv((size_t)SL_COUNT * nvec);
+  double *h = hv.data();
+  auto H = [&](int v, int slot) { return h[(size_t)v * SL_COUNT + slot]; };
 
+  if (c->frozen_dirty) {
+    set_frozen(s, c->scal.p, nvec, 0ull);
+    c->frozen_dirty = false;
+  }
   EpiArgs<double> er;
-  er.y = r; er.b = F;
+  er.y = r; er.b = F; er.nvec = nvec;
   level_apply<double>(c, 0, U, EPI_RESID, er);
-  dot_into(c, F, F, n, SL_BB);
-  dot_into(c, r, r, n, SL_RR);
-  read_scal(c, h);
-  const double bnorm = std::sqrt(h[SL_BB]);
-  double rn = std::sqrt(h[SL_RR]);
-  const double rnorm0 = bnorm > 0 ? bnorm : rn;
-  const double ttol = std::max(rtol * rnorm0, 1e-50);
-  const int conv = (rtol * rnorm0 > 1e-50) ? TOFE_CONVERGED_RTOL : TOFE_CONVERGED_ATOL;
-  *bn_out = bnorm;
-  int k = 0, reason = 0;
-  if (std::isnan(rn) || std::isinf(rn)) reason = TOFE_DIVERGED_NANORINF;
-  else if (rn <= ttol) reason = conv;
-  if (!reason) {
-    precondition(c, r, z);
-    dot_into(c, r, z, n, SL_RZ);
-    TOFE_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  dot_into(c, F, F, n, SL_BB, POST_NONE, nvec);
+  dot_into(c, r, r, n, SL_RR, POST_NONE, nvec);
+  scal_to_host_async(c, nvec);
+  scal_wait(c, h, nvec);
+  std::vector<double> rn(nvec), rnorm0(nvec), ttol(nvec), rn_prev(nvec), rate(nvec, 0.5);
+  std::vector<int> conv(nvec), reason(nvec, 0), its(nvec, 0);
+  int live = 0;
+  unsigned long long done_mask = 0;
+  for (int v = 0; v < nvec; ++v) {
+    const double bnorm = std::sqrt(H(v, SL_BB));
+    rn[v] = std::sqrt(H(v, SL_RR));
+    rnorm0[v] = bnorm > 0 ? bnorm : rn[v];
+    ttol[v] = std::max(rtol * rnorm0[v], 1e-50);
+    conv[v] = (rtol * rnorm0[v] > 1e-50) ? TOFE_CONVERGED_RTOL : TOFE_CONVERGED_ATOL;
+    bn_out[v] = bnorm;
+    if (std::isnan(rn[v]) || std::isinf(rn[v])) reason[v] = TOFE_DIVERGED_NANORINF;
+    else if (rn[v] <= ttol[v]) reason[v] = conv[v];
+    if (reason[v]) done_mask |= 1ull << v;
+    else ++live;
+    rn_prev[v] = rn[v];
+  }
+  int k = 0;
+  if (live) {
+    if (done_mask) {  // right-hand sides already solved take no steps
+      set_frozen(s, c->scal.p, nvec, done_mask);
+      c->frozen_dirty = true;
+    }
+    precondition(c, r, z, nullptr, false, nvec);
+    dot_into(c, r, z, n, SL_RZ, POST_NONE, nvec);
+    TOFE_HIP(hipMemcpyAsync(p, z, n * nvec * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
   // The next preconditioner application and search direction depend on the device
-  // scalars only, so they are queued before the host reads this step's residual norm
-  // whenever the step is predicted (from the last contraction rate) not to converge: the
-  // device then never waits for the host's test.  A converged step leaves them unused.
-  double rn_prev = rn, rate = 0.5;
+  // scalars only, so they are queued before the host reads this step's residual norms
+  // whenever no right-hand side is predicted (from its last contraction rate) to converge:
+  // the device then never waits for the host's test.  A converged step leaves them unused.
   // mixed: the FCG update also forms the next V-cycle's first fine-level step
   const FirstCheb first = flex ? vcycle_first_step(c) : FirstCheb();
   // CG and flexible CG apply the iterate's update x += alpha p in the next search-direction
@@ -596,56 +669,89 @@ int solve_one(tofe_ctx *c, const double *F, double *U, double rtol, int maxit, i
   bool x_pending = false;
   auto next_direction = [&] {
     if (flex) {
-      const int nparts = precondition(c, r, z, q, first.b32 != nullptr);
-      fcg_update_xp(s, p, z, U, n, c->partial.p, nparts, c->scal.p);
-      x_pending = false;
+      const int nparts = precondition(c, r, z, q, first.b32 != nullptr, nvec);
+      fcg_update_xp(s, p, z, U, n, c->partial.p, nparts, c->scal.p, nvec);
     } else {
-      precondition(c, r, z);
-      dot_into(c, r, z, n, SL_TMP, POST_BETA_CG);
-      cg_update_xp(s, p, z, U, n, c->scal.p, SL_BETA, SL_ALPHA);
-      x_pending = false;
+      precondition(c, r, z, nullptr, false, nvec);
+      dot_into(c, r, z, n, SL_TMP, POST_BETA_CG, nvec);
+      cg_update_xp(s, p, z, U, n, c->scal.p, SL_BETA, SL_ALPHA, nvec);
     }
+    x_pending = false;
   };
-  while (!reason) {
-    if (k >= maxit) { reason = TOFE_DIVERGED_ITS; break; }
+  while (live) {
+    if (k >= maxit) {
+      for (int v = 0; v < nvec; ++v)
+        if (!reason[v]) {
+          reason[v] = TOFE_DIVERGED_ITS;
+          its[v] = k;
+        }
+      break;
+    }
     EpiArgs<double> ep;
-    ep.y = q; ep.partial = c->partial.p;
+    ep.y = q; ep.partial = c->partial.p; ep.nvec = nvec;
     if (flex) {
       ep.b = r;
       ep.partial2 = c->partial2.p;
       level_apply<double>(c, 0, p, EPI_APPLY_DOT2, ep);
-      fcg_update_r(s, r, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p, first);
+      fcg_update_r(s, r, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p, first,
+                   nvec);
       x_pending = true;
-      reduce_final(s, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p);
+      reduce_final(s, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p, nvec);
     } else {
       level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
-      reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p);
-      cg_update_r(s, r, q, n, c->scal.p, SL_ALPHA, c->partial.p);
+      reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p, nvec);
+      cg_update_r(s, r, q, n, c->scal.p, SL_ALPHA, c->partial.p, nvec);
       x_pending = true;
-      reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p);
+      reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p, nvec);
     }
-    scal_to_host_async(c);
-    const bool ahead = k + 1 < maxit && rn_prev * std::min(rate, 1.0) > 3 * ttol;
+    scal_to_host_async(c, nvec);
+    bool ahead = k + 1 < maxit;
+    for (int v = 0; v < nvec && ahead; ++v)
+      if (!reason[v] && !(rn_prev[v] * std::min(rate[v], 1.0) > 3 * ttol[v])) ahead = false;
     if (ahead) next_direction();
-    scal_wait(c, h);
+    scal_wait(c, h, nvec);
     ++k;
-    if (!flex && h[SL_RZ] < 0) { reason = TOFE_DIVERGED_INDEFINITE_PC; break; }
-    if (!(h[SL_PQ] > 0)) {
-      reason = std::isnan(h[SL_PQ]) ? TOFE_DIVERGED_NANORINF : TOFE_DIVERGED_INDEFINITE_MAT;
-      break;
+    unsigned long long newly = 0;
+    for (int v = 0; v < nvec; ++v) {
+      if (reason[v]) continue;
+      int rs = 0;
+      const double rnv = std::sqrt(H(v, SL_RR));
+      if (!flex && H(v, SL_RZ) < 0) rs = TOFE_DIVERGED_INDEFINITE_PC;
+      else if (!(H(v, SL_PQ) > 0))
+        rs = std::isnan(H(v, SL_PQ)) ? TOFE_DIVERGED_NANORINF : TOFE_DIVERGED_INDEFINITE_MAT;
+      else {
+        rn[v] = rnv;
+        if (std::isnan(rnv) || std::isinf(rnv)) rs = TOFE_DIVERGED_NANORINF;
+        else if (rnv <= ttol[v]) rs = conv[v];
+        else if (rnv >= 1e5 * rnorm0[v]) rs = TOFE_DIVERGED_DTOL;
+      }
+      if (rs) {
+        reason[v] = rs;
+        its[v] = k;
+        newly |= 1ull << v;
+        --live;
+      } else {
+        if (rn_prev[v] > 0) rate[v] = rnv / rn_prev[v];
+        rn_prev[v] = rnv;
+      }
     }
-    rn = std::sqrt(h[SL_RR]);
-    if (std::isnan(rn) || std::isinf(rn)) { reason = TOFE_DIVERGED_NANORINF; break; }
-    if (rn <= ttol) { reason = conv; break; }
-    if (rn >= 1e5 * rnorm0) { reason = TOFE_DIVERGED_DTOL; break; }
-    if (rn_prev > 0) rate = rn / rn_prev;
-    rn_prev = rn;
+    if (!live) break;
+    if (newly) {
+      // frozen from here on; a finished right-hand side's owed update of this step is still
+      // applied by the next search-direction kernel (its alpha is already formed)
+      done_mask |= newly;
+      set_frozen(s, c->scal.p, nvec, done_mask);
+      c->frozen_dirty = true;
+    }
     if (!ahead) next_direction();
   }
-  if (x_pending) fcg_update_x(s, U, p, n, c->scal.p);
-  *its_out = k;
-  *rn_out = rn;
-  return reason;
+  // the last step's owed update (right-hand sides frozen earlier have alpha = 0)
+  if (x_pending) fcg_update_x(s, U, p, n, c->scal.p, nvec);
+  for (int v = 0; v < nvec; ++v) {
+    its_out[v] = its[v];
+    reason_out[v] = reason[v];
+    rn_out[v] = rn[v];
+  }
 }
 
 // lambda_max(D^-1 A) of level l from eig_its steps of Jacobi-preconditioned CG: the CG
@@ -701,8 +807,8 @@ void lanczos_fine_enqueue(tofe_ctx *c)
         c->lv32.emplace_back(new Level<float>());
         alloc_level32(c, *c->lv32.back(), l);
       }
-      c->vr32.alloc(c->ndof);
-      c->vz32.alloc(c->ndof);
+      c->vr32.alloc((size_t)c->ndof * c->nvcap);
+      c->vz32.alloc((size_t)c->ndof * c->nvcap);
       TOFE_HIP(hipStreamSynchronize(c->stream));
     }
     convert<double, float>(s4, c->lv64[0]->dinv.p, c->lv32[0]->dinv.p, c->ndof);
@@ -830,12 +936,12 @@ void coarse_inverse_enqueue(tofe_ctx *c)
   if (c->precision == TOFE_PREC_MIXED) {
     c->X32r.alloc((size_t)n * c->ldc);
     c->XT32r.alloc((size_t)n * c->ldc);
-    c->ctmp32.alloc(n);
+    c->ctmp32.alloc((size_t)n * c->nvcap);
     tri_pack<float>(s, c->chol_x.p, n, c->ldc, c->X32r.p, c->XT32r.p);
   } else {
     c->X64r.alloc((size_t)n * c->ldc);
     c->XT64r.alloc((size_t)n * c->ldc);
-    c->ctmp64.alloc(n);
+    c->ctmp64.alloc((size_t)n * c->nvcap);
     tri_pack<double>(s, c->chol_x.p, n, c->ldc, c->X64r.p, c->XT64r.p);
   }
 }
@@ -992,18 +1098,46 @@ void solve_dev(tofe_ctx *c, const double *dF, double *dU, int nrhs, double rtol,
   if (nrhs < 1 || !dF || !dU) throw Error(TOFE_ERR_ARG, "bad solve arguments");
   Range range(c->built_precision == TOFE_PREC_MIXED ? "solve:mixed" : "solve:fp64");
   const double t0 = now_s();
-  int k = 0, rs = 0;
-  double rn = 0, bn = 0;
-  for (int i = 0; i < nrhs; ++i)
-    rs = solve_one(c, dF + (size_t)i * c->ndof, dU + (size_t)i * c->ndof, rtol, maxit, &k, &rn, &bn);
+  ensure_nvec(c, std::min(nrhs, kMaxBlockVec));
+  int kmax = 0, rs_out = 0;
+  double rn_last = 0, bn_last = 0;
+  for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec) {
+    const int nv = std::min(kMaxBlockVec, nrhs - i0);
+    int k[kMaxBlockVec], rs[kMaxBlockVec];
+    double rn[kMaxBlockVec], bn[kMaxBlockVec];
+    solve_block(c, dF + (size_t)i0 * c->ndof, dU + (size_t)i0 * c->ndof, nv, rtol, maxit, k, rs,
+                rn, bn);
+    for (int v = 0; v < nv; ++v) {
+      kmax = std::max(kmax, k[v]);
+      // the first failing right-hand side's reason, else the last one's
+      if (rs_out >= 0) rs_out = rs[v];
+      rn_last = rn[v];
+      bn_last = bn[v];
+    }
+  }
   TOFE_HIP(hipStreamSynchronize(c->stream));
   c->stats.solve_seconds = now_s() - t0;
-  c->stats.its = k;
-  c->stats.reason = rs;
-  c->stats.rnorm = rn;
-  c->stats.bnorm = bn;
-  if (its) *its = k;
-  if (reason) *reason = rs;
+  c->stats.its = kmax;
+  c->stats.reason = rs_out;
+  c->stats.rnorm = rn_last;
+  c->stats.bnorm = bn_last;
+  if (its) *its = kmax;
+  if (reason) *reason = rs_out;
+}
+
+// Y = M X for nrhs vectors with the assembled mass form (chunks of up to 16 vectors per
+// launch for the consistent form)
+void apply_mass(tofe_ctx *c, const double *X, double *Y, int nrhs)
+{
+  if (c->mass_form == TOFE_MASS_CONSISTENT) {
+    for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec)
+      apply_mass_consistent(c->stream, c->mV.p, c->mass_m0, c->has_mlump ? c->mlump.p : nullptr,
+                            c->has_fix ? c->dfix.p : nullptr, c->desc.nx, c->desc.ny, c->desc.nz,
+                            X + (size_t)i0 * c->ndof, Y + (size_t)i0 * c->ndof,
+                            std::min(kMaxBlockVec, nrhs - i0));
+  } else {
+    diag_mul(c->stream, c->Mdiag.p, X, Y, c->ndof, nrhs);
+  }
 }
 
 void h2d(tofe_ctx *c, double *d, const double *h, size_t n)
@@ -1466,15 +1600,24 @@ int tofe_solve(tofe_ctx *c, const double *F, double *U, int nrhs, double rtol, i
 {
   return guarded(c, "tofe_solve", [&] {
     if (!F || !U || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad solve arguments");
-    int k = 0, rs = 0;
-    for (int i = 0; i < nrhs; ++i) {
-      h2d(c, c->vF.p, F + (size_t)i * c->ndof, c->ndof);
-      h2d(c, c->vU.p, U + (size_t)i * c->ndof, c->ndof);
-      solve_dev(c, c->vF.p, c->vU.p, 1, rtol, maxit, &k, &rs);
-      d2h(c, U + (size_t)i * c->ndof, c->vU.p, c->ndof);
+    need_assembled(c);
+    ensure_nvec(c, std::min(nrhs, kMaxBlockVec));
+    int kmax = 0, rs_out = 0;
+    for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec) {
+      const int nv = std::min(kMaxBlockVec, nrhs - i0);
+      const size_t off = (size_t)i0 * c->ndof, len = (size_t)nv * c->ndof;
+      h2d(c, c->vF.p, F + off, len);
+      h2d(c, c->vU.p, U + off, len);
+      int k = 0, rs = 0;
+      solve_dev(c, c->vF.p, c->vU.p, nv, rtol, maxit, &k, &rs);
+      d2h(c, U + off, c->vU.p, len);
+      kmax = std::max(kmax, k);
+      if (rs_out >= 0) rs_out = rs;
     }
-    if (its) *its = k;
-    if (reason) *reason = rs;
+    c->stats.its = kmax;
+    c->stats.reason = rs_out;
+    if (its) *its = kmax;
+    if (reason) *reason = rs_out;
   });
 }
 
@@ -1483,10 +1626,11 @@ int tofe_apply_K_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
   return guarded(c, "tofe_apply_K_dev", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!dX || !dY || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
-    for (int i = 0; i < nrhs; ++i) {
+    for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec) {  // one launch per chunk of vectors
       EpiArgs<double> ep;
-      ep.y = dY + (size_t)i * c->ndof;
-      level_apply<double>(c, 0, dX + (size_t)i * c->ndof, EPI_APPLY, ep);
+      ep.y = dY + (size_t)i0 * c->ndof;
+      ep.nvec = std::min(kMaxBlockVec, nrhs - i0);
+      level_apply<double>(c, 0, dX + (size_t)i0 * c->ndof, EPI_APPLY, ep);
     }
     TOFE_HIP(hipStreamSynchronize(c->stream));
   });
@@ -1497,12 +1641,16 @@ int tofe_apply_K(tofe_ctx *c, const double *X, double *Y, int nrhs)
   return guarded(c, "tofe_apply_K", [&] {
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!X || !Y || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
-    for (int i = 0; i < nrhs; ++i) {
-      h2d(c, c->vp.p, X + (size_t)i * c->ndof, c->ndof);
+    ensure_nvec(c, std::min(nrhs, kMaxBlockVec));
+    for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec) {
+      const int nv = std::min(kMaxBlockVec, nrhs - i0);
+      const size_t off = (size_t)i0 * c->ndof, len = (size_t)nv * c->ndof;
+      h2d(c, c->vp.p, X + off, len);
       EpiArgs<double> ep;
       ep.y = c->vq.p;
+      ep.nvec = nv;
       level_apply<double>(c, 0, c->vp.p, EPI_APPLY, ep);
-      d2h(c, Y + (size_t)i * c->ndof, c->vq.p, c->ndof);
+      d2h(c, Y + off, c->vq.p, len);
     }
   });
 }
@@ -1518,16 +1666,38 @@ int tofe_assemble_mass(tofe_ctx *c, double density, const double *mlump)
     elem *= c->desc.hy;
     elem *= c->desc.hz;
     const double m0 = 1.0 / std::pow(2.0, 3) / 3 * elem * density;
-    if (!c->Mdiag.p) c->Mdiag.alloc(c->ndof);
-    const double *dl = nullptr;
-    if (mlump) {
-      h2d(c, c->vq.p, mlump, c->ndof);
-      dl = c->vq.p;
+    c->mass_m0 = m0;
+    if (c->mass_form == TOFE_MASS_CONSISTENT) {
+      c->mV.alloc(c->ne);
+      TOFE_HIP(hipMemcpyAsync(c->mV.p, c->rho.p, c->ne * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+      c->has_mlump = mlump != nullptr;
+      if (mlump) {
+        c->mlump.alloc(c->ndof);
+        h2d(c, c->mlump.p, mlump, c->ndof);
+      }
+    } else {
+      if (!c->Mdiag.p) c->Mdiag.alloc(c->ndof);
+      const double *dl = nullptr;
+      if (mlump) {
+        h2d(c, c->vq.p, mlump, c->ndof);
+        dl = c->vq.p;
+      }
+      mass_diag(c->stream, c->rho.p, c->desc.nx, c->desc.ny, c->desc.nz, m0, dl,
+                c->has_fix ? c->dfix.p : nullptr, c->Mdiag.p);
     }
-    mass_diag(c->stream, c->rho.p, c->desc.nx, c->desc.ny, c->desc.nz, m0, dl,
-              c->has_fix ? c->dfix.p : nullptr, c->Mdiag.p);
     TOFE_HIP(hipStreamSynchronize(c->stream));
     c->has_mass = true;
+  });
+}
+
+int tofe_set_mass_form(tofe_ctx *c, int form)
+{
+  return guarded(c, "tofe_set_mass_form", [&] {
+    if (form != TOFE_MASS_LUMPED && form != TOFE_MASS_CONSISTENT)
+      throw Error(TOFE_ERR_ARG, "unknown mass form");
+    if (form != c->mass_form) c->has_mass = false;  // assemble the mass again
+    c->mass_form = form;
   });
 }
 
@@ -1536,7 +1706,7 @@ int tofe_apply_M_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
   return guarded(c, "tofe_apply_M_dev", [&] {
     if (!c->has_mass) throw Error(TOFE_ERR_STATE, "no mass matrix (call tofe_assemble_mass)");
     if (!dX || !dY || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
-    diag_mul(c->stream, c->Mdiag.p, dX, dY, c->ndof, nrhs);
+    apply_mass(c, dX, dY, nrhs);
     TOFE_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -1546,10 +1716,13 @@ int tofe_apply_M(tofe_ctx *c, const double *X, double *Y, int nrhs)
   return guarded(c, "tofe_apply_M", [&] {
     if (!c->has_mass) throw Error(TOFE_ERR_STATE, "no mass matrix (call tofe_assemble_mass)");
     if (!X || !Y || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
-    for (int i = 0; i < nrhs; ++i) {
-      h2d(c, c->vp.p, X + (size_t)i * c->ndof, c->ndof);
-      diag_mul(c->stream, c->Mdiag.p, c->vp.p, c->vq.p, c->ndof, 1);
-      d2h(c, Y + (size_t)i * c->ndof, c->vq.p, c->ndof);
+    ensure_nvec(c, std::min(nrhs, kMaxBlockVec));
+    for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec) {
+      const int nv = std::min(kMaxBlockVec, nrhs - i0);
+      const size_t off = (size_t)i0 * c->ndof, len = (size_t)nv * c->ndof;
+      h2d(c, c->vp.p, X + off, len);
+      apply_mass(c, c->vp.p, c->vq.p, nv);
+      d2h(c, Y + off, c->vq.p, len);
     }
   });
 }
@@ -1565,7 +1738,7 @@ int tofe_element_quadratic(tofe_ctx *c, const double *phi, int nvec, double *kq,
     if (mq) dm.alloc(ne);
     h2d(c, dphi.p, phi, nd);
     elem_quad(c->stream, dphi.p, c->dKE.p, c->desc.nx, c->desc.ny, c->desc.nz, c->ndof, nvec, dk.p,
-              mq ? dm.p : nullptr);
+              mq ? dm.p : nullptr, c->mass_form == TOFE_MASS_CONSISTENT);
     d2h(c, kq, dk.p, ne);
     if (mq) d2h(c, mq, dm.p, ne);
   });
@@ -1600,9 +1773,10 @@ int tofe_apply_Ks_dev(tofe_ctx *c, const double *dX, double *dY, int nrhs)
     if (!c->has_stress) throw Error(TOFE_ERR_STATE, "no stress state (call tofe_set_stress)");
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!dX || !dY || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
-    for (int i = 0; i < nrhs; ++i)
+    for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec)
       apply_ks(c->stream, c->Hs.p, c->Es.p, c->has_fix ? c->dfix.p : nullptr, c->desc.nx,
-               c->desc.ny, c->desc.nz, dX + (size_t)i * c->ndof, dY + (size_t)i * c->ndof);
+               c->desc.ny, c->desc.nz, dX + (size_t)i0 * c->ndof, dY + (size_t)i0 * c->ndof,
+               std::min(kMaxBlockVec, nrhs - i0));
     TOFE_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -1613,11 +1787,14 @@ int tofe_apply_Ks(tofe_ctx *c, const double *X, double *Y, int nrhs)
     if (!c->has_stress) throw Error(TOFE_ERR_STATE, "no stress state (call tofe_set_stress)");
     if (!c->design_set) throw Error(TOFE_ERR_STATE, "no design set");
     if (!X || !Y || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad apply arguments");
-    for (int i = 0; i < nrhs; ++i) {
-      h2d(c, c->vp.p, X + (size_t)i * c->ndof, c->ndof);
+    ensure_nvec(c, std::min(nrhs, kMaxBlockVec));
+    for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec) {
+      const int nv = std::min(kMaxBlockVec, nrhs - i0);
+      const size_t off = (size_t)i0 * c->ndof, len = (size_t)nv * c->ndof;
+      h2d(c, c->vp.p, X + off, len);
       apply_ks(c->stream, c->Hs.p, c->Es.p, c->has_fix ? c->dfix.p : nullptr, c->desc.nx,
-               c->desc.ny, c->desc.nz, c->vp.p, c->vq.p);
-      d2h(c, Y + (size_t)i * c->ndof, c->vq.p, c->ndof);
+               c->desc.ny, c->desc.nz, c->vp.p, c->vq.p, nv);
+      d2h(c, Y + off, c->vq.p, len);
     }
   });
 }
@@ -1670,7 +1847,10 @@ int tofe_pc_apply_dev(tofe_ctx *c, const double *dR, double *dZ, int nrhs)
   return guarded(c, "tofe_pc_apply_dev", [&] {
     need_assembled(c);
     if (!dR || !dZ || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad pc arguments");
-    for (int i = 0; i < nrhs; ++i) precondition(c, dR + (size_t)i * c->ndof, dZ + (size_t)i * c->ndof);
+    ensure_nvec(c, std::min(nrhs, kMaxBlockVec));
+    for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec)  // one V-cycle pass per chunk of vectors
+      precondition(c, dR + (size_t)i0 * c->ndof, dZ + (size_t)i0 * c->ndof, nullptr, false,
+                   std::min(kMaxBlockVec, nrhs - i0));
     TOFE_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -1680,10 +1860,13 @@ int tofe_pc_apply(tofe_ctx *c, const double *R, double *Z, int nrhs)
   return guarded(c, "tofe_pc_apply", [&] {
     need_assembled(c);
     if (!R || !Z || nrhs < 1) throw Error(TOFE_ERR_ARG, "bad pc arguments");
-    for (int i = 0; i < nrhs; ++i) {
-      h2d(c, c->vp.p, R + (size_t)i * c->ndof, c->ndof);
-      precondition(c, c->vp.p, c->vq.p);
-      d2h(c, Z + (size_t)i * c->ndof, c->vq.p, c->ndof);
+    ensure_nvec(c, std::min(nrhs, kMaxBlockVec));
+    for (int i0 = 0; i0 < nrhs; i0 += kMaxBlockVec) {
+      const int nv = std::min(kMaxBlockVec, nrhs - i0);
+      const size_t off = (size_t)i0 * c->ndof, len = (size_t)nv * c->ndof;
+      h2d(c, c->vp.p, R + off, len);
+      precondition(c, c->vp.p, c->vq.p, nullptr, false, nv);
+      d2h(c, Z + off, c->vq.p, len);
     }
   });
 }

@@ -56,6 +56,10 @@ struct EpiArgs {
   double *xout64 = nullptr;
   const double *q64 = nullptr;
   T c1 = 0, c2 = 0;
+  // Vectors per launch: vector v (blockIdx.y) of u, y, b, d, xout, xout64 and q64 starts v
+  // level-DOF counts after the first and its partial sums v gridDim.x entries after the
+  // first; dinv and the operator are shared by all vectors.
+  int nvec = 1;
 };
 
 // Matrix-free fine operator y = K(E) u with MatZeroRowsColumns Dirichlet rows/cols.
@@ -143,9 +147,11 @@ void coarse_inverse_factor(hipStream_t s, double *A, double *Lb, double *X, int 
 // zero outside the triangle.
 template <typename T>
 void tri_pack(hipStream_t s, const double *X, int n, int ld, T *Xr, T *XTr);
-// x = A^-1 b = X^T (X b) with the packed triangles; tmp holds n values.
+// x = A^-1 b = X^T (X b) with the packed triangles for nvec right-hand sides; b, tmp and x
+// hold nvec vectors of n values one after another.
 template <typename T>
-void coarse_solve_tri(hipStream_t s, const T *Xr, const T *XTr, int n, int ld, const T *b, T *tmp, T *x);
+void coarse_solve_tri(hipStream_t s, const T *Xr, const T *XTr, int n, int ld, const T *b, T *tmp, T *x,
+                      int nvec = 1);
 
 // ---- level-1 Galerkin operator from E and KE (k_galerkin.hip) -------------
 struct MaskedElems {
@@ -161,15 +167,21 @@ void fine_diag(hipStream_t s, const double *E, const double *KE, const uint8_t *
                int nz, double *diag, double *dinv);
 
 // ---- vector kernels (k_vec.hip) -------------------------------------------
+// nvec > 1: nvec vectors of n values stored one after another (vector v at v * n), each
+// with its own per-block partial sums (v * grid blocks on) and scalars (v * SL_COUNT on);
+// dinv, the operator and the transfer weights are shared.
 template <typename T>
-void dot_partial(hipStream_t s, const T *a, const T *b, int64_t n, double *partial);
+void dot_partial(hipStream_t s, const T *a, const T *b, int64_t n, double *partial, int nvec = 1);
 void reduce_final(hipStream_t s, const double *partial, int nparts, double *out, int post,
-                  double *scal);
+                  double *scal, int nvec = 1);
+// Right-hand sides of a block solve that have converged (bit v of mask): their alpha and
+// beta are forced to zero from the next reduction on.
+void set_frozen(hipStream_t s, double *scal, int nvec, unsigned long long mask);
 template <typename T>
-void cheb_zero(hipStream_t s, const T *b, const T *dinv, T c2, T *d, T *x, int64_t n);
+void cheb_zero(hipStream_t s, const T *b, const T *dinv, T c2, T *d, T *x, int64_t n, int nvec = 1);
 // first Chebyshev step from an FP64 right-hand side: b32 = (float) b64, d = x = c2 dinv b32
 void cheb_zero_from64(hipStream_t s, const double *b64, float *b32, const float *dinv, float c2,
-                      float *d, float *x, int64_t n);
+                      float *d, float *x, int64_t n, int nvec = 1);
 // Optional output of restrict_op: the coarse level's first Chebyshev step from x = 0,
 // d = x = c2 dinv b (k_cheb_zero), when d is set.
 template <typename T>
@@ -180,18 +192,18 @@ struct FirstStep {
 };
 template <typename T>
 void restrict_op(hipStream_t s, const T *rf, int Nfx, int Nfy, int Nfz, T *bc, int Ncx, int Ncy,
-                 int Ncz, const FirstStep<T> &fs = FirstStep<T>());
+                 int Ncz, const FirstStep<T> &fs = FirstStep<T>(), int nvec = 1);
 template <typename T>
 void prolong_add(hipStream_t s, const T *xc, int Ncx, int Ncy, int Ncz, T *xf, int Nfx, int Nfy,
-                 int Nfz);
+                 int Nfz, int nvec = 1);
 template <typename Ta, typename Tb>
 void convert(hipStream_t s, const Ta *a, Tb *b, int64_t n);
 void fill_d(hipStream_t s, double *a, double v, int64_t n);
 void cg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *scal,
-                 int alpha_slot, double *partial);
+                 int alpha_slot, double *partial, int nvec = 1);
 // x += scal[alpha_slot] p (the iterate's update, deferred to where p is read), p = z + beta p
 void cg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n, const double *scal,
-                  int beta_slot, int alpha_slot);
+                  int beta_slot, int alpha_slot, int nvec = 1);
 template <typename T>
 void power_start(hipStream_t s, T *v, int64_t n);
 template <typename T>
@@ -211,10 +223,11 @@ struct FirstCheb {
 };
 void fcg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *pq_part,
                   const double *pr_part, int nparts, double *scal, double *partial,
-                  const FirstCheb &fc = FirstCheb());
+                  const FirstCheb &fc = FirstCheb(), int nvec = 1);
 void fcg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n,
-                   const double *zq_part, int nparts, double *scal);
-void fcg_update_x(hipStream_t s, double *x, const double *p, int64_t n, const double *scal);
+                   const double *zq_part, int nparts, double *scal, int nvec = 1);
+void fcg_update_x(hipStream_t s, double *x, const double *p, int64_t n, const double *scal,
+                  int nvec = 1);
 // Fused Jacobi-CG steps of the eigenvalue estimate (no separate reduction launches):
 // lanczos_r: alpha = scal[rz_slot] / sum(pq_part[0..npq)), recorded; r -= alpha q; partials
 // of r.(D^-1 r).  lanczos_p: beta = sum(rz_part) / scal[rz_old], recorded; scal[rz_new] =
@@ -229,7 +242,7 @@ void lanczos_p(hipStream_t s, T *p, const T *r, const T *dinv, int64_t n, const 
 // device scalar slots (ctx->scal)
 enum Slot : int {
   SL_RZ = 0, SL_PQ = 1, SL_RR = 2, SL_ALPHA = 3, SL_BETA = 4, SL_BB = 5, SL_PR = 6, SL_ZQ = 7,
-  SL_RZA = 8, SL_RZB = 9, SL_TMP = 11, SL_COUNT = 16
+  SL_RZA = 8, SL_RZB = 9, SL_TMP = 11, SL_FROZEN = 12, SL_COUNT = 16
 };
 // reduce_final post-processing
 enum Post : int {
@@ -277,8 +290,9 @@ struct ShapeData {
 };
 void stress_h(hipStream_t s, const double *U, int nx, int ny, int nz, const ShapeData &sd,
               double *H);
+// y = Ks x for nvec vectors stored one after another (H read once for up to four of them)
 void apply_ks(hipStream_t s, const double *H, const double *Es, const uint8_t *fix, int nx,
-              int ny, int nz, const double *x, double *y);
+              int ny, int nz, const double *x, double *y, int nvec = 1);
 void ks_quad(hipStream_t s, const double *H, const double *phi, int nx, int ny, int nz,
              int64_t ndof, int nvec, double *q);
 void stab_rhs(hipStream_t s, const double *phi, const double *Es, const uint8_t *fix, int nx,
@@ -288,8 +302,15 @@ void elem_bilinear(hipStream_t s, const double *A, const double *B, const double
 void mass_diag(hipStream_t s, const double *V, int nx, int ny, int nz, double m0,
                const double *mlump, const uint8_t *fix, double *M);
 void diag_mul(hipStream_t s, const double *d, const double *x, double *y, int64_t n, int nrhs);
+// kq = phi_e^T KE phi_e; mq = sum of phi_e^2 (lumped mass) or phi_e^T (Q (x) I_3) phi_e
+// (consistent mass over m0), per element and vector
 void elem_quad(hipStream_t s, const double *phi, const double *KE, int nx, int ny, int nz,
-               int64_t ndof, int nvec, double *kq, double *mq);
+               int64_t ndof, int nvec, double *kq, double *mq, bool consistent = false);
+// y = M x for nvec vectors with the consistent mass sum_e V_e m0 Q (x) I_3 + diag(mlump) and
+// the Dirichlet rows / columns of MatZeroRowsColumns(M, fixed, 1e-8)
+void apply_mass_consistent(hipStream_t s, const double *V, double m0, const double *mlump,
+                           const uint8_t *fix, int nx, int ny, int nz, const double *x, double *y,
+                           int nvec);
 void sensitivity(hipStream_t s, const double *U, const double *dEdz, const double *KE, int nx,
                  int ny, int nz, double *g);
 
