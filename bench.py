@@ -284,6 +284,12 @@ def main():
     ap.add_argument("--small", action="store_true", help="48x24x24 instead of 192x96x96")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="print the phase-split kernel table of a rocprofv3 trace and exit")
+    ap.add_argument("--workload", choices=["cantilever", "buckling", "vibration", "batch"],
+                    default="cantilever",
+                    help="cantilever: the headline solve (default); buckling: the 128x64x64 "
+                         "column, 6 modes, adjoint gradients and central differences; vibration: "
+                         "the 160x80x80 cantilever, 12 modes with the consistent mass; batch: 32 "
+                         "uneven cantilevers, 4 load cases each, one graph-captured solve")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only launch the fine FP32 Chebyshev kernel and the level-1 FP32 stencil "
                          "Chebyshev kernel 6 times each and a 512 MiB device copy, for rocprofv3 "
@@ -296,6 +302,21 @@ def main():
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box in this "
               "setting has a single MI355X and the path is not sharded here", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "cantilever":
+        import torch
+
+        from topopt3d_amd import workloads
+
+        assert torch.cuda.is_available(), "bench.py needs a GPU"
+        t0 = time.perf_counter()
+        res = getattr(workloads, args.workload)()
+        wall = time.perf_counter() - t0
+        value = res["batch_s"] if args.workload == "batch" else res["precisions"]["mixed"]["eigen_s"]
+        res.update({"metric": f"{args.workload} analysis on one MI355X", "unit": "s",
+                    "value": value, "wall_s": wall,
+                    "n_gpus": 1, "data": "synthetic"})
+        print(json.dumps(res))
+        return
 
     import torch
 

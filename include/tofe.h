@@ -315,6 +315,41 @@ typedef struct tofe_kernel_time {
 int tofe_time_kernels(tofe_ctx *ctx, int start, tofe_kernel_time *out, int max_kinds,
                       int *nkinds);
 
+/* Launch shapes from measurement (no reference counterpart: PETSc has no such knob): for
+ * the precision of the last assemble, times the fine operator's Chebyshev step over
+ * candidate z-chunk lengths and every coarse level's stencil kernel one thread per node
+ * and one per DOF (nrep launches each), keeps the fastest chunk length and the node-count
+ * threshold below which levels run per DOF that minimises their summed time.  Outputs
+ * (may be NULL): the chunk length and the threshold. */
+int tofe_autotune(tofe_ctx *ctx, int nrep, int *fine_zlen, int64_t *split_nodes);
+
+/* ---- batches of independent instances ---------------------------------------------
+ * Solves of several contexts (uneven meshes, nrhs right-hand sides each) as one graph:
+ * after each instance's start (r = F - K U, norms, first preconditioner application),
+ * `chunk` iterations of every instance are captured from the instances' streams into one
+ * HIP graph, with the convergence test on the device (a converged right-hand side freezes
+ * itself), and the graph is replayed until every right-hand side has finished or maxit
+ * iterations have run.  Results equal tofe_solve_dev on each instance bit for bit.  No
+ * reference counterpart: it replaces a host loop of FESolve's KSPSolve
+ * (FEAnalysis.cpp:673) over independent problems.  A context belongs to at most one batch
+ * call at a time; the divergence test ||r|| >= 1e5 ||b|| of tofe_solve is not applied. */
+typedef struct tofe_batch tofe_batch;
+typedef struct tofe_batch_stats {
+  double seconds;          /* last tofe_batch_solve_dev, wall time */
+  double capture_seconds;  /* of which: capturing and instantiating the graph */
+  double replay_seconds;   /* of which: graph replays and their convergence reads */
+  int replays;             /* graph launches */
+  int iterations;          /* iterations captured times replays */
+} tofe_batch_stats;
+int tofe_batch_create(tofe_ctx *const *ctxs, int n, tofe_batch **batch);
+int tofe_batch_destroy(tofe_batch *batch);
+const char *tofe_batch_last_error(const tofe_batch *batch);
+/* d_F[i], d_U[i]: nrhs vectors of instance i's ndof back to back (U is the initial guess);
+ * its / reasons (may be NULL): n * nrhs values, instance-major. */
+int tofe_batch_solve_dev(tofe_batch *batch, const double *const *d_F, double *const *d_U,
+                         int nrhs, double rtol, int maxit, int chunk, int *its, int *reasons);
+int tofe_batch_get_stats(const tofe_batch *batch, tofe_batch_stats *stats);
+
 /* ---- on-disk formats ------------------------------------------------------------
  * PETSc binary files as PetscViewerBinaryOpen + VecView / MatView write them (big-endian;
  * Vec: classid 1211214, n, n doubles; sequential AIJ: classid 1211216, rows, cols, nnz,

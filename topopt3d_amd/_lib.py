@@ -47,7 +47,8 @@ EXPORTS = (
     "tofe_petsc_aij_write", "tofe_petsc_aij_read", "tofe_mesh_out", "tofe_print_vals",
     "tofe_set_stress", "tofe_set_stress_dev", "tofe_apply_Ks", "tofe_apply_Ks_dev",
     "tofe_stability_terms", "tofe_element_bilinear", "tofe_compliance_noalloc",
-    "tofe_set_mass_form",
+    "tofe_set_mass_form", "tofe_autotune", "tofe_batch_create", "tofe_batch_destroy",
+    "tofe_batch_last_error", "tofe_batch_solve_dev", "tofe_batch_get_stats",
 )
 
 
@@ -77,6 +78,11 @@ class TofeStats(C.Structure):
 class TofeKernelTime(C.Structure):
     _fields_ = [("level", C.c_int), ("precision", C.c_int), ("epilogue", C.c_int),
                 ("count", C.c_int), ("total_ms", C.c_double)]
+
+
+class TofeBatchStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("capture_seconds", C.c_double),
+                ("replay_seconds", C.c_double), ("replays", C.c_int), ("iterations", C.c_int)]
 
 
 class TofeError(RuntimeError):
@@ -149,6 +155,13 @@ def lib():
         "tofe_stability_terms": ([vp, dp, C.c_int, dp, dp], C.c_int),
         "tofe_element_bilinear": ([vp, dp, C.c_int, dp, dp], C.c_int),
         "tofe_set_mass_form": ([vp, C.c_int], C.c_int),
+        "tofe_autotune": ([vp, C.c_int, ip, i64p], C.c_int),
+        "tofe_batch_create": ([C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
+        "tofe_batch_destroy": ([vp], C.c_int),
+        "tofe_batch_last_error": ([vp], C.c_char_p),
+        "tofe_batch_solve_dev": ([vp, C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_double, C.c_int,
+                                  C.c_int, ip, ip], C.c_int),
+        "tofe_batch_get_stats": ([vp, C.POINTER(TofeBatchStats)], C.c_int),
         "tofe_compliance_noalloc": ([C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double, vp, vp, vp, dp, vp],
                                     C.c_int),
@@ -479,7 +492,56 @@ class Context:
         return [{k: getattr(out[j], k) for k, _ in TofeKernelTime._fields_}
                 for j in range(min(n.value, 256))]
 
+    def autotune(self, nrep=10):
+        """Measured launch shapes for the current precision: (fine z-chunk, split threshold)."""
+        z, t = C.c_int(), C.c_int64()
+        self._check(lib().tofe_autotune(self.h, nrep, C.byref(z), C.byref(t)))
+        return z.value, t.value
+
     def level_lambda(self, level):
         lam = C.c_double()
         self._check(lib().tofe_get_level_lambda(self.h, level, C.byref(lam)))
         return lam.value
+
+
+class Batch:
+    """tofe_batch over contexts of independent instances: their solves as one HIP graph of
+    chunk iterations, replayed until every right-hand side has finished."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h.value for c in self.contexts])
+        self.h = C.c_void_p()
+        rc = lib().tofe_batch_create(arr, len(self.contexts), C.byref(self.h))
+        if rc != TOFE_OK:
+            raise TofeError(rc, "tofe_batch_create failed")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().tofe_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve_dev(self, F_ptrs, U_ptrs, nrhs=1, rtol=1e-8, maxit=1000, chunk=4):
+        """(its, reasons) as (n, nrhs) arrays."""
+        n = len(self.contexts)
+        Fa = (C.c_void_p * n)(*F_ptrs)
+        Ua = (C.c_void_p * n)(*U_ptrs)
+        its = np.zeros(n * nrhs, np.int32)
+        rs = np.zeros(n * nrhs, np.int32)
+        rc = lib().tofe_batch_solve_dev(self.h, Fa, Ua, nrhs, rtol, maxit, chunk,
+                                        its.ctypes.data_as(C.POINTER(C.c_int)),
+                                        rs.ctypes.data_as(C.POINTER(C.c_int)))
+        if rc != TOFE_OK:
+            raise TofeError(rc, lib().tofe_batch_last_error(self.h).decode())
+        return its.reshape(n, nrhs), rs.reshape(n, nrhs)
+
+    def stats(self):
+        st = TofeBatchStats()
+        lib().tofe_batch_get_stats(self.h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in TofeBatchStats._fields_}

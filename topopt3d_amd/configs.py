@@ -157,3 +157,36 @@ def config5(Emin=1e-9) -> Problem:
                     mg_levels=5, name="cantilever-160x80x80")
     pb.extra.update(density=1.0, nev=12)
     return pb
+
+
+def batch_instances(n=32, sizes=tuple(range(16, 97, 2)), seed=4, nrhs=4, lo=0.05, hi=1.0,
+                    Emin=1e-9):
+    """Independent cantilevers of uneven size: instance i has nx_i drawn from `sizes`
+    (SplitMix64(seed)), ny = nz = nx_i / 2, densities U[lo, hi) from SplitMix64(seed * 1000 + i),
+    the x = 0 face clamped, and nrhs load cases: unit point loads at nodes of the x = max face
+    drawn from SplitMix64(seed * 1000 + 500 + i), in directions (u0, u1, u2) / |u| with
+    u_d uniform in [-1, 1) from the same stream."""
+    pick = splitmix64(seed, n)
+    out = []
+    for i in range(n):
+        nx = int(sizes[int(pick[i] % np.uint64(len(sizes)))])
+        nel = (nx, nx // 2, nx // 2)
+        pb = cantilever(nel, seed=seed * 1000 + i, lo=lo, hi=hi, Emin=Emin, mg_levels=0,
+                        name=f"cantilever-{nel[0]}x{nel[1]}x{nel[2]}")
+        r = uniform(seed * 1000 + 500 + i, 5 * nrhs)
+        F = np.zeros((nrhs, pb.ndof))
+        for k in range(nrhs):
+            j = min(int(r[5 * k] * (nel[1] + 1)), nel[1])
+            kz = min(int(r[5 * k + 1] * (nel[2] + 1)), nel[2])
+            d = 2.0 * r[5 * k + 2:5 * k + 5] - 1.0
+            d /= np.linalg.norm(d)
+            F[k, 3 * node_index(nel, nx, j, kz):3 * node_index(nel, nx, j, kz) + 3] = d
+        pb.F = F
+        out.append(pb)
+    return out
+
+
+def config4(nrhs=4) -> list:
+    """32 cantilevers with nx in {16, 18, ..., 96} (seed 4), ny = nz = nx / 2, densities
+    U[0.05,1], p = 3, 4 unit point-load cases each on the x = max face."""
+    return batch_instances(32, tuple(range(16, 97, 2)), seed=4, nrhs=nrhs)

@@ -51,9 +51,37 @@ __global__ __launch_bounds__(NT) void k_reduce_final(const double *__restrict__ 
         break;
       case POST_ALPHA_FCG: scal[SL_ALPHA] = frozen ? 0.0 : scal[SL_PR] / v; break;
       case POST_BETA_FCG: scal[SL_BETA] = frozen ? 0.0 : -v / scal[SL_PQ]; break;
+      case POST_CONV_FCG:
+      case POST_CONV_CG:
+        if (!frozen) {
+          scal[SL_ITS] += 1.0;
+          const double pq = scal[SL_PQ];
+          double rs = 0.0;
+          if (post == POST_CONV_CG && scal[SL_RZ] < 0) rs = TOFE_DIVERGED_INDEFINITE_PC;
+          else if (!(pq > 0)) rs = (pq != pq) ? TOFE_DIVERGED_NANORINF : TOFE_DIVERGED_INDEFINITE_MAT;
+          else if (!(v == v) || v > 1.7976931348623157e308) rs = TOFE_DIVERGED_NANORINF;
+          else if (sqrt(v) <= scal[SL_TTOL]) rs = scal[SL_CONVCODE];
+          if (rs != 0.0) {
+            scal[SL_REASON] = rs;
+            scal[SL_FROZEN] = 1.0;
+          }
+        }
+        break;
       default: break;
     }
   }
+}
+
+__global__ void k_set_conv_state(double *scal, int nvec, ConvState cs, unsigned long long mask)
+{
+  const int v = threadIdx.x;
+  if (v >= nvec) return;
+  double *sv = scal + (int64_t)v * SL_COUNT;
+  sv[SL_TTOL] = cs.ttol[v];
+  sv[SL_CONVCODE] = (double)cs.conv[v];
+  sv[SL_ITS] = 0.0;
+  sv[SL_REASON] = 0.0;
+  sv[SL_FROZEN] = ((mask >> v) & 1ull) ? 1.0 : 0.0;
 }
 
 // scal[v * SL_COUNT + SL_FROZEN] = bit v of mask, v < nvec (the right-hand sides of a block
@@ -460,6 +488,14 @@ void reduce_final(hipStream_t s, const double *partial, int nparts, double *out,
                   double *scal, int nvec)
 {
   k_reduce_final<<<dim3(1, nvec), dim3(NT), 0, s>>>(partial, nparts, out, post, scal);
+  TOFE_LAUNCH_CHECK();
+}
+
+void set_conv_state(hipStream_t s, double *scal, int nvec, const ConvState &cs,
+                    unsigned long long frozen_mask)
+{
+  if (nvec < 1 || nvec > 16) throw Error(TOFE_ERR_ARG, "set_conv_state: 1..16 vectors");
+  k_set_conv_state<<<dim3(1), dim3(64), 0, s>>>(scal, nvec, cs, frozen_mask);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -1140,6 +1140,168 @@ void apply_mass(tofe_ctx *c, const double *X, double *Y, int nrhs)
   }
 }
 
+// Average ms per launch of nrep back-to-back launches of one level operator kernel on the
+// context's stream (HIP events), on a deterministic input with the level's work vectors.
+double time_level_op(tofe_ctx *c, int level, bool f32, Epi epi, int nrep)
+{
+  const hipStream_t s = c->stream;
+  const int64_t nd = c->grids[level].ndof();
+  hipEvent_t e0, e1;
+  TOFE_HIP(hipEventCreate(&e0));
+  TOFE_HIP(hipEventCreate(&e1));
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    auto &lv = *levels<T>(c)[level];
+    Level<double> &l64 = *c->lv64[level];
+    power_start(s, l64.r.p, nd);
+    if constexpr (sizeof(T) == 4) convert<double, float>(s, l64.r.p, lv.x.p, nd);
+    else TOFE_HIP(hipMemcpyAsync(lv.x.p, l64.r.p, nd * sizeof(double), hipMemcpyDeviceToDevice, s));
+    TOFE_HIP(hipMemsetAsync(lv.d.p, 0, nd * sizeof(T), s));
+    EpiArgs<T> ep;
+    ep.y = lv.r.p;
+    ep.b = lv.x.p;
+    ep.dinv = lv.dinv.p;
+    ep.d = lv.d.p;
+    ep.xout = lv.x2.p;
+    ep.c1 = (T)0.5;
+    ep.c2 = (T)0.5;
+    level_apply<T>(c, level, lv.x.p, epi, ep);  // warm-up
+    TOFE_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < nrep; ++k) level_apply<T>(c, level, lv.x.p, epi, ep);
+    TOFE_HIP(hipEventRecord(e1, s));
+  };
+  if (f32) run(float{});
+  else run(double{});
+  TOFE_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  TOFE_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)t / nrep;
+}
+
+// Launch shapes of the current precision's operators from measurement: the fine operator's
+// z-chunk length (element-centric form) over candidate lengths, and the node-count
+// threshold below which a stencil level runs one thread per DOF, chosen to minimise the
+// summed Chebyshev-step time of the coarse levels.  The chosen shapes stay in the context.
+void autotune(tofe_ctx *c, int nrep, int *zlen_out, int64_t *split_out)
+{
+  need_assembled(c);
+  const bool f32 = c->built_precision == TOFE_PREC_MIXED;
+  int best_z = f32 ? c->wht_zlen32 : c->wht_zlen;
+  if (c->ke_wht && !c->fine_mfma) {
+    double best = 1e300;
+    const int keep = best_z;
+    for (int zl : {4, 5, 6, 8, 10, 12, 16, 20, 24}) {
+      if (zl > c->desc.nz + 1 && zl != keep) continue;
+      (f32 ? c->wht_zlen32 : c->wht_zlen) = zl;
+      const double t = time_level_op(c, 0, f32, EPI_CHEB, nrep);
+      if (t < best * 0.995) {  // a longer chunk must win clearly (fewer blocks, less halo)
+        best = t;
+        best_z = zl;
+      }
+    }
+    (f32 ? c->wht_zlen32 : c->wht_zlen) = best_z;
+  }
+  // per coarse level, the time of both stencil modes; thresholds between level sizes
+  std::vector<double> t_node(c->L, 0.0), t_dof(c->L, 0.0);
+  const int64_t keep_split = c->split_nodes;
+  for (int l = 1; l < c->L; ++l) {
+    c->split_nodes = 0;  // every grid one thread per node
+    t_node[l] = time_level_op(c, l, f32, EPI_CHEB, nrep);
+    c->split_nodes = INT64_MAX;  // every grid one thread per DOF
+    t_dof[l] = time_level_op(c, l, f32, EPI_CHEB, nrep);
+  }
+  int64_t best_split = keep_split;
+  double best_sum = 1e300;
+  std::vector<int64_t> cand = {0};
+  for (int l = 1; l < c->L; ++l) cand.push_back(c->grids[l].nn() + 1);
+  for (int64_t T : cand) {
+    double sum = 0;
+    for (int l = 1; l < c->L; ++l) sum += c->grids[l].nn() < T ? t_dof[l] : t_node[l];
+    if (sum < best_sum) {
+      best_sum = sum;
+      best_split = T;
+    }
+  }
+  c->split_nodes = c->L > 1 ? best_split : keep_split;
+  if (zlen_out) *zlen_out = best_z;
+  if (split_out) *split_out = c->split_nodes;
+}
+
+// One iteration of the solve loop for nvec right-hand sides with its convergence test on the
+// device (the reduction of r.r freezes converged columns): the same kernels as solve_block's
+// step and search-direction update, with nothing read back, so it can be captured in a graph.
+void iteration_on_device(tofe_ctx *c, double *U, int nvec)
+{
+  const hipStream_t s = c->stream;
+  const int64_t n = c->ndof;
+  double *r = c->vr.p, *z = c->vz.p, *p = c->vp.p, *q = c->vq.p;
+  const int nfb = level_blocks<double>(c, 0);
+  EpiArgs<double> ep;
+  ep.y = q; ep.partial = c->partial.p; ep.nvec = nvec;
+  if (c->built_precision == TOFE_PREC_MIXED) {
+    const FirstCheb first = vcycle_first_step(c);
+    ep.b = r;
+    ep.partial2 = c->partial2.p;
+    level_apply<double>(c, 0, p, EPI_APPLY_DOT2, ep);
+    fcg_update_r(s, r, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p, first,
+                 nvec);
+    reduce_final(s, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_CONV_FCG, c->scal.p, nvec);
+    const int nparts = precondition(c, r, z, q, first.b32 != nullptr, nvec);
+    fcg_update_xp(s, p, z, U, n, c->partial.p, nparts, c->scal.p, nvec);
+  } else {
+    level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
+    reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p, nvec);
+    cg_update_r(s, r, q, n, c->scal.p, SL_ALPHA, c->partial.p, nvec);
+    reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_CONV_CG, c->scal.p, nvec);
+    precondition(c, r, z, nullptr, false, nvec);
+    dot_into(c, r, z, n, SL_TMP, POST_BETA_CG, nvec);
+    cg_update_xp(s, p, z, U, n, c->scal.p, SL_BETA, SL_ALPHA, nvec);
+  }
+}
+
+// Start of a solve tested on the device, queued on the context's stream: r = F - K U, ||F||,
+// ||r||, read back (the host forms each column's tolerance and start-up reason), then the
+// device test's state, z = M^-1 r, r.z and p = z.  Returns the columns still to solve.
+int start_on_device(tofe_ctx *c, const double *F, double *U, int nvec, double rtol, int *reason)
+{
+  const hipStream_t s = c->stream;
+  const int64_t n = c->ndof;
+  double *r = c->vr.p, *z = c->vz.p, *p = c->vp.p;
+  EpiArgs<double> er;
+  er.y = r; er.b = F; er.nvec = nvec;
+  level_apply<double>(c, 0, U, EPI_RESID, er);
+  dot_into(c, F, F, n, SL_BB, POST_NONE, nvec);
+  dot_into(c, r, r, n, SL_RR, POST_NONE, nvec);
+  std::vector<double> h((size_t)SL_COUNT * nvec);
+  scal_to_host_async(c, nvec);
+  scal_wait(c, h.data(), nvec);
+  ConvState cs{};
+  unsigned long long done = 0;
+  int live = 0;
+  for (int v = 0; v < nvec; ++v) {
+    const double bnorm = std::sqrt(h[(size_t)v * SL_COUNT + SL_BB]);
+    const double rn = std::sqrt(h[(size_t)v * SL_COUNT + SL_RR]);
+    const double rnorm0 = bnorm > 0 ? bnorm : rn;
+    cs.ttol[v] = std::max(rtol * rnorm0, 1e-50);
+    cs.conv[v] = (rtol * rnorm0 > 1e-50) ? TOFE_CONVERGED_RTOL : TOFE_CONVERGED_ATOL;
+    reason[v] = 0;
+    if (std::isnan(rn) || std::isinf(rn)) reason[v] = TOFE_DIVERGED_NANORINF;
+    else if (rn <= cs.ttol[v]) reason[v] = cs.conv[v];
+    if (reason[v]) done |= 1ull << v;
+    else ++live;
+  }
+  set_conv_state(s, c->scal.p, nvec, cs, done);
+  c->frozen_dirty = true;
+  if (live) {
+    precondition(c, r, z, nullptr, false, nvec);
+    dot_into(c, r, z, n, SL_RZ, POST_NONE, nvec);
+    TOFE_HIP(hipMemcpyAsync(p, z, n * nvec * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  return live;
+}
+
 void h2d(tofe_ctx *c, double *d, const double *h, size_t n)
 {
   TOFE_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -2126,41 +2288,7 @@ int tofe_time_operator(tofe_ctx *c, int level, int precision, int epilogue, int 
     const bool f32 = precision == TOFE_PREC_MIXED;
     if (f32 && c->built_precision != TOFE_PREC_MIXED)
       throw Error(TOFE_ERR_STATE, "FP32 kernels need an assemble in mixed precision");
-    const hipStream_t s = c->stream;
-    const int64_t nd = c->grids[level].ndof();
-    hipEvent_t e0, e1;
-    TOFE_HIP(hipEventCreate(&e0));
-    TOFE_HIP(hipEventCreate(&e1));
-    auto run = [&](auto tag) {
-      using T = decltype(tag);
-      auto &lv = *levels<T>(c)[level];
-      Level<double> &l64 = *c->lv64[level];
-      power_start(s, l64.r.p, nd);
-      if constexpr (sizeof(T) == 4) convert<double, float>(s, l64.r.p, lv.x.p, nd);
-      else TOFE_HIP(hipMemcpyAsync(lv.x.p, l64.r.p, nd * sizeof(double), hipMemcpyDeviceToDevice, s));
-      TOFE_HIP(hipMemsetAsync(lv.d.p, 0, nd * sizeof(T), s));
-      EpiArgs<T> ep;
-      ep.y = lv.r.p;
-      ep.b = lv.x.p;
-      ep.dinv = lv.dinv.p;
-      ep.d = lv.d.p;
-      ep.xout = lv.x2.p;
-      ep.c1 = (T)0.5;
-      ep.c2 = (T)0.5;
-      const Epi epi = epilogue == 1 ? EPI_CHEB : EPI_APPLY;
-      level_apply<T>(c, level, lv.x.p, epi, ep);  // warm-up
-      TOFE_HIP(hipEventRecord(e0, s));
-      for (int k = 0; k < nrep; ++k) level_apply<T>(c, level, lv.x.p, epi, ep);
-      TOFE_HIP(hipEventRecord(e1, s));
-    };
-    if (f32) run(float{});
-    else run(double{});
-    TOFE_HIP(hipEventSynchronize(e1));
-    float t = 0;
-    TOFE_HIP(hipEventElapsedTime(&t, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *ms = (double)t / nrep;
+    *ms = time_level_op(c, level, f32, epilogue == 1 ? EPI_CHEB : EPI_APPLY, nrep);
   });
 }
 
@@ -2214,6 +2342,228 @@ int tofe_get_level_lambda(const tofe_ctx *c, int level, double *lambda)
   if (!c || !lambda || level < 0 || level >= c->L) return TOFE_ERR_ARG;
   if (!c->assembled) return TOFE_ERR_STATE;
   *lambda = c->lam[level];
+  return TOFE_OK;
+}
+
+int tofe_autotune(tofe_ctx *c, int nrep, int *fine_zlen, int64_t *split_nodes)
+{
+  return guarded(c, "tofe_autotune", [&] {
+    if (nrep < 1) throw Error(TOFE_ERR_ARG, "nrep must be positive");
+    autotune(c, nrep, fine_zlen, split_nodes);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+}  // extern "C"
+
+// ---- batches of independent contexts --------------------------------------------------
+
+struct tofe_batch {
+  std::vector<tofe_ctx *> ctx;
+  hipStream_t origin = nullptr;  // the stream the graph is captured from and launched on
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
+  tofe_batch_stats stats{};
+  std::string err;
+
+  tofe_batch() = default;
+  tofe_batch(const tofe_batch &) = delete;
+  tofe_batch &operator=(const tofe_batch &) = delete;
+  ~tofe_batch()
+  {
+    if (origin) (void)hipStreamSynchronize(origin);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    for (hipEvent_t e : ev_join)
+      if (e) (void)hipEventDestroy(e);
+    if (origin) (void)hipStreamDestroy(origin);
+  }
+};
+
+namespace {
+
+template <typename F>
+int batch_guarded(tofe_batch *b, const char *name, F &&f)
+{
+  if (!b) return TOFE_ERR_ARG;
+  Range range(name);
+  try {
+    f();
+    return TOFE_OK;
+  } catch (const Error &e) {
+    b->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    b->err = "host allocation failed";
+    return TOFE_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    b->err = e.what();
+    return TOFE_ERR_HIP;
+  }
+}
+
+// Graph of `chunk` solve iterations of every instance that still has columns to solve: each
+// instance's kernels on its own stream, forked from and joined back to the origin stream.
+hipGraphExec_t capture_iterations(tofe_batch *b, const std::vector<double *> &U,
+                                  const std::vector<int> &live, int nvec, int chunk)
+{
+  hipGraph_t g = nullptr;
+  TOFE_HIP(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+  try {
+    TOFE_HIP(hipEventRecord(b->ev_fork, b->origin));
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+      if (!live[i]) continue;
+      tofe_ctx *c = b->ctx[i];
+      TOFE_HIP(hipStreamWaitEvent(c->stream, b->ev_fork, 0));
+      for (int k = 0; k < chunk; ++k) iteration_on_device(c, U[i], nvec);
+      TOFE_HIP(hipEventRecord(b->ev_join[i], c->stream));
+      TOFE_HIP(hipStreamWaitEvent(b->origin, b->ev_join[i], 0));
+    }
+  } catch (...) {
+    (void)hipStreamEndCapture(b->origin, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  TOFE_HIP(hipStreamEndCapture(b->origin, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    throw Error(TOFE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return exec;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tofe_batch_create(tofe_ctx *const *ctxs, int n, tofe_batch **out)
+{
+  if (!ctxs || n < 1 || !out) return TOFE_ERR_ARG;
+  *out = nullptr;
+  std::unique_ptr<tofe_batch> b(new tofe_batch());
+  try {
+    for (int i = 0; i < n; ++i) {
+      if (!ctxs[i]) return TOFE_ERR_ARG;
+      for (int j = 0; j < i; ++j)
+        if (ctxs[j] == ctxs[i]) return TOFE_ERR_ARG;  // one stream per instance
+      if (ctxs[i]->device != ctxs[0]->device) return TOFE_ERR_ARG;
+      b->ctx.push_back(ctxs[i]);
+    }
+    TOFE_HIP(hipSetDevice(ctxs[0]->device));
+    TOFE_HIP(hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking));
+    TOFE_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    b->ev_join.assign(n, nullptr);
+    for (int i = 0; i < n; ++i) TOFE_HIP(hipEventCreateWithFlags(&b->ev_join[i], hipEventDisableTiming));
+  } catch (const Error &e) {
+    return e.code;
+  } catch (...) {
+    return TOFE_ERR_HIP;
+  }
+  *out = b.release();
+  return TOFE_OK;
+}
+
+int tofe_batch_destroy(tofe_batch *b)
+{
+  delete b;
+  return TOFE_OK;
+}
+
+const char *tofe_batch_last_error(const tofe_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int tofe_batch_solve_dev(tofe_batch *b, const double *const *d_F, double *const *d_U, int nrhs,
+                         double rtol, int maxit, int chunk, int *its, int *reasons)
+{
+  return batch_guarded(b, "tofe_batch_solve_dev", [&] {
+    const int n = (int)b->ctx.size();
+    if (!d_F || !d_U || nrhs < 1 || nrhs > kMaxBlockVec || chunk < 1 || maxit < 0)
+      throw Error(TOFE_ERR_ARG, "bad batch solve arguments");
+    for (int i = 0; i < n; ++i) {
+      tofe_ctx *c = b->ctx[i];
+      need_assembled(c);
+      if (c->desc.nu_pre < 1)
+        throw Error(TOFE_ERR_ARG, "batch solves need at least one pre-smoothing step");
+      if (!d_F[i] || !d_U[i]) throw Error(TOFE_ERR_ARG, "null right-hand side or solution");
+      ensure_nvec(c, nrhs);
+    }
+    const double t0 = now_s();
+    // start of every instance (each on its own stream), then the graph of `chunk` steps
+    std::vector<int> live(n), rs((size_t)n * nrhs);
+    std::vector<double *> U(d_U, d_U + n);
+    int any = 0;
+    for (int i = 0; i < n; ++i) {
+      live[i] = start_on_device(b->ctx[i], d_F[i], d_U[i], nrhs, rtol, &rs[(size_t)i * nrhs]);
+      any += live[i] > 0;
+    }
+    for (int i = 0; i < n; ++i) TOFE_HIP(hipStreamSynchronize(b->ctx[i]->stream));
+    int done_its = 0;
+    double t_graph = 0, t_replay = 0;
+    int replays = 0;
+    if (any && maxit > 0) {
+      const double tg = now_s();
+      hipGraphExec_t exec = capture_iterations(b, U, live, nrhs, chunk);
+      t_graph = now_s() - tg;
+      const double tr = now_s();
+      try {
+        while (done_its < maxit) {
+          TOFE_HIP(hipGraphLaunch(exec, b->origin));
+          ++replays;
+          done_its += chunk;
+          for (int i = 0; i < n; ++i)
+            TOFE_HIP(hipMemcpyAsync(b->ctx[i]->hscal, b->ctx[i]->scal.p,
+                                    (size_t)SL_COUNT * nrhs * sizeof(double), hipMemcpyDeviceToHost,
+                                    b->origin));
+          TOFE_HIP(hipStreamSynchronize(b->origin));
+          bool all = true;
+          for (int i = 0; i < n && all; ++i)
+            for (int v = 0; v < nrhs; ++v)
+              if (b->ctx[i]->hscal[(size_t)v * SL_COUNT + SL_FROZEN] == 0.0) {
+                all = false;
+                break;
+              }
+          if (all) break;
+        }
+      } catch (...) {
+        (void)hipGraphExecDestroy(exec);
+        throw;
+      }
+      (void)hipGraphExecDestroy(exec);
+      t_replay = now_s() - tr;
+    }
+    // per-column iteration counts and reasons from the device state
+    for (int i = 0; i < n; ++i) {
+      tofe_ctx *c = b->ctx[i];
+      std::vector<double> h((size_t)SL_COUNT * nrhs);
+      TOFE_HIP(hipMemcpy(h.data(), c->scal.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+      int kmax = 0, rmin = 0;
+      for (int v = 0; v < nrhs; ++v) {
+        const size_t o = (size_t)i * nrhs + v;
+        int k = 0, r = rs[o];
+        if (!r) {
+          k = (int)h[(size_t)v * SL_COUNT + SL_ITS];
+          r = (int)h[(size_t)v * SL_COUNT + SL_REASON];
+          if (!r) r = TOFE_DIVERGED_ITS;
+        }
+        if (its) its[o] = k;
+        if (reasons) reasons[o] = r;
+        kmax = std::max(kmax, k);
+        if (rmin >= 0) rmin = r;
+      }
+      c->stats.its = kmax;
+      c->stats.reason = rmin;
+    }
+    b->stats.seconds = now_s() - t0;
+    b->stats.capture_seconds = t_graph;
+    b->stats.replay_seconds = t_replay;
+    b->stats.replays = replays;
+    b->stats.iterations = done_its;
+  });
+}
+
+int tofe_batch_get_stats(const tofe_batch *b, tofe_batch_stats *st)
+{
+  if (!b || !st) return TOFE_ERR_ARG;
+  *st = b->stats;
   return TOFE_OK;
 }
 

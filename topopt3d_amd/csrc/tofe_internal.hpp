@@ -242,7 +242,8 @@ void lanczos_p(hipStream_t s, T *p, const T *r, const T *dinv, int64_t n, const 
 // device scalar slots (ctx->scal)
 enum Slot : int {
   SL_RZ = 0, SL_PQ = 1, SL_RR = 2, SL_ALPHA = 3, SL_BETA = 4, SL_BB = 5, SL_PR = 6, SL_ZQ = 7,
-  SL_RZA = 8, SL_RZB = 9, SL_TMP = 11, SL_FROZEN = 12, SL_COUNT = 16
+  SL_RZA = 8, SL_RZB = 9, SL_CONVCODE = 10, SL_TMP = 11, SL_FROZEN = 12, SL_TTOL = 13,
+  SL_ITS = 14, SL_REASON = 15, SL_COUNT = 16
 };
 // reduce_final post-processing
 enum Post : int {
@@ -250,8 +251,22 @@ enum Post : int {
   POST_ALPHA_CG = 1,   // scal[ALPHA] = scal[RZ] / v
   POST_BETA_CG = 2,    // scal[BETA] = v / scal[RZ]; scal[RZ] = v
   POST_ALPHA_FCG = 3,  // scal[ALPHA] = scal[PR] / v   (v = p.Ap)
-  POST_BETA_FCG = 4    // scal[BETA] = -v / scal[PQ]  (v = z.Ap)
+  POST_BETA_FCG = 4,   // scal[BETA] = -v / scal[PQ]  (v = z.Ap)
+  // v = r.r after a step of a solve whose test runs on the device: unless frozen, count the
+  // step in scal[ITS] and freeze with scal[REASON] on convergence (sqrt(v) <= scal[TTOL]:
+  // reason scal[CONVCODE]), on p.Ap <= 0 or NaN, on a non-finite residual, or (CG only) on
+  // r.z < 0
+  POST_CONV_FCG = 5,
+  POST_CONV_CG = 6
 };
+// Per-column state of a solve tested on the device: tolerances, convergence reason codes,
+// iteration counts and reasons reset, frozen flags as in the mask.
+struct ConvState {
+  double ttol[16];
+  int conv[16];
+};
+void set_conv_state(hipStream_t s, double *scal, int nvec, const ConvState &cs,
+                    unsigned long long frozen_mask);
 
 // ---- design-side kernels (k_design.hip) -----------------------------------
 struct FilterTemplate {

@@ -53,13 +53,16 @@ def _svqb(G, drop=1e-12):
 
 
 def lobpcg(apply_A, apply_B, apply_T, X0, largest, tol=1e-8, maxit=500, mask=None,
-           log=None):
-    """Eigenpairs of A x = theta B x (B symmetric positive definite) for the k = X0.shape[0]
-    largest or smallest theta.  apply_* map a (m, n) tensor of row vectors to their images
+           log=None, nwant=None):
+    """Eigenpairs of A x = theta B x (B symmetric positive definite): the block of
+    k = X0.shape[0] vectors iterates, and the nwant (default k) largest or smallest theta are
+    tested and returned -- the other k - nwant are guard vectors that speed up convergence
+    near clustered eigenvalues.  apply_* map a (m, n) tensor of row vectors to their images
     (m <= 16 per call is one block pass); mask (n,) zeroes the Dirichlet DOFs of every new
     search direction."""
     torch = _torch()
     k = X0.shape[0]
+    nw = k if nwant is None else min(nwant, k)
 
     def clean(Y):
         return Y * mask if mask is not None else Y
@@ -90,9 +93,9 @@ def lobpcg(apply_A, apply_B, apply_T, X0, largest, tol=1e-8, maxit=500, mask=Non
         bn = torch.linalg.vector_norm(BX, dim=1).cpu().numpy()
         res = rn / np.maximum(np.abs(theta) * bn, 1e-300)
         if log:
-            log(it, theta, res)
-        if np.all(res < tol):
-            return EigResult(theta, X, it, res, True)
+            log(it, theta[:nw], res[:nw])
+        if np.all(res[:nw] < tol):
+            return EigResult(theta[:nw], X[:nw], it, res[:nw], True)
         W = clean(apply_T(R))
         AW, BW = apply_A(W), apply_B(W)
         blocks = [(X, AX, BX), (W, AW, BW)] + ([(P, AP, BP)] if P is not None else [])
@@ -117,7 +120,7 @@ def lobpcg(apply_A, apply_B, apply_T, X0, largest, tol=1e-8, maxit=500, mask=Non
         Cpt = torch.from_numpy(np.ascontiguousarray(Cp.T)).to(X)
         P, AP, BP = Cpt @ S, Cpt @ AS, Cpt @ BS
         X, AX, BX = Xn, AXn, BXn
-    return EigResult(theta, X, it, res, False)
+    return EigResult(theta[:nw], X[:nw], it, res[:nw], False)
 
 
 class _Ops:
@@ -155,15 +158,21 @@ def _start_block(n, k, fixed, seed, device):
     return X, (mask.to(device) if mask is not None else None)
 
 
+def block_size(nev):
+    """Vectors iterated for nev wanted eigenpairs: up to four guard vectors, at most one block
+    pass (16)."""
+    return max(nev, min(16, nev + 4))
+
+
 def buckling_modes(ctx, U, fixed_dof, nev=6, tol=1e-8, maxit=500, seed=0, log=None):
     """The nev largest mu of Ks(U) phi = mu K phi (Stability::Function's eigenproblem,
     Set_Operators(Ks, K) and target LR at Buckling.cpp:67-70); phi K-orthonormal.
     U is a (n,) CUDA tensor; the context must be assembled for the current design."""
     ctx.set_stress_dev(U.data_ptr())
     ops = _Ops(ctx)
-    X0, mask = _start_block(ctx.ndof, nev, fixed_dof, seed, U.device)
+    X0, mask = _start_block(ctx.ndof, block_size(nev), fixed_dof, seed, U.device)
     return lobpcg(ops.Ks, ops.K, ops.T, X0, largest=True, tol=tol, maxit=maxit, mask=mask,
-                  log=log)
+                  log=log, nwant=nev)
 
 
 def vibration_modes(ctx, fixed_dof, nev=12, tol=1e-8, maxit=500, seed=0, device="cuda",
@@ -173,9 +182,9 @@ def vibration_modes(ctx, fixed_dof, nev=12, tol=1e-8, maxit=500, seed=0, device=
     omega^2 = 1 / lambda.  Returns the EigResult of (M, K) (phi K-orthonormal; values are
     lambda, largest first).  Needs tofe_assemble and tofe_assemble_mass."""
     ops = _Ops(ctx)
-    X0, mask = _start_block(ctx.ndof, nev, fixed_dof, seed, device)
+    X0, mask = _start_block(ctx.ndof, block_size(nev), fixed_dof, seed, device)
     return lobpcg(ops.M, ops.K, ops.T, X0, largest=True, tol=tol, maxit=maxit, mask=mask,
-                  log=log)
+                  log=log, nwant=nev)
 
 
 def buckling_sensitivities(ctx, U, mu, phi, rtol=1e-10, maxit=2000):
