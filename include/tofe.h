@@ -154,6 +154,13 @@ int tofe_set_E(tofe_ctx *ctx, const double *E);
 /* Copy out one of the TOFE_FIELD_* vectors. */
 int tofe_get_field(const tofe_ctx *ctx, int field, double *out);
 
+/* Spring supports: K + diag(spk) on the free DOFs (3*nnode values; NULL removes them), the
+ * reference's MatDiagonalSet(K, spKVec, ADD_VALUES) before its Dirichlet rows
+ * (FEAnalysis.cpp:360-368, 471), so Dirichlet DOFs keep their unit row.  The operator, the
+ * Jacobi diagonal and the Galerkin hierarchy (P^T diag(spk) P on level 1) include it from the
+ * next tofe_assemble. */
+int tofe_set_springs(tofe_ctx *ctx, const double *spk);
+
 /* Rebuild the solver for the current E: fine Jacobi diagonal, Galerkin coarse
  * operators P^T A P, Chebyshev eigenvalue bounds and the coarsest factorisation.
  * Replaces TopOpt::FEAssemble (FEAnalysis.cpp:432-490) plus PCSetUp of PCMG
@@ -329,7 +336,8 @@ int tofe_autotune(tofe_ctx *ctx, int nrep, int *fine_zlen, int64_t *split_nodes)
  * `chunk` iterations of every instance are captured from the instances' streams into one
  * HIP graph, with the convergence test on the device (a converged right-hand side freezes
  * itself), and the graph is replayed until every right-hand side has finished or maxit
- * iterations have run.  Results equal tofe_solve_dev on each instance bit for bit.  No
+ * iterations have run; once an instance has finished, the graph is captured again without
+ * it.  Results equal tofe_solve_dev on each instance bit for bit.  No
  * reference counterpart: it replaces a host loop of FESolve's KSPSolve
  * (FEAnalysis.cpp:673) over independent problems.  A context belongs to at most one batch
  * call at a time; the divergence test ||r|| >= 1e5 ||b|| of tofe_solve is not applied. */
@@ -339,7 +347,7 @@ typedef struct tofe_batch_stats {
   double capture_seconds;  /* of which: capturing and instantiating the graph */
   double replay_seconds;   /* of which: graph replays and their convergence reads */
   int replays;             /* graph launches */
-  int iterations;          /* iterations captured times replays */
+  int iterations;          /* iterations run by the longest-running instance (chunk x replays) */
 } tofe_batch_stats;
 int tofe_batch_create(tofe_ctx *const *ctxs, int n, tofe_batch **batch);
 int tofe_batch_destroy(tofe_batch *batch);

@@ -48,7 +48,7 @@ EXPORTS = (
     "tofe_set_stress", "tofe_set_stress_dev", "tofe_apply_Ks", "tofe_apply_Ks_dev",
     "tofe_stability_terms", "tofe_element_bilinear", "tofe_compliance_noalloc",
     "tofe_set_mass_form", "tofe_autotune", "tofe_batch_create", "tofe_batch_destroy",
-    "tofe_batch_last_error", "tofe_batch_solve_dev", "tofe_batch_get_stats",
+    "tofe_batch_last_error", "tofe_batch_solve_dev", "tofe_batch_get_stats", "tofe_set_springs",
 )
 
 
@@ -155,6 +155,7 @@ def lib():
         "tofe_stability_terms": ([vp, dp, C.c_int, dp, dp], C.c_int),
         "tofe_element_bilinear": ([vp, dp, C.c_int, dp, dp], C.c_int),
         "tofe_set_mass_form": ([vp, C.c_int], C.c_int),
+        "tofe_set_springs": ([vp, dp], C.c_int),
         "tofe_autotune": ([vp, C.c_int, ip, i64p], C.c_int),
         "tofe_batch_create": ([C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
         "tofe_batch_destroy": ([vp], C.c_int),
@@ -385,6 +386,10 @@ class Context:
     def assemble_mass(self, density, mlump=None):
         self._check(lib().tofe_assemble_mass(self.h, density,
                                              None if mlump is None else _dp(_f64(mlump))))
+
+    def set_springs(self, spk):
+        """Spring stiffness per DOF (None removes them); takes effect at the next assemble."""
+        self._check(lib().tofe_set_springs(self.h, None if spk is None else _dp(_f64(spk))))
 
     def set_mass_form(self, form):
         self._check(lib().tofe_set_mass_form(self.h, form))

@@ -337,7 +337,7 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *u0,
       for (int c = 0; c < 3; ++c) {
         const int64_t dof = 3 * n + c;
         const T uval = u[dof];
-        const T Au = ((fm >> c) & 1) ? uval : alo[c];
+        const T Au = ((fm >> c) & 1) ? uval : (op.spk ? alo[c] + op.spk[dof] * uval : alo[c]);
         apply_epilogue<T, EPI>(ep, dof, Au, uval, dsum, dsum2);
       }
     }

@@ -190,7 +190,8 @@ __global__ __launch_bounds__(NTH) void k_fine_mfma(FineOp<T> op, const T *u0,
         for (int c = 0; c < 3; ++c) {
           const int64_t dof = 3 * n + c;
           const T uval = u[dof];
-          const T Au = ((fm >> c) & 1) ? uval : sacc[lo][lb + c];
+          const T Ku = sacc[lo][lb + c];
+          const T Au = ((fm >> c) & 1) ? uval : (op.spk ? Ku + op.spk[dof] * uval : Ku);
           apply_epilogue<T, EPI>(ep, dof, Au, uval, dsum, dsum2);
         }
       }

@@ -250,7 +250,13 @@ __global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *u0, EpiA
       alo[c] = ahi[c] + (plo[c] + sp[pb][c][tb]);
       ahi[c] = phi[c] + sp[pb][3 + c][tb];
     }
-    if (emit) epi_store<T, EPI>(ep, n, ein, alo, dsum, dsum2);
+    if (emit) {
+      if (op.spk) {  // spring supports: K + diag(spk) on free DOFs (fixed rows return u below)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) alo[c] += op.spk[3 * n + c] * ein.u[c];
+      }
+      epi_store<T, EPI>(ep, n, ein, alo, dsum, dsum2);
+    }
     Ecur = Enext;
   }
   if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2 || EPI == EPI_CHEB_OUT) {

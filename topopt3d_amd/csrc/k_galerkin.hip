@@ -240,7 +240,7 @@ __global__ __launch_bounds__(NT) void k_galerkin_fine(const double *__restrict__
 // FEAnalysis.cpp:448-464), 1 on fixed DOFs, and diag <= 0 -> 1 (476-484).
 __global__ void k_fine_diag(const double *__restrict__ E, const double *__restrict__ KE,
                             const uint8_t *__restrict__ fix, int nx, int ny, int nz,
-                            double *diag, double *dinv)
+                            double *diag, double *dinv, const double *__restrict__ spk)
 {
 #pragma clang fp contract(off)
   const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
@@ -263,13 +263,69 @@ __global__ void k_fine_diag(const double *__restrict__ E, const double *__restri
       }
     }
   }
+  if (spk) s = s + spk[t];  // MatDiagonalSet(K, spKVec, ADD_VALUES), FEAnalysis.cpp:471
   if (fix && ((fix[n] >> a) & 1)) s = 1.0;
   if (!(s > 0)) s = 1.0;
   diag[t] = s;
   dinv[t] = 1.0 / s;
 }
 
+// Sc(I, I + o) += sum over fine nodes f shared by the supports of I and I + o of
+// w(f, I) w(f, I + o) spk(f) on the diagonal of the 3x3 block (one thread per coarse node and
+// offset o = blockIdx.y); identity: coarse == fine, offset 13 only.
+__global__ __launch_bounds__(NT) void k_galerkin_springs(const double *__restrict__ spk, int Nfx,
+                                                         int Nfy, int Nfz, int identity,
+                                                         double *__restrict__ Sc)
+{
+  const int Ncx = identity ? Nfx : (Nfx + 1) / 2, Ncy = identity ? Nfy : (Nfy + 1) / 2;
+  const int Ncz = identity ? Nfz : (Nfz + 1) / 2;
+  const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
+  const int64_t I = (int64_t)blockIdx.x * NT + threadIdx.x;
+  const int o = identity ? 13 : (int)blockIdx.y;
+  if (I >= ncn) return;
+  int Ix, Iy, Iz;
+  node_ijk(I, Ncx, Ncy, Ix, Iy, Iz);
+  const int Jx = Ix + o % 3 - 1, Jy = Iy + (o / 3) % 3 - 1, Jz = Iz + o / 9 - 1;
+  if (Jx < 0 || Jx >= Ncx || Jy < 0 || Jy >= Ncy || Jz < 0 || Jz >= Ncz) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (identity) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] = spk[3 * I + a];
+  } else {
+    for (int fz = 2 * Iz - 1; fz <= 2 * Iz + 1; ++fz) {
+      if (fz < 0 || fz >= Nfz) continue;
+      const double wz = w1d(fz, Iz, Nfz) * w1d(fz, Jz, Nfz);
+      if (wz == 0.0) continue;
+      for (int fy = 2 * Iy - 1; fy <= 2 * Iy + 1; ++fy) {
+        if (fy < 0 || fy >= Nfy) continue;
+        const double wy = w1d(fy, Iy, Nfy) * w1d(fy, Jy, Nfy);
+        if (wy == 0.0) continue;
+        for (int fx = 2 * Ix - 1; fx <= 2 * Ix + 1; ++fx) {
+          if (fx < 0 || fx >= Nfx) continue;
+          const double w = w1d(fx, Ix, Nfx) * w1d(fx, Jx, Nfx) * wy * wz;
+          if (w == 0.0) continue;
+          const int64_t f = fx + (int64_t)Nfx * (fy + (int64_t)Nfy * fz);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) acc[a] += w * spk[3 * f + a];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) Sc[(int64_t)(o * 9 + a * 3 + a) * ncn + I] += acc[a];
+}
+
 }  // namespace
+
+void galerkin_springs(hipStream_t s, const double *spk, int Nfx, int Nfy, int Nfz, bool identity,
+                      double *Sc)
+{
+  const int64_t ncn = identity ? (int64_t)Nfx * Nfy * Nfz
+                               : (int64_t)((Nfx + 1) / 2) * ((Nfy + 1) / 2) * ((Nfz + 1) / 2);
+  k_galerkin_springs<<<dim3((unsigned)((ncn + NT - 1) / NT), identity ? 1 : 27), dim3(NT), 0, s>>>(
+      spk, Nfx, Nfy, Nfz, identity ? 1 : 0, Sc);
+  TOFE_LAUNCH_CHECK();
+}
 
 void galerkin_fine(hipStream_t s, const double *E, const int *emap, const double *KEC,
                    const MaskedElems &me, const double *KE, const double *Qforms, double *KCm,
@@ -288,11 +344,11 @@ void galerkin_fine(hipStream_t s, const double *E, const int *emap, const double
 }
 
 void fine_diag(hipStream_t s, const double *E, const double *KE, const uint8_t *fix, int nx, int ny,
-               int nz, double *diag, double *dinv)
+               int nz, double *diag, double *dinv, const double *spk)
 {
   const int64_t nn = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
   k_fine_diag<<<dim3((unsigned)((3 * nn + NT - 1) / NT)), dim3(NT), 0, s>>>(E, KE, fix, nx, ny, nz,
-                                                                          diag, dinv);
+                                                                          diag, dinv, spk);
   TOFE_LAUNCH_CHECK();
 }
 

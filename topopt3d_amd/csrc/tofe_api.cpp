@@ -133,6 +133,11 @@ struct tofe_ctx {
   std::vector<int> rtpl_off;
   DBuf<int> drtpl_off;
   DBuf<double> drtpl_w, REdge, y, rhoq, ebuf3;
+  // spring stiffness per DOF (spKVec, FEAnalysis.cpp:360-368), zero at Dirichlet DOFs, FP64 and
+  // FP32 (the mixed V-cycle's fine operator)
+  DBuf<double> spk;
+  DBuf<float> spk32;
+  bool has_springs = false;
   // mass matrix: the lumped diagonal (Frequency::DiagMassFnc), or the consistent element mass
   // V_e m0 Q (x) I_3 applied matrix-free from a snapshot of V and the point masses
   DBuf<double> Mdiag, mV, mlump;
@@ -269,6 +274,7 @@ FineOp<double> fine_of<double>(tofe_ctx *c)
   f.wht = c->ke_wht;
   f.wht_cube = c->ke_wht_cube;
   if (f.wht) f.zlen = c->wht_zlen;
+  f.spk = c->has_springs ? c->spk.p : nullptr;
   return f;
 }
 template <>
@@ -284,6 +290,7 @@ FineOp<float> fine_of<float>(tofe_ctx *c)
   f.wht = c->ke_wht;
   f.wht_cube = c->ke_wht_cube;
   if (f.wht) f.zlen = c->wht_zlen32;
+  f.spk = c->has_springs ? c->spk32.p : nullptr;
   return f;
 }
 
@@ -966,8 +973,13 @@ void assemble(tofe_ctx *c)
   c->diag0.alloc(c->ndof);
   for (auto &lv : c->lv64) lv->su_ok = false;  // the stencils are formed anew below
   const uint8_t *fix = c->has_fix ? c->dfix.p : nullptr;
+  const double *spk = c->has_springs ? c->spk.p : nullptr;
   fine_diag(s, c->E.p, c->dKE.p, fix, c->desc.nx, c->desc.ny, c->desc.nz, c->diag0.p,
-            c->lv64[0]->dinv.p);
+            c->lv64[0]->dinv.p, spk);
+  if (spk && c->precision == TOFE_PREC_MIXED) {
+    c->spk32.alloc(c->ndof);
+    convert<double, float>(s, spk, c->spk32.p, c->ndof);
+  }
   if (L > 1) {
     MaskedElems me;
     me.elem = c->dme_elem.p;
@@ -976,10 +988,14 @@ void assemble(tofe_ctx *c)
     me.count = c->n_masked;
     galerkin_fine(s, c->E.p, c->demap.p, c->dKEC.p, me, c->dKE.p, c->dQforms.p, c->dKCm.p, fix,
                   c->desc.nx, c->desc.ny, c->desc.nz, c->lv64[1]->S.p);
+    if (spk) galerkin_springs(s, spk, c->grids[0].Nx, c->grids[0].Ny, c->grids[0].Nz, false,
+                              c->lv64[1]->S.p);
     stencil_diag(s, c->lv64[1]->S.p, c->grids[1].nn(), nullptr, c->lv64[1]->dinv.p);
   } else {
     c->fineS.alloc((size_t)243 * c->nn);
     fine_stencil_build(s, fine_of<double>(c), c->fineS.p);
+    if (spk) galerkin_springs(s, spk, c->grids[0].Nx, c->grids[0].Ny, c->grids[0].Nz, true,
+                              c->fineS.p);
   }
   for (int l = 1; l + 1 < L; ++l) {
     const Grid &gf = c->grids[l], &gc = c->grids[l + 1];
@@ -1853,6 +1869,28 @@ int tofe_assemble_mass(tofe_ctx *c, double density, const double *mlump)
   });
 }
 
+int tofe_set_springs(tofe_ctx *c, const double *spk)
+{
+  return guarded(c, "tofe_set_springs", [&] {
+    c->assembled = false;  // the diagonal and the Galerkin hierarchy change
+    if (!spk) {
+      c->has_springs = false;
+      return;
+    }
+    std::vector<double> h(spk, spk + c->ndof);
+    if (c->has_fix) {  // MatZeroRowsColumns after MatDiagonalSet: nothing stays on fixed rows
+      std::vector<uint8_t> fm(c->nn);
+      TOFE_HIP(hipMemcpy(fm.data(), c->dfix.p, c->nn, hipMemcpyDeviceToHost));
+      for (int64_t d = 0; d < c->ndof; ++d)
+        if ((fm[d / 3] >> (d % 3)) & 1) h[d] = 0.0;
+    }
+    c->spk.alloc(c->ndof);
+    h2d(c, c->spk.p, h.data(), c->ndof);
+    TOFE_HIP(hipStreamSynchronize(c->stream));
+    c->has_springs = true;
+  });
+}
+
 int tofe_set_mass_form(tofe_ctx *c, int form)
 {
   return guarded(c, "tofe_set_mass_form", [&] {
@@ -2363,6 +2401,11 @@ struct tofe_batch {
   hipStream_t origin = nullptr;  // the stream the graph is captured from and launched on
   hipEvent_t ev_fork = nullptr;
   std::vector<hipEvent_t> ev_join;
+  // how the chunks of iterations run: one graph holding every unfinished instance as a branch
+  // (TOFE_BATCH_GRAPH=single, the default), one graph per instance on the instance's own stream
+  // (=instance), or no graph, the chunk's kernels enqueued on the instances' streams in turn
+  // (=streams)
+  enum Form { SINGLE, INSTANCE, STREAMS } form = SINGLE;
   tofe_batch_stats stats{};
   std::string err;
 
@@ -2432,6 +2475,27 @@ hipGraphExec_t capture_iterations(tofe_batch *b, const std::vector<double *> &U,
   return exec;
 }
 
+// Graph of `chunk` solve iterations of one instance, captured from its own stream.
+hipGraphExec_t capture_instance(tofe_ctx *c, double *U, int nvec, int chunk)
+{
+  hipGraph_t g = nullptr;
+  TOFE_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int k = 0; k < chunk; ++k) iteration_on_device(c, U, nvec);
+  } catch (...) {
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  TOFE_HIP(hipStreamEndCapture(c->stream, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    throw Error(TOFE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return exec;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2454,6 +2518,10 @@ int tofe_batch_create(tofe_ctx *const *ctxs, int n, tofe_batch **out)
     TOFE_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
     b->ev_join.assign(n, nullptr);
     for (int i = 0; i < n; ++i) TOFE_HIP(hipEventCreateWithFlags(&b->ev_join[i], hipEventDisableTiming));
+    if (const char *m = std::getenv("TOFE_BATCH_GRAPH"))
+      b->form = std::strcmp(m, "instance") == 0  ? tofe_batch::INSTANCE
+                : std::strcmp(m, "streams") == 0 ? tofe_batch::STREAMS
+                                                 : tofe_batch::SINGLE;
   } catch (const Error &e) {
     return e.code;
   } catch (...) {
@@ -2496,39 +2564,92 @@ int tofe_batch_solve_dev(tofe_batch *b, const double *const *d_F, double *const 
       any += live[i] > 0;
     }
     for (int i = 0; i < n; ++i) TOFE_HIP(hipStreamSynchronize(b->ctx[i]->stream));
+    // Replays: the graph covers the instances with columns still to solve; when an instance
+    // has finished, the next replay's graph is captured without it, so finished instances
+    // cost nothing while the others go on.
     int done_its = 0;
     double t_graph = 0, t_replay = 0;
     int replays = 0;
     if (any && maxit > 0) {
-      const double tg = now_s();
-      hipGraphExec_t exec = capture_iterations(b, U, live, nrhs, chunk);
-      t_graph = now_s() - tg;
+      hipGraphExec_t exec = nullptr;           // single-graph form
+      std::vector<hipGraphExec_t> execs(n);    // per-instance form
+      std::vector<int> captured;
+      auto release = [&] {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        for (hipGraphExec_t &e : execs)
+          if (e) (void)hipGraphExecDestroy(e);
+      };
       const double tr = now_s();
       try {
+        if (b->form == tofe_batch::INSTANCE) {
+          const double tg = now_s();
+          for (int i = 0; i < n; ++i)
+            if (live[i]) execs[i] = capture_instance(b->ctx[i], U[i], nrhs, chunk);
+          t_graph += now_s() - tg;
+        }
         while (done_its < maxit) {
-          TOFE_HIP(hipGraphLaunch(exec, b->origin));
+          if (b->form == tofe_batch::STREAMS) {
+            // the chunk's iterations on every unfinished instance's stream, one iteration of
+            // each instance in turn, joined on the batch stream
+            TOFE_HIP(hipEventRecord(b->ev_fork, b->origin));
+            for (int i = 0; i < n; ++i)
+              if (live[i]) TOFE_HIP(hipStreamWaitEvent(b->ctx[i]->stream, b->ev_fork, 0));
+            for (int k = 0; k < chunk; ++k)
+              for (int i = 0; i < n; ++i)
+                if (live[i]) iteration_on_device(b->ctx[i], U[i], nrhs);
+            for (int i = 0; i < n; ++i) {
+              if (!live[i]) continue;
+              TOFE_HIP(hipEventRecord(b->ev_join[i], b->ctx[i]->stream));
+              TOFE_HIP(hipStreamWaitEvent(b->origin, b->ev_join[i], 0));
+            }
+          } else if (b->form == tofe_batch::INSTANCE) {
+            // each unfinished instance's graph on its own stream, joined on the batch stream
+            TOFE_HIP(hipEventRecord(b->ev_fork, b->origin));
+            for (int i = 0; i < n; ++i) {
+              if (!live[i]) continue;
+              tofe_ctx *c = b->ctx[i];
+              TOFE_HIP(hipStreamWaitEvent(c->stream, b->ev_fork, 0));
+              TOFE_HIP(hipGraphLaunch(execs[i], c->stream));
+              TOFE_HIP(hipEventRecord(b->ev_join[i], c->stream));
+              TOFE_HIP(hipStreamWaitEvent(b->origin, b->ev_join[i], 0));
+            }
+          } else {
+            if (live != captured) {
+              if (exec) (void)hipGraphExecDestroy(exec);
+              exec = nullptr;
+              const double tg = now_s();
+              exec = capture_iterations(b, U, live, nrhs, chunk);
+              t_graph += now_s() - tg;
+              captured = live;
+            }
+            TOFE_HIP(hipGraphLaunch(exec, b->origin));
+          }
           ++replays;
           done_its += chunk;
           for (int i = 0; i < n; ++i)
-            TOFE_HIP(hipMemcpyAsync(b->ctx[i]->hscal, b->ctx[i]->scal.p,
-                                    (size_t)SL_COUNT * nrhs * sizeof(double), hipMemcpyDeviceToHost,
-                                    b->origin));
+            if (live[i])
+              TOFE_HIP(hipMemcpyAsync(b->ctx[i]->hscal, b->ctx[i]->scal.p,
+                                      (size_t)SL_COUNT * nrhs * sizeof(double),
+                                      hipMemcpyDeviceToHost, b->origin));
           TOFE_HIP(hipStreamSynchronize(b->origin));
-          bool all = true;
-          for (int i = 0; i < n && all; ++i)
+          int remaining = 0;
+          for (int i = 0; i < n; ++i) {
+            if (!live[i]) continue;
+            int open = 0;
             for (int v = 0; v < nrhs; ++v)
-              if (b->ctx[i]->hscal[(size_t)v * SL_COUNT + SL_FROZEN] == 0.0) {
-                all = false;
-                break;
-              }
-          if (all) break;
+              open += b->ctx[i]->hscal[(size_t)v * SL_COUNT + SL_FROZEN] == 0.0;
+            live[i] = open;
+            remaining += open > 0;
+          }
+          if (!remaining) break;
         }
       } catch (...) {
-        (void)hipGraphExecDestroy(exec);
+        release();
         throw;
       }
-      (void)hipGraphExecDestroy(exec);
-      t_replay = now_s() - tr;
+      release();
+      t_replay = now_s() - tr - t_graph;
     }
     // per-column iteration counts and reasons from the device state
     for (int i = 0; i < n; ++i) {

@@ -73,6 +73,7 @@ struct FineOp {
   bool sym = false;            // cubic element: KE[576 + k] holds the cube-orbit values of KE
   bool wht = false;            // KE[592 + k] holds its Walsh-Hadamard blocks: use k_fine_wht
   bool wht_cube = false;       // cubic element: KE[625 + k] holds the 9 cube block values
+  const T *spk = nullptr;      // spring stiffness per DOF added to free diagonals (may be null)
 };
 
 // Launch shape of the fine operator: one of kFineTiles compiled column tiles and the number of
@@ -164,7 +165,12 @@ void galerkin_fine(hipStream_t s, const double *E, const int *emap, const double
                    const MaskedElems &me, const double *KE, const double *Qforms, double *KCm,
                    const uint8_t *fix, int nx, int ny, int nz, double *Sc);
 void fine_diag(hipStream_t s, const double *E, const double *KE, const uint8_t *fix, int nx, int ny,
-               int nz, double *diag, double *dinv);
+               int nz, double *diag, double *dinv, const double *spk = nullptr);
+// Spring supports in the Galerkin hierarchy: Sc += P^T diag(spk) P on the coarse grid of a fine
+// grid of Nfx x Nfy x Nfz nodes (spk per fine DOF, zero at Dirichlet DOFs), or, with identity
+// transfer (coarse == fine), the diagonal itself.
+void galerkin_springs(hipStream_t s, const double *spk, int Nfx, int Nfy, int Nfz, bool identity,
+                      double *Sc);
 
 // ---- vector kernels (k_vec.hip) -------------------------------------------
 // nvec > 1: nvec vectors of n values stored one after another (vector v at v * n), each

@@ -179,6 +179,8 @@ class TopOpt:
                            fixed_dof=self.fixed_dof, rmax=self.Rmax, vd_min=self.vdMin,
                            vd_penal=self.vdPenal, interpolation=int(self.interpolation),
                            interp_param=tuple(self.interp_param))
+        if self.spKVec.any():  # MatDiagonalSet(K, spKVec) in FEAssemble (FEAnalysis.cpp:471)
+            self.ctx.set_springs(self.spKVec)
         self.ke = [self.ctx.ke()]
 
     # -- design iteration ---------------------------------------------------------
