@@ -289,7 +289,9 @@ def main():
                     help="cantilever: the headline solve (default); buckling: the 128x64x64 "
                          "column, 6 modes, adjoint gradients and central differences; vibration: "
                          "the 160x80x80 cantilever, 12 modes with the consistent mass; batch: 32 "
-                         "uneven cantilevers, 4 load cases each, one graph-captured solve")
+                         "uneven cantilevers, 4 load cases each, graph-captured batch solves")
+    ap.add_argument("--batch-forms", default="merged,single,instance,streams",
+                    help="batch workload: the batch forms to time (comma-separated)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only launch the fine FP32 Chebyshev kernel and the level-1 FP32 stencil "
                          "Chebyshev kernel 6 times each and a 512 MiB device copy, for rocprofv3 "
@@ -309,7 +311,10 @@ def main():
 
         assert torch.cuda.is_available(), "bench.py needs a GPU"
         t0 = time.perf_counter()
-        res = getattr(workloads, args.workload)()
+        if args.workload == "batch":
+            res = workloads.batch(forms=tuple(args.batch_forms.split(",")))
+        else:
+            res = getattr(workloads, args.workload)()
         wall = time.perf_counter() - t0
         value = res["batch_s"] if args.workload == "batch" else res["precisions"]["mixed"]["eigen_s"]
         res.update({"metric": f"{args.workload} analysis on one MI355X", "unit": "s",

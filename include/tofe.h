@@ -348,6 +348,9 @@ typedef struct tofe_batch_stats {
   double replay_seconds;   /* of which: graph replays and their convergence reads */
   int replays;             /* graph launches */
   int iterations;          /* iterations run by the longest-running instance (chunk x replays) */
+  int launches;            /* merged form: launches per iteration in the first graph of the solve */
+  int unbatched;           /* ... of which serve a single instance */
+  int instance_launches;   /* ... and the launches the instances' own iterations make, summed */
 } tofe_batch_stats;
 int tofe_batch_create(tofe_ctx *const *ctxs, int n, tofe_batch **batch);
 int tofe_batch_destroy(tofe_batch *batch);

@@ -1,9 +1,11 @@
 """Batches of independent instances of uneven size (the reference solves each problem with
 its own KSPSolve, FEAnalysis.cpp:673): tofe_batch_solve_dev captures `chunk` iterations of
 every instance into one HIP graph, with the convergence test on the device, and replays it.
-Every replay must leave each instance exactly where the eager block solve stands after the
-same number of iterations (bit for bit), and the converged batch must equal the eager solves
-and meet the oracle's residual; the autotuner must keep the operator's results unchanged."""
+In the merged form each step of the iteration is one launch over every instance (table-driven
+batched kernels); in the single form the graph holds every instance's own launches.  Every
+replay must leave each instance exactly where the eager block solve stands after the same
+number of iterations (bit for bit), and the converged batch must equal the eager solves and
+meet the oracle's residual; the autotuner must keep the operator's results unchanged."""
 import numpy as np
 import pytest
 
@@ -26,11 +28,13 @@ def _instances(tofe, precision, n=5, sizes=(8, 10, 12, 14, 16), nrhs=3):
     return pbs, ctxs, F
 
 
+@pytest.mark.parametrize("form", ["merged", "single"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_batch_graph_matches_eager_every_replay(oracle, tofe, precision):
+def test_batch_graph_matches_eager_every_replay(oracle, tofe, precision, form, monkeypatch):
     import torch
 
-    print(f"case batch precision={precision}", flush=True)
+    print(f"case batch precision={precision} form={form}", flush=True)
+    monkeypatch.setenv("TOFE_BATCH_GRAPH", form)
     pbs, ctxs, F = _instances(tofe, precision)
     nrhs, chunk = 3, 2
     batch = tofe.Batch(ctxs)
@@ -47,6 +51,11 @@ def test_batch_graph_matches_eager_every_replay(oracle, tofe, precision):
             assert (rs == tofe.CONVERGED_RTOL).all()
             st = batch.stats()
             assert st["replays"] >= 1 and st["iterations"] >= its.max()
+            if form == "merged":
+                # the five instances' launches share far fewer launches (in FP64 the CG updates
+                # and the dot products of the preconditioned residual have no batched form)
+                bound = st["instance_launches"] / (3 if precision else 1)
+                assert 0 < st["launches"] < bound, st
             for i, pb in enumerate(pbs):
                 E = ctxs[i].field(tofe.FIELD_E)
                 K = oracle.assemble_K(*pb.nel, oracle.ke(), E, pb.fixed_dof).to_scipy()

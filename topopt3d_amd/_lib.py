@@ -82,7 +82,8 @@ class TofeKernelTime(C.Structure):
 
 class TofeBatchStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("capture_seconds", C.c_double),
-                ("replay_seconds", C.c_double), ("replays", C.c_int), ("iterations", C.c_int)]
+                ("replay_seconds", C.c_double), ("replays", C.c_int), ("iterations", C.c_int),
+                ("launches", C.c_int), ("unbatched", C.c_int), ("instance_launches", C.c_int)]
 
 
 class TofeError(RuntimeError):

@@ -20,9 +20,9 @@ constexpr int NT = 256;
 // neighbours' stencil blocks (read again as the transposed lower half) then sit in the
 // same L2.
 template <bool SPLIT>
-__device__ inline void split_index(int64_t nn, int64_t &n, int &a0)
+__device__ inline void split_index(int64_t nn, int blk, int nb, int64_t &n, int &a0)
 {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = xcd_remap(blk, nb);
   if constexpr (SPLIT) {
     const int nbn = (int)((nn + NT - 1) / NT);
     a0 = b / nbn;
@@ -38,11 +38,11 @@ __global__ __launch_bounds__(NT) void k_stencil_op(StencilOp<T> op, const T *u0,
                                                    EpiArgs<T> ep, int64_t nn)
 {
   __shared__ double sred[NT / 64];
-  epi_select_vector(ep, u0, 3 * nn);  // vector blockIdx.y
+  epi_select_vector(ep, u0, 3 * nn, gridDim.x);  // vector blockIdx.y
   const T *__restrict__ u = u0;
   int64_t n;
   int a0;
-  split_index<SPLIT>(nn, n, a0);
+  split_index<SPLIT>(nn, blockIdx.x, gridDim.x, n, a0);
   const T *__restrict__ S = op.S;
   double dsum = 0, dsum2 = 0;
   if (n < nn) {
@@ -262,16 +262,17 @@ __host__ __device__ inline int64_t su_index(int v, int64_t n)
 #ifndef TOFE_SYM64_G
 #define TOFE_SYM64_G 9
 #endif
+// One block (blk of nb per vector) of the upper-half stencil operator.
 template <typename T, int EPI, bool SPLIT>
-__global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T *u0,
-                                                       EpiArgs<T> ep, int64_t nn)
+__device__ __forceinline__ void stencil_sym_block(const StencilOp<T> &op, const T *u0,
+                                                  EpiArgs<T> ep, int64_t nn, int blk, int nb)
 {
   __shared__ double sred[NT / 64];
-  epi_select_vector(ep, u0, 3 * nn);  // vector blockIdx.y
+  epi_select_vector(ep, u0, 3 * nn, nb);  // vector blockIdx.y
   const T *__restrict__ u = u0;
   int64_t n;
   int a0;
-  split_index<SPLIT>(nn, n, a0);
+  split_index<SPLIT>(nn, blk, nb, n, a0);
   const T *__restrict__ S = op.S;
   double dsum = 0, dsum2 = 0;
   if (n < nn) {
@@ -349,13 +350,35 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
   }
   if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2 || EPI == EPI_CHEB_OUT) {
     const double t = block_sum<NT>(dsum, sred);
-    if (threadIdx.x == 0) ep.partial[blockIdx.x] = t;
+    if (threadIdx.x == 0) ep.partial[blk] = t;
     if constexpr (EPI == EPI_APPLY_DOT2) {
       __syncthreads();
       const double t2 = block_sum<NT>(dsum2, sred);
-      if (threadIdx.x == 0) ep.partial2[blockIdx.x] = t2;
+      if (threadIdx.x == 0) ep.partial2[blk] = t2;
     }
   }
+}
+
+template <typename T, int EPI, bool SPLIT>
+__global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T *u0,
+                                                       EpiArgs<T> ep, int64_t nn)
+{
+  stencil_sym_block<T, EPI, SPLIT>(op, u0, ep, nn, blockIdx.x, gridDim.x);
+}
+
+// The operator of several instances' levels in one grid: entry e owns blocks
+// [b0[e], b0[e + 1]); grids of either form (one thread per node or per DOF) share the launch.
+template <typename T, int EPI>
+__global__ __launch_bounds__(NT) void k_stencil_op_sym_batch(const StencilLaunch<T> *__restrict__ tab,
+                                                             const int *__restrict__ b0, int n)
+{
+  const int e = batch_entry(b0, n, blockIdx.x);
+  const StencilLaunch<T> L = tab[e];
+  const int blk = blockIdx.x - b0[e];
+  if (L.nn < L.op.split_nodes)
+    stencil_sym_block<T, EPI, true>(L.op, L.u, L.ep, L.nn, blk, L.nblocks);
+  else
+    stencil_sym_block<T, EPI, false>(L.op, L.u, L.ep, L.nn, blk, L.nblocks);
 }
 
 // FP64 full stencil -> upper half (offsets 13..26) in the node-blocked layout, as To;
@@ -421,8 +444,8 @@ __global__ void k_tri_pack(const double *__restrict__ X, int n, int ld, T *__res
 // wavefront), x staged in LDS, 16-byte loads along the row, products summed in FP64 in a
 // fixed order.
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(NT) void k_trigemv(const T *__restrict__ A, int n, int ld,
-                                                const T *__restrict__ x, T *__restrict__ y)
+__device__ __forceinline__ void trigemv_block(const T *__restrict__ A, int n, int ld,
+                                              const T *__restrict__ x, T *__restrict__ y, int blk)
 {
   x += (int64_t)blockIdx.y * n;  // vector blockIdx.y of a block of right-hand sides
   y += (int64_t)blockIdx.y * n;
@@ -430,7 +453,7 @@ __global__ __launch_bounds__(NT) void k_trigemv(const T *__restrict__ A, int n, 
   T *xs = reinterpret_cast<T *>(xs_raw);
   for (int i = threadIdx.x; i < ld; i += NT) xs[i] = i < n ? x[i] : T(0);
   __syncthreads();
-  const int row = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const int row = blk * (NT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
   constexpr int V = 16 / (int)sizeof(T);  // values per 16-byte load
@@ -454,6 +477,23 @@ __global__ __launch_bounds__(NT) void k_trigemv(const T *__restrict__ A, int n, 
   }
   s = wave_sum(s);
   if (lane == 0) y[row] = (T)s;
+}
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(NT) void k_trigemv(const T *__restrict__ A, int n, int ld,
+                                                const T *__restrict__ x, T *__restrict__ y)
+{
+  trigemv_block<T, LOWER>(A, n, ld, x, y, blockIdx.x);
+}
+
+// Several instances' products in one grid (the dynamic LDS holds the largest ld).
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(NT) void k_trigemv_batch(const TrigemvLaunch<T> *__restrict__ tab,
+                                                      const int *__restrict__ b0, int n)
+{
+  const int e = batch_entry(b0, n, blockIdx.x);
+  const TrigemvLaunch<T> L = tab[e];
+  trigemv_block<T, LOWER>(L.A, L.n, L.ld, L.x, L.y, blockIdx.x - b0[e]);
 }
 
 // ---- dense coarse solve: banded blocked Cholesky, X = L^-1, A^-1 = X^T X ------------
@@ -873,6 +913,39 @@ void stencil_op_sym(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, 
   TOFE_STENCIL_SWITCH(k_stencil_op_sym);
 }
 
+template <typename T, int EPI>
+void launch_sym_batch(hipStream_t s, const StencilLaunch<T> *tab, const int *b0, int n,
+                      int nblocks_total, int nvec)
+{
+  k_stencil_op_sym_batch<T, EPI><<<dim3(nblocks_total, nvec), dim3(NT), 0, s>>>(tab, b0, n);
+}
+
+template <typename T>
+void stencil_sym_batch(hipStream_t s, Epi epi, const StencilLaunch<T> *tab, const int *b0, int n,
+                       int nblocks_total, int nvec)
+{
+  switch (epi) {
+    case EPI_APPLY: launch_sym_batch<T, EPI_APPLY>(s, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_APPLY_DOT: launch_sym_batch<T, EPI_APPLY_DOT>(s, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_APPLY_DOT2: launch_sym_batch<T, EPI_APPLY_DOT2>(s, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_RESID: launch_sym_batch<T, EPI_RESID>(s, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_CHEB: launch_sym_batch<T, EPI_CHEB>(s, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_CHEB_OUT: launch_sym_batch<T, EPI_CHEB_OUT>(s, tab, b0, n, nblocks_total, nvec); break;
+  }
+  TOFE_LAUNCH_CHECK();
+}
+
+template <typename T>
+void trigemv_batch(hipStream_t s, bool lower, const TrigemvLaunch<T> *tab, const int *b0, int n,
+                   int nblocks_total, int nvec, int ld_max)
+{
+  const dim3 grid(nblocks_total, nvec), block(NT);
+  const size_t lds = sizeof(T) * (size_t)ld_max;
+  if (lower) k_trigemv_batch<T, true><<<grid, block, lds, s>>>(tab, b0, n);
+  else k_trigemv_batch<T, false><<<grid, block, lds, s>>>(tab, b0, n);
+  TOFE_LAUNCH_CHECK();
+}
+
 template <typename To>
 void stencil_upper(hipStream_t s, const double *S, int64_t nn, To *Su)
 {
@@ -954,6 +1027,14 @@ template void coarse_solve_tri<double>(hipStream_t, const double *, const double
 template void coarse_solve_tri<float>(hipStream_t, const float *, const float *, int, int, const float *,
                                       float *, float *, int);
 
+template void stencil_sym_batch<float>(hipStream_t, Epi, const StencilLaunch<float> *, const int *,
+                                       int, int, int);
+template void stencil_sym_batch<double>(hipStream_t, Epi, const StencilLaunch<double> *, const int *,
+                                        int, int, int);
+template void trigemv_batch<float>(hipStream_t, bool, const TrigemvLaunch<float> *, const int *, int,
+                                   int, int, int);
+template void trigemv_batch<double>(hipStream_t, bool, const TrigemvLaunch<double> *, const int *,
+                                    int, int, int, int);
 template void stencil_upper<float>(hipStream_t, const double *, int64_t, float *);
 template void stencil_upper<double>(hipStream_t, const double *, int64_t, double *);
 template void stencil_op_sym<double>(hipStream_t, const StencilOp<double> &, const double *, Epi,

@@ -77,15 +77,28 @@ __host__ __device__ inline double w1d(int i, int J, int Nf)
   return 0.5;
 }
 
+// Launches over several instances: one grid over the blocks of all of them, entry e owning
+// blocks [b0[e], b0[e + 1]).  The entry of block b (uniform in the block).
+__device__ inline int batch_entry(const int *__restrict__ b0, int n, int b)
+{
+  int lo = 0, hi = n - 1;  // the largest e with b0[e] <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Epilogue arguments and input of vector blockIdx.y of a launch over ep.nvec vectors stored
 // back to back (ndof values each); the per-block partial sums of vector v start at
-// v * gridDim.x.  Vector 0 keeps the pointers as given.
+// v * nblocks (the launch's blocks per vector).  Vector 0 keeps the pointers as given.
 template <typename T>
-__device__ inline void epi_select_vector(EpiArgs<T> &ep, const T *&u, int64_t ndof)
+__device__ inline void epi_select_vector(EpiArgs<T> &ep, const T *&u, int64_t ndof, int nblocks)
 {
   const int64_t v = blockIdx.y;
   if (v == 0) return;
-  const int64_t vo = v * ndof, po = v * (int64_t)gridDim.x;
+  const int64_t vo = v * ndof, po = v * (int64_t)nblocks;
   u += vo;
   if (ep.y) ep.y += vo;
   if (ep.b) ep.b += vo;

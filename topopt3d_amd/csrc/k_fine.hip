@@ -181,7 +181,7 @@ __global__ __launch_bounds__(NT) void k_fine_op(FineOp<T> op, const T *u0,
 
   const int nx = op.nx, ny = op.ny, nz = op.nz;
   const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
-  epi_select_vector(ep, u0, 3 * (int64_t)Nx * Ny * Nz);  // vector blockIdx.y
+  epi_select_vector(ep, u0, 3 * (int64_t)Nx * Ny * Nz, gridDim.x);  // vector blockIdx.y
   const T *__restrict__ u = u0;
   const int bid = xcd_remap(blockIdx.x, nblocks);
   const int txy = tiles_x * tiles_y;

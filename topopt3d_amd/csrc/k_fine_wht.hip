@@ -103,9 +103,11 @@ __device__ inline void element_product(T (&v)[3][8], const T (&kb)[NV], T Ee)
   wht8(v[2]);
 }
 
+// One block of the operator: block `bx` of the launch's `nblocks` (per vector).
 template <typename T, int EPI, bool CUBE>
-__global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *u0, EpiArgs<T> ep,
-                                                 int tiles_x, int tiles_y, int nblocks, int zlen)
+__device__ __forceinline__ void fine_wht_block(const FineOp<T> &op, const T *u0, EpiArgs<T> ep,
+                                               int tiles_x, int tiles_y, int nblocks, int zlen,
+                                               int blk)
 {
   // partial sums handed to the row below (lower, upper node): double-buffered in FP32; one
   // buffer and a second barrier per step in FP64, which keeps the block at 33 KB of LDS (four
@@ -117,9 +119,9 @@ __global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *u0, EpiA
 
   const int nx = op.nx, ny = op.ny, nz = op.nz;
   const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
-  epi_select_vector(ep, u0, 3 * (int64_t)Nx * Ny * Nz);  // vector blockIdx.y
+  epi_select_vector(ep, u0, 3 * (int64_t)Nx * Ny * Nz, nblocks);  // vector blockIdx.y
   const T *__restrict__ u = u0;
-  const int bid = xcd_remap(blockIdx.x, nblocks);
+  const int bid = xcd_remap(blk, nblocks);
   const int txy = tiles_x * tiles_y;
   const int bz = bid / txy;
   const int rem = bid - bz * txy;
@@ -262,13 +264,33 @@ __global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *u0, EpiA
   if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2 || EPI == EPI_CHEB_OUT) {
     __syncthreads();
     const double s1 = block_sum<NW>(dsum, sred);
-    if (t == 0) ep.partial[blockIdx.x] = s1;
+    if (t == 0) ep.partial[blk] = s1;
     if constexpr (EPI == EPI_APPLY_DOT2) {
       __syncthreads();
       const double s2 = block_sum<NW>(dsum2, sred);
-      if (t == 0) ep.partial2[blockIdx.x] = s2;
+      if (t == 0) ep.partial2[blk] = s2;
     }
   }
+}
+
+template <typename T, int EPI, bool CUBE>
+__global__ __launch_bounds__(NW) void k_fine_wht(FineOp<T> op, const T *u0, EpiArgs<T> ep,
+                                                 int tiles_x, int tiles_y, int nblocks, int zlen)
+{
+  fine_wht_block<T, EPI, CUBE>(op, u0, ep, tiles_x, tiles_y, nblocks, zlen, blockIdx.x);
+}
+
+// The operator of several instances in one grid: entry e of the table owns blocks
+// [b0[e], b0[e + 1]).  (Cubic and general elements take separate launches: one kernel holding
+// both bodies would hold both bodies' LDS arrays.)
+template <typename T, int EPI, bool CUBE>
+__global__ __launch_bounds__(NW) void k_fine_wht_batch(const FineLaunch<T> *__restrict__ tab,
+                                                       const int *__restrict__ b0, int n)
+{
+  const int e = batch_entry(b0, n, blockIdx.x);
+  const FineLaunch<T> L = tab[e];
+  fine_wht_block<T, EPI, CUBE>(L.op, L.u, L.ep, L.tiles_x, L.tiles_y, L.nblocks, L.zlen,
+                               blockIdx.x - b0[e]);
 }
 
 struct WhtGrid {
@@ -341,6 +363,45 @@ void fine_wht(hipStream_t s, const FineOp<T> &op, const T *u, Epi epi, const Epi
   TOFE_LAUNCH_CHECK();
 }
 
+template <typename T>
+int fine_wht_grid(const FineOp<T> &op, int *tiles_x, int *tiles_y)
+{
+  const WhtGrid g = wht_grid(op);
+  *tiles_x = g.tiles_x;
+  *tiles_y = g.tiles_y;
+  return g.nblocks;
+}
+
+template <typename T, int EPI>
+void launch_wht_batch(hipStream_t s, bool cube, const FineLaunch<T> *tab, const int *b0, int n,
+                      int nblocks_total, int nvec)
+{
+  dim3 grid(nblocks_total, nvec), block(NW);
+  if (cube) k_fine_wht_batch<T, EPI, true><<<grid, block, 0, s>>>(tab, b0, n);
+  else k_fine_wht_batch<T, EPI, false><<<grid, block, 0, s>>>(tab, b0, n);
+}
+
+template <typename T>
+void fine_wht_batch(hipStream_t s, Epi epi, bool cube, const FineLaunch<T> *tab, const int *b0,
+                    int n, int nblocks_total, int nvec)
+{
+  switch (epi) {
+    case EPI_APPLY: launch_wht_batch<T, EPI_APPLY>(s, cube, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_APPLY_DOT: launch_wht_batch<T, EPI_APPLY_DOT>(s, cube, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_APPLY_DOT2: launch_wht_batch<T, EPI_APPLY_DOT2>(s, cube, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_RESID: launch_wht_batch<T, EPI_RESID>(s, cube, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_CHEB: launch_wht_batch<T, EPI_CHEB>(s, cube, tab, b0, n, nblocks_total, nvec); break;
+    case EPI_CHEB_OUT: launch_wht_batch<T, EPI_CHEB_OUT>(s, cube, tab, b0, n, nblocks_total, nvec); break;
+  }
+  TOFE_LAUNCH_CHECK();
+}
+
+template int fine_wht_grid<double>(const FineOp<double> &, int *, int *);
+template int fine_wht_grid<float>(const FineOp<float> &, int *, int *);
+template void fine_wht_batch<double>(hipStream_t, Epi, bool, const FineLaunch<double> *, const int *,
+                                     int, int, int);
+template void fine_wht_batch<float>(hipStream_t, Epi, bool, const FineLaunch<float> *, const int *,
+                                    int, int, int);
 template int fine_wht_blocks<double>(const FineOp<double> &);
 template int fine_wht_blocks<float>(const FineOp<float> &);
 template void fine_wht<double>(hipStream_t, const FineOp<double> &, const double *, Epi, const EpiArgs<double> &);

@@ -27,9 +27,9 @@ __global__ __launch_bounds__(NT) void k_dot_partial(const T *__restrict__ a,
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(NT) void k_reduce_final(const double *__restrict__ partial,
-                                                     int nparts, double *out, int post,
-                                                     double *scal)
+__device__ __forceinline__ void reduce_final_block(const double *__restrict__ partial,
+                                                   int nparts, double *out, int post,
+                                                   double *scal)
 {
   __shared__ double sred[NT / 64];
   // vector blockIdx.y: its partials, and its scalars (out included) SL_COUNT further on
@@ -70,6 +70,20 @@ __global__ __launch_bounds__(NT) void k_reduce_final(const double *__restrict__ 
       default: break;
     }
   }
+}
+
+__global__ __launch_bounds__(NT) void k_reduce_final(const double *__restrict__ partial,
+                                                     int nparts, double *out, int post,
+                                                     double *scal)
+{
+  reduce_final_block(partial, nparts, out, post, scal);
+}
+
+__global__ __launch_bounds__(NT) void k_reduce_final_batch(const ReduceLaunch *__restrict__ tab,
+                                                           const int *__restrict__ b0, int n)
+{
+  const ReduceLaunch L = tab[batch_entry(b0, n, blockIdx.x)];
+  reduce_final_block(L.partial, L.nparts, L.out, L.post, L.scal);
 }
 
 __global__ void k_set_conv_state(double *scal, int nvec, ConvState cs, unsigned long long mask)
@@ -132,9 +146,9 @@ __global__ void k_cheb_zero_from64(const double *__restrict__ b64, float *__rest
 
 // b_c(I) = sum over fine nodes i within one fine step of 2I of w(i,I) r_f(i)  (P^T r)
 template <typename T>
-__global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int Nfx, int Nfy,
-                                                 int Nfz, T *__restrict__ bc, int Ncx, int Ncy,
-                                                 int Ncz, FirstStep<T> fs)
+__device__ __forceinline__ void restrict_block(const T *__restrict__ rf, int Nfx, int Nfy, int Nfz,
+                                               T *__restrict__ bc, int Ncx, int Ncy, int Ncz,
+                                               FirstStep<T> fs, int blk)
 {
   const int64_t ncn = (int64_t)Ncx * Ncy * Ncz;
   {  // vector blockIdx.y of fine residuals and coarse right-hand sides; dinv is shared
@@ -146,7 +160,7 @@ __global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int N
       fs.x += v * 3 * ncn;
     }
   }
-  const int64_t I = (int64_t)blockIdx.x * NT + threadIdx.x;
+  const int64_t I = (int64_t)blk * NT + threadIdx.x;
   if (I >= ncn) return;
   int Ix, Iy, Iz;
   node_ijk(I, Ncx, Ncy, Ix, Iy, Iz);
@@ -187,16 +201,32 @@ __global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int N
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(NT) void k_restrict(const T *__restrict__ rf, int Nfx, int Nfy,
+                                                 int Nfz, T *__restrict__ bc, int Ncx, int Ncy,
+                                                 int Ncz, FirstStep<T> fs)
+{
+  restrict_block<T>(rf, Nfx, Nfy, Nfz, bc, Ncx, Ncy, Ncz, fs, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_restrict_batch(const RestrictLaunch<T> *__restrict__ tab,
+                                                       const int *__restrict__ b0, int n)
+{
+  const int e = batch_entry(b0, n, blockIdx.x);
+  const RestrictLaunch<T> L = tab[e];
+  restrict_block<T>(L.rf, L.Nfx, L.Nfy, L.Nfz, L.bc, L.Ncx, L.Ncy, L.Ncz, L.fs, blockIdx.x - b0[e]);
+}
+
 // x_f(i) += sum over coarse parents J of w(i,J) x_c(J)  (P x_c)
 template <typename T>
-__global__ __launch_bounds__(NT) void k_prolong_add(const T *__restrict__ xc, int Ncx, int Ncy,
-                                                    int Ncz, T *__restrict__ xf, int Nfx, int Nfy,
-                                                    int Nfz)
+__device__ __forceinline__ void prolong_block(const T *__restrict__ xc, int Ncx, int Ncy, int Ncz,
+                                              T *__restrict__ xf, int Nfx, int Nfy, int Nfz, int blk)
 {
   const int64_t nfn = (int64_t)Nfx * Nfy * Nfz;
   xc += (int64_t)blockIdx.y * 3 * ((int64_t)Ncx * Ncy * Ncz);  // vector blockIdx.y
   xf += (int64_t)blockIdx.y * 3 * nfn;
-  const int64_t f = (int64_t)blockIdx.x * NT + threadIdx.x;
+  const int64_t f = (int64_t)blk * NT + threadIdx.x;
   if (f >= nfn) return;
   int fx, fy, fz;
   node_ijk(f, Nfx, Nfy, fx, fy, fz);
@@ -228,6 +258,23 @@ __global__ __launch_bounds__(NT) void k_prolong_add(const T *__restrict__ xc, in
       }
 #pragma unroll
   for (int q = 0; q < 3; ++q) xf[3 * f + q] = (T)((double)xf[3 * f + q] + acc[q]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_prolong_add(const T *__restrict__ xc, int Ncx, int Ncy,
+                                                    int Ncz, T *__restrict__ xf, int Nfx, int Nfy,
+                                                    int Nfz)
+{
+  prolong_block<T>(xc, Ncx, Ncy, Ncz, xf, Nfx, Nfy, Nfz, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_prolong_add_batch(const ProlongLaunch<T> *__restrict__ tab,
+                                                          const int *__restrict__ b0, int n)
+{
+  const int e = batch_entry(b0, n, blockIdx.x);
+  const ProlongLaunch<T> L = tab[e];
+  prolong_block<T>(L.xc, L.Ncx, L.Ncy, L.Ncz, L.xf, L.Nfx, L.Nfy, L.Nfz, blockIdx.x - b0[e]);
 }
 
 template <typename Ta, typename Tb>
@@ -334,11 +381,11 @@ __device__ inline double all_sum(const double *__restrict__ partial, int nparts,
 // Flexible-CG step without separate reduction launches, first half: alpha = p.r / p.Ap
 // from the operator's two sets of partials (block 0 stores p.r, p.Ap and alpha in scal);
 // r -= alpha q, partials of r.r (the iterate's update x += alpha p follows in k_fcg_p).
-__global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const double *__restrict__ q,
-                                               int64_t n,
-                                               const double *__restrict__ pq_part,
-                                               const double *__restrict__ pr_part, int nparts,
-                                               double *scal, double *partial, FirstCheb fc)
+__device__ __forceinline__ void fcg_xr_block(double *__restrict__ r, const double *__restrict__ q,
+                                             int64_t n, const double *__restrict__ pq_part,
+                                             const double *__restrict__ pr_part, int nparts,
+                                             double *scal, double *partial, FirstCheb fc, int blk,
+                                             int nb)
 {
   __shared__ double sred[NT / 64];
   {  // vector blockIdx.y: its vectors, partials and scalars; dinv is shared
@@ -348,7 +395,7 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const dou
     pq_part += v * nparts;
     pr_part += v * nparts;
     scal += v * SL_COUNT;
-    partial += v * gridDim.x;
+    partial += v * nb;
     if (fc.b32) {
       fc.b32 += vo;
       fc.d += vo;
@@ -359,13 +406,13 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const dou
   const double pq = all_sum(pq_part, nparts, sred);
   // a frozen right-hand side (converged in a block solve) takes no further steps
   const double alpha = scal[SL_FROZEN] != 0.0 ? 0.0 : pr / pq;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
     scal[SL_PR] = pr;
     scal[SL_PQ] = pq;
     scal[SL_ALPHA] = alpha;
   }
   double s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+  for (int64_t i = (int64_t)blk * NT + threadIdx.x; i < n; i += (int64_t)nb * NT) {
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     s += ri * ri;
@@ -378,15 +425,33 @@ __global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const dou
     }
   }
   const double t = block_sum<NT>(s, sred);
-  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  if (threadIdx.x == 0) partial[blk] = t;
+}
+
+__global__ __launch_bounds__(NT) void k_fcg_xr(double *__restrict__ r, const double *__restrict__ q,
+                                               int64_t n,
+                                               const double *__restrict__ pq_part,
+                                               const double *__restrict__ pr_part, int nparts,
+                                               double *scal, double *partial, FirstCheb fc)
+{
+  fcg_xr_block(r, q, n, pq_part, pr_part, nparts, scal, partial, fc, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(NT) void k_fcg_xr_batch(const FcgXrLaunch *__restrict__ tab,
+                                                     const int *__restrict__ b0, int n)
+{
+  const int e = batch_entry(b0, n, blockIdx.x);
+  const FcgXrLaunch L = tab[e];
+  fcg_xr_block(L.r, L.q, L.n, L.pq_part, L.pr_part, L.nparts, L.scal, L.partial, L.fc,
+               blockIdx.x - b0[e], L.nblocks);
 }
 
 // Second half: beta = -z.Ap / p.Ap from the z.q partials (block 0 stores z.Ap and beta);
 // x += alpha p with the first half's alpha, then p = z + beta p.
-__global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const double *__restrict__ z,
-                                              double *__restrict__ x, int64_t n,
-                                              const double *__restrict__ zq_part, int nparts,
-                                              double *scal)
+__device__ __forceinline__ void fcg_p_block(double *__restrict__ p, const double *__restrict__ z,
+                                            double *__restrict__ x, int64_t n,
+                                            const double *__restrict__ zq_part, int nparts,
+                                            double *scal, int blk, int nb)
 {
   __shared__ double sred[NT / 64];
   {  // vector blockIdx.y
@@ -400,15 +465,31 @@ __global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const doub
   const double zq = all_sum(zq_part, nparts, sred);
   const double beta = scal[SL_FROZEN] != 0.0 ? 0.0 : -zq / scal[SL_PQ];
   const double alpha = scal[SL_ALPHA];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
     scal[SL_ZQ] = zq;
     scal[SL_BETA] = beta;
   }
-  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+  for (int64_t i = (int64_t)blk * NT + threadIdx.x; i < n; i += (int64_t)nb * NT) {
     const double pi = p[i];
     x[i] += alpha * pi;  // the iterate's update of the step that formed alpha
     p[i] = z[i] + beta * pi;
   }
+}
+
+__global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const double *__restrict__ z,
+                                              double *__restrict__ x, int64_t n,
+                                              const double *__restrict__ zq_part, int nparts,
+                                              double *scal)
+{
+  fcg_p_block(p, z, x, n, zq_part, nparts, scal, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(NT) void k_fcg_p_batch(const FcgPLaunch *__restrict__ tab,
+                                                    const int *__restrict__ b0, int n)
+{
+  const int e = batch_entry(b0, n, blockIdx.x);
+  const FcgPLaunch L = tab[e];
+  fcg_p_block(L.p, L.z, L.x, L.n, L.zq_part, L.nparts, L.scal, blockIdx.x - b0[e], L.nblocks);
 }
 
 // x += alpha p with alpha = scal[SL_ALPHA]: the iterate's update when no k_fcg_p / k_cg_xp follows.
@@ -602,6 +683,45 @@ void fcg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t
   TOFE_LAUNCH_CHECK();
 }
 
+int fcg_update_r_blocks() { return kRedBlocks; }
+int fcg_update_xp_blocks(int64_t n) { return (int)stride_blocks(n); }
+
+void reduce_final_batch(hipStream_t s, const ReduceLaunch *tab, const int *b0, int n, int nvec)
+{
+  k_reduce_final_batch<<<dim3(n, nvec), dim3(NT), 0, s>>>(tab, b0, n);
+  TOFE_LAUNCH_CHECK();
+}
+
+void fcg_update_r_batch(hipStream_t s, const FcgXrLaunch *tab, const int *b0, int n,
+                        int nblocks_total, int nvec)
+{
+  k_fcg_xr_batch<<<dim3(nblocks_total, nvec), dim3(NT), 0, s>>>(tab, b0, n);
+  TOFE_LAUNCH_CHECK();
+}
+
+void fcg_update_xp_batch(hipStream_t s, const FcgPLaunch *tab, const int *b0, int n,
+                         int nblocks_total, int nvec)
+{
+  k_fcg_p_batch<<<dim3(nblocks_total, nvec), dim3(NT), 0, s>>>(tab, b0, n);
+  TOFE_LAUNCH_CHECK();
+}
+
+template <typename T>
+void restrict_batch(hipStream_t s, const RestrictLaunch<T> *tab, const int *b0, int n,
+                    int nblocks_total, int nvec)
+{
+  k_restrict_batch<T><<<dim3(nblocks_total, nvec), dim3(NT), 0, s>>>(tab, b0, n);
+  TOFE_LAUNCH_CHECK();
+}
+
+template <typename T>
+void prolong_batch(hipStream_t s, const ProlongLaunch<T> *tab, const int *b0, int n,
+                   int nblocks_total, int nvec)
+{
+  k_prolong_add_batch<T><<<dim3(nblocks_total, nvec), dim3(NT), 0, s>>>(tab, b0, n);
+  TOFE_LAUNCH_CHECK();
+}
+
 void fcg_update_x(hipStream_t s, double *x, const double *p, int64_t n, const double *scal, int nvec)
 {
   k_fcg_x<<<dim3(stride_blocks(n), nvec), dim3(NT), 0, s>>>(x, p, n, scal);
@@ -632,6 +752,8 @@ void lanczos_p(hipStream_t s, T *p, const T *r, const T *dinv, int64_t n, const 
   template void restrict_op<T>(hipStream_t, const T *, int, int, int, T *, int, int, int,        \
                                const FirstStep<T> &, int);                                    \
   template void prolong_add<T>(hipStream_t, const T *, int, int, int, T *, int, int, int, int); \
+  template void restrict_batch<T>(hipStream_t, const RestrictLaunch<T> *, const int *, int, int, int); \
+  template void prolong_batch<T>(hipStream_t, const ProlongLaunch<T> *, const int *, int, int, int); \
 
 TOFE_INST_T(double)
 TOFE_INST_T(float)

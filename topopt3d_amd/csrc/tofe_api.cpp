@@ -14,7 +14,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <functional>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include <rocprofiler-sdk-roctx/roctx.h>
@@ -93,6 +96,64 @@ double now_s()
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// The CU-masked streams of one device, shared by every context of the process.  A CU mask is
+// a property of a hardware queue, so each CU-masked stream holds a queue of its own; once a
+// process holds more queues than the hardware scheduler maps at once, the scheduler
+// time-slices all of them, and busy kernels then stall for whole quanta (measured: 21 ms
+// stalls every 31.5 ms of busy time with 16 contexts of two masked streams each).  Created on
+// first use and kept until the process ends.
+struct MaskedStreams {
+  hipStream_t a = nullptr, b = nullptr;
+};
+MaskedStreams masked_streams(int device, int ncu, int first_cu)
+{
+  static std::mutex mu;
+  static std::map<int, MaskedStreams> per_device;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = per_device.find(device);
+  if (it != per_device.end()) return it->second;
+  std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+  for (int cu = first_cu; cu < ncu; ++cu) mask[cu / 32] |= 1u << (cu % 32);
+  MaskedStreams m;
+  TOFE_HIP(hipExtStreamCreateWithCUMask(&m.a, (uint32_t)mask.size(), mask.data()));
+  TOFE_HIP(hipExtStreamCreateWithCUMask(&m.b, (uint32_t)mask.size(), mask.data()));
+  per_device[device] = m;
+  return m;
+}
+
+// Launches of one instance's solve iteration, recorded instead of enqueued, so that a batch
+// of instances can run each step as one launch over all of them.  An op of a kind with a
+// batched kernel keeps its launch parameters (desc, blocks per vector); any other op is kept
+// as a closure that enqueues it on a given stream.
+enum BatchKind : int {
+  BK_CLOSURE = 0,
+  BK_FINE = 1,      // fine operator, element-centric Walsh-Hadamard form
+  BK_STENCIL = 2,   // upper-half stencil operator of a coarse level
+  BK_RESTRICT = 3,
+  BK_PROLONG = 4,
+  BK_TRI_LO = 5,    // coarse solve, tmp = X b
+  BK_TRI_UP = 6,    // coarse solve, x = X^T tmp
+  BK_FCG_XR = 7,
+  BK_FCG_P = 8,
+  BK_REDUCE = 9,
+};
+// kind, precision, element form (fine operator) and epilogue in one key: ops with equal keys
+// share a launch
+constexpr int batch_key(int kind, bool fp32, int epi = 0, bool cube = false)
+{
+  return (kind << 8) | ((fp32 ? 1 : 0) << 7) | ((cube ? 1 : 0) << 6) | epi;
+}
+struct RecOp {
+  int key = BK_CLOSURE;
+  int nblocks = 0;  // blocks per vector
+  int ld = 0;       // coarse solve: padded row length (dynamic LDS)
+  std::vector<unsigned char> desc;
+  std::function<void(hipStream_t)> run;
+};
+struct Recorder {
+  std::vector<RecOp> ops;
+};
+
 }  // namespace
 
 struct tofe_ctx {
@@ -102,6 +163,7 @@ struct tofe_ctx {
   std::vector<Grid> grids;
   int64_t ne = 0, nn = 0, ndof = 0;
   hipStream_t stream = nullptr;
+  Recorder *rec = nullptr;  // set while a batch records this context's iteration
   // second stream: the coarsest-level factorisation runs beside the Lanczos estimates
   hipStream_t stream2 = nullptr;
   // third stream: the Lanczos estimates of the coarser smoothed levels run beside the
@@ -109,8 +171,10 @@ struct tofe_ctx {
   hipStream_t stream3 = nullptr;
   // fourth stream: the fine level's Lanczos estimate.  The third and fourth streams are
   // kept off the first kCoarseCUs compute units, which the latency-bound coarse
-  // factorisation on the second stream then has to itself.
+  // factorisation on the second stream then has to itself; those CU-masked streams are the
+  // device's pair shared by every context (masked_streams), not destroyed with the context.
   hipStream_t stream4 = nullptr;
+  bool shared_34 = false;
   hipEvent_t ev_dense = nullptr, ev_inv = nullptr, ev_ops = nullptr, ev_lz = nullptr;
   hipEvent_t ev_scal = nullptr, ev_lz0 = nullptr;
   double *hscal = nullptr;  // pinned host copy of the device scalars
@@ -221,8 +285,8 @@ struct tofe_ctx {
     if (hscal) (void)hipHostFree(hscal);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
-    if (stream3) (void)hipStreamDestroy(stream3);
-    if (stream4) (void)hipStreamDestroy(stream4);
+    if (stream3 && !shared_34) (void)hipStreamDestroy(stream3);
+    if (stream4 && !shared_34) (void)hipStreamDestroy(stream4);
   }
 };
 
@@ -309,11 +373,83 @@ void kernel_timer_end(tofe_ctx *c, hipStream_t s)
   ++c->kt_n;
 }
 
+template <typename D>
+void rec_push(tofe_ctx *c, int key, const D &d, int nblocks, int ld = 0)
+{
+  static_assert(std::is_trivially_copyable<D>::value, "launch parameters are copied as bytes");
+  RecOp op;
+  op.key = key;
+  op.nblocks = nblocks;
+  op.ld = ld;
+  op.desc.resize(sizeof(D));
+  std::memcpy(op.desc.data(), &d, sizeof(D));
+  c->rec->ops.push_back(std::move(op));
+}
+
+// Enqueue f(stream) on the context's stream, or record it while a batch records the context.
+void issue(tofe_ctx *c, std::function<void(hipStream_t)> f)
+{
+  if (c->rec) {
+    RecOp op;
+    op.run = std::move(f);
+    c->rec->ops.push_back(std::move(op));
+  } else {
+    f(c->stream);
+  }
+}
+
+template <typename T>
+void level_apply_now(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep, hipStream_t s);
+
+// Records a level's operator launch (recording mode of level_apply).
+template <typename T>
+void level_record(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep)
+{
+  constexpr bool f32 = sizeof(T) == 4;
+  if (l == 0) {
+    const FineOp<T> f = fine_of<T>(c);
+    if (f.wht && !c->fine_mfma) {
+      FineLaunch<T> d;
+      d.op = f;
+      d.u = u;
+      d.ep = ep;
+      d.nblocks = fine_wht_grid<T>(f, &d.tiles_x, &d.tiles_y);
+      d.zlen = f.zlen;
+      rec_push(c, batch_key(BK_FINE, f32, epi, f.wht_cube), d, d.nblocks);
+      return;
+    }
+  } else {
+    auto &lv = *levels<T>(c)[l];
+    if (f32 || lv.su_ok) {
+      StencilLaunch<T> d;
+      d.op.S = f32 ? lv.S.p : lv.Su.p;
+      d.op.Nx = lv.g.Nx; d.op.Ny = lv.g.Ny; d.op.Nz = lv.g.Nz;
+      d.op.split_nodes = c->split_nodes;
+      d.u = u;
+      d.ep = ep;
+      d.nn = lv.g.nn();
+      d.nblocks = stencil_op_blocks<T>(d.op);
+      rec_push(c, batch_key(BK_STENCIL, f32, epi), d, d.nblocks);
+      return;
+    }
+  }
+  issue(c, [=](hipStream_t s) { level_apply_now<T>(c, l, u, epi, ep, s); });
+}
+
 template <typename T>
 void level_apply(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep,
                  hipStream_t s = nullptr)
 {
-  if (!s) s = c->stream;
+  if (c->rec && (!s || s == c->stream)) {
+    level_record<T>(c, l, u, epi, ep);
+    return;
+  }
+  level_apply_now<T>(c, l, u, epi, ep, s ? s : c->stream);
+}
+
+template <typename T>
+void level_apply_now(tofe_ctx *c, int l, const T *u, Epi epi, const EpiArgs<T> &ep, hipStream_t s)
+{
   if (l == 0) {
     const FineOp<T> f = fine_of<T>(c);
     const bool timed = kernel_timer_begin(c, s, l, sizeof(T) == 4, epi);
@@ -419,6 +555,23 @@ void ensure_nvec(tofe_ctx *c, int nvec)
 template <typename T>
 void coarse_apply(tofe_ctx *c, const T *b, T *x, int nvec = 1)
 {
+  if (c->rec) {
+    TrigemvLaunch<T> lo, up;
+    lo.n = up.n = (int)c->nc;
+    lo.ld = up.ld = c->ldc;
+    lo.nblocks = up.nblocks = (lo.n + 3) / 4;
+    T *tmp;
+    if constexpr (sizeof(T) == 4) {
+      lo.A = c->X32r.p; up.A = c->XT32r.p; tmp = c->ctmp32.p;
+    } else {
+      lo.A = c->X64r.p; up.A = c->XT64r.p; tmp = c->ctmp64.p;
+    }
+    lo.x = b; lo.y = tmp;
+    up.x = tmp; up.y = x;
+    rec_push(c, batch_key(BK_TRI_LO, sizeof(T) == 4), lo, lo.nblocks, lo.ld);
+    rec_push(c, batch_key(BK_TRI_UP, sizeof(T) == 4), up, up.nblocks, up.ld);
+    return;
+  }
   if constexpr (sizeof(T) == 4)
     coarse_solve_tri<float>(c->stream, c->X32r.p, c->XT32r.p, (int)c->nc, c->ldc, b, c->ctmp32.p, x,
                             nvec);
@@ -467,7 +620,11 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
   const hipStream_t s = c->stream;
   const bool mixed_io = io && sizeof(T) == 4;
   if (L == 1) {
-    if (mixed_io && io->b64) convert<double, T>(s, io->b64, const_cast<T *>(b0), c->ndof * nvec);
+    if (mixed_io && io->b64) {
+      const double *b64 = io->b64;
+      const int64_t nd = c->ndof * nvec;
+      issue(c, [=](hipStream_t st) { convert<double, T>(st, b64, const_cast<T *>(b0), nd); });
+    }
     coarse_apply<T>(c, b0, x0, nvec);
     return;
   }
@@ -482,19 +639,26 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
     const int64_t nd = lv.g.ndof();
     const double *c1 = &c->c1[(size_t)l * km], *c2 = &c->c2[(size_t)l * km];
     const bool from64 = l == 0 && mixed_io && io->b64;
-    if (from64 && nu1 == 0) convert<double, T>(s, io->b64, const_cast<T *>(b0), nd * nvec);
+    const double *b64 = from64 ? io->b64 : nullptr;
+    if (from64 && nu1 == 0)
+      issue(c, [=](hipStream_t st) { convert<double, T>(st, b64, const_cast<T *>(b0), nd * nvec); });
     if (nu1 > 0) {
       // the first step from x = 0 on levels l >= 1 was formed by the restriction into l
+      const T *dinv = lv.dinv.p;
+      T *d = lv.d.p;
+      const T c20 = (T)c2[0];
       if constexpr (sizeof(T) == 4) {
         if (from64 && io->first_done) {
           // formed by the kernel that produced b64
         } else if (from64)
-          cheb_zero_from64(s, io->b64, const_cast<float *>(b0), lv.dinv.p, (float)c2[0], lv.d.p,
-                           xc, nd, nvec);
+          issue(c, [=](hipStream_t st) {
+            cheb_zero_from64(st, b64, const_cast<float *>(b0), dinv, c20, d, xc, nd, nvec);
+          });
         else if (l == 0)
-          cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd, nvec);
+          issue(c, [=](hipStream_t st) { cheb_zero<T>(st, b, dinv, c20, d, xc, nd, nvec); });
       } else {
-        if (l == 0) cheb_zero<T>(s, b, lv.dinv.p, (T)c2[0], lv.d.p, xc, nd, nvec);
+        if (l == 0)
+          issue(c, [=](hipStream_t st) { cheb_zero<T>(st, b, dinv, c20, d, xc, nd, nvec); });
       }
       for (int k = 1; k < nu1; ++k) {
         EpiArgs<T> ep;
@@ -505,7 +669,7 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
         std::swap(xc, xo);
       }
     } else {
-      TOFE_HIP(hipMemsetAsync(xc, 0, nd * nvec * sizeof(T), s));
+      issue(c, [=](hipStream_t st) { TOFE_HIP(hipMemsetAsync(xc, 0, nd * nvec * sizeof(T), st)); });
     }
     EpiArgs<T> er;
     er.y = lv.r.p; er.b = b;
@@ -520,7 +684,16 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
       fs.d = ln.d.p;
       fs.x = (flips & 1) ? ln.x2.p : ln.x.p;  // the level's xc (same flip count on every level)
     }
-    restrict_op<T>(s, lv.r.p, gf.Nx, gf.Ny, gf.Nz, LV[l + 1]->b.p, gc.Nx, gc.Ny, gc.Nz, fs, nvec);
+    if (c->rec) {
+      RestrictLaunch<T> d;
+      d.rf = lv.r.p; d.Nfx = gf.Nx; d.Nfy = gf.Ny; d.Nfz = gf.Nz;
+      d.bc = LV[l + 1]->b.p; d.Ncx = gc.Nx; d.Ncy = gc.Ny; d.Ncz = gc.Nz;
+      d.fs = fs;
+      d.nblocks = (int)((gc.nn() + 255) / 256);
+      rec_push(c, batch_key(BK_RESTRICT, sizeof(T) == 4), d, d.nblocks);
+    } else {
+      restrict_op<T>(s, lv.r.p, gf.Nx, gf.Ny, gf.Nz, LV[l + 1]->b.p, gc.Nx, gc.Ny, gc.Nz, fs, nvec);
+    }
     cur[l] = xc;
     oth[l] = xo;
   }
@@ -532,7 +705,15 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
     const T *b = (l == 0) ? b0 : lv.b.p;
     T *xc = cur[l], *xo = oth[l];
     const Grid &gf = lv.g, &gc = LV[l + 1]->g;
-    prolong_add<T>(s, cur[l + 1], gc.Nx, gc.Ny, gc.Nz, xc, gf.Nx, gf.Ny, gf.Nz, nvec);
+    if (c->rec) {
+      ProlongLaunch<T> d;
+      d.xc = cur[l + 1]; d.Ncx = gc.Nx; d.Ncy = gc.Ny; d.Ncz = gc.Nz;
+      d.xf = xc; d.Nfx = gf.Nx; d.Nfy = gf.Ny; d.Nfz = gf.Nz;
+      d.nblocks = (int)((gf.nn() + 255) / 256);
+      rec_push(c, batch_key(BK_PROLONG, sizeof(T) == 4), d, d.nblocks);
+    } else {
+      prolong_add<T>(s, cur[l + 1], gc.Nx, gc.Ny, gc.Nz, xc, gf.Nx, gf.Ny, gf.Nz, nvec);
+    }
     const double *c1 = &c->c1[(size_t)l * km], *c2 = &c->c2[(size_t)l * km];
     for (int k = 0; k < nu2; ++k) {
       EpiArgs<T> ep;
@@ -550,8 +731,13 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
     }
     cur[l] = xc;
   }
-  if (cur[0] != x0)
-    TOFE_HIP(hipMemcpyAsync(x0, cur[0], c->ndof * nvec * sizeof(T), hipMemcpyDeviceToDevice, s));
+  if (cur[0] != x0) {
+    const T *src = cur[0];
+    const size_t bytes = c->ndof * nvec * sizeof(T);
+    issue(c, [=](hipStream_t st) {
+      TOFE_HIP(hipMemcpyAsync(x0, src, bytes, hipMemcpyDeviceToDevice, st));
+    });
+  }
 }
 
 // z = M^-1 r.  With q given (flexible CG), also the partial sums of z.q into c->partial;
@@ -569,21 +755,39 @@ int precondition(tofe_ctx *c, const double *r, double *z, const double *q = null
     }
     vcycle<float>(c, c->vr32.p, c->vz32.p, &io, nvec);
     if (io.fused_out) return level_blocks<float>(c, 0);
-    convert<float, double>(s, c->vz32.p, z, c->ndof * nvec);
+    const float *z32 = c->vz32.p;
+    const int64_t nd = c->ndof * nvec;
+    issue(c, [=](hipStream_t st) { convert<float, double>(st, z32, z, nd); });
   } else {
     vcycle<double>(c, r, z, nullptr, nvec);
   }
   if (!q) return 0;
-  dot_partial<double>(s, z, q, c->ndof, c->partial.p, nvec);
+  double *part = c->partial.p;
+  const int64_t nd = c->ndof;
+  issue(c, [=](hipStream_t st) { dot_partial<double>(st, z, q, nd, part, nvec); });
   return kRedBlocks;
+}
+
+// Final stage of a reduction (k_reduce_final), enqueued or recorded.
+void reduce_into(tofe_ctx *c, const double *partial, int nparts, double *out, int post, double *scal,
+                 int nvec)
+{
+  if (c->rec) {
+    ReduceLaunch d;
+    d.partial = partial; d.nparts = nparts; d.out = out; d.post = post; d.scal = scal;
+    rec_push(c, batch_key(BK_REDUCE, false), d, 1);
+  } else {
+    reduce_final(c->stream, partial, nparts, out, post, scal, nvec);
+  }
 }
 
 // scal[v * SL_COUNT + slot] = a_v . b_v for the nvec vectors of length n
 void dot_into(tofe_ctx *c, const double *a, const double *b, int64_t n, int slot, int post = POST_NONE,
               int nvec = 1)
 {
-  dot_partial<double>(c->stream, a, b, n, c->partial.p, nvec);
-  reduce_final(c->stream, c->partial.p, kRedBlocks, c->scal.p + slot, post, c->scal.p, nvec);
+  double *part = c->partial.p;
+  issue(c, [=](hipStream_t st) { dot_partial<double>(st, a, b, n, part, nvec); });
+  reduce_into(c, c->partial.p, kRedBlocks, c->scal.p + slot, post, c->scal.p, nvec);
 }
 
 // Device scalars (of nvec vectors) to the host through the pinned buffer: queue the copy and
@@ -1261,19 +1465,36 @@ void iteration_on_device(tofe_ctx *c, double *U, int nvec)
     ep.b = r;
     ep.partial2 = c->partial2.p;
     level_apply<double>(c, 0, p, EPI_APPLY_DOT2, ep);
-    fcg_update_r(s, r, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p, first,
-                 nvec);
-    reduce_final(s, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_CONV_FCG, c->scal.p, nvec);
+    if (c->rec) {
+      FcgXrLaunch d;
+      d.r = r; d.q = q; d.n = n; d.pq_part = c->partial.p; d.pr_part = c->partial2.p;
+      d.nparts = nfb; d.scal = c->scal.p; d.partial = c->partial3.p; d.fc = first;
+      d.nblocks = fcg_update_r_blocks();
+      rec_push(c, batch_key(BK_FCG_XR, false), d, d.nblocks);
+    } else {
+      fcg_update_r(s, r, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p, first,
+                   nvec);
+    }
+    reduce_into(c, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_CONV_FCG, c->scal.p, nvec);
     const int nparts = precondition(c, r, z, q, first.b32 != nullptr, nvec);
-    fcg_update_xp(s, p, z, U, n, c->partial.p, nparts, c->scal.p, nvec);
+    if (c->rec) {
+      FcgPLaunch d;
+      d.p = p; d.z = z; d.x = U; d.n = n; d.zq_part = c->partial.p; d.nparts = nparts;
+      d.scal = c->scal.p;
+      d.nblocks = fcg_update_xp_blocks(n);
+      rec_push(c, batch_key(BK_FCG_P, false), d, d.nblocks);
+    } else {
+      fcg_update_xp(s, p, z, U, n, c->partial.p, nparts, c->scal.p, nvec);
+    }
   } else {
     level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
-    reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p, nvec);
-    cg_update_r(s, r, q, n, c->scal.p, SL_ALPHA, c->partial.p, nvec);
-    reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_CONV_CG, c->scal.p, nvec);
+    reduce_into(c, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p, nvec);
+    double *scal = c->scal.p, *part = c->partial.p;
+    issue(c, [=](hipStream_t st) { cg_update_r(st, r, q, n, scal, SL_ALPHA, part, nvec); });
+    reduce_into(c, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_CONV_CG, c->scal.p, nvec);
     precondition(c, r, z, nullptr, false, nvec);
     dot_into(c, r, z, n, SL_TMP, POST_BETA_CG, nvec);
-    cg_update_xp(s, p, z, U, n, c->scal.p, SL_BETA, SL_ALPHA, nvec);
+    issue(c, [=](hipStream_t st) { cg_update_xp(st, p, z, U, n, scal, SL_BETA, SL_ALPHA, nvec); });
   }
 }
 
@@ -1426,10 +1647,10 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
         c->fine_tile = c->fine_tile32 = FineTile{t, z};
       }
       if (ncu >= 4 * kCoarseCUs) {
-        std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
-        for (int cu = kCoarseCUs; cu < ncu; ++cu) mask[cu / 32] |= 1u << (cu % 32);
-        TOFE_HIP(hipExtStreamCreateWithCUMask(&c->stream3, (uint32_t)mask.size(), mask.data()));
-        TOFE_HIP(hipExtStreamCreateWithCUMask(&c->stream4, (uint32_t)mask.size(), mask.data()));
+        const MaskedStreams m = masked_streams(c->device, ncu, kCoarseCUs);
+        c->stream3 = m.a;
+        c->stream4 = m.b;
+        c->shared_34 = true;
       } else {
         TOFE_HIP(hipStreamCreate(&c->stream3));
         TOFE_HIP(hipStreamCreate(&c->stream4));
@@ -2401,11 +2622,15 @@ struct tofe_batch {
   hipStream_t origin = nullptr;  // the stream the graph is captured from and launched on
   hipEvent_t ev_fork = nullptr;
   std::vector<hipEvent_t> ev_join;
-  // how the chunks of iterations run: one graph holding every unfinished instance as a branch
-  // (TOFE_BATCH_GRAPH=single, the default), one graph per instance on the instance's own stream
-  // (=instance), or no graph, the chunk's kernels enqueued on the instances' streams in turn
-  // (=streams)
-  enum Form { SINGLE, INSTANCE, STREAMS } form = SINGLE;
+  // how the chunks of iterations run: one graph of batched launches, each step of the
+  // iteration one launch over every unfinished instance (TOFE_BATCH_GRAPH=merged, the
+  // default); one graph holding every unfinished instance's launches as a branch (=single);
+  // one graph per instance on the instance's own stream (=instance); or no graph, the chunk's
+  // kernels enqueued on the instances' streams in turn (=streams)
+  enum Form { MERGED, SINGLE, INSTANCE, STREAMS } form = MERGED;
+  // of the first merged program of the last solve, per iteration: launches, those serving one
+  // instance, and the launches the instances' own iterations make
+  int program_launches = 0, program_unbatched = 0, program_recorded = 0;
   tofe_batch_stats stats{};
   std::string err;
 
@@ -2496,6 +2721,169 @@ hipGraphExec_t capture_instance(tofe_ctx *c, double *U, int nvec, int chunk)
   return exec;
 }
 
+// One solve iteration of several instances as a program of batched launches.  Each
+// instance's launches are recorded (iteration_on_device in recording mode); the sequences are
+// then merged step by step: the instance with the most launches left leads, and every
+// instance whose next launch has the lead's key joins that step's launch.  Each instance's
+// launches keep their order and each block of a batched launch does what the instance's own
+// launch does with that block, so the results are those of the instances' own iterations,
+// bit for bit.  Launches without a batched form run on their own, in their place.
+struct BatchProgram {
+  struct Step {
+    int key = BK_CLOSURE;
+    int n = 0, total = 0, ld_max = 0;
+    size_t tab = 0, b0 = 0;  // offsets of the parameter table and block offsets in the arena
+    std::function<void(hipStream_t)> run;
+  };
+  std::vector<Step> steps;
+  DBuf<unsigned char> arena;
+  int nvec = 1;
+  int batched = 0, single = 0;  // launches per iteration with several instances / one
+  int recorded = 0;             // the instances' own launches per iteration, summed
+
+  void build(const std::vector<tofe_ctx *> &ctx, const std::vector<double *> &U,
+             const std::vector<int> &live, int nv)
+  {
+    nvec = nv;
+    steps.clear();
+    batched = single = recorded = 0;
+    const size_t n = ctx.size();
+    std::vector<Recorder> rec(n);
+    for (size_t i = 0; i < n; ++i) {
+      if (!live[i]) continue;
+      struct Stop {
+        tofe_ctx *c;
+        ~Stop() { c->rec = nullptr; }
+      } stop{ctx[i]};
+      ctx[i]->rec = &rec[i];
+      iteration_on_device(ctx[i], U[i], nvec);
+      recorded += (int)rec[i].ops.size();
+    }
+    std::vector<size_t> pos(n, 0);
+    std::vector<unsigned char> host;
+    auto align = [&host] { host.resize((host.size() + 255) / 256 * 256); };
+    for (;;) {
+      size_t lead = n, most = 0;
+      for (size_t i = 0; i < n; ++i) {
+        const size_t left = rec[i].ops.size() - pos[i];
+        if (left > most) {
+          most = left;
+          lead = i;
+        }
+      }
+      if (lead == n) break;
+      const RecOp &head = rec[lead].ops[pos[lead]];
+      Step st;
+      st.key = head.key;
+      if (head.key == BK_CLOSURE) {
+        st.run = head.run;
+        ++pos[lead];
+        steps.push_back(std::move(st));
+        ++single;
+        continue;
+      }
+      std::vector<size_t> grp;
+      for (size_t i = 0; i < n; ++i)
+        if (pos[i] < rec[i].ops.size() && rec[i].ops[pos[i]].key == head.key) grp.push_back(i);
+      align();
+      st.tab = host.size();
+      std::vector<int> b0;
+      int64_t total = 0;
+      for (size_t i : grp) {
+        const RecOp &o = rec[i].ops[pos[i]];
+        host.insert(host.end(), o.desc.begin(), o.desc.end());
+        b0.push_back((int)total);
+        total += o.nblocks;
+        st.ld_max = std::max(st.ld_max, o.ld);
+        ++pos[i];
+      }
+      if (total >= (int64_t(1) << 31)) throw Error(TOFE_ERR_ARG, "batched launch of 2^31 blocks or more");
+      b0.push_back((int)total);
+      align();
+      st.b0 = host.size();
+      const unsigned char *pb = reinterpret_cast<const unsigned char *>(b0.data());
+      host.insert(host.end(), pb, pb + b0.size() * sizeof(int));
+      st.n = (int)grp.size();
+      st.total = (int)total;
+      steps.push_back(std::move(st));
+      ++(grp.size() > 1 ? batched : single);
+    }
+    arena.alloc(host.size());
+    if (!host.empty()) TOFE_HIP(hipMemcpy(arena.p, host.data(), host.size(), hipMemcpyHostToDevice));
+  }
+
+  template <typename D>
+  const D *tab(const Step &st) const
+  {
+    return reinterpret_cast<const D *>(arena.p + st.tab);
+  }
+
+  void enqueue(hipStream_t s) const
+  {
+    for (const Step &st : steps) {
+      if (st.key == BK_CLOSURE) {
+        st.run(s);
+        continue;
+      }
+      const int kind = st.key >> 8, epi = st.key & 63;
+      const bool f32 = (st.key >> 7) & 1, cube = (st.key >> 6) & 1;
+      const int *b0 = reinterpret_cast<const int *>(arena.p + st.b0);
+      switch (kind) {
+        case BK_FINE:
+          if (f32) fine_wht_batch<float>(s, (Epi)epi, cube, tab<FineLaunch<float>>(st), b0, st.n, st.total, nvec);
+          else fine_wht_batch<double>(s, (Epi)epi, cube, tab<FineLaunch<double>>(st), b0, st.n, st.total, nvec);
+          break;
+        case BK_STENCIL:
+          if (f32) stencil_sym_batch<float>(s, (Epi)epi, tab<StencilLaunch<float>>(st), b0, st.n, st.total, nvec);
+          else stencil_sym_batch<double>(s, (Epi)epi, tab<StencilLaunch<double>>(st), b0, st.n, st.total, nvec);
+          break;
+        case BK_RESTRICT:
+          if (f32) restrict_batch<float>(s, tab<RestrictLaunch<float>>(st), b0, st.n, st.total, nvec);
+          else restrict_batch<double>(s, tab<RestrictLaunch<double>>(st), b0, st.n, st.total, nvec);
+          break;
+        case BK_PROLONG:
+          if (f32) prolong_batch<float>(s, tab<ProlongLaunch<float>>(st), b0, st.n, st.total, nvec);
+          else prolong_batch<double>(s, tab<ProlongLaunch<double>>(st), b0, st.n, st.total, nvec);
+          break;
+        case BK_TRI_LO:
+        case BK_TRI_UP: {
+          const bool lower = kind == BK_TRI_LO;
+          if (f32)
+            trigemv_batch<float>(s, lower, tab<TrigemvLaunch<float>>(st), b0, st.n, st.total, nvec, st.ld_max);
+          else
+            trigemv_batch<double>(s, lower, tab<TrigemvLaunch<double>>(st), b0, st.n, st.total, nvec, st.ld_max);
+          break;
+        }
+        case BK_FCG_XR: fcg_update_r_batch(s, tab<FcgXrLaunch>(st), b0, st.n, st.total, nvec); break;
+        case BK_FCG_P: fcg_update_xp_batch(s, tab<FcgPLaunch>(st), b0, st.n, st.total, nvec); break;
+        case BK_REDUCE: reduce_final_batch(s, tab<ReduceLaunch>(st), b0, st.n, nvec); break;
+        default: throw Error(TOFE_ERR_STATE, "batched launch of an unknown kind");
+      }
+    }
+  }
+};
+
+// Graph of `chunk` iterations of a merged program, captured from the batch's stream.
+hipGraphExec_t capture_program(tofe_batch *b, const BatchProgram &prog, int chunk)
+{
+  hipGraph_t g = nullptr;
+  TOFE_HIP(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int k = 0; k < chunk; ++k) prog.enqueue(b->origin);
+  } catch (...) {
+    (void)hipStreamEndCapture(b->origin, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  TOFE_HIP(hipStreamEndCapture(b->origin, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    throw Error(TOFE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return exec;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2521,7 +2909,8 @@ int tofe_batch_create(tofe_ctx *const *ctxs, int n, tofe_batch **out)
     if (const char *m = std::getenv("TOFE_BATCH_GRAPH"))
       b->form = std::strcmp(m, "instance") == 0  ? tofe_batch::INSTANCE
                 : std::strcmp(m, "streams") == 0 ? tofe_batch::STREAMS
-                                                 : tofe_batch::SINGLE;
+                : std::strcmp(m, "single") == 0  ? tofe_batch::SINGLE
+                                                 : tofe_batch::MERGED;
   } catch (const Error &e) {
     return e.code;
   } catch (...) {
@@ -2555,6 +2944,7 @@ int tofe_batch_solve_dev(tofe_batch *b, const double *const *d_F, double *const 
       ensure_nvec(c, nrhs);
     }
     const double t0 = now_s();
+    b->program_launches = b->program_unbatched = b->program_recorded = 0;
     // start of every instance (each on its own stream), then the graph of `chunk` steps
     std::vector<int> live(n), rs((size_t)n * nrhs);
     std::vector<double *> U(d_U, d_U + n);
@@ -2571,8 +2961,9 @@ int tofe_batch_solve_dev(tofe_batch *b, const double *const *d_F, double *const 
     double t_graph = 0, t_replay = 0;
     int replays = 0;
     if (any && maxit > 0) {
-      hipGraphExec_t exec = nullptr;           // single-graph form
+      hipGraphExec_t exec = nullptr;           // single-graph and merged forms
       std::vector<hipGraphExec_t> execs(n);    // per-instance form
+      BatchProgram prog;                       // merged form
       std::vector<int> captured;
       auto release = [&] {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -2615,13 +3006,27 @@ int tofe_batch_solve_dev(tofe_batch *b, const double *const *d_F, double *const 
               TOFE_HIP(hipStreamWaitEvent(b->origin, b->ev_join[i], 0));
             }
           } else {
-            if (live != captured) {
+            // the graph covers the instances with columns still to solve (merged: the program
+            // is rebuilt for them)
+            std::vector<int> alive(n);
+            for (int i = 0; i < n; ++i) alive[i] = live[i] > 0;
+            if (alive != captured) {
               if (exec) (void)hipGraphExecDestroy(exec);
               exec = nullptr;
               const double tg = now_s();
-              exec = capture_iterations(b, U, live, nrhs, chunk);
+              if (b->form == tofe_batch::MERGED) {
+                prog.build(b->ctx, U, alive, nrhs);
+                if (replays == 0) {  // the first program, over every instance still to solve
+                  b->program_launches = (int)prog.steps.size();
+                  b->program_unbatched = prog.single;
+                  b->program_recorded = prog.recorded;
+                }
+                exec = capture_program(b, prog, chunk);
+              } else {
+                exec = capture_iterations(b, U, alive, nrhs, chunk);
+              }
               t_graph += now_s() - tg;
-              captured = live;
+              captured = alive;
             }
             TOFE_HIP(hipGraphLaunch(exec, b->origin));
           }
@@ -2678,6 +3083,9 @@ int tofe_batch_solve_dev(tofe_batch *b, const double *const *d_F, double *const 
     b->stats.replay_seconds = t_replay;
     b->stats.replays = replays;
     b->stats.iterations = done_its;
+    b->stats.launches = b->form == tofe_batch::MERGED ? b->program_launches : 0;
+    b->stats.unbatched = b->form == tofe_batch::MERGED ? b->program_unbatched : 0;
+    b->stats.instance_launches = b->form == tofe_batch::MERGED ? b->program_recorded : 0;
   });
 }
 

@@ -108,6 +108,47 @@ struct StencilOp {
 constexpr int kRedBlocks = 1024;   // grid of the grid-stride reduction kernels
 
 
+// ---- launches over several instances (batches) ----------------------------
+// Parameters of one instance's part of a batched launch; a table of them (device memory) and
+// the first block of each entry (n + 1 offsets) drive one grid over every instance's blocks.
+template <typename T>
+struct FineLaunch {
+  FineOp<T> op;
+  const T *u = nullptr;
+  EpiArgs<T> ep;
+  int tiles_x = 0, tiles_y = 0, nblocks = 0, zlen = 0;
+};
+template <typename T>
+struct StencilLaunch {
+  StencilOp<T> op;
+  const T *u = nullptr;
+  EpiArgs<T> ep;
+  int64_t nn = 0;
+  int nblocks = 0;
+};
+template <typename T>
+struct TrigemvLaunch {
+  const T *A = nullptr;
+  int n = 0, ld = 0;
+  const T *x = nullptr;
+  T *y = nullptr;
+  int nblocks = 0;
+};
+// Batched launches (one grid, nblocks_total blocks, nvec rows of vectors): table `tab` and
+// block offsets `b0` (n + 1 entries) in device memory.
+template <typename T>
+void fine_wht_batch(hipStream_t s, Epi epi, bool cube, const FineLaunch<T> *tab, const int *b0,
+                    int n, int nblocks_total, int nvec);
+template <typename T>
+int fine_wht_grid(const FineOp<T> &op, int *tiles_x, int *tiles_y);
+template <typename T>
+void stencil_sym_batch(hipStream_t s, Epi epi, const StencilLaunch<T> *tab, const int *b0, int n,
+                       int nblocks_total, int nvec);
+// coarse solve halves (lower: tmp = X b, upper: x = X^T tmp); the LDS holds ld_max values
+template <typename T>
+void trigemv_batch(hipStream_t s, bool lower, const TrigemvLaunch<T> *tab, const int *b0, int n,
+                   int nblocks_total, int nvec, int ld_max);
+
 // ---- fine operator (k_fine.hip) -------------------------------------------
 template <typename T>
 int fine_op_blocks(const FineOp<T> &op);
@@ -234,6 +275,67 @@ void fcg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t
                    const double *zq_part, int nparts, double *scal, int nvec = 1);
 void fcg_update_x(hipStream_t s, double *x, const double *p, int64_t n, const double *scal,
                   int nvec = 1);
+// Batched forms of the V-cycle's transfers and of the flexible-CG updates: one entry per
+// instance, with the blocks per vector of the single launch (restriction: coarse nodes / 256,
+// prolongation: fine nodes / 256, fcg_update_r_blocks, fcg_update_xp_blocks, reduction: 1).
+template <typename T>
+struct RestrictLaunch {
+  const T *rf = nullptr;
+  int Nfx = 0, Nfy = 0, Nfz = 0;
+  T *bc = nullptr;
+  int Ncx = 0, Ncy = 0, Ncz = 0;
+  FirstStep<T> fs;
+  int nblocks = 0;
+};
+template <typename T>
+struct ProlongLaunch {
+  const T *xc = nullptr;
+  int Ncx = 0, Ncy = 0, Ncz = 0;
+  T *xf = nullptr;
+  int Nfx = 0, Nfy = 0, Nfz = 0;
+  int nblocks = 0;
+};
+struct FcgXrLaunch {
+  double *r = nullptr;
+  const double *q = nullptr;
+  int64_t n = 0;
+  const double *pq_part = nullptr, *pr_part = nullptr;
+  int nparts = 0;
+  double *scal = nullptr, *partial = nullptr;
+  FirstCheb fc;
+  int nblocks = 0;
+};
+struct FcgPLaunch {
+  double *p = nullptr;
+  const double *z = nullptr;
+  double *x = nullptr;
+  int64_t n = 0;
+  const double *zq_part = nullptr;
+  int nparts = 0;
+  double *scal = nullptr;
+  int nblocks = 0;
+};
+struct ReduceLaunch {
+  const double *partial = nullptr;
+  int nparts = 0;
+  double *out = nullptr;
+  int post = 0;
+  double *scal = nullptr;
+  int nblocks = 1;
+};
+int fcg_update_r_blocks();
+int fcg_update_xp_blocks(int64_t n);
+void reduce_final_batch(hipStream_t s, const ReduceLaunch *tab, const int *b0, int n, int nvec);
+void fcg_update_r_batch(hipStream_t s, const FcgXrLaunch *tab, const int *b0, int n,
+                        int nblocks_total, int nvec);
+void fcg_update_xp_batch(hipStream_t s, const FcgPLaunch *tab, const int *b0, int n,
+                         int nblocks_total, int nvec);
+template <typename T>
+void restrict_batch(hipStream_t s, const RestrictLaunch<T> *tab, const int *b0, int n,
+                    int nblocks_total, int nvec);
+template <typename T>
+void prolong_batch(hipStream_t s, const ProlongLaunch<T> *tab, const int *b0, int n,
+                   int nblocks_total, int nvec);
 // Fused Jacobi-CG steps of the eigenvalue estimate (no separate reduction launches):
 // lanczos_r: alpha = scal[rz_slot] / sum(pq_part[0..npq)), recorded; r -= alpha q; partials
 // of r.(D^-1 r).  lanczos_p: beta = sum(rz_part) / scal[rz_old], recorded; scal[rz_new] =

@@ -203,11 +203,12 @@ def vibration(nel=None, nev=12, eig_tol=1e-8, mass_form=_lib.MASS_CONSISTENT, re
     return out
 
 
-def batch(n=32, nrhs=4, rtol=1e-8, chunk=2, autotune=True):
+def batch(n=32, nrhs=4, rtol=1e-8, chunk=2, autotune=True,
+          forms=("merged", "single", "instance", "streams")):
     """Config 4: 32 cantilevers of uneven size (nx in {16..96}), 4 load cases each, FP32
     V-cycle inside FP64 flexible CG to 1e-8.  Times the eager solves one instance after the
-    other against one graph-captured batch solve of all instances, and checks that both give
-    the same bits."""
+    other against graph-captured batch solves of all instances in each batch form, and checks
+    that every form gives the eager solves' bits."""
     import torch
 
     pbs = configs.config4(nrhs=nrhs)[:n]
@@ -249,12 +250,13 @@ def batch(n=32, nrhs=4, rtol=1e-8, chunk=2, autotune=True):
         its_e.append(c.solve_dev(f.data_ptr(), u.data_ptr(), nrhs=nrhs, rtol=rtol, maxit=500))
     _sync()
     t_eager = time.perf_counter() - t0
-    # graph batch: one graph per instance on the instances' streams, and one graph with every
-    # instance as a branch (TOFE_BATCH_GRAPH)
+    # graph batch (TOFE_BATCH_GRAPH): one launch per step over every instance (merged), one
+    # graph with every instance's own launches as a branch (single), one graph per instance on
+    # the instances' streams (instance), no graph (streams)
     import os
 
-    forms = {}
-    for form in ("single", "instance", "streams"):
+    timed = {}
+    for form in forms:
         os.environ["TOFE_BATCH_GRAPH"] = form
         b = _lib.Batch(ctxs)
         Ub = [torch.zeros_like(f) for f in F]
@@ -270,7 +272,7 @@ def batch(n=32, nrhs=4, rtol=1e-8, chunk=2, autotune=True):
         t_batch = time.perf_counter() - t0
         st = b.stats()
         same = all(bool(torch.equal(a, e)) for a, e in zip(Ub, Ue))
-        forms[form] = {"batch_s": t_batch, "stats": st, "identical_to_eager": same,
+        timed[form] = {"batch_s": t_batch, "stats": st, "identical_to_eager": same,
                        "all_converged": bool((rs == _lib.CONVERGED_RTOL).all()),
                        "its_max": int(its.max()), "its_min": int(its.min())}
         _log(f"batch ({form} graph): {t_batch * 1e3:.1f} ms (capture "
@@ -278,11 +280,11 @@ def batch(n=32, nrhs=4, rtol=1e-8, chunk=2, autotune=True):
              f"its {its.min()}..{its.max()}")
         b.close()
     os.environ.pop("TOFE_BATCH_GRAPH", None)
-    best = min(forms, key=lambda k: forms[k]["batch_s"])
+    best = min(timed, key=lambda k: timed[k]["batch_s"])
     out.update({
-        "eager_s": t_eager, "batch_s": forms[best]["batch_s"], "batch_form": best,
-        "forms": forms, "batch_over_eager": forms[best]["batch_s"] / t_eager,
-        "identical_to_eager": all(f["identical_to_eager"] for f in forms.values()),
+        "eager_s": t_eager, "batch_s": timed[best]["batch_s"], "batch_form": best,
+        "forms": timed, "batch_over_eager": timed[best]["batch_s"] / t_eager,
+        "identical_to_eager": all(f["identical_to_eager"] for f in timed.values()),
         "eager_its_max": int(max(k for k, _ in its_e)),
     })
     _log(f"batch: eager {t_eager * 1e3:.1f} ms")
