@@ -43,6 +43,25 @@ def test_block_operator_and_vcycle_match_single(tofe, nel, L, precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("k", [2, 4, 6])
+def test_multivector_stencils_match_single(tofe, k, precision, monkeypatch):
+    """The coarse levels' smoothing steps and residuals with V = 2, 4 or 3 vectors per thread
+    (forced on whatever the grid size) give each column the bits of its one-vector launch, in
+    the V-cycle and in the block solve."""
+    print(f"case multivector stencil k={k} precision={precision}", flush=True)
+    monkeypatch.setenv("TOFE_STENCIL_MV", "2")
+    pb, ctx = _ctx(tofe, (16, 8, 8), 3, precision)
+    X = _rhs(pb, k)
+    ZX = ctx.pc_apply(X)
+    U, its, reason = ctx.solve(X, rtol=1e-10)
+    assert reason == tofe.CONVERGED_RTOL
+    for j in range(k):
+        np.testing.assert_array_equal(ZX[j], ctx.pc_apply(X[j]))
+        np.testing.assert_array_equal(U[j], ctx.solve(X[j], rtol=1e-10)[0])
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_block_solve_matches_single_and_oracle(oracle, tofe, precision):
     nel, L = (16, 8, 8), 3
     print(f"case block solve nel={nel} precision={precision}", flush=True)

@@ -6,6 +6,8 @@
 // coarse operator couples each coarse node only with its 26 neighbours, so the
 // stencil is exact, including the Dirichlet identity rows carried up from the fine
 // level and the degenerate last cell of grids with an odd element count.
+#include <cstdlib>
+
 #include "k_common.hpp"
 
 namespace tofe {
@@ -300,10 +302,10 @@ __device__ __forceinline__ void stencil_sym_block(const StencilOp<T> &op, const 
           T t;
           if (o >= 13) {
             const T *B = S + su_index((o - 13) * 9, n);
-            t = B[(a * 3 + 0) * 64] * u0 + B[(a * 3 + 1) * 64] * u1 + B[(a * 3 + 2) * 64] * u2;
+            t = row3(B[(a * 3 + 0) * 64], B[(a * 3 + 1) * 64], B[(a * 3 + 2) * 64], u0, u1, u2);
           } else {
             const T *B = S + su_index((13 - o) * 9, m);  // block (m, 26 - o), transposed
-            t = B[(0 * 3 + a) * 64] * u0 + B[(1 * 3 + a) * 64] * u1 + B[(2 * 3 + a) * 64] * u2;
+            t = row3(B[(0 * 3 + a) * 64], B[(1 * 3 + a) * 64], B[(2 * 3 + a) * 64], u0, u1, u2);
           }
           acc[c] = in ? acc[c] + t : acc[c];
         }
@@ -335,8 +337,8 @@ __device__ __forceinline__ void stencil_sym_block(const StencilOp<T> &op, const 
 #pragma unroll
           for (int c = 0; c < NA; ++c) {
             const int a = SPLIT ? a0 : c;
-            const T t = B[(a * sr + 0 * sc) * 64] * u0 + B[(a * sr + 1 * sc) * 64] * u1 +
-                        B[(a * sr + 2 * sc) * 64] * u2;
+            const T t = row3(B[(a * sr + 0 * sc) * 64], B[(a * sr + 1 * sc) * 64],
+                             B[(a * sr + 2 * sc) * 64], u0, u1, u2);
             acc[c] = __builtin_fma(w, t, acc[c]);
           }
         }
@@ -364,6 +366,113 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym(StencilOp<T> op, const T 
                                                        EpiArgs<T> ep, int64_t nn)
 {
   stencil_sym_block<T, EPI, SPLIT>(op, u0, ep, nn, blockIdx.x, gridDim.x);
+}
+
+// The same operator for V vectors per thread (blockIdx.y numbers groups of V vectors): each
+// stored stencil value is read once for the group instead of once per vector.  Per vector,
+// the neighbour terms (row3), their order and the epilogue are those of stencil_sym_block, so
+// every vector's result equals its one-vector launch bit for bit.
+template <typename T, int EPI, bool SPLIT, int V>
+__device__ __forceinline__ void stencil_sym_block_mv(const StencilOp<T> &op, const T *u0,
+                                                     const EpiArgs<T> &ep0, int64_t nn, int blk,
+                                                     int nb)
+{
+  __shared__ double sred[NT / 64];
+  const int64_t ndof = 3 * nn;
+  const int v0 = blockIdx.y * V;
+  int64_t n;
+  int a0;
+  split_index<SPLIT>(nn, blk, nb, n, a0);
+  const T *__restrict__ S = op.S;
+  constexpr int NA = SPLIT ? 1 : 3;
+  T acc[V][NA];
+#pragma unroll
+  for (int j = 0; j < V; ++j)
+#pragma unroll
+    for (int c = 0; c < NA; ++c) acc[j][c] = 0;
+  double dsum[V], dsum2[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) dsum[j] = dsum2[j] = 0;
+  if (n < nn) {
+    const int Nx = op.Nx, Ny = op.Ny, Nz = op.Nz;
+    int i, j, k;
+    node_ijk(n, Nx, Ny, i, j, k);
+#pragma unroll 3
+    for (int o = 0; o < 27; ++o) {
+      const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+      const int mi = i + dx, mj = j + dy, mk = k + dz;
+      // unconditional loads (a missing neighbour reads node n), selected contribution
+      const bool in = ((unsigned)mi < (unsigned)Nx) & ((unsigned)mj < (unsigned)Ny) &
+                      ((unsigned)mk < (unsigned)Nz);
+      const int64_t m = in ? n + dx + (int64_t)Nx * (dy + (int64_t)Ny * dz) : n;
+      const bool up = o >= 13;
+      const T *B = S + (up ? su_index((o - 13) * 9, n) : su_index((13 - o) * 9, m));
+      const int sr = up ? 3 : 1, sc = up ? 1 : 3;
+      T bv[NA][3];
+#pragma unroll
+      for (int c = 0; c < NA; ++c) {
+        const int a = SPLIT ? a0 : c;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) bv[c][q] = B[(a * sr + q * sc) * 64];
+      }
+#pragma unroll
+      for (int jv = 0; jv < V; ++jv) {
+        const T *__restrict__ u = u0 + (int64_t)(v0 + jv) * ndof;
+        const T x0 = u[3 * m], x1 = u[3 * m + 1], x2 = u[3 * m + 2];
+#pragma unroll
+        for (int c = 0; c < NA; ++c) {
+          const T t = row3(bv[c][0], bv[c][1], bv[c][2], x0, x1, x2);
+          acc[jv][c] = in ? acc[jv][c] + t : acc[jv][c];
+        }
+      }
+    }
+#pragma unroll
+    for (int jv = 0; jv < V; ++jv) {
+      EpiArgs<T> ep = ep0;
+      epi_offset(ep, v0 + jv, ndof, nb);
+      const T *__restrict__ u = u0 + (int64_t)(v0 + jv) * ndof;
+#pragma unroll
+      for (int c = 0; c < NA; ++c) {
+        const int64_t dof = 3 * n + (SPLIT ? a0 : c);
+        apply_epilogue<T, EPI>(ep, dof, acc[jv][c], u[dof], dsum[jv], dsum2[jv]);
+      }
+    }
+  }
+  if constexpr (EPI == EPI_APPLY_DOT || EPI == EPI_APPLY_DOT2 || EPI == EPI_CHEB_OUT) {
+#pragma unroll
+    for (int jv = 0; jv < V; ++jv) {
+      EpiArgs<T> ep = ep0;
+      epi_offset(ep, v0 + jv, ndof, nb);
+      const double t = block_sum<NT>(dsum[jv], sred);
+      if (threadIdx.x == 0) ep.partial[blk] = t;
+      __syncthreads();
+      if constexpr (EPI == EPI_APPLY_DOT2) {
+        const double t2 = block_sum<NT>(dsum2[jv], sred);
+        if (threadIdx.x == 0) ep.partial2[blk] = t2;
+        __syncthreads();
+      }
+    }
+  }
+}
+
+template <typename T, int EPI, bool SPLIT, int V>
+__global__ __launch_bounds__(NT) void k_stencil_op_sym_mv(StencilOp<T> op, const T *u0,
+                                                          EpiArgs<T> ep, int64_t nn)
+{
+  stencil_sym_block_mv<T, EPI, SPLIT, V>(op, u0, ep, nn, blockIdx.x, gridDim.x);
+}
+
+template <typename T, int EPI, int V>
+__global__ __launch_bounds__(NT) void k_stencil_op_sym_mv_batch(const StencilLaunch<T> *__restrict__ tab,
+                                                                const int *__restrict__ b0, int n)
+{
+  const int e = batch_entry(b0, n, blockIdx.x);
+  const StencilLaunch<T> L = tab[e];
+  const int blk = blockIdx.x - b0[e];
+  if (L.nn < L.op.split_nodes)
+    stencil_sym_block_mv<T, EPI, true, V>(L.op, L.u, L.ep, L.nn, blk, L.nblocks);
+  else
+    stencil_sym_block_mv<T, EPI, false, V>(L.op, L.u, L.ep, L.nn, blk, L.nblocks);
 }
 
 // The operator of several instances' levels in one grid: entry e owns blocks
@@ -907,9 +1016,51 @@ void stencil_op(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, cons
   TOFE_STENCIL_SWITCH(k_stencil_op);
 }
 
+// Vectors per thread of the multi-vector stencil kernels for a launch over nvec vectors of
+// nblocks blocks each (1: the one-vector kernel).  Only the V-cycle's epilogues (Chebyshev
+// step, residual) have them, and only launches that still fill the GPU with V vectors per
+// thread (kMvMinBlocks blocks) take them: on a small grid the vectors' blocks running side by
+// side finish sooner than one block doing V vectors one after the other.  TOFE_STENCIL_MV=0
+// keeps every launch on the one-vector kernel, TOFE_STENCIL_MV=2 puts every eligible launch on
+// the multi-vector kernels whatever its grid.
+constexpr int64_t kMvMinBlocks = 1024;
+int stencil_mv_width(int nvec, Epi epi, int64_t nblocks)
+{
+  const char *e = std::getenv("TOFE_STENCIL_MV");
+  const int mode = e ? std::atoi(e) : 1;
+  if (mode == 0 || (epi != EPI_CHEB && epi != EPI_RESID)) return 1;
+  for (int V : {4, 3, 2})
+    if (nvec % V == 0 && (mode == 2 || nblocks * (nvec / V) >= kMvMinBlocks)) return V;
+  return 1;
+}
+
+template <typename T, int EPI, int V>
+void launch_sym_mv(hipStream_t s, const StencilOp<T> &op, const T *u, const EpiArgs<T> &ep)
+{
+  const int64_t nn = (int64_t)op.Nx * op.Ny * op.Nz;
+  const dim3 grid((unsigned)stencil_op_blocks<T>(op), ep.nvec / V), block(NT);
+  if (nn < op.split_nodes) k_stencil_op_sym_mv<T, EPI, true, V><<<grid, block, 0, s>>>(op, u, ep, nn);
+  else k_stencil_op_sym_mv<T, EPI, false, V><<<grid, block, 0, s>>>(op, u, ep, nn);
+}
+
+template <typename T, int EPI>
+void launch_sym_mv(hipStream_t s, const StencilOp<T> &op, const T *u, const EpiArgs<T> &ep, int V)
+{
+  if (V == 4) launch_sym_mv<T, EPI, 4>(s, op, u, ep);
+  else if (V == 3) launch_sym_mv<T, EPI, 3>(s, op, u, ep);
+  else launch_sym_mv<T, EPI, 2>(s, op, u, ep);
+}
+
 template <typename T>
 void stencil_op_sym(hipStream_t s, const StencilOp<T> &op, const T *u, Epi epi, const EpiArgs<T> &ep)
 {
+  const int V = stencil_mv_width(ep.nvec, epi, stencil_op_blocks<T>(op));
+  if (V > 1) {
+    if (epi == EPI_CHEB) launch_sym_mv<T, EPI_CHEB>(s, op, u, ep, V);
+    else launch_sym_mv<T, EPI_RESID>(s, op, u, ep, V);
+    TOFE_LAUNCH_CHECK();
+    return;
+  }
   TOFE_STENCIL_SWITCH(k_stencil_op_sym);
 }
 
@@ -920,10 +1071,27 @@ void launch_sym_batch(hipStream_t s, const StencilLaunch<T> *tab, const int *b0,
   k_stencil_op_sym_batch<T, EPI><<<dim3(nblocks_total, nvec), dim3(NT), 0, s>>>(tab, b0, n);
 }
 
+template <typename T, int EPI>
+void launch_sym_mv_batch(hipStream_t s, const StencilLaunch<T> *tab, const int *b0, int n,
+                         int nblocks_total, int nvec, int V)
+{
+  const dim3 grid(nblocks_total, nvec / V), block(NT);
+  if (V == 4) k_stencil_op_sym_mv_batch<T, EPI, 4><<<grid, block, 0, s>>>(tab, b0, n);
+  else if (V == 3) k_stencil_op_sym_mv_batch<T, EPI, 3><<<grid, block, 0, s>>>(tab, b0, n);
+  else k_stencil_op_sym_mv_batch<T, EPI, 2><<<grid, block, 0, s>>>(tab, b0, n);
+}
+
 template <typename T>
 void stencil_sym_batch(hipStream_t s, Epi epi, const StencilLaunch<T> *tab, const int *b0, int n,
                        int nblocks_total, int nvec)
 {
+  const int V = stencil_mv_width(nvec, epi, nblocks_total);
+  if (V > 1) {
+    if (epi == EPI_CHEB) launch_sym_mv_batch<T, EPI_CHEB>(s, tab, b0, n, nblocks_total, nvec, V);
+    else launch_sym_mv_batch<T, EPI_RESID>(s, tab, b0, n, nblocks_total, nvec, V);
+    TOFE_LAUNCH_CHECK();
+    return;
+  }
   switch (epi) {
     case EPI_APPLY: launch_sym_batch<T, EPI_APPLY>(s, tab, b0, n, nblocks_total, nvec); break;
     case EPI_APPLY_DOT: launch_sym_batch<T, EPI_APPLY_DOT>(s, tab, b0, n, nblocks_total, nvec); break;

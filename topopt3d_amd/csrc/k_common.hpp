@@ -94,12 +94,10 @@ __device__ inline int batch_entry(const int *__restrict__ b0, int n, int b)
 // back to back (ndof values each); the per-block partial sums of vector v start at
 // v * nblocks (the launch's blocks per vector).  Vector 0 keeps the pointers as given.
 template <typename T>
-__device__ inline void epi_select_vector(EpiArgs<T> &ep, const T *&u, int64_t ndof, int nblocks)
+__device__ inline void epi_offset(EpiArgs<T> &ep, int64_t v, int64_t ndof, int nblocks)
 {
-  const int64_t v = blockIdx.y;
   if (v == 0) return;
   const int64_t vo = v * ndof, po = v * (int64_t)nblocks;
-  u += vo;
   if (ep.y) ep.y += vo;
   if (ep.b) ep.b += vo;
   if (ep.d) ep.d += vo;
@@ -108,6 +106,27 @@ __device__ inline void epi_select_vector(EpiArgs<T> &ep, const T *&u, int64_t nd
   if (ep.q64) ep.q64 += vo;
   if (ep.partial) ep.partial += po;
   if (ep.partial2) ep.partial2 += po;
+}
+
+// Vector blockIdx.y of a launch over several vectors (epi_offset and the operand u).
+template <typename T>
+__device__ inline void epi_select_vector(EpiArgs<T> &ep, const T *&u, int64_t ndof, int nblocks)
+{
+  const int64_t v = blockIdx.y;
+  epi_offset(ep, v, ndof, nblocks);
+  u += v * ndof;
+}
+
+// b0 u0 + b1 u1 + b2 u2 rounded as fma(b2, u2, fma(b1, u1, b0 u0)): the one operation order of
+// a stencil block row times a node's three values, so that every kernel forming it (one vector
+// or several per thread) gets the same bits.
+template <typename T>
+__device__ __forceinline__ T row3(T b0, T b1, T b2, T u0, T u1, T u2)
+{
+  if constexpr (sizeof(T) == 4)
+    return __builtin_fmaf(b2, u2, __builtin_fmaf(b1, u1, b0 * u0));
+  else
+    return __builtin_fma(b2, u2, __builtin_fma(b1, u1, b0 * u0));
 }
 
 template <typename T, int EPI>

@@ -385,7 +385,7 @@ __device__ __forceinline__ void fcg_xr_block(double *__restrict__ r, const doubl
                                              int64_t n, const double *__restrict__ pq_part,
                                              const double *__restrict__ pr_part, int nparts,
                                              double *scal, double *partial, FirstCheb fc, int blk,
-                                             int nb)
+                                             int nb, bool pre = false)
 {
   __shared__ double sred[NT / 64];
   {  // vector blockIdx.y: its vectors, partials and scalars; dinv is shared
@@ -402,14 +402,19 @@ __device__ __forceinline__ void fcg_xr_block(double *__restrict__ r, const doubl
       fc.x += vo;
     }
   }
-  const double pr = all_sum(pr_part, nparts, sred);
-  const double pq = all_sum(pq_part, nparts, sred);
-  // a frozen right-hand side (converged in a block solve) takes no further steps
-  const double alpha = scal[SL_FROZEN] != 0.0 ? 0.0 : pr / pq;
-  if (blk == 0 && threadIdx.x == 0) {
-    scal[SL_PR] = pr;
-    scal[SL_PQ] = pq;
-    scal[SL_ALPHA] = alpha;
+  double alpha;
+  if (pre) {
+    alpha = scal[SL_ALPHA];
+  } else {
+    const double pr = all_sum(pr_part, nparts, sred);
+    const double pq = all_sum(pq_part, nparts, sred);
+    // a frozen right-hand side (converged in a block solve) takes no further steps
+    alpha = scal[SL_FROZEN] != 0.0 ? 0.0 : pr / pq;
+    if (blk == 0 && threadIdx.x == 0) {
+      scal[SL_PR] = pr;
+      scal[SL_PQ] = pq;
+      scal[SL_ALPHA] = alpha;
+    }
   }
   double s = 0;
   for (int64_t i = (int64_t)blk * NT + threadIdx.x; i < n; i += (int64_t)nb * NT) {
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(NT) void k_fcg_xr_batch(const FcgXrLaunch *__restri
   const int e = batch_entry(b0, n, blockIdx.x);
   const FcgXrLaunch L = tab[e];
   fcg_xr_block(L.r, L.q, L.n, L.pq_part, L.pr_part, L.nparts, L.scal, L.partial, L.fc,
-               blockIdx.x - b0[e], L.nblocks);
+               blockIdx.x - b0[e], L.nblocks, L.pre != 0);
 }
 
 // Second half: beta = -z.Ap / p.Ap from the z.q partials (block 0 stores z.Ap and beta);
@@ -451,7 +456,7 @@ __global__ __launch_bounds__(NT) void k_fcg_xr_batch(const FcgXrLaunch *__restri
 __device__ __forceinline__ void fcg_p_block(double *__restrict__ p, const double *__restrict__ z,
                                             double *__restrict__ x, int64_t n,
                                             const double *__restrict__ zq_part, int nparts,
-                                            double *scal, int blk, int nb)
+                                            double *scal, int blk, int nb, bool pre = false)
 {
   __shared__ double sred[NT / 64];
   {  // vector blockIdx.y
@@ -462,13 +467,18 @@ __device__ __forceinline__ void fcg_p_block(double *__restrict__ p, const double
     zq_part += v * nparts;
     scal += v * SL_COUNT;
   }
-  const double zq = all_sum(zq_part, nparts, sred);
-  const double beta = scal[SL_FROZEN] != 0.0 ? 0.0 : -zq / scal[SL_PQ];
-  const double alpha = scal[SL_ALPHA];
-  if (blk == 0 && threadIdx.x == 0) {
-    scal[SL_ZQ] = zq;
-    scal[SL_BETA] = beta;
+  double beta;
+  if (pre) {
+    beta = scal[SL_BETA];
+  } else {
+    const double zq = all_sum(zq_part, nparts, sred);
+    beta = scal[SL_FROZEN] != 0.0 ? 0.0 : -zq / scal[SL_PQ];
+    if (blk == 0 && threadIdx.x == 0) {
+      scal[SL_ZQ] = zq;
+      scal[SL_BETA] = beta;
+    }
   }
+  const double alpha = scal[SL_ALPHA];
   for (int64_t i = (int64_t)blk * NT + threadIdx.x; i < n; i += (int64_t)nb * NT) {
     const double pi = p[i];
     x[i] += alpha * pi;  // the iterate's update of the step that formed alpha
@@ -489,7 +499,8 @@ __global__ __launch_bounds__(NT) void k_fcg_p_batch(const FcgPLaunch *__restrict
 {
   const int e = batch_entry(b0, n, blockIdx.x);
   const FcgPLaunch L = tab[e];
-  fcg_p_block(L.p, L.z, L.x, L.n, L.zq_part, L.nparts, L.scal, blockIdx.x - b0[e], L.nblocks);
+  fcg_p_block(L.p, L.z, L.x, L.n, L.zq_part, L.nparts, L.scal, blockIdx.x - b0[e], L.nblocks,
+              L.pre != 0);
 }
 
 // x += alpha p with alpha = scal[SL_ALPHA]: the iterate's update when no k_fcg_p / k_cg_xp follows.

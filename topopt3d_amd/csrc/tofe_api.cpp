@@ -1466,10 +1466,16 @@ void iteration_on_device(tofe_ctx *c, double *U, int nvec)
     ep.partial2 = c->partial2.p;
     level_apply<double>(c, 0, p, EPI_APPLY_DOT2, ep);
     if (c->rec) {
+      // p.r and p.Ap (then alpha) by two small reductions instead of in every block of the
+      // update: in a batched launch the update's blocks, each re-summing the partials, would
+      // cost more than the update itself
+      reduce_into(c, c->partial2.p, nfb, c->scal.p + SL_PR, POST_NONE, c->scal.p, nvec);
+      reduce_into(c, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_FCG, c->scal.p, nvec);
       FcgXrLaunch d;
       d.r = r; d.q = q; d.n = n; d.pq_part = c->partial.p; d.pr_part = c->partial2.p;
       d.nparts = nfb; d.scal = c->scal.p; d.partial = c->partial3.p; d.fc = first;
       d.nblocks = fcg_update_r_blocks();
+      d.pre = 1;
       rec_push(c, batch_key(BK_FCG_XR, false), d, d.nblocks);
     } else {
       fcg_update_r(s, r, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p, first,
@@ -1478,10 +1484,12 @@ void iteration_on_device(tofe_ctx *c, double *U, int nvec)
     reduce_into(c, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_CONV_FCG, c->scal.p, nvec);
     const int nparts = precondition(c, r, z, q, first.b32 != nullptr, nvec);
     if (c->rec) {
+      reduce_into(c, c->partial.p, nparts, c->scal.p + SL_ZQ, POST_BETA_FCG, c->scal.p, nvec);
       FcgPLaunch d;
       d.p = p; d.z = z; d.x = U; d.n = n; d.zq_part = c->partial.p; d.nparts = nparts;
       d.scal = c->scal.p;
       d.nblocks = fcg_update_xp_blocks(n);
+      d.pre = 1;
       rec_push(c, batch_key(BK_FCG_P, false), d, d.nblocks);
     } else {
       fcg_update_xp(s, p, z, U, n, c->partial.p, nparts, c->scal.p, nvec);

@@ -295,6 +295,8 @@ struct ProlongLaunch {
   int Nfx = 0, Nfy = 0, Nfz = 0;
   int nblocks = 0;
 };
+// pre != 0: alpha (FcgXr) or beta (FcgP) already stands in the scalars, formed by reduction
+// launches (POST_ALPHA_FCG, POST_BETA_FCG) with the sum every block would otherwise form
 struct FcgXrLaunch {
   double *r = nullptr;
   const double *q = nullptr;
@@ -304,6 +306,7 @@ struct FcgXrLaunch {
   double *scal = nullptr, *partial = nullptr;
   FirstCheb fc;
   int nblocks = 0;
+  int pre = 0;
 };
 struct FcgPLaunch {
   double *p = nullptr;
@@ -314,6 +317,7 @@ struct FcgPLaunch {
   int nparts = 0;
   double *scal = nullptr;
   int nblocks = 0;
+  int pre = 0;
 };
 struct ReduceLaunch {
   const double *partial = nullptr;
