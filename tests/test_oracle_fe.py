@@ -131,6 +131,71 @@ def test_galerkin_and_mg_solve_small(oracle):
     assert x @ pb.F == pytest.approx(x @ A0 @ x, rel=1e-12)
 
 
+def test_coarsest_operator_as_element_sum(oracle):
+    """The identity the library's direct coarsest operator (k_direct.hip) rests on: when every
+    element count is a multiple of r = 2^(L-1), the Galerkin chain's coarsest operator equals
+    sum_e E_e W_e^T KE_masked,e W_e + sum over fixed DOFs of w w^T + sum over free spring DOFs of
+    s w w^T, with W_e the trilinear weights of the element's nodes on the coarsest cell's nodes."""
+    nel, L = (8, 4, 4), 3
+    r = 2 ** (L - 1)
+    nx, ny, nz = nel
+    pb = configs.cantilever(nel, seed=11, lo=0.05, mg_levels=L)
+    KE = oracle.ke()
+    E = oracle.simp(oracle.filter_csr(*nel, 1.5).matvec(pb.x))[0]
+    fixed = pb.fixed_dof.astype(bool)
+    Nf = (nx + 1, ny + 1, nz + 1)
+    spk = np.zeros(3 * Nf[0] * Nf[1] * Nf[2])
+    spk[3 * np.arange(Nf[0] * Nf[1] * Nf[2])[::7] + 2] = 0.3  # springs, some on fixed DOFs
+    free_spk = spk * ~fixed
+    import scipy.sparse as sp
+    K = (oracle.assemble_K(nx, ny, nz, KE, E, pb.fixed_dof).to_scipy() + sp.diags(free_spk)).tocsr()
+    mg = oracle.MG(oracle.CSR.from_scipy(K), Nf, L)
+    chain = mg.A(L - 1).to_scipy().toarray()
+    Nc = (nx // r + 1, ny // r + 1, nz // r + 1)
+    nc = 3 * Nc[0] * Nc[1] * Nc[2]
+
+    def weights(fx, fy, fz):  # (coarse node, weight) of a fine node
+        out = []
+        for Iz in range(Nc[2]):
+            for Iy in range(Nc[1]):
+                for Ix in range(Nc[0]):
+                    wt = (max(0.0, 1 - abs(fx - r * Ix) / r) * max(0.0, 1 - abs(fy - r * Iy) / r)
+                          * max(0.0, 1 - abs(fz - r * Iz) / r))
+                    if wt:
+                        out.append((Ix + Nc[0] * (Iy + Nc[1] * Iz), wt))
+        return out
+
+    LN = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    A = np.zeros((nc, nc))
+    for ez in range(nz):
+        for ey in range(ny):
+            for ex in range(nx):
+                e = ex + nx * (ey + ny * ez)
+                W = np.zeros((24, nc))
+                dofs = []
+                for k, (a, b, c) in enumerate(LN):
+                    n = (ex + a) + Nf[0] * ((ey + b) + Nf[1] * (ez + c))
+                    dofs += [3 * n, 3 * n + 1, 3 * n + 2]
+                    for J, wt in weights(ex + a, ey + b, ez + c):
+                        for d in range(3):
+                            W[3 * k + d, 3 * J + d] = wt
+                Km = KE.copy()
+                m = fixed[dofs]
+                Km[m, :] = 0
+                Km[:, m] = 0
+                A += E[e] * W.T @ Km @ W
+    for n in range(Nf[0] * Nf[1] * Nf[2]):
+        f = (n % Nf[0], (n // Nf[0]) % Nf[1], n // (Nf[0] * Nf[1]))
+        for d in range(3):
+            s = 1.0 if fixed[3 * n + d] else free_spk[3 * n + d]
+            if s:
+                w = np.zeros(nc)
+                for J, wt in weights(*f):
+                    w[3 * J + d] = wt
+                A += s * np.outer(w, w)
+    assert np.abs(A - chain).max() <= 1e-14 * np.abs(chain).max()
+
+
 def test_pcg_reason_codes(oracle):
     nel = (8, 4, 4)
     pb = configs.cantilever(nel)

@@ -72,6 +72,7 @@ class TofeStats(C.Structure):
         ("levels", C.c_int), ("coarse_size", C.c_int64), ("its", C.c_int), ("reason", C.c_int),
         ("rnorm", C.c_double), ("bnorm", C.c_double),
         ("setup_seconds", C.c_double), ("solve_seconds", C.c_double), ("fine_form", C.c_int),
+        ("coarse_form", C.c_int),
     ]
 
 

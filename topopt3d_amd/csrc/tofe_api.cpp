@@ -176,7 +176,7 @@ struct tofe_ctx {
   hipStream_t stream4 = nullptr;
   bool shared_34 = false;
   hipEvent_t ev_dense = nullptr, ev_inv = nullptr, ev_ops = nullptr, ev_lz = nullptr;
-  hipEvent_t ev_scal = nullptr, ev_lz0 = nullptr;
+  hipEvent_t ev_scal = nullptr, ev_lz0 = nullptr, ev_diag = nullptr;
   double *hscal = nullptr;  // pinned host copy of the device scalars
   double KE[576];
   std::vector<int> tpl_off;
@@ -224,6 +224,13 @@ struct tofe_ctx {
   int n_masked = 0;
   DBuf<double> gal_t1, gal_t2;  // axis-by-axis Galerkin scratch
   DBuf<double> Ainv;
+  // coarsest operator formed from E directly (k_direct.hip) when the mesh allows it
+  bool direct = false;
+  int direct_nmasked = 0;
+  DBuf<double> direct_M, direct_part, direct_cells, direct_corr, direct_table, direct_cconst;
+  DBuf<int> direct_range, direct_tidx, direct_slot;
+  DBuf<int64_t> direct_elem;
+  std::vector<double> direct_cfix;  // host: the cells' fixed-DOF part of cconst
   // X = L^-1 of the coarse operator as padded row-major triangles X and X^T, in the V-cycle's
   // precision, and an n-vector for the solve's intermediate X b
   DBuf<double> X64r, XT64r, ctmp64;
@@ -278,7 +285,7 @@ struct tofe_ctx {
     if (stream2) (void)hipStreamSynchronize(stream2);
     if (stream3) (void)hipStreamSynchronize(stream3);
     if (stream4) (void)hipStreamSynchronize(stream4);
-    for (hipEvent_t e : {ev_dense, ev_inv, ev_ops, ev_lz, ev_scal, ev_lz0})
+    for (hipEvent_t e : {ev_dense, ev_inv, ev_ops, ev_lz, ev_scal, ev_lz0, ev_diag})
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : kt_ev)
       if (e) (void)hipEventDestroy(e);
@@ -1005,13 +1012,18 @@ void lanczos_coefficients(tofe_ctx *c, int l, hipStream_t s, T *r, T *p, T *q, d
 // stencil into its upper-half layout there.  ev_lz / ev_lz0 mark their ends.
 // The fine level's estimate on the fourth stream, once the fine diagonal (main stream) is
 // formed: it needs nothing of the Galerkin hierarchy.  ev_lz0 marks its end.
-void lanczos_fine_enqueue(tofe_ctx *c)
+// after: an event of the main stream recorded once E and the fine diagonal are formed (null:
+// record one now)
+void lanczos_fine_enqueue(tofe_ctx *c, hipEvent_t after = nullptr)
 {
   const int L = c->L;
   const hipStream_t s4 = c->stream4;
   c->eig_ab.alloc((size_t)2 * L * kMaxEig);
-  TOFE_HIP(hipEventRecord(c->ev_ops, c->stream));
-  TOFE_HIP(hipStreamWaitEvent(s4, c->ev_ops, 0));
+  if (!after) {
+    TOFE_HIP(hipEventRecord(c->ev_ops, c->stream));
+    after = c->ev_ops;
+  }
+  TOFE_HIP(hipStreamWaitEvent(s4, after, 0));
   if (c->precision == TOFE_PREC_MIXED) {
     if (c->lv32.empty()) {
       for (int l = 0; l < L; ++l) {
@@ -1132,6 +1144,43 @@ void chebyshev_coefficients(tofe_ctx *c)
 // then X and X^T packed as padded row-major triangles for the per-V-cycle solve
 // X^T (X b).  The coarse operator couples DOFs at most bw apart (27-point stencil,
 // x-fastest nodes) and the factor's fill stays inside that band.
+// Tables of the direct coarsest operator (k_direct.hip) when TOFE_DIRECT_COARSE=1 and the mesh
+// allows it: M_p, the Dirichlet elements' table rows, the cells' constant part.  Off by default:
+// the set-up is bound by the bandwidth of the Galerkin chain and the Lanczos estimates, which the
+// coarse factorisation overlaps either way (192x96x96 cantilever, set-up per call: mixed 2.53 ms
+// direct / 2.54 ms chain, FP64 3.48 / 3.36 ms).
+void direct_setup(tofe_ctx *c, const uint8_t *fixmask)
+{
+  const tofe_desc &d = c->desc;
+  const char *env = std::getenv("TOFE_DIRECT_COARSE");
+  c->direct = direct_coarse_ok(d.nx, d.ny, d.nz, c->L) && env && env[0] == '1';
+  if (!c->direct) return;
+  const int r = 1 << (c->L - 1);
+  const int ncell = (d.nx / r) * (d.ny / r) * (d.nz / r);
+  std::vector<double> M((size_t)r * r * r * 576);
+  direct_coarse_table(c->KE, r, M.data());
+  c->direct_M.alloc(M.size());
+  TOFE_HIP(hipMemcpy(c->direct_M.p, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+  DirectStatic st;
+  direct_coarse_static(c->KE, d.nx, d.ny, d.nz, r, fixmask, st);
+  c->direct_nmasked = (int)st.cells.size();
+  auto up = [](auto &buf, const auto &v) {
+    buf.alloc(std::max<size_t>(v.size(), 1));
+    if (!v.empty())
+      TOFE_HIP(hipMemcpy(buf.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+  };
+  up(c->direct_range, st.range);
+  up(c->direct_tidx, st.tidx);
+  up(c->direct_elem, st.elem);
+  up(c->direct_table, st.table);
+  up(c->direct_slot, st.slot_of);
+  up(c->direct_cconst, st.cconst);
+  c->direct_cfix = std::move(st.cconst);
+  c->direct_part.alloc((size_t)direct_coarse_chunks(ncell, r) * ncell * 576);
+  c->direct_cells.alloc((size_t)ncell * 576);
+  c->direct_corr.alloc((size_t)std::max(c->direct_nmasked, 1) * 576 * kDirectMaskSplit);
+}
+
 void coarse_inverse_enqueue(tofe_ctx *c)
 {
   const hipStream_t s = c->stream2;
@@ -1184,6 +1233,35 @@ void assemble(tofe_ctx *c)
     c->spk32.alloc(c->ndof);
     convert<double, float>(s, spk, c->spk32.p, c->ndof);
   }
+  TOFE_HIP(hipEventRecord(c->ev_diag, s));
+  const Grid &gL = c->grids[L - 1];
+  c->nc = gL.ndof();
+  c->Ainv.alloc((size_t)c->nc * c->nc);
+  c->dinfo.alloc(2);
+  if (c->direct) {
+    // The coarsest operator straight from E on the second stream, and its factorisation right
+    // after, beside the Galerkin chain of the smoothed levels on the main stream (enqueued
+    // first: it is the longest chain of the set-up).
+    TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_diag, 0));
+    DirectCoarse dc;
+    dc.E = c->E.p;
+    dc.M = c->direct_M.p;
+    dc.nmasked = c->direct_nmasked;
+    dc.range = c->direct_range.p;
+    dc.elem = c->direct_elem.p;
+    dc.tidx = c->direct_tidx.p;
+    dc.table = c->direct_table.p;
+    dc.slot_of = c->direct_slot.p;
+    dc.cconst = c->direct_cconst.p;
+    dc.part = c->direct_part.p;
+    dc.cells = c->direct_cells.p;
+    dc.corr = c->direct_corr.p;
+    dc.nx = c->desc.nx; dc.ny = c->desc.ny; dc.nz = c->desc.nz;
+    dc.r = 1 << (L - 1);
+    direct_coarse(c->stream2, dc, c->Ainv.p);
+    coarse_inverse_enqueue(c);
+    TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
+  }
   if (L > 1) {
     MaskedElems me;
     me.elem = c->dme_elem.p;
@@ -1211,21 +1289,25 @@ void assemble(tofe_ctx *c)
     galerkin(s, c->lv64[l]->S.p, gf.Nx, gf.Ny, gf.Nz, c->lv64[l + 1]->S.p, c->gal_t1.p, c->gal_t2.p);
     stencil_diag(s, c->lv64[l + 1]->S.p, gc.nn(), nullptr, c->lv64[l + 1]->dinv.p);
   }
-  // coarsest level: dense Cholesky factor and explicit inverse (FP64)
-  const Grid &gL = c->grids[L - 1];
-  const double *SL = (L == 1) ? c->fineS.p : c->lv64[L - 1]->S.p;
-  c->nc = gL.ndof();
-  c->Ainv.alloc((size_t)c->nc * c->nc);
-  c->dinfo.alloc(2);
-  stencil_to_dense(s, SL, gL.Nx, gL.Ny, gL.Nz, c->Ainv.p);
-  TOFE_HIP(hipEventRecord(c->ev_dense, s));
+  if (!c->direct) {
+    // coarsest level from the chain's last stencil: dense Cholesky factor and explicit inverse
+    // (FP64) on the second stream
+    const double *SL = (L == 1) ? c->fineS.p : c->lv64[L - 1]->S.p;
+    stencil_to_dense(s, SL, gL.Nx, gL.Ny, gL.Nz, c->Ainv.p);
+    TOFE_HIP(hipEventRecord(c->ev_dense, s));
+    TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
+    coarse_inverse_enqueue(c);
+    TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
+  }
+  // The fine level's Lanczos estimate needs E and the fine diagonal only, but the fourth stream
+  // starts it after the Galerkin chain: the chain feeds the other estimates, and the estimate's
+  // bandwidth-bound kernels beside it slow it down (measured: 0.07 ms more set-up per call on the
+  // 192x96x96 cantilever when started with the chain).
+  lanczos_fine_enqueue(c);
   // Four streams from here: the coarsest level's factorisation on the second, the Lanczos
-  // estimates of level 1 on the third, of the fine level on the fourth and of levels 2.. on
-  // the main stream, which then joins the other three.
-  TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
-  coarse_inverse_enqueue(c);
-  TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
-  lanczos_enqueue(c, false);
+  // estimates of level 1 on the third, of the fine level on the fourth (enqueued above) and of
+  // levels 2.. on the main stream, which then joins the other three.
+  lanczos_enqueue(c, true);
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz, 0));
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz0, 0));
   lanczos_lambdas(c);
@@ -1664,7 +1746,8 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
         TOFE_HIP(hipStreamCreate(&c->stream4));
       }
     }
-    for (hipEvent_t *e : {&c->ev_dense, &c->ev_inv, &c->ev_ops, &c->ev_lz, &c->ev_scal, &c->ev_lz0})
+    for (hipEvent_t *e : {&c->ev_dense, &c->ev_inv, &c->ev_ops, &c->ev_lz, &c->ev_scal, &c->ev_lz0,
+                          &c->ev_diag})
       TOFE_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     TOFE_HIP(hipHostMalloc((void **)&c->hscal, SL_COUNT * sizeof(double), hipHostMallocDefault));
 
@@ -1795,8 +1878,10 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
       TOFE_HIP(hipMemcpyAsync(c->dfix.p, m.data(), c->nn, hipMemcpyHostToDevice, c->stream));
       TOFE_HIP(hipStreamSynchronize(c->stream));
       c->has_fix = true;
+      direct_setup(c.get(), m.data());
     } else {
       c->dfix.zero(c->stream);
+      direct_setup(c.get(), nullptr);
     }
 
     {
@@ -2104,6 +2189,9 @@ int tofe_set_springs(tofe_ctx *c, const double *spk)
     c->assembled = false;  // the diagonal and the Galerkin hierarchy change
     if (!spk) {
       c->has_springs = false;
+      if (c->direct)
+        TOFE_HIP(hipMemcpy(c->direct_cconst.p, c->direct_cfix.data(),
+                           c->direct_cfix.size() * sizeof(double), hipMemcpyHostToDevice));
       return;
     }
     std::vector<double> h(spk, spk + c->ndof);
@@ -2117,6 +2205,11 @@ int tofe_set_springs(tofe_ctx *c, const double *spk)
     h2d(c, c->spk.p, h.data(), c->ndof);
     TOFE_HIP(hipStreamSynchronize(c->stream));
     c->has_springs = true;
+    if (c->direct) {  // the direct coarsest operator's constant part: fixed diagonal + springs
+      std::vector<double> cc = c->direct_cfix;
+      direct_coarse_add_springs(c->desc.nx, c->desc.ny, c->desc.nz, 1 << (c->L - 1), h.data(), cc.data());
+      TOFE_HIP(hipMemcpy(c->direct_cconst.p, cc.data(), cc.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
   });
 }
 
@@ -2529,6 +2622,7 @@ int tofe_get_stats(const tofe_ctx *c, tofe_stats *st)
   if (!c || !st) return TOFE_ERR_ARG;
   *st = c->stats;
   st->fine_form = c->ke_wht ? (c->ke_wht_cube ? 3 : 2) : c->ke_sym ? 1 : 0;
+  st->coarse_form = c->direct ? 1 : 0;
   return TOFE_OK;
 }
 

@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/tofe.h"
 
@@ -180,6 +181,41 @@ void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double
 int64_t galerkin_scratch(int Nfx, int Nfy, int Nfz, int64_t *n2);
 void stencil_diag(hipStream_t s, const double *S, int64_t nn, double *diag, double *dinv);
 void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A);
+// ---- coarsest operator straight from the fine level (k_direct.hip) ---------
+// Valid when levels >= 2 and every element count is a multiple of r = 2^(levels-1) <= 16.
+constexpr int kDirectMaxR = 16;
+constexpr int kDirectMaskSplit = 8;  // slices of a cell's Dirichlet elements (corr rows per cell)
+bool direct_coarse_ok(int nx, int ny, int nz, int levels);
+// M (host, r^3 x 576): the element matrix at each cell position seen from the cell's 8 nodes
+void direct_coarse_table(const double KE[576], int r, double *M);
+struct DirectCoarse {
+  const double *E = nullptr;       // FP64 element scales
+  const double *M = nullptr;       // device table of M_p
+  // Dirichlet elements (per context): nmasked cells, element ranges, element indices, table rows
+  int nmasked = 0;
+  const int *range = nullptr;
+  const int64_t *elem = nullptr;
+  const int *tidx = nullptr;
+  const double *table = nullptr;
+  const int *slot_of = nullptr;    // cell -> its row of corr, -1 without Dirichlet elements
+  const double *cconst = nullptr;  // cells x 576: fixed DOFs' unit diagonal and springs
+  double *part = nullptr, *cells = nullptr, *corr = nullptr;  // scratch (direct_coarse_chunks)
+  int nx = 0, ny = 0, nz = 0, r = 0;
+};
+// Host tables of the Dirichlet part (fixmask: per node, bit a = component a fixed; may be null).
+struct DirectStatic {
+  std::vector<int> cells, range, tidx, slot_of;
+  std::vector<int64_t> elem;
+  std::vector<double> table, cconst;
+};
+void direct_coarse_static(const double KE[576], int nx, int ny, int nz, int r, const uint8_t *fixmask,
+                          DirectStatic &st);
+// cconst += the springs (per DOF, zero on fixed DOFs) of every cell's owned nodes
+void direct_coarse_add_springs(int nx, int ny, int nz, int r, const double *spk, double *cconst);
+int direct_coarse_chunks(int ncell, int r);
+// Dense coarsest matrix (column-major) into A, as stencil_to_dense of the Galerkin chain's last
+// level gives it (to rounding).
+void direct_coarse(hipStream_t s, const DirectCoarse &d, double *A);
 constexpr int kCholNB = 64;  // tile of the coarse Cholesky factorisation
 // A (n x n SPD, column-major, half-bandwidth bw): banded blocked Cholesky A = L L^T into
 // Lb and X = L^-1 into X (column-major, lower triangle), so A^-1 = X^T X.  info[0] = first
