@@ -111,7 +111,8 @@ typedef struct tofe_stats {
   double solve_seconds;    /* last tofe_solve wall time */
   int fine_form;           /* fine operator kernel: 0 node-centric with KE rows, 1 node-centric
                               with KE's cube-orbit values, 2 element-centric Walsh-Hadamard
-                              (box element), 3 the same with the cubic element's 9 values */
+                              (box element), 3 the same with the cubic element's 9 values,
+                              4 element-centric on the matrix cores (TOFE_FINE_MFMA=1) */
   int coarse_form;         /* coarsest operator: 0 the Galerkin chain's last level, 1 formed
                               straight from E (TOFE_DIRECT_COARSE=1 at tofe_create; element
                               counts multiples of 2^(levels-1) <= 16) */

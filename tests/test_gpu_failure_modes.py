@@ -1,0 +1,79 @@
+"""Failure modes of the analysis, forced from the host with ordinary inputs (no kernel faults):
+a NaN design variable, a structure without Dirichlet DOFs (singular K), and an all-void design.
+The reference reports the first two through PETSc (KSP_DIVERGED_NANORINF, a failed factorisation
+of the coarse operator in PCSetUp; FEAnalysis.cpp:673-690 reads the KSPConvergedReason); the
+library either rejects the set-up with TOFE_ERR_NUMERIC (the coarse Cholesky meets a non-positive
+or NaN pivot) or returns a diverged reason from the solve.  An all-void design is not a failure:
+E = Emin everywhere, so K is Emin times the solid K and U scales by 1/Emin exactly up to the
+solve's tolerance."""
+import numpy as np
+import pytest
+
+from topopt3d_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+NEL = (16, 8, 8)
+
+
+def _solve_or_error(tofe, ctx, F):
+    try:
+        ctx.assemble()
+    except tofe.TofeError as e:
+        return "error", e.code
+    U, its, reason = ctx.solve(F, rtol=1e-8, maxit=60)
+    return "reason", reason
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_nan_design_variable_is_reported(tofe, precision):
+    print(f"case nan design precision={precision}", flush=True)
+    pb = configs.cantilever(NEL, seed=3, lo=0.05, hi=1.0, mg_levels=3)
+    x = pb.x.copy()
+    x[123] = np.nan
+    ctx = tofe.Context(NEL, mg_levels=3, fixed_dof=pb.fixed_dof, precision=precision)
+    ctx.set_design(x)
+    assert np.isnan(ctx.field(tofe.FIELD_E)).any()
+    kind, code = _solve_or_error(tofe, ctx, pb.F)
+    assert (kind, code) in (("error", tofe.TOFE_ERR_NUMERIC), ("reason", tofe.DIVERGED_NANORINF))
+    # the context stays usable: a finite design assembles and solves again
+    ctx.set_design(pb.x)
+    ctx.assemble()
+    U, its, reason = ctx.solve(pb.F, rtol=1e-8)
+    assert reason == tofe.CONVERGED_RTOL and np.isfinite(U).all()
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_no_dirichlet_dofs_is_not_a_silent_success(tofe, precision):
+    """Without supports K has the six rigid-body modes in its null space and a point load has a
+    component outside its range, so no solve can meet the tolerance."""
+    print(f"case no dirichlet precision={precision}", flush=True)
+    pb = configs.cantilever(NEL, seed=3, lo=0.05, hi=1.0, mg_levels=3)
+    ctx = tofe.Context(NEL, mg_levels=3, fixed_dof=None, precision=precision)
+    ctx.set_design(pb.x)
+    kind, code = _solve_or_error(tofe, ctx, pb.F)
+    if kind == "error":
+        assert code == tofe.TOFE_ERR_NUMERIC
+    else:
+        assert code < 0
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_all_void_design_scales_the_solid_solution(tofe, precision):
+    print(f"case all void precision={precision}", flush=True)
+    pb = configs.cantilever(NEL, seed=3, lo=0.05, hi=1.0, mg_levels=3)
+    Emin = 1e-6
+    out = []
+    for value in (0.0, 1.0):
+        ctx = tofe.Context(NEL, Emin=Emin, mg_levels=3, fixed_dof=pb.fixed_dof, precision=precision)
+        ctx.set_design(np.full(pb.nelem, value))
+        ctx.assemble()
+        U, its, reason = ctx.solve(pb.F, rtol=1e-10)
+        assert reason == tofe.CONVERGED_RTOL
+        out.append(U)
+        ctx.close()
+    void, solid = out
+    # E = Emin (void) and E = Emin + (1 - Emin) = 1 (solid): K_void = Emin K_solid
+    assert np.abs(void * Emin - solid).max() <= 1e-8 * np.abs(solid).max()

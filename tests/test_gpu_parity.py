@@ -77,7 +77,7 @@ def test_operator_matches_assembled_K(oracle, tofe, nel, L):
         ctx.close()
 
 
-@pytest.mark.parametrize("mode", ["wht", "whtbox", "sym", "rows"])
+@pytest.mark.parametrize("mode", ["wht", "whtbox", "sym", "rows", "mfma"])
 @pytest.mark.parametrize("h,E0,nu", [((1.0, 1.0, 1.0), 1.0, 0.3),
                                      ((0.5, 0.5, 0.5), 2.5, 0.27),
                                      ((0.5, 0.75, 1.25), 1.0, 0.3),
@@ -87,7 +87,10 @@ def test_operator_element_shapes(oracle, tofe, h, E0, nu, mode, monkeypatch):
     (E0, nu, h): the element-centric Walsh-Hadamard form (default; 9 block values for cubic
     elements, 33 for box elements or with TOFE_FINE_WHT_CUBE=0), the node-centric form with
     KE from its 12 cube-symmetry orbit values (cubic elements; TOFE_FINE_WHT=0) and the
-    node-centric form reading KE's rows (TOFE_FINE_WHT=0 TOFE_FINE_SYM=0, and box elements)."""
+    node-centric form reading KE's rows (TOFE_FINE_WHT=0 TOFE_FINE_SYM=0, and box elements), and
+    the element-centric form on the matrix cores (TOFE_FINE_MFMA=1: KE x (E_e u_e) as 16-element
+    v_mfma_{f32,f64}_16x16x4 GEMMs)."""
+    monkeypatch.setenv("TOFE_FINE_MFMA", "1" if mode == "mfma" else "0")
     monkeypatch.setenv("TOFE_FINE_WHT", "1" if mode.startswith("wht") else "0")
     monkeypatch.setenv("TOFE_FINE_WHT_CUBE", "0" if mode == "whtbox" else "1")
     monkeypatch.setenv("TOFE_FINE_SYM", "1" if mode == "sym" else "0")
@@ -96,7 +99,7 @@ def test_operator_element_shapes(oracle, tofe, h, E0, nu, mode, monkeypatch):
     print(f"h={h} E0={E0} nu={nu} mode={mode}", flush=True)
     ctx = tofe.Context(nel, h=h, E0=E0, nu=nu, mg_levels=2, fixed_dof=pb.fixed_dof)
     cube = h[0] == h[1] == h[2]
-    want = {"wht": 3 if cube else 2, "whtbox": 2, "sym": 1 if cube else 0, "rows": 0}[mode]
+    want = {"wht": 3 if cube else 2, "whtbox": 2, "sym": 1 if cube else 0, "rows": 0, "mfma": 4}[mode]
     assert ctx.stats()["fine_form"] == want
     ctx.set_design(pb.x)
     E = ctx.field(tofe.FIELD_E)

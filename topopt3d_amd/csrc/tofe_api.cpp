@@ -2621,7 +2621,7 @@ int tofe_get_stats(const tofe_ctx *c, tofe_stats *st)
 {
   if (!c || !st) return TOFE_ERR_ARG;
   *st = c->stats;
-  st->fine_form = c->ke_wht ? (c->ke_wht_cube ? 3 : 2) : c->ke_sym ? 1 : 0;
+  st->fine_form = c->fine_mfma ? 4 : c->ke_wht ? (c->ke_wht_cube ? 3 : 2) : c->ke_sym ? 1 : 0;
   st->coarse_form = c->direct ? 1 : 0;
   return TOFE_OK;
 }
