@@ -165,7 +165,7 @@ import json, sys, time
 sys.path.insert(0, {root!r})
 from oracle import oracle as O
 from topopt3d_amd import configs
-pb = configs.config2()
+pb = configs.cantilever_192x96x96()
 t0 = time.perf_counter()
 rho = O.filter_csr(*pb.nel, pb.rmin).matvec(pb.x)
 E = O.simp(rho, pb.penal, pb.Emin)[0]
@@ -330,7 +330,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     pb = configs.cantilever((48, 24, 24), seed=2, lo=0.01, hi=1.0, Emin=1e-9, mg_levels=3) \
-        if args.small else configs.config2()
+        if args.small else configs.cantilever_192x96x96()
     log(f"bench: {pb.name} ndof={pb.ndof} nelem={pb.nelem}")
     ctx = _lib.Context(pb.nel, Emin=pb.Emin, penal=pb.penal, rmin=pb.rmin,
                        mg_levels=pb.mg_levels, fixed_dof=pb.fixed_dof)

@@ -1,4 +1,4 @@
-"""Writes tests/golden/cfg1_direct.npz: an independent sparse-direct solution of the
+"""Writes tests/golden/cantilever48_direct.npz: an independent sparse-direct solution of the
 48x24x24 cantilever (91,875 DOF, x = 0.3, R = 1.5, p = 3, eps = 1e-10).
 
 Nothing here uses the oracle or the HIP library: the element matrix comes from exact
@@ -7,7 +7,7 @@ stiffness matrix is assembled with scipy.sparse COO, the Dirichlet rows/columns 
 MatZeroRowsColumns(K, fixed, 1.0), and the system is solved with SciPy's SuperLU.
 The fixture stores scalars and seeded samples only (a few KB).
 
-Run:  python tests/golden/make_cfg1_direct.py      (about 4 minutes on 8 cores)
+Run:  python tests/golden/make_cantilever48_direct.py      (about 4 minutes on 8 cores)
 """
 import itertools
 import os
@@ -77,7 +77,7 @@ def main():
     rng = np.random.default_rng(1)
     dof_s = np.sort(rng.choice(ndof, 64, replace=False))
     el_s = np.sort(rng.choice(nx * ny * nz, 32, replace=False))
-    out = os.path.join(HERE, "cfg1_direct.npz")
+    out = os.path.join(HERE, "cantilever48_direct.npz")
     np.savez(out, compliance=c, residual=res, U_load=U[3 * load:3 * load + 3], dof_samples=dof_s,
              U_samples=U[dof_s], U_norm=np.linalg.norm(U), el_samples=el_s, g_samples=g[el_s],
              g_sum=g.sum())

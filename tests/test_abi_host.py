@@ -74,14 +74,14 @@ def test_splitmix64_known_answers():
 
 
 def test_cantilever_boundary_conditions():
-    pb = configs.config1()
+    pb = configs.cantilever_48x24x24()
     assert pb.ndof == 91875 and pb.nelem == 27648
     assert pb.fixed_dof.sum() == 25 * 25 * 3
     fixed_nodes = np.flatnonzero(pb.fixed_dof[0::3])
     assert np.all(fixed_nodes % 49 == 0)
     load = np.flatnonzero(pb.F)
     assert load.tolist() == [3 * (48 + 49 * 12) + 2] and pb.F[load[0]] == -1.0
-    pb2 = configs.config2()
+    pb2 = configs.cantilever_192x96x96()
     assert pb2.ndof == 5447811 and pb2.nelem == 1769472 and pb2.fixed_dof.sum() == 97 * 97 * 3
 
 

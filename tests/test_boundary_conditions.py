@@ -62,5 +62,5 @@ def test_config3_builder():
     f = pb.fixed_dof.reshape(5, 5, 9, 3)
     assert f[0].all() and not f[1:].any()
     assert pb.x.min() >= 0.1 and pb.x.max() < 1.0 and pb.rmin == 2.0
-    c3 = configs.config3()
+    c3 = configs.column_128x64x64()
     assert c3.ndof == 1_635_075 and c3.nelem == 128 * 64 * 64

@@ -25,7 +25,7 @@ pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
 @pytest.fixture(scope="module")
 def full(tofe):
-    pb = configs.config2()
+    pb = configs.cantilever_192x96x96()
     print(f"case {pb.name} ndof={pb.ndof} nelem={pb.nelem}", flush=True)
     ctx = tofe.Context(pb.nel, Emin=pb.Emin, penal=pb.penal, rmin=pb.rmin,
                        mg_levels=pb.mg_levels, fixed_dof=pb.fixed_dof)

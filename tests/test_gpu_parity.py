@@ -274,11 +274,11 @@ def test_pc_apply_is_symmetric(oracle, tofe):
 
 def test_config1_against_sparse_direct_fixture(tofe):
     """48x24x24 cantilever (x = 0.3) on the GPU against the committed SciPy sparse-direct
-    fixture (tests/golden/make_cfg1_direct.py, independent of the oracle and the GPU)."""
+    fixture (tests/golden/make_cantilever48_direct.py, independent of the oracle and the GPU)."""
     import os
 
-    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg1_direct.npz"))
-    pb = configs.config1()
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "cantilever48_direct.npz"))
+    pb = configs.cantilever_48x24x24()
     ctx = tofe.Context(pb.nel, mg_levels=3, fixed_dof=pb.fixed_dof)
     ctx.set_design(pb.x)
     ctx.assemble()

@@ -211,12 +211,12 @@ def test_pcg_reason_codes(oracle):
 
 def test_config1_against_sparse_direct(oracle):
     """The 48x24x24 cantilever (91,875 DOF) by the oracle's 3-level MG-CG to 1e-12
-    against the committed SciPy sparse-direct fixture (tests/golden/make_cfg1_direct.py,
+    against the committed SciPy sparse-direct fixture (tests/golden/make_cantilever48_direct.py,
     independent of the oracle): displacements, compliance and element sensitivities."""
     import os
 
-    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg1_direct.npz"))
-    pb = configs.config1()
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "cantilever48_direct.npz"))
+    pb = configs.cantilever_48x24x24()
     nel = pb.nel
     KE = oracle.ke()
     rho = oracle.filter_csr(*nel, 1.5).matvec(pb.x)

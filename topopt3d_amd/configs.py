@@ -95,13 +95,13 @@ def cantilever(nel, x=None, rmin=1.5, penal=3.0, Emin=1e-10, mg_levels=0, seed=N
                    mg_levels)
 
 
-def config1(Emin=1e-10) -> Problem:
+def cantilever_48x24x24(Emin=1e-10) -> Problem:
     """48x24x24 cantilever, uniform density 0.3, filter R = 1.5, 3-level GMG."""
     return cantilever((48, 24, 24), x=0.3, rmin=1.5, penal=3.0, Emin=Emin, mg_levels=3,
                       name="cantilever-48x24x24")
 
 
-def config2(Emin=1e-9) -> Problem:
+def cantilever_192x96x96(Emin=1e-9) -> Problem:
     """192x96x96 cantilever (5,447,811 DOF), densities U[0.01,1] seed 2, R = 1.5, p = 3,
     5-level V(2,2) Chebyshev GMG.  The densities are filtered inside tofe_set_design."""
     return cantilever((192, 96, 96), seed=2, lo=0.01, hi=1.0, rmin=1.5, penal=3.0, Emin=Emin,
@@ -142,14 +142,14 @@ def column(nel, seed=3, lo=0.1, hi=1.0, rmin=2.0, penal=3.0, Emin=1e-9, mg_level
     return Problem(name, tuple(nel), fixed, F, x, rmin, penal, Emin, mg_levels)
 
 
-def config3(Emin=1e-9) -> Problem:
+def column_128x64x64(Emin=1e-9) -> Problem:
     """128x64x64 column (1,635,075 DOF), bottom clamped, unit pressure on the top face,
     densities U[0.1,1] seed 3, filter R = 2.0, p = 3, 5-level GMG; 6 buckling modes."""
     return column((128, 64, 64), seed=3, lo=0.1, hi=1.0, rmin=2.0, penal=3.0, Emin=Emin,
                   mg_levels=5, name="column-128x64x64")
 
 
-def config5(Emin=1e-9) -> Problem:
+def vibration_160x80x80(Emin=1e-9) -> Problem:
     """Free vibration of the 160x80x80 cantilever (3,168,963 DOF): x = 0 face clamped,
     densities U[0.05,1] seed 5 through the R = 1.5 filter, SIMP p = 3 on the stiffness, the
     mass scaled by V = rho (p = 1) with density 1, 5-level GMG; 12 lowest modes."""
@@ -186,7 +186,7 @@ def batch_instances(n=32, sizes=tuple(range(16, 97, 2)), seed=4, nrhs=4, lo=0.05
     return out
 
 
-def config4(nrhs=4) -> list:
+def cantilevers_batch32(nrhs=4) -> list:
     """32 cantilevers with nx in {16, 18, ..., 96} (seed 4), ny = nz = nx / 2, densities
     U[0.05,1], p = 3, 4 unit point-load cases each on the x = max face."""
     return batch_instances(32, tuple(range(16, 97, 2)), seed=4, nrhs=nrhs)

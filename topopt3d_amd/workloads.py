@@ -35,10 +35,11 @@ def _sync():
 
 
 def buckling(nel=None, nev=6, fd_elements=32, fd_steps=(1e-6, 1e-3), eig_tol=1e-9):
-    """Config 3 (or a smaller column with nel): both precisions, then the FD check in FP64."""
+    """The 128x64x64 column (or a smaller column with nel): both precisions, then the FD check in
+    FP64."""
     import torch
 
-    pb = configs.config3() if nel is None else configs.column(nel, seed=3, rmin=2.0, mg_levels=0)
+    pb = configs.column_128x64x64() if nel is None else configs.column(nel, seed=3, rmin=2.0, mg_levels=0)
     dev = torch.device("cuda:0")
     F = torch.from_numpy(pb.F).to(dev)
     out = {"workload": f"{pb.name}: {pb.nelem} elements, {pb.ndof} DOF, R={pb.rmin}, "
@@ -137,11 +138,11 @@ def buckling(nel=None, nev=6, fd_elements=32, fd_steps=(1e-6, 1e-3), eig_tol=1e-
 
 
 def vibration(nel=None, nev=12, eig_tol=1e-8, mass_form=_lib.MASS_CONSISTENT, reps=10):
-    """Config 5 (or a smaller cantilever with nel): 12 lowest modes in both precisions, and the
+    """The 160x80x80 cantilever (or a smaller cantilever with nel): 12 lowest modes in both precisions, and the
     block-pass timings of K, M and the V-cycle on 12 vectors."""
     import torch
 
-    pb = configs.config5() if nel is None else configs.cantilever(nel, seed=5, lo=0.05,
+    pb = configs.vibration_160x80x80() if nel is None else configs.cantilever(nel, seed=5, lo=0.05,
                                                                     Emin=1e-9, mg_levels=0)
     dev = torch.device("cuda:0")
     out = {"workload": f"{pb.name}: {pb.nelem} elements, {pb.ndof} DOF, {nev} modes, "
@@ -205,13 +206,13 @@ def vibration(nel=None, nev=12, eig_tol=1e-8, mass_form=_lib.MASS_CONSISTENT, re
 
 def batch(n=32, nrhs=4, rtol=1e-8, chunk=2, autotune=True,
           forms=("merged", "single", "instance", "streams")):
-    """Config 4: 32 cantilevers of uneven size (nx in {16..96}), 4 load cases each, FP32
+    """32 cantilevers of uneven size (nx in {16..96}), 4 load cases each, FP32
     V-cycle inside FP64 flexible CG to 1e-8.  Times the eager solves one instance after the
     other against graph-captured batch solves of all instances in each batch form, and checks
     that every form gives the eager solves' bits."""
     import torch
 
-    pbs = configs.config4(nrhs=nrhs)[:n]
+    pbs = configs.cantilevers_batch32(nrhs=nrhs)[:n]
     dev = torch.device("cuda:0")
     out = {"workload": f"{len(pbs)} cantilevers nx in {{16..96}}, ny = nz = nx/2, "
                        f"{sum(p.ndof for p in pbs)} DOF in total, {nrhs} load cases each, "
