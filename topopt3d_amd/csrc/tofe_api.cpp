@@ -103,7 +103,8 @@ double now_s()
 // stalls every 31.5 ms of busy time with 16 contexts of two masked streams each).  Created on
 // first use and kept until the process ends.
 struct MaskedStreams {
-  hipStream_t a = nullptr, b = nullptr;
+  hipStream_t a = nullptr, b = nullptr;  // off the first first_cu compute units
+  hipStream_t full = nullptr;            // every compute unit, a hardware queue of its own
 };
 MaskedStreams masked_streams(int device, int ncu, int first_cu)
 {
@@ -117,6 +118,9 @@ MaskedStreams masked_streams(int device, int ncu, int first_cu)
   MaskedStreams m;
   TOFE_HIP(hipExtStreamCreateWithCUMask(&m.a, (uint32_t)mask.size(), mask.data()));
   TOFE_HIP(hipExtStreamCreateWithCUMask(&m.b, (uint32_t)mask.size(), mask.data()));
+  std::vector<uint32_t> all((ncu + 31) / 32, 0u);
+  for (int cu = 0; cu < ncu; ++cu) all[cu / 32] |= 1u << (cu % 32);
+  TOFE_HIP(hipExtStreamCreateWithCUMask(&m.full, (uint32_t)all.size(), all.data()));
   per_device[device] = m;
   return m;
 }
@@ -164,17 +168,22 @@ struct tofe_ctx {
   int64_t ne = 0, nn = 0, ndof = 0;
   hipStream_t stream = nullptr;
   Recorder *rec = nullptr;  // set while a batch records this context's iteration
-  // second stream: the coarsest-level factorisation runs beside the Lanczos estimates
+  // second stream: the coarsest-level factorisation runs beside the Galerkin chain and the
+  // Lanczos estimates
   hipStream_t stream2 = nullptr;
   // third stream: the Lanczos estimates of the coarser smoothed levels run beside the
   // fine level's
   hipStream_t stream3 = nullptr;
   // fourth stream: the fine level's Lanczos estimate.  The third and fourth streams are
   // kept off the first kCoarseCUs compute units, which the latency-bound coarse
-  // factorisation on the second stream then has to itself; those CU-masked streams are the
-  // device's pair shared by every context (masked_streams), not destroyed with the context.
+  // factorisation on the second stream then has to itself.  The second to fourth streams are
+  // CU-masked streams shared by every context of the device (masked_streams), not destroyed
+  // with the context: a masked stream owns its hardware queue, where ordinary streams share
+  // the process's few queues round the pool, so that two of a context's streams could land on
+  // one queue and run in turn (measured: set-up of the 192x96x96 cantilever 2.5 or 3.5 ms
+  // depending on the streams other contexts had created before).
   hipStream_t stream4 = nullptr;
-  bool shared_34 = false;
+  bool shared_helpers = false;
   hipEvent_t ev_dense = nullptr, ev_inv = nullptr, ev_ops = nullptr, ev_lz = nullptr;
   hipEvent_t ev_scal = nullptr, ev_lz0 = nullptr, ev_diag = nullptr;
   double *hscal = nullptr;  // pinned host copy of the device scalars
@@ -291,9 +300,9 @@ struct tofe_ctx {
       if (e) (void)hipEventDestroy(e);
     if (hscal) (void)hipHostFree(hscal);
     if (stream) (void)hipStreamDestroy(stream);
-    if (stream2) (void)hipStreamDestroy(stream2);
-    if (stream3 && !shared_34) (void)hipStreamDestroy(stream3);
-    if (stream4 && !shared_34) (void)hipStreamDestroy(stream4);
+    if (stream2 && !shared_helpers) (void)hipStreamDestroy(stream2);
+    if (stream3 && !shared_helpers) (void)hipStreamDestroy(stream3);
+    if (stream4 && !shared_helpers) (void)hipStreamDestroy(stream4);
   }
 };
 
@@ -1714,7 +1723,6 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
                                     " nodes); use more multigrid levels");
 
     TOFE_HIP(hipStreamCreate(&c->stream));
-    TOFE_HIP(hipStreamCreate(&c->stream2));
     {
       int ncu = 0;
       TOFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
@@ -1738,10 +1746,12 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
       }
       if (ncu >= 4 * kCoarseCUs) {
         const MaskedStreams m = masked_streams(c->device, ncu, kCoarseCUs);
+        c->stream2 = m.full;
         c->stream3 = m.a;
         c->stream4 = m.b;
-        c->shared_34 = true;
+        c->shared_helpers = true;
       } else {
+        TOFE_HIP(hipStreamCreate(&c->stream2));
         TOFE_HIP(hipStreamCreate(&c->stream3));
         TOFE_HIP(hipStreamCreate(&c->stream4));
       }
