@@ -68,7 +68,7 @@ __device__ inline int elem_form(int ex, int nel, int Nf)
 // 0..1 (D = -1), so positions, local node indices and the interpolation form of a regular
 // element are compile-time constants and its Q^T KE Q block comes through uniform
 // (scalar) loads.  Elements with a Dirichlet DOF (their own masked matrix) and the last
-// element of a grid with an even node count (single-parent form) take a general path.
+// element of a grid with an even node count (single-parent form) add their own blocks.
 template <int O>
 __device__ inline void galerkin_block(const double *__restrict__ E, const int *__restrict__ emap,
                                       const double *__restrict__ KEC,
@@ -79,49 +79,57 @@ __device__ inline void galerkin_block(const double *__restrict__ E, const int *_
   constexpr int X0 = DX > 0 ? 2 : 0, X1 = DX < 0 ? 2 : 4;
   constexpr int Y0 = DY > 0 ? 2 : 0, Y1 = DY < 0 ? 2 : 4;
   constexpr int Z0 = DZ > 0 ? 2 : 0, Z1 = DZ < 0 ? 2 : 4;
+  // Every element of the window is visited without a branch, so its loads are issued with
+  // the others: an element outside the grid (clamped load, weight 0) or of a special form adds
+  // w = 0 times its regular block, which leaves the sum unchanged, so the sum runs over the same
+  // elements in the same order as with those skipped.  A special element then adds its own
+  // block right away (a branch taken only near Dirichlet faces and at the last element of an
+  // axis with an even node count).
 #pragma unroll
   for (int dz = Z0; dz < Z1; ++dz) {
     const int ez = 2 * Ic[2] - 2 + dz;
-    if (ez < 0 || ez >= nel[2]) continue;
+    const bool zin = ez >= 0 && ez < nel[2];
 #pragma unroll
     for (int dy = Y0; dy < Y1; ++dy) {
       const int ey = 2 * Ic[1] - 2 + dy;
-      if (ey < 0 || ey >= nel[1]) continue;
+      const bool yin = ey >= 0 && ey < nel[1];
 #pragma unroll
       for (int dx = X0; dx < X1; ++dx) {
         const int ex = 2 * Ic[0] - 2 + dx;
-        if (ex < 0 || ex >= nel[0]) continue;
-        const int LI = lidx(1 - (dx >> 1), 1 - (dy >> 1), 1 - (dz >> 1));
-        const int LJ = lidx(1 - (dx >> 1) + DX, 1 - (dy >> 1) + DY, 1 - (dz >> 1) + DZ);
-        const int64_t e = ex + (int64_t)nel[0] * (ey + (int64_t)nel[1] * ez);
-        const int m = emap[e];
+        const bool in = zin && yin && ex >= 0 && ex < nel[0];
+        const int li = lidx(1 - (dx >> 1), 1 - (dy >> 1), 1 - (dz >> 1));
+        const int lj = lidx(1 - (dx >> 1) + DX, 1 - (dy >> 1) + DY, 1 - (dz >> 1) + DZ);
+        const int64_t e = in ? ex + (int64_t)nel[0] * (ey + (int64_t)nel[1] * ez) : 0;
+        const int m = in ? emap[e] : -1;
+        const double Ee = E[e];
         // single-parent form: even position (even ex) that is the last element of a grid
         // with an even node count
         const bool sx = !(dx & 1) && (Nf[0] & 1) == 0 && ex == Nf[0] - 2;
         const bool sy = !(dy & 1) && (Nf[1] & 1) == 0 && ey == Nf[1] - 2;
         const bool sz = !(dz & 1) && (Nf[2] & 1) == 0 && ez == Nf[2] - 2;
-        if (m < 0 && !(sx || sy || sz)) {
-          const int F = (dx & 1) + 3 * (dy & 1) + 9 * (dz & 1);
-          const double Ee = E[e];
-          const double *K = KEC + F * 576 + 3 * LI * 24 + 3 * LJ;
+        const bool special = in && (m >= 0 || sx || sy || sz);
+        const double w = (in && !special) ? Ee : 0.0;
+        const int F = (dx & 1) + 3 * (dy & 1) + 9 * (dz & 1);
+        const double *K = KEC + F * 576 + 3 * li * 24 + 3 * lj;
 #pragma unroll
-          for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += Ee * K[a * 24 + b];
-        } else if (m < 0) {
-          const int F = (sx ? 2 : (dx & 1)) + 3 * (sy ? 2 : (dy & 1)) + 9 * (sz ? 2 : (dz & 1));
-          const double Ee = E[e];
-          const double *K = KEC + (int64_t)F * 576 + 3 * LI * 24 + 3 * LJ;
+          for (int b = 0; b < 3; ++b) acc[a * 3 + b] = __builtin_fma(w, K[a * 24 + b], acc[a * 3 + b]);
+        if (special) {
+          if (m < 0) {
+            const int Fs = (sx ? 2 : (dx & 1)) + 3 * (sy ? 2 : (dy & 1)) + 9 * (sz ? 2 : (dz & 1));
+            const double *Ks = KEC + (int64_t)Fs * 576 + 3 * li * 24 + 3 * lj;
 #pragma unroll
-          for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += Ee * K[a * 24 + b];
-        } else {
-          const double *K = KCm + (int64_t)m * 576 + 3 * LI * 24 + 3 * LJ;
+              for (int b = 0; b < 3; ++b) acc[a * 3 + b] = __builtin_fma(Ee, Ks[a * 24 + b], acc[a * 3 + b]);
+          } else {
+            const double *Km = KCm + (int64_t)m * 576 + 3 * li * 24 + 3 * lj;
 #pragma unroll
-          for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 3; ++b) acc[a * 3 + b] += K[a * 24 + b];
+              for (int b = 0; b < 3; ++b) acc[a * 3 + b] += Km[a * 24 + b];
+          }
         }
       }
     }
