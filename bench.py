@@ -350,6 +350,24 @@ def main():
         log(f"bench: pmc probe: copy 536870912 B read + written; fine fp32 cheb x6 "
             f"{ms0 * 1e3:.1f} us, level-1 fp32 cheb x6 {ms1 * 1e3:.1f} us")
         ctx.close()
+        # the same fine Chebyshev step in the matrix-core form (element-centric KE . [u_e] on
+        # MFMA, FP32 and FP64), for the MFMA counters beside the Walsh-Hadamard form above
+        os.environ["TOFE_FINE_MFMA"] = "1"
+        try:
+            cm = _lib.Context(pb.nel, Emin=pb.Emin, penal=pb.penal, rmin=pb.rmin,
+                              mg_levels=pb.mg_levels, fixed_dof=pb.fixed_dof,
+                              precision=_lib.PREC_MIXED)
+        finally:
+            del os.environ["TOFE_FINE_MFMA"]
+        cm.set_design_dev(xd.data_ptr())
+        cm.assemble()
+        mm32 = cm.time_operator(0, _lib.PREC_MIXED, 1, 5)
+        cm.set_precision(_lib.PREC_FP64)
+        cm.assemble()
+        mm64 = cm.time_operator(0, _lib.PREC_FP64, 1, 5)
+        log(f"bench: pmc probe: matrix-core fine cheb x6 fp32 {mm32 * 1e3:.1f} us, "
+            f"fp64 {mm64 * 1e3:.1f} us (fine_form {cm.stats()['fine_form']})")
+        cm.close()
         return
     F = torch.from_numpy(pb.F).to(dev)
     U = torch.zeros_like(F)

@@ -56,7 +56,7 @@ def test_mass_spring_eigen_node_sets():
     np.testing.assert_array_equal(t.eigenSupports, [[True, False, True]])
 
 
-def test_config3_builder():
+def test_column_128x64x64_builder():
     pb = configs.column((8, 4, 4), seed=3)
     assert pb.F.sum() == -4.0 * 8 * 4
     f = pb.fixed_dof.reshape(5, 5, 9, 3)

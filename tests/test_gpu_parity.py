@@ -272,7 +272,7 @@ def test_pc_apply_is_symmetric(oracle, tofe):
     ctx.close()
 
 
-def test_config1_against_sparse_direct_fixture(tofe):
+def test_cantilever48_against_sparse_direct_fixture(tofe):
     """48x24x24 cantilever (x = 0.3) on the GPU against the committed SciPy sparse-direct
     fixture (tests/golden/make_cantilever48_direct.py, independent of the oracle and the GPU)."""
     import os

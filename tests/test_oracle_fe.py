@@ -209,7 +209,7 @@ def test_pcg_reason_codes(oracle):
     assert its2 == 0 and reason2 == 2
 
 
-def test_config1_against_sparse_direct(oracle):
+def test_cantilever48_against_sparse_direct(oracle):
     """The 48x24x24 cantilever (91,875 DOF) by the oracle's 3-level MG-CG to 1e-12
     against the committed SciPy sparse-direct fixture (tests/golden/make_cantilever48_direct.py,
     independent of the oracle): displacements, compliance and element sensitivities."""
