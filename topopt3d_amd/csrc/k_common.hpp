@@ -101,6 +101,7 @@ __device__ inline void epi_offset(EpiArgs<T> &ep, int64_t v, int64_t ndof, int n
   if (ep.y) ep.y += vo;
   if (ep.b) ep.b += vo;
   if (ep.d) ep.d += vo;
+  if (ep.dout) ep.dout += vo;
   if (ep.xout) ep.xout += vo;
   if (ep.xout64) ep.xout64 += vo;
   if (ep.q64) ep.q64 += vo;
@@ -147,7 +148,7 @@ __device__ inline void apply_epilogue(const EpiArgs<T> &ep, int64_t dof, T Au, T
   } else if constexpr (EPI == EPI_CHEB) {
     const T r = ep.dinv[dof] * (ep.b[dof] - Au);
     const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * ep.d[dof] + ep.c2 * r;
-    ep.d[dof] = dn;
+    (ep.dout ? ep.dout : ep.d)[dof] = dn;
     ep.xout[dof] = uval + dn;
   } else if constexpr (EPI == EPI_CHEB_OUT) {
     const T r = ep.dinv[dof] * (ep.b[dof] - Au);
@@ -223,7 +224,7 @@ __device__ inline void epi_store(const EpiArgs<T> &ep, int64_t n, const EpiIn<T,
     } else if constexpr (EPI == EPI_CHEB) {
       const T r = in.dinv[c] * (in.b[c] - Au);
       const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * in.d[c] + ep.c2 * r;
-      ep.d[dof] = dn;
+      (ep.dout ? ep.dout : ep.d)[dof] = dn;
       ep.xout[dof] = uval + dn;
     } else if constexpr (EPI == EPI_CHEB_OUT) {
       const T r = in.dinv[c] * (in.b[c] - Au);

@@ -400,7 +400,7 @@ __device__ __forceinline__ void fcg_xr_block(double *__restrict__ r, const doubl
     if (fc.b32) {
       fc.b32 += vo;
       fc.d += vo;
-      fc.x += vo;
+      if (fc.x) fc.x += vo;
     }
   }
   double alpha;
@@ -427,7 +427,7 @@ __device__ __forceinline__ void fcg_xr_block(double *__restrict__ r, const doubl
       fc.b32[i] = b;
       const float v = fc.c2 * (fc.dinv[i] * b);
       fc.d[i] = v;
-      fc.x[i] = v;
+      if (fc.x) fc.x[i] = v;
     }
   }
   const double t = block_sum<NT>(s, sred);

@@ -620,7 +620,9 @@ FirstCheb vcycle_first_step(tofe_ctx *c)
   const int flips = std::max(nu1 - 1, 0) + nu2;
   f.b32 = c->vr32.p;
   f.d = lv.d.p;
-  f.x = (flips & 1) ? lv.x2.p : c->vz32.p;
+  // With a second pre-smoothing step, that step reads the iterate from d (the two are equal
+  // after the first step from zero), so the first step does not write it.
+  f.x = nu1 >= 2 ? nullptr : (flips & 1) ? lv.x2.p : c->vz32.p;
   f.dinv = lv.dinv.p;
   f.c2 = (float)c->c2[0];
   return f;
@@ -645,7 +647,7 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
     return;
   }
   const int nu1 = c->desc.nu_pre, nu2 = c->desc.nu_post, km = c->kmax();
-  std::vector<T *> cur(L, nullptr), oth(L, nullptr);
+  std::vector<T *> cur(L, nullptr), oth(L, nullptr), dcur(L, nullptr);  // dcur: where d lives
   for (int l = 0; l < L - 1; ++l) {
     Level<T> &lv = *LV[l];
     const T *b = (l == 0) ? b0 : lv.b.p;
@@ -676,14 +678,30 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
         if (l == 0)
           issue(c, [=](hipStream_t st) { cheb_zero<T>(st, b, dinv, c20, d, xc, nd, nvec); });
       }
+      // After a first step formed without the iterate (vcycle_first_step), the iterate equals
+      // d: the second step reads its operand from d and writes the new d to the iterate's unused
+      // buffer (not over d, which other blocks read as operand), and d lives there from then on.
+      bool x_in_d = l == 0 && from64 && io->first_done && nu1 >= 2;
+      T *dc = lv.d.p;
       for (int k = 1; k < nu1; ++k) {
         EpiArgs<T> ep;
-        ep.b = b; ep.dinv = lv.dinv.p; ep.d = lv.d.p; ep.xout = xo;
+        ep.b = b; ep.dinv = lv.dinv.p; ep.d = dc; ep.xout = xo;
         ep.c1 = (T)c1[k]; ep.c2 = (T)c2[k];
         ep.nvec = nvec;
-        level_apply<T>(c, l, xc, EPI_CHEB, ep);
-        std::swap(xc, xo);
+        if (x_in_d) {
+          ep.dout = xc;
+          level_apply<T>(c, l, dc, EPI_CHEB, ep);
+          T *const freed = dc;
+          dc = xc;
+          xc = xo;
+          xo = freed;
+          x_in_d = false;
+        } else {
+          level_apply<T>(c, l, xc, EPI_CHEB, ep);
+          std::swap(xc, xo);
+        }
       }
+      dcur[l] = dc;
     } else {
       issue(c, [=](hipStream_t st) { TOFE_HIP(hipMemsetAsync(xc, 0, nd * nvec * sizeof(T), st)); });
     }
@@ -733,7 +751,7 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
     const double *c1 = &c->c1[(size_t)l * km], *c2 = &c->c2[(size_t)l * km];
     for (int k = 0; k < nu2; ++k) {
       EpiArgs<T> ep;
-      ep.b = b; ep.dinv = lv.dinv.p; ep.d = lv.d.p; ep.xout = xo;
+      ep.b = b; ep.dinv = lv.dinv.p; ep.d = dcur[l] ? dcur[l] : lv.d.p; ep.xout = xo;
       ep.c1 = (T)(k == 0 ? 0.0 : c1[k]); ep.c2 = (T)c2[k];
       ep.nvec = nvec;
       if (l == 0 && k == nu2 - 1 && mixed_io && io->x64) {

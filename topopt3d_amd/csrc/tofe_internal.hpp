@@ -51,6 +51,7 @@ struct EpiArgs {
   const T *b = nullptr;
   const T *dinv = nullptr;
   T *d = nullptr;
+  T *dout = nullptr;  // Chebyshev steps: where the new d goes (null: over d, in place)
   T *xout = nullptr;
   double *partial = nullptr;
   double *partial2 = nullptr;
@@ -300,7 +301,8 @@ void record_scalar(hipStream_t s, const double *scal, int slot, double *dst);
 // Optional output of fcg_update_r: the mixed V-cycle's first Chebyshev step on the new
 // residual (b32 = (float) r, d = x = c2 dinv b32), when b32 is set.
 struct FirstCheb {
-  float *b32 = nullptr, *d = nullptr, *x = nullptr;
+  float *b32 = nullptr, *d = nullptr;
+  float *x = nullptr;  // null: the iterate is not written (it equals d after the first step)
   const float *dinv = nullptr;
   float c2 = 0;
 };
