@@ -153,8 +153,10 @@ __device__ inline void apply_epilogue(const EpiArgs<T> &ep, int64_t dof, T Au, T
   } else if constexpr (EPI == EPI_CHEB_OUT) {
     const T r = ep.dinv[dof] * (ep.b[dof] - Au);
     const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * ep.d[dof] + ep.c2 * r;
-    const double x = (double)(uval + dn);
-    ep.xout64[dof] = x;
+    const T xt = uval + dn;
+    const double x = (double)xt;
+    if (ep.xout64) ep.xout64[dof] = x;
+    else ep.xout[dof] = xt;  // the same value, kept in the working precision
     dsum += x * ep.q64[dof];
   }
 }
@@ -229,8 +231,10 @@ __device__ inline void epi_store(const EpiArgs<T> &ep, int64_t n, const EpiIn<T,
     } else if constexpr (EPI == EPI_CHEB_OUT) {
       const T r = in.dinv[c] * (in.b[c] - Au);
       const T dn = (ep.c1 == T(0)) ? ep.c2 * r : ep.c1 * in.d[c] + ep.c2 * r;
-      const double x = (double)(uval + dn);
-      ep.xout64[dof] = x;
+      const T xt = uval + dn;
+      const double x = (double)xt;
+      if (ep.xout64) ep.xout64[dof] = x;
+      else ep.xout[dof] = xt;  // the same value, kept in the working precision
       dsum += x * in.q[c];
     }
   }

@@ -454,7 +454,9 @@ __global__ __launch_bounds__(NT) void k_fcg_xr_batch(const FcgXrLaunch *__restri
 
 // Second half: beta = -z.Ap / p.Ap from the z.q partials (block 0 stores z.Ap and beta);
 // x += alpha p with the first half's alpha, then p = z + beta p.
+// z comes in FP32 (z32, the FP32 V-cycle's last step, whose values are FP32 numbers) or FP64.
 __device__ __forceinline__ void fcg_p_block(double *__restrict__ p, const double *__restrict__ z,
+                                            const float *__restrict__ z32,
                                             double *__restrict__ x, int64_t n,
                                             const double *__restrict__ zq_part, int nparts,
                                             double *scal, int blk, int nb, bool pre = false)
@@ -463,7 +465,8 @@ __device__ __forceinline__ void fcg_p_block(double *__restrict__ p, const double
   {  // vector blockIdx.y
     const int64_t v = blockIdx.y, vo = v * n;
     p += vo;
-    z += vo;
+    if (z32) z32 += vo;
+    else z += vo;
     x += vo;
     zq_part += v * nparts;
     scal += v * SL_COUNT;
@@ -486,16 +489,18 @@ __device__ __forceinline__ void fcg_p_block(double *__restrict__ p, const double
     // displace from the memory-side cache (256 MB) the FP32 vectors and level-1 stencil that
     // the V-cycle reuses (measured: mixed solve of the 192x96x96 cantilever 6.24 -> 6.15 ms).
     __builtin_nontemporal_store(__builtin_nontemporal_load(x + i) + alpha * pi, x + i);
-    p[i] = z[i] + beta * pi;
+    const double zi = z32 ? (double)z32[i] : z[i];
+    p[i] = zi + beta * pi;
   }
 }
 
 __global__ __launch_bounds__(NT) void k_fcg_p(double *__restrict__ p, const double *__restrict__ z,
+                                              const float *__restrict__ z32,
                                               double *__restrict__ x, int64_t n,
                                               const double *__restrict__ zq_part, int nparts,
                                               double *scal)
 {
-  fcg_p_block(p, z, x, n, zq_part, nparts, scal, blockIdx.x, gridDim.x);
+  fcg_p_block(p, z, z32, x, n, zq_part, nparts, scal, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(NT) void k_fcg_p_batch(const FcgPLaunch *__restrict__ tab,
@@ -503,8 +508,8 @@ __global__ __launch_bounds__(NT) void k_fcg_p_batch(const FcgPLaunch *__restrict
 {
   const int e = batch_entry(b0, n, blockIdx.x);
   const FcgPLaunch L = tab[e];
-  fcg_p_block(L.p, L.z, L.x, L.n, L.zq_part, L.nparts, L.scal, blockIdx.x - b0[e], L.nblocks,
-              L.pre != 0);
+  fcg_p_block(L.p, L.z, L.z32, L.x, L.n, L.zq_part, L.nparts, L.scal, blockIdx.x - b0[e],
+              L.nblocks, L.pre != 0);
 }
 
 // x += alpha p with alpha = scal[SL_ALPHA]: the iterate's update when no k_fcg_p / k_cg_xp follows.
@@ -691,10 +696,10 @@ void fcg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const do
   TOFE_LAUNCH_CHECK();
 }
 
-void fcg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n,
-                   const double *zq_part, int nparts, double *scal, int nvec)
+void fcg_update_xp(hipStream_t s, double *p, const double *z, const float *z32, double *x,
+                   int64_t n, const double *zq_part, int nparts, double *scal, int nvec)
 {
-  k_fcg_p<<<dim3(stride_blocks(n), nvec), dim3(NT), 0, s>>>(p, z, x, n, zq_part, nparts, scal);
+  k_fcg_p<<<dim3(stride_blocks(n), nvec), dim3(NT), 0, s>>>(p, z, z32, x, n, zq_part, nparts, scal);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -606,6 +606,10 @@ struct VcycleIO {
   double *partial = nullptr;
   bool first_done = false;  // b0, d and the first iterate already hold the first step
   bool fused_out = false;   // set when the last step wrote x64 and the partials
+  // want_z32: the caller reads the result in FP32 (its values are FP32 numbers): the last step
+  // then writes it to the level's free FP32 buffer, returned in z32, instead of x64
+  bool want_z32 = false;
+  const float *z32 = nullptr;
 };
 
 // Where the mixed V-cycle's first fine-level Chebyshev step goes (b0, d, the iterate it
@@ -755,9 +759,12 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
       ep.c1 = (T)(k == 0 ? 0.0 : c1[k]); ep.c2 = (T)c2[k];
       ep.nvec = nvec;
       if (l == 0 && k == nu2 - 1 && mixed_io && io->x64) {
-        ep.xout64 = io->x64; ep.q64 = io->q64; ep.partial = io->partial;
+        ep.xout64 = io->want_z32 ? nullptr : io->x64;
+        ep.q64 = io->q64; ep.partial = io->partial;
         level_apply<T>(c, l, xc, EPI_CHEB_OUT, ep);
         io->fused_out = true;
+        if constexpr (sizeof(T) == 4)
+          if (io->want_z32) io->z32 = xo;
         return;
       }
       level_apply<T>(c, l, xc, EPI_CHEB, ep);
@@ -775,20 +782,26 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
 }
 
 // z = M^-1 r.  With q given (flexible CG), also the partial sums of z.q into c->partial;
-// returns how many there are.
+// returns how many there are.  With z32 given, a mixed V-cycle whose last step forms z.q may
+// leave z in FP32 instead (*z32 then points to it, else *z32 = null and z holds it).
 int precondition(tofe_ctx *c, const double *r, double *z, const double *q = nullptr,
-                 bool first_done = false, int nvec = 1)
+                 bool first_done = false, int nvec = 1, const float **z32 = nullptr)
 {
   const hipStream_t s = c->stream;
+  if (z32) *z32 = nullptr;
   if (c->built_precision == TOFE_PREC_MIXED) {
     VcycleIO io;
     io.b64 = r;
     io.first_done = first_done;
     if (q && c->L > 1 && !c->fine_mfma) {
       io.x64 = z; io.q64 = q; io.partial = c->partial.p;
+      io.want_z32 = z32 != nullptr;
     }
     vcycle<float>(c, c->vr32.p, c->vz32.p, &io, nvec);
-    if (io.fused_out) return level_blocks<float>(c, 0);
+    if (io.fused_out) {
+      if (z32) *z32 = io.z32;
+      return level_blocks<float>(c, 0);
+    }
     const float *z32 = c->vz32.p;
     const int64_t nd = c->ndof * nvec;
     issue(c, [=](hipStream_t st) { convert<float, double>(st, z32, z, nd); });
@@ -914,8 +927,9 @@ void solve_block(tofe_ctx *c, const double *F, double *U, int nvec, double rtol,
   bool x_pending = false;
   auto next_direction = [&] {
     if (flex) {
-      const int nparts = precondition(c, r, z, q, first.b32 != nullptr, nvec);
-      fcg_update_xp(s, p, z, U, n, c->partial.p, nparts, c->scal.p, nvec);
+      const float *z32 = nullptr;
+      const int nparts = precondition(c, r, z, q, first.b32 != nullptr, nvec, &z32);
+      fcg_update_xp(s, p, z, z32, U, n, c->partial.p, nparts, c->scal.p, nvec);
     } else {
       precondition(c, r, z, nullptr, false, nvec);
       dot_into(c, r, z, n, SL_TMP, POST_BETA_CG, nvec);
@@ -1591,17 +1605,18 @@ void iteration_on_device(tofe_ctx *c, double *U, int nvec)
                    nvec);
     }
     reduce_into(c, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_CONV_FCG, c->scal.p, nvec);
-    const int nparts = precondition(c, r, z, q, first.b32 != nullptr, nvec);
+    const float *z32 = nullptr;
+    const int nparts = precondition(c, r, z, q, first.b32 != nullptr, nvec, &z32);
     if (c->rec) {
       reduce_into(c, c->partial.p, nparts, c->scal.p + SL_ZQ, POST_BETA_FCG, c->scal.p, nvec);
       FcgPLaunch d;
-      d.p = p; d.z = z; d.x = U; d.n = n; d.zq_part = c->partial.p; d.nparts = nparts;
+      d.p = p; d.z = z; d.z32 = z32; d.x = U; d.n = n; d.zq_part = c->partial.p; d.nparts = nparts;
       d.scal = c->scal.p;
       d.nblocks = fcg_update_xp_blocks(n);
       d.pre = 1;
       rec_push(c, batch_key(BK_FCG_P, false), d, d.nblocks);
     } else {
-      fcg_update_xp(s, p, z, U, n, c->partial.p, nparts, c->scal.p, nvec);
+      fcg_update_xp(s, p, z, z32, U, n, c->partial.p, nparts, c->scal.p, nvec);
     }
   } else {
     level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);

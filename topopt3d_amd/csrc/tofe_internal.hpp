@@ -42,7 +42,8 @@ enum Epi : int {
   EPI_CHEB = 3,       // r = dinv (b - Au); d = c1 d + c2 r; xout = u + d
   EPI_APPLY_DOT2 = 4, // y = Au, partial += u.Au, partial2 += u.b
   EPI_CHEB_OUT = 5    // last Chebyshev step: x = u + (c1 d + c2 dinv (b - Au)) written in
-                      // FP64 to xout64, partial += x.q64 (d is not updated)
+                      // FP64 to xout64 (or, with xout64 null, to xout), partial += x.q64 (d is
+                      // not updated)
 };
 
 template <typename T>
@@ -309,8 +310,8 @@ struct FirstCheb {
 void fcg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *pq_part,
                   const double *pr_part, int nparts, double *scal, double *partial,
                   const FirstCheb &fc = FirstCheb(), int nvec = 1);
-void fcg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n,
-                   const double *zq_part, int nparts, double *scal, int nvec = 1);
+void fcg_update_xp(hipStream_t s, double *p, const double *z, const float *z32, double *x,
+                   int64_t n, const double *zq_part, int nparts, double *scal, int nvec = 1);
 void fcg_update_x(hipStream_t s, double *x, const double *p, int64_t n, const double *scal,
                   int nvec = 1);
 // Batched forms of the V-cycle's transfers and of the flexible-CG updates: one entry per
@@ -349,6 +350,7 @@ struct FcgXrLaunch {
 struct FcgPLaunch {
   double *p = nullptr;
   const double *z = nullptr;
+  const float *z32 = nullptr;  // z held in FP32 (the V-cycle's last step), or null: z
   double *x = nullptr;
   int64_t n = 0;
   const double *zq_part = nullptr;
