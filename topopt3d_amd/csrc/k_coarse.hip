@@ -560,28 +560,46 @@ __device__ __forceinline__ void trigemv_block(const T *__restrict__ A, int n, in
   y += (int64_t)blockIdx.y * n;
   extern __shared__ unsigned char xs_raw[];
   T *xs = reinterpret_cast<T *>(xs_raw);
-  for (int i = threadIdx.x; i < ld; i += NT) xs[i] = i < n ? x[i] : T(0);
-  __syncthreads();
   const int row = blk * (NT / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (row >= n) return;
+  const bool live = row < n;
   constexpr int V = 16 / (int)sizeof(T);  // values per 16-byte load
-  const int c_lo = LOWER ? 0 : row / V, c_hi = LOWER ? row / V + 1 : ld / V;
-  const T *a = A + (int64_t)row * ld;
+  typedef T V16 __attribute__((ext_vector_type(V)));
+  const int c_lo = LOWER ? 0 : (live ? row / V : 0), c_hi = LOWER ? (live ? row / V + 1 : 0) : ld / V;
+  const V16 *a4 = reinterpret_cast<const V16 *>(A + (int64_t)(live ? row : 0) * ld);
+  // The row's first U loads per lane are issued before x is staged, so their latency overlaps
+  // the staging and its barrier instead of following it one dependent load after another.
+  constexpr int U = 8;
+  V16 va[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int c = c_lo + lane + 64 * u;
+    if (live && c < c_hi) va[u] = a4[c];
+  }
+  for (int i = threadIdx.x; i < ld; i += NT) xs[i] = i < n ? x[i] : T(0);
+  __syncthreads();
+  if (!live) return;
+  const V16 *x4 = reinterpret_cast<const V16 *>(xs);
   double s = 0;
-  if constexpr (sizeof(T) == 4) {
-    const float4 *a4 = reinterpret_cast<const float4 *>(a);
-    const float4 *x4 = reinterpret_cast<const float4 *>(xs);
-    for (int c = c_lo + lane; c < c_hi; c += 64) {
-      const float4 v = a4[c], w = x4[c];
-      s += (double)(v.x * w.x) + (double)(v.y * w.y) + (double)(v.z * w.z) + (double)(v.w * w.w);
+  // the products are summed in ascending c per lane, as one load at a time would
+  for (int c0 = c_lo + lane; c0 < c_hi; c0 += 64 * U) {
+    if (c0 != c_lo + lane) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int c = c0 + 64 * u;
+        if (c < c_hi) va[u] = a4[c];
+      }
     }
-  } else {
-    const double2 *a2 = reinterpret_cast<const double2 *>(a);
-    const double2 *x2 = reinterpret_cast<const double2 *>(xs);
-    for (int c = c_lo + lane; c < c_hi; c += 64) {
-      const double2 v = a2[c], w = x2[c];
-      s += v.x * w.x + v.y * w.y;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = c0 + 64 * u;
+      if (c < c_hi) {
+        const V16 v = va[u], w = x4[c];
+        if constexpr (sizeof(T) == 4)
+          s += (double)(v.x * w.x) + (double)(v.y * w.y) + (double)(v.z * w.z) + (double)(v.w * w.w);
+        else
+          s += v.x * w.x + v.y * w.y;
+      }
     }
   }
   s = wave_sum(s);
