@@ -493,12 +493,19 @@ __global__ __launch_bounds__(NT) void k_stencil_op_sym_batch(const StencilLaunch
 // FP64 full stencil -> upper half (offsets 13..26) in the node-blocked layout, as To;
 // blockIdx.y is the stored value v (0..125), so no thread divides by nn.
 template <typename To>
-__global__ void k_stencil_upper(const double *__restrict__ S, int64_t nn, To *__restrict__ Su)
+__global__ __launch_bounds__(NT) void k_stencil_upper(const double *__restrict__ S, int64_t nn,
+                                                      To *__restrict__ Su)
 {
-  const int v = blockIdx.y;
-  const double *__restrict__ src = S + (int64_t)(13 * 9 + v) * nn;
-  for (int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x; n < nn; n += (int64_t)gridDim.x * NT)
-    Su[su_index(v, n)] = (To)src[n];
+  // One thread per node, all 126 values: a block's four 64-node groups of the node-blocked
+  // layout (32 KB each) are written in order and fill whole lines in the cache, where one value
+  // per block left 256-byte pieces 32 KB apart for other blocks to fill much later; the reads
+  // stay contiguous runs of NT values per stencil plane.
+  const int64_t n = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (n >= nn) return;
+  const double *__restrict__ src = S + (int64_t)(13 * 9) * nn + n;
+  To *__restrict__ dst = Su + su_index(0, n);
+#pragma unroll 9
+  for (int v = 0; v < 126; ++v) dst[(int64_t)v * 64] = (To)src[(int64_t)v * nn];
 }
 
 __global__ void k_stencil_diag(const double *__restrict__ S, int64_t nn, double *diag,
@@ -1136,8 +1143,7 @@ template <typename To>
 void stencil_upper(hipStream_t s, const double *S, int64_t nn, To *Su)
 {
   if (nn <= 0) return;
-  const unsigned nb = (unsigned)std::min<int64_t>((nn + NT - 1) / NT, 1024);
-  k_stencil_upper<To><<<dim3(nb, 126), dim3(NT), 0, s>>>(S, nn, Su);
+  k_stencil_upper<To><<<dim3((unsigned)((nn + NT - 1) / NT)), dim3(NT), 0, s>>>(S, nn, Su);
   TOFE_LAUNCH_CHECK();
 }
 
