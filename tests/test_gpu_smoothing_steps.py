@@ -59,7 +59,7 @@ def test_vcycle_and_solve_with_step_counts(oracle, tofe, nu1, nu2):
         ctx.close()
 
 
-@pytest.mark.parametrize("nu1,nu2", [(1, 2), (3, 1)])
+@pytest.mark.parametrize("nu1,nu2", [(0, 2), (1, 2), (3, 1)])
 def test_merged_batch_with_step_counts(tofe, nu1, nu2, monkeypatch):
     import torch
 

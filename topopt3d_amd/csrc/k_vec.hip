@@ -284,7 +284,8 @@ __global__ void k_convert(const Ta *__restrict__ a, Tb *__restrict__ b, int64_t 
     b[i] = (Tb)a[i];
 }
 
-__global__ void k_fill(double *a, double v, int64_t n)
+template <typename T>
+__global__ void k_fill(T *a, T v, int64_t n)
 {
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
     a[i] = v;
@@ -648,11 +649,16 @@ void convert(hipStream_t s, const Ta *a, Tb *b, int64_t n)
   TOFE_LAUNCH_CHECK();
 }
 
-void fill_d(hipStream_t s, double *a, double v, int64_t n)
+template <typename T>
+void fill(hipStream_t s, T *a, T v, int64_t n)
 {
-  k_fill<<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(a, v, n);
+  k_fill<T><<<dim3(stride_blocks(n)), dim3(NT), 0, s>>>(a, v, n);
   TOFE_LAUNCH_CHECK();
 }
+template void fill<float>(hipStream_t, float *, float, int64_t);
+template void fill<double>(hipStream_t, double *, double, int64_t);
+
+void fill_d(hipStream_t s, double *a, double v, int64_t n) { fill<double>(s, a, v, n); }
 
 void cg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *scal,
                  int alpha_slot, double *partial, int nvec)

@@ -707,7 +707,7 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
       }
       dcur[l] = dc;
     } else {
-      issue(c, [=](hipStream_t st) { TOFE_HIP(hipMemsetAsync(xc, 0, nd * nvec * sizeof(T), st)); });
+      issue(c, [=](hipStream_t st) { fill<T>(st, xc, T(0), nd * nvec); });
     }
     EpiArgs<T> er;
     er.y = lv.r.p; er.b = b;
@@ -3083,8 +3083,6 @@ int tofe_batch_solve_dev(tofe_batch *b, const double *const *d_F, double *const 
     for (int i = 0; i < n; ++i) {
       tofe_ctx *c = b->ctx[i];
       need_assembled(c);
-      if (c->desc.nu_pre < 1)
-        throw Error(TOFE_ERR_ARG, "batch solves need at least one pre-smoothing step");
       if (!d_F[i] || !d_U[i]) throw Error(TOFE_ERR_ARG, "null right-hand side or solution");
       ensure_nvec(c, nrhs);
     }

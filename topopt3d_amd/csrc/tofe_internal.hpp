@@ -284,6 +284,9 @@ void prolong_add(hipStream_t s, const T *xc, int Ncx, int Ncy, int Ncz, T *xf, i
 template <typename Ta, typename Tb>
 void convert(hipStream_t s, const Ta *a, Tb *b, int64_t n);
 void fill_d(hipStream_t s, double *a, double v, int64_t n);
+// a fill kernel, not hipMemsetAsync: it may be captured into a graph (no memset nodes there)
+template <typename T>
+void fill(hipStream_t s, T *a, T v, int64_t n);
 void cg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *scal,
                  int alpha_slot, double *partial, int nvec = 1);
 // x += scal[alpha_slot] p (the iterate's update, deferred to where p is read), p = z + beta p
