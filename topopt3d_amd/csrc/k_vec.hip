@@ -291,15 +291,17 @@ __global__ void k_fill(T *a, T v, int64_t n)
     a[i] = v;
 }
 
-// r -= alpha q; partial ||r||^2 (the iterate's update x += alpha p follows in k_cg_xp)
+// r -= alpha q; partial ||r||^2 (the iterate's update x += alpha p follows in k_cg_xp); with
+// fc.d, also the V-cycle's first fine-level Chebyshev step d = c2 D^-1 r (as k_cheb_zero).
 __global__ __launch_bounds__(NT) void k_cg_r(double *__restrict__ r, const double *__restrict__ q,
                                              int64_t n, const double *__restrict__ scal, int alpha_slot,
-                                             double *partial)
+                                             double *partial, FirstCheb64 fc)
 {
   __shared__ double sred[NT / 64];
   const int64_t vo = (int64_t)blockIdx.y * n;  // vector blockIdx.y
   r += vo;
   q += vo;
+  if (fc.d) fc.d += vo;
   scal += (int64_t)blockIdx.y * SL_COUNT;
   partial += (int64_t)blockIdx.y * gridDim.x;
   const double alpha = scal[alpha_slot];
@@ -308,6 +310,7 @@ __global__ __launch_bounds__(NT) void k_cg_r(double *__restrict__ r, const doubl
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     s += ri * ri;
+    if (fc.d) fc.d[i] = fc.c2 * (fc.dinv[i] * ri);
   }
   const double t = block_sum<NT>(s, sred);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -661,9 +664,9 @@ template void fill<double>(hipStream_t, double *, double, int64_t);
 void fill_d(hipStream_t s, double *a, double v, int64_t n) { fill<double>(s, a, v, n); }
 
 void cg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *scal,
-                 int alpha_slot, double *partial, int nvec)
+                 int alpha_slot, double *partial, int nvec, const FirstCheb64 &fc)
 {
-  k_cg_r<<<dim3(kRedBlocks, nvec), dim3(NT), 0, s>>>(r, q, n, scal, alpha_slot, partial);
+  k_cg_r<<<dim3(kRedBlocks, nvec), dim3(NT), 0, s>>>(r, q, n, scal, alpha_slot, partial, fc);
   TOFE_LAUNCH_CHECK();
 }
 

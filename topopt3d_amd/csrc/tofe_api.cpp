@@ -632,6 +632,18 @@ FirstCheb vcycle_first_step(tofe_ctx *c)
   return f;
 }
 
+// The FP64 V-cycle's first fine-level Chebyshev step for the CG residual update to form
+// (with a second pre-smoothing step, which then reads its operand from d); d == nullptr: none.
+FirstCheb64 vcycle_first_step64(tofe_ctx *c)
+{
+  FirstCheb64 f;
+  if (c->built_precision != TOFE_PREC_FP64 || c->L == 1 || c->desc.nu_pre < 2) return f;
+  f.d = c->lv64[0]->d.p;
+  f.dinv = c->lv64[0]->dinv.p;
+  f.c2 = c->c2[0];
+  return f;
+}
+
 // Multigrid V-cycle: x0 = M^-1 b0 (PCMG V-cycle, Chebyshev-Jacobi smoothers,
 // Galerkin coarse operators, direct coarse solve).
 template <typename T>
@@ -679,13 +691,13 @@ void vcycle(tofe_ctx *c, const T *b0, T *x0, VcycleIO *io = nullptr, int nvec = 
         else if (l == 0)
           issue(c, [=](hipStream_t st) { cheb_zero<T>(st, b, dinv, c20, d, xc, nd, nvec); });
       } else {
-        if (l == 0)
+        if (l == 0 && !(io && io->first_done))  // else formed by the CG residual update
           issue(c, [=](hipStream_t st) { cheb_zero<T>(st, b, dinv, c20, d, xc, nd, nvec); });
       }
       // After a first step formed without the iterate (vcycle_first_step), the iterate equals
       // d: the second step reads its operand from d and writes the new d to the iterate's unused
       // buffer (not over d, which other blocks read as operand), and d lives there from then on.
-      bool x_in_d = l == 0 && from64 && io->first_done && nu1 >= 2;
+      bool x_in_d = l == 0 && io && io->first_done && nu1 >= 2 && (sizeof(T) == 8 || from64);
       T *dc = lv.d.p;
       for (int k = 1; k < nu1; ++k) {
         EpiArgs<T> ep;
@@ -806,7 +818,9 @@ int precondition(tofe_ctx *c, const double *r, double *z, const double *q = null
     const int64_t nd = c->ndof * nvec;
     issue(c, [=](hipStream_t st) { convert<float, double>(st, z32, z, nd); });
   } else {
-    vcycle<double>(c, r, z, nullptr, nvec);
+    VcycleIO io;
+    io.first_done = first_done;  // the CG residual update formed the first fine-level step
+    vcycle<double>(c, r, z, first_done ? &io : nullptr, nvec);
   }
   if (!q) return 0;
   double *part = c->partial.p;
@@ -922,6 +936,8 @@ void solve_block(tofe_ctx *c, const double *F, double *U, int nvec, double rtol,
   // the device then never waits for the host's test.  A converged step leaves them unused.
   // mixed: the FCG update also forms the next V-cycle's first fine-level step
   const FirstCheb first = flex ? vcycle_first_step(c) : FirstCheb();
+  // CG: the residual update forms the next V-cycle's first fine-level step
+  const FirstCheb64 first64 = flex ? FirstCheb64() : vcycle_first_step64(c);
   // CG and flexible CG apply the iterate's update x += alpha p in the next search-direction
   // kernel, which reads p anyway; x_pending marks a step whose update is still owed
   bool x_pending = false;
@@ -931,7 +947,7 @@ void solve_block(tofe_ctx *c, const double *F, double *U, int nvec, double rtol,
       const int nparts = precondition(c, r, z, q, first.b32 != nullptr, nvec, &z32);
       fcg_update_xp(s, p, z, z32, U, n, c->partial.p, nparts, c->scal.p, nvec);
     } else {
-      precondition(c, r, z, nullptr, false, nvec);
+      precondition(c, r, z, nullptr, first64.d != nullptr, nvec);
       dot_into(c, r, z, n, SL_TMP, POST_BETA_CG, nvec);
       cg_update_xp(s, p, z, U, n, c->scal.p, SL_BETA, SL_ALPHA, nvec);
     }
@@ -959,7 +975,7 @@ void solve_block(tofe_ctx *c, const double *F, double *U, int nvec, double rtol,
     } else {
       level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
       reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p, nvec);
-      cg_update_r(s, r, q, n, c->scal.p, SL_ALPHA, c->partial.p, nvec);
+      cg_update_r(s, r, q, n, c->scal.p, SL_ALPHA, c->partial.p, nvec, first64);
       x_pending = true;
       reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p, nvec);
     }
@@ -1622,9 +1638,10 @@ void iteration_on_device(tofe_ctx *c, double *U, int nvec)
     level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
     reduce_into(c, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p, nvec);
     double *scal = c->scal.p, *part = c->partial.p;
-    issue(c, [=](hipStream_t st) { cg_update_r(st, r, q, n, scal, SL_ALPHA, part, nvec); });
+    const FirstCheb64 first64 = vcycle_first_step64(c);
+    issue(c, [=](hipStream_t st) { cg_update_r(st, r, q, n, scal, SL_ALPHA, part, nvec, first64); });
     reduce_into(c, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_CONV_CG, c->scal.p, nvec);
-    precondition(c, r, z, nullptr, false, nvec);
+    precondition(c, r, z, nullptr, first64.d != nullptr, nvec);
     dot_into(c, r, z, n, SL_TMP, POST_BETA_CG, nvec);
     issue(c, [=](hipStream_t st) { cg_update_xp(st, p, z, U, n, scal, SL_BETA, SL_ALPHA, nvec); });
   }

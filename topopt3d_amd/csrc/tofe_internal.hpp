@@ -287,8 +287,9 @@ void fill_d(hipStream_t s, double *a, double v, int64_t n);
 // a fill kernel, not hipMemsetAsync: it may be captured into a graph (no memset nodes there)
 template <typename T>
 void fill(hipStream_t s, T *a, T v, int64_t n);
+struct FirstCheb64;
 void cg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *scal,
-                 int alpha_slot, double *partial, int nvec = 1);
+                 int alpha_slot, double *partial, int nvec, const FirstCheb64 &fc);
 // x += scal[alpha_slot] p (the iterate's update, deferred to where p is read), p = z + beta p
 void cg_update_xp(hipStream_t s, double *p, const double *z, double *x, int64_t n, const double *scal,
                   int beta_slot, int alpha_slot, int nvec = 1);
@@ -309,6 +310,13 @@ struct FirstCheb {
   float *x = nullptr;  // null: the iterate is not written (it equals d after the first step)
   const float *dinv = nullptr;
   float c2 = 0;
+};
+// The FP64 V-cycle's first fine-level Chebyshev step formed by the CG residual update:
+// d = c2 D^-1 r (d == nullptr: none).  The iterate is not written (it equals d).
+struct FirstCheb64 {
+  double *d = nullptr;
+  const double *dinv = nullptr;
+  double c2 = 0;
 };
 void fcg_update_r(hipStream_t s, double *r, const double *q, int64_t n, const double *pq_part,
                   const double *pr_part, int nparts, double *scal, double *partial,
