@@ -168,7 +168,10 @@ int tofe_set_springs(tofe_ctx *ctx, const double *spk);
 /* Rebuild the solver for the current E: fine Jacobi diagonal, Galerkin coarse
  * operators P^T A P, Chebyshev eigenvalue bounds and the coarsest factorisation.
  * Replaces TopOpt::FEAssemble (FEAnalysis.cpp:432-490) plus PCSetUp of PCMG
- * (FEAnalysis.cpp:541, 555-573). */
+ * (FEAnalysis.cpp:541, 555-573).  Returns when the eigenvalue bounds are known; the
+ * coarsest factorisation may still be running (PETSc factors inside KSPSolve's PCSetUp):
+ * tofe_solve_dev overlaps its start with it and reports a failed factorisation
+ * (TOFE_ERR_NUMERIC) when it ends, every other call before it starts. */
 int tofe_assemble(tofe_ctx *ctx);
 
 /* Solve K U = F for nrhs right-hand sides stored one after another (ndof each),

@@ -3,7 +3,8 @@ a NaN design variable, a structure without Dirichlet DOFs (singular K), and an a
 The reference reports the first two through PETSc (KSP_DIVERGED_NANORINF, a failed factorisation
 of the coarse operator in PCSetUp; FEAnalysis.cpp:673-690 reads the KSPConvergedReason); the
 library either rejects the set-up with TOFE_ERR_NUMERIC (the coarse Cholesky meets a non-positive
-or NaN pivot) or returns a diverged reason from the solve.  An all-void design is not a failure:
+or NaN pivot; tofe_assemble leaves the factorisation running, so the call after it reports this, as
+PETSc's factorisation inside KSPSolve does) or returns a diverged reason from the solve.  An all-void design is not a failure:
 E = Emin everywhere, so K is Emin times the solid K and U scales by 1/Emin exactly up to the
 solve's tolerance."""
 import numpy as np
@@ -19,9 +20,9 @@ NEL = (16, 8, 8)
 def _solve_or_error(tofe, ctx, F):
     try:
         ctx.assemble()
+        U, its, reason = ctx.solve(F, rtol=1e-8, maxit=60)
     except tofe.TofeError as e:
         return "error", e.code
-    U, its, reason = ctx.solve(F, rtol=1e-8, maxit=60)
     return "reason", reason
 
 

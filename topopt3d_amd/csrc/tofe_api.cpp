@@ -264,6 +264,10 @@ struct tofe_ctx {
   DBuf<double> ebuf, ebuf2;
 
   bool assembled = false;
+  // The coarsest level's factorisation (second stream) after tofe_assemble: coarse_unjoined
+  // while the main stream has not waited for it, coarse_unchecked while its pivot report has
+  // not been read (coarse_join, coarse_settle).
+  bool coarse_unjoined = false, coarse_unchecked = false;
   bool fine_mfma = false;  // TOFE_FINE_MFMA=1: element-centric MFMA fine operator
   int64_t split_nodes = kStencilSplitNodes;  // TOFE_STENCIL_SPLIT_NODES: one thread per DOF below
   FineTile fine_tile{0, 8}, fine_tile32{0, 8};  // k_fine_op launch shapes, FP64 and FP32
@@ -566,6 +570,16 @@ void ensure_nvec(tofe_ctx *c, int nvec)
   TOFE_HIP(hipStreamSynchronize(c->stream));
 }
 
+// tofe_assemble leaves the coarsest level's factorisation running on the second stream (PETSc
+// factors inside the solve's PCSetUp): the main stream waits for it just before its first use,
+// so the solve's work up to the first coarse solve overlaps the factorisation's end.
+void coarse_join(tofe_ctx *c)
+{
+  if (!c->coarse_unjoined) return;
+  TOFE_HIP(hipStreamWaitEvent(c->stream, c->ev_inv, 0));
+  c->coarse_unjoined = false;
+}
+
 // Exact solve with the coarsest operator, x = A^-1 b = X^T (X b) with X = L^-1, in the
 // V-cycle's precision.
 template <typename T>
@@ -588,6 +602,7 @@ void coarse_apply(tofe_ctx *c, const T *b, T *x, int nvec = 1)
     rec_push(c, batch_key(BK_TRI_UP, sizeof(T) == 4), up, up.nblocks, up.ld);
     return;
   }
+  coarse_join(c);
   if constexpr (sizeof(T) == 4)
     coarse_solve_tri<float>(c->stream, c->X32r.p, c->XT32r.p, (int)c->nc, c->ldc, b, c->ctmp32.p, x,
                             nvec);
@@ -1273,6 +1288,22 @@ void coarse_inverse_finish(tofe_ctx *c)
                                       std::to_string(info[0]) + ")");
 }
 
+// Joins the factorisation and reads its pivot report: at the end of a solve, and at the start
+// of every other call on the context (its buffers and the second stream are then settled).  A
+// failed factorisation leaves the context unassembled.
+void coarse_settle(tofe_ctx *c)
+{
+  coarse_join(c);
+  if (!c->coarse_unchecked) return;
+  c->coarse_unchecked = false;
+  try {
+    coarse_inverse_finish(c);
+  } catch (...) {
+    c->assembled = false;
+    throw;
+  }
+}
+
 void assemble(tofe_ctx *c)
 {
   Range range(c->precision == TOFE_PREC_MIXED ? "assemble:mixed" : "assemble:fp64");
@@ -1369,8 +1400,8 @@ void assemble(tofe_ctx *c)
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz0, 0));
   lanczos_lambdas(c);
   chebyshev_coefficients(c);
-  TOFE_HIP(hipStreamWaitEvent(s, c->ev_inv, 0));
-  coarse_inverse_finish(c);
+  // the factorisation is joined by the first coarse solve and checked by coarse_settle
+  c->coarse_unjoined = c->coarse_unchecked = true;
   TOFE_HIP(hipStreamSynchronize(s));
   c->built_precision = c->precision;
   c->assembled = true;
@@ -1383,13 +1414,17 @@ void check_ctx(const tofe_ctx *c)
   TOFE_HIP(hipSetDevice(c->device));
 }
 
+// Runs an entry point: errors become return codes and the last error message.  The coarse
+// factorisation of the last tofe_assemble is settled first, except for the solves
+// (settle_first false), which settle it at their end.
 template <typename F>
-int guarded(tofe_ctx *c, const char *name, F &&f)
+int guarded(tofe_ctx *c, const char *name, F &&f, bool settle_first = true)
 {
   if (!c) return TOFE_ERR_ARG;
   Range range(name);
   try {
     check_ctx(c);
+    if (settle_first) coarse_settle(c);
     f();
     return TOFE_OK;
   } catch (const Error &e) {
@@ -1475,6 +1510,7 @@ void solve_dev(tofe_ctx *c, const double *dF, double *dU, int nrhs, double rtol,
     }
   }
   TOFE_HIP(hipStreamSynchronize(c->stream));
+  coarse_settle(c);
   c->stats.solve_seconds = now_s() - t0;
   c->stats.its = kmax;
   c->stats.reason = rs_out;
@@ -2144,7 +2180,8 @@ int tofe_assemble(tofe_ctx *c)
 int tofe_solve_dev(tofe_ctx *c, const double *dF, double *dU, int nrhs, double rtol, int maxit,
                    int *its, int *reason)
 {
-  return guarded(c, "tofe_solve_dev", [&] { solve_dev(c, dF, dU, nrhs, rtol, maxit, its, reason); });
+  return guarded(
+      c, "tofe_solve_dev", [&] { solve_dev(c, dF, dU, nrhs, rtol, maxit, its, reason); }, false);
 }
 
 int tofe_solve(tofe_ctx *c, const double *F, double *U, int nrhs, double rtol, int maxit,
@@ -3099,6 +3136,8 @@ int tofe_batch_solve_dev(tofe_batch *b, const double *const *d_F, double *const 
       throw Error(TOFE_ERR_ARG, "bad batch solve arguments");
     for (int i = 0; i < n; ++i) {
       tofe_ctx *c = b->ctx[i];
+      TOFE_HIP(hipSetDevice(c->device));
+      coarse_settle(c);  // before any recording or capture of its iteration
       need_assembled(c);
       if (!d_F[i] || !d_U[i]) throw Error(TOFE_ERR_ARG, "null right-hand side or solution");
       ensure_nvec(c, nrhs);
