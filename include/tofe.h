@@ -107,7 +107,8 @@ typedef struct tofe_stats {
   int reason;              /* KSPConvergedReason of the last solve */
   double rnorm;            /* final unpreconditioned residual 2-norm */
   double bnorm;            /* ||F|| of the last solve */
-  double setup_seconds;    /* last tofe_assemble (Galerkin, diagonals, eigen bounds, coarse factor) */
+  double setup_seconds;    /* last tofe_assemble (Galerkin, diagonals, eigen bounds; the coarse
+                              factorisation it starts may end inside the next solve) */
   double solve_seconds;    /* last tofe_solve wall time */
   int fine_form;           /* fine operator kernel: 0 node-centric with KE rows, 1 node-centric
                               with KE's cube-orbit values, 2 element-centric Walsh-Hadamard
