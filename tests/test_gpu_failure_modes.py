@@ -78,3 +78,39 @@ def test_all_void_design_scales_the_solid_solution(tofe, precision):
     void, solid = out
     # E = Emin (void) and E = Emin + (1 - Emin) = 1 (solid): K_void = Emin K_solid
     assert np.abs(void * Emin - solid).max() <= 1e-8 * np.abs(solid).max()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_solve_reports_the_coarse_factorisation(tofe, precision):
+    """tofe_assemble returns before the coarsest factorisation ends; tofe_solve_dev waits for it
+    inside its first V-cycle and reads its pivot report when it ends.  A NaN design makes the
+    factorisation fail (or the solve diverge); a reported failure leaves the context unassembled
+    until the next tofe_assemble, and a finite design then solves again."""
+    import torch
+
+    print(f"case device solve after failed factorisation precision={precision}", flush=True)
+    pb = configs.cantilever(NEL, seed=3, lo=0.05, hi=1.0, mg_levels=3)
+    x = pb.x.copy()
+    x[123] = np.nan
+    ctx = tofe.Context(NEL, mg_levels=3, fixed_dof=pb.fixed_dof, precision=precision)
+    ctx.set_design(x)
+    ctx.assemble()  # the pivot report is not read here
+    F = torch.from_numpy(pb.F).cuda()
+    U = torch.zeros_like(F)
+    try:
+        its, reason = ctx.solve_dev(F.data_ptr(), U.data_ptr(), rtol=1e-8, maxit=60)
+        outcome = ("reason", reason)
+    except tofe.TofeError as e:
+        outcome = ("error", e.code)
+    print(f"outcome {outcome}", flush=True)
+    assert outcome in (("error", tofe.TOFE_ERR_NUMERIC), ("reason", tofe.DIVERGED_NANORINF))
+    if outcome[0] == "error":
+        with pytest.raises(tofe.TofeError) as err:
+            ctx.solve_dev(F.data_ptr(), U.data_ptr(), rtol=1e-8, maxit=60)
+        assert err.value.code == tofe.TOFE_ERR_STATE
+    ctx.set_design(pb.x)
+    ctx.assemble()
+    U.zero_()
+    its, reason = ctx.solve_dev(F.data_ptr(), U.data_ptr(), rtol=1e-8)
+    assert reason == tofe.CONVERGED_RTOL and torch.isfinite(U).all()
+    ctx.close()
