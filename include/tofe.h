@@ -89,7 +89,9 @@ typedef struct tofe_desc {
   int smoother;            /* TOFE_SMOOTHER_* */
   int nu_pre, nu_post;     /* smoothing steps (V(nu_pre, nu_post)); 0 -> 2 */
   double cheb_lo, cheb_hi; /* Chebyshev interval as fractions of lambda_max(D^-1 A); 0 -> 0.1, 1.1 */
-  int eig_its;             /* power-iteration steps for lambda_max; 0 -> 10 */
+  int eig_its;             /* Jacobi-CG (Lanczos) steps of each level's lambda_max estimate,
+                              1..64 (the reference's runtime smoother options,
+                              FEAnalysis.cpp:497-499); 0 -> 10 */
   int precision;           /* TOFE_PREC_*: FP64, or FP32 V-cycle inside FP64 flexible CG */
   const uint8_t *fixed_dof;/* 3*nnode flags, 1 = Dirichlet DOF (fixedDof, FEAnalysis.cpp:236-246); may be NULL */
   /* maximum length scale and interpolation (Material_Interpolation.cpp:37-116) */

@@ -86,3 +86,41 @@ def test_merged_batch_with_step_counts(tofe, nu1, nu2, monkeypatch):
     batch.close()
     for c in ctxs:
         c.close()
+
+
+@pytest.mark.parametrize("eig_its,lo,hi", [(4, 0.1, 1.1), (7, 0.05, 1.2), (20, 0.2, 1.0)])
+def test_estimate_steps_and_chebyshev_interval(oracle, tofe, eig_its, lo, hi):
+    """The smoother settings the reference takes as runtime options (FEAnalysis.cpp:497-499):
+    the number of Jacobi-CG steps of each level's lambda_max estimate and the Chebyshev interval
+    [lo, hi] * lambda.  The GPU's level estimates, V-cycle and FP64 solve against the oracle's
+    with the same settings (the estimate's tridiagonal has eig_its rows, so a wrong step count or
+    bound moves lambda or the coefficients well beyond the tolerances); the mixed build within
+    FP32 rounding.  More steps than the estimate's buffers hold is rejected at create."""
+    print(f"case eig_its={eig_its} interval=({lo}, {hi})", flush=True)
+    pb = configs.cantilever(NEL, seed=11, lo=0.05, hi=1.0, mg_levels=L)
+    E = oracle.simp(oracle.filter_csr(*NEL, pb.rmin).matvec(pb.x), pb.penal, 1e-10)[0]
+    K = oracle.assemble_K(*NEL, oracle.ke(), E, pb.fixed_dof)
+    mg = oracle.MG(K, tuple(n + 1 for n in NEL), L, lo=lo, hi=hi, eig_its=eig_its)
+    r = np.random.default_rng(5).standard_normal(pb.ndof)
+    r[pb.fixed_dof.astype(bool)] = 0
+    zo = mg.vcycle(r)
+    _, its_o, reason_o, _ = mg.solve(pb.F, rtol=1e-10, maxit=200)
+    for precision, ltol, ztol in ((0, 1e-10, 1e-11), (1, 1e-4, 1e-4)):
+        ctx = tofe.Context(NEL, rmin=pb.rmin, penal=pb.penal, mg_levels=L, fixed_dof=pb.fixed_dof,
+                           precision=precision, cheb_lo=lo, cheb_hi=hi, eig_its=eig_its)
+        ctx.set_design(pb.x)
+        ctx.assemble()
+        for lvl in range(L - 1):
+            assert ctx.level_lambda(lvl) == pytest.approx(mg.lam(lvl), rel=ltol)
+        assert _rel(ctx.pc_apply(r), zo) <= ztol
+        U, its, reason = ctx.solve(pb.F, rtol=1e-10, maxit=200)
+        print(f"precision {precision}: its {its} reason {reason}; oracle CG its {its_o} "
+              f"reason {reason_o}", flush=True)
+        if precision == 0:
+            assert reason == reason_o
+            assert abs(its - its_o) <= 1
+        elif reason_o == tofe.CONVERGED_RTOL:
+            assert reason == tofe.CONVERGED_RTOL
+        ctx.close()
+    with pytest.raises(tofe.TofeError):
+        tofe.Context(NEL, mg_levels=L, fixed_dof=pb.fixed_dof, eig_its=65)

@@ -1756,6 +1756,7 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
   if (!(d.cheb_hi > 0)) d.cheb_hi = 1.1;
   if (!(d.cheb_lo > 0)) d.cheb_lo = 0.1;
   if (d.eig_its <= 0) d.eig_its = 10;
+  if (d.eig_its > kMaxEig || !(d.cheb_lo < d.cheb_hi)) return TOFE_ERR_ARG;
   if (d.penal == 0) d.penal = 3.0;
   if (d.vd_penal == 0) d.vd_penal = 1.0;
   if (d.interpolation < TOFE_INTERP_SIMP || d.interpolation > TOFE_INTERP_SIMP_LOGISTIC)
