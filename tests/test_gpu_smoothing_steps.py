@@ -124,3 +124,38 @@ def test_estimate_steps_and_chebyshev_interval(oracle, tofe, eig_its, lo, hi):
         ctx.close()
     with pytest.raises(tofe.TofeError):
         tofe.Context(NEL, mg_levels=L, fixed_dof=pb.fixed_dof, eig_its=65)
+
+
+@pytest.mark.parametrize("nu1,nu2", [(2, 2), (1, 2)])
+def test_damped_jacobi_smoother(oracle, tofe, nu1, nu2):
+    """The level smoother as damped Jacobi, x += (2 / (lo + hi) lambda) D^-1 (b - A x) (the
+    Chebyshev iteration's first step repeated; PETSc's -kuf_mg_levels_ksp_type richardson with a
+    Jacobi PC, FEAnalysis.cpp:497-499): level estimates, V-cycle and FP64 solve against the
+    oracle's with smoother 1; the mixed build within FP32 rounding and converged."""
+    print(f"case Jacobi V({nu1},{nu2})", flush=True)
+    pb = configs.cantilever(NEL, seed=13, lo=0.05, hi=1.0, mg_levels=L)
+    E = oracle.simp(oracle.filter_csr(*NEL, pb.rmin).matvec(pb.x), pb.penal, 1e-10)[0]
+    K = oracle.assemble_K(*NEL, oracle.ke(), E, pb.fixed_dof)
+    mg = oracle.MG(K, tuple(n + 1 for n in NEL), L, smoother=1, nu1=nu1, nu2=nu2)
+    r = np.random.default_rng(6).standard_normal(pb.ndof)
+    r[pb.fixed_dof.astype(bool)] = 0
+    zo = mg.vcycle(r)
+    _, its_o, reason_o, _ = mg.solve(pb.F, rtol=1e-10, maxit=300)
+    for precision, ltol, ztol in ((0, 1e-10, 1e-11), (1, 1e-4, 1e-4)):
+        ctx = tofe.Context(NEL, rmin=pb.rmin, penal=pb.penal, mg_levels=L, fixed_dof=pb.fixed_dof,
+                           precision=precision, smoother=tofe.SMOOTHER_JACOBI, nu_pre=nu1,
+                           nu_post=nu2)
+        ctx.set_design(pb.x)
+        ctx.assemble()
+        for lvl in range(L - 1):
+            assert ctx.level_lambda(lvl) == pytest.approx(mg.lam(lvl), rel=ltol)
+        assert _rel(ctx.pc_apply(r), zo) <= ztol
+        U, its, reason = ctx.solve(pb.F, rtol=1e-10, maxit=300)
+        print(f"precision {precision}: its {its} reason {reason}; oracle CG its {its_o} "
+              f"reason {reason_o}", flush=True)
+        if precision == 0:
+            assert reason == reason_o
+            assert abs(its - its_o) <= 1
+        elif reason_o == tofe.CONVERGED_RTOL:
+            assert reason == tofe.CONVERGED_RTOL
+        ctx.close()
