@@ -12,7 +12,8 @@ from topopt3d_amd import configs
 
 pytestmark = pytest.mark.gpu
 
-CASES = [((8, 4, 4), 2), ((7, 5, 3), 2), ((9, 6, 5), 3), ((16, 8, 8), 3), ((48, 24, 24), 3)]
+CASES = [((8, 4, 4), 2), ((7, 5, 3), 2), ((9, 6, 5), 3), ((16, 8, 8), 3), ((48, 24, 24), 3),
+         ((32, 16, 16), 5)]
 
 
 def rel(a, b):
