@@ -1349,6 +1349,7 @@ void assemble(tofe_ctx *c)
     direct_coarse(c->stream2, dc, c->Ainv.p);
     coarse_inverse_enqueue(c);
     TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
+    c->coarse_unjoined = c->coarse_unchecked = true;
   }
   if (L > 1) {
     MaskedElems me;
@@ -1386,6 +1387,7 @@ void assemble(tofe_ctx *c)
     TOFE_HIP(hipStreamWaitEvent(c->stream2, c->ev_dense, 0));
     coarse_inverse_enqueue(c);
     TOFE_HIP(hipEventRecord(c->ev_inv, c->stream2));
+    c->coarse_unjoined = c->coarse_unchecked = true;
   }
   // The fine level's Lanczos estimate needs E and the fine diagonal only, but the fourth stream
   // starts it after the Galerkin chain: the chain feeds the other estimates, and the estimate's
@@ -1400,8 +1402,8 @@ void assemble(tofe_ctx *c)
   TOFE_HIP(hipStreamWaitEvent(s, c->ev_lz0, 0));
   lanczos_lambdas(c);
   chebyshev_coefficients(c);
-  // the factorisation is joined by the first coarse solve and checked by coarse_settle
-  c->coarse_unjoined = c->coarse_unchecked = true;
+  // the factorisation (flagged where it was enqueued, so that a call after a failure here
+  // still waits for it) is joined by the first coarse solve and checked by coarse_settle
   TOFE_HIP(hipStreamSynchronize(s));
   c->built_precision = c->precision;
   c->assembled = true;
