@@ -520,8 +520,9 @@ __global__ void k_stencil_diag(const double *__restrict__ S, int64_t nn, double 
   dinv[t] = 1.0 / d;
 }
 
+// inv (may be null): the dense matrix's index of each natural DOF 3 node + c
 __global__ void k_stencil_dense(const double *__restrict__ S, int Nx, int Ny, int Nz,
-                                double *__restrict__ A)
+                                double *__restrict__ A, const int *__restrict__ inv)
 {
   const int64_t nn = (int64_t)Nx * Ny * Nz, n = 3 * nn;
   const int64_t t = (int64_t)blockIdx.x * NT + threadIdx.x;
@@ -537,7 +538,10 @@ __global__ void k_stencil_dense(const double *__restrict__ S, int Nx, int Ny, in
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b) A[(3 * I + a) * n + 3 * J + b] = S[(int64_t)(o * 9 + a * 3 + b) * nn + I];
+    for (int b = 0; b < 3; ++b) {
+      const int64_t r = inv ? inv[3 * I + a] : 3 * I + a, c = inv ? inv[3 * J + b] : 3 * J + b;
+      A[r * n + c] = S[(int64_t)(o * 9 + a * 3 + b) * nn + I];
+    }
 }
 
 // X = L^-1 (column-major, lower triangular; entries above the diagonal are not stored)
@@ -558,10 +562,12 @@ __global__ void k_tri_pack(const double *__restrict__ X, int n, int ld, T *__res
 // y = A x for a row-major triangular A (padded rows of ld, zeros outside the triangle):
 // LOWER rows use columns [0, row], upper rows [row, n).  Four rows per block (one per
 // wavefront), x staged in LDS, 16-byte loads along the row, products summed in FP64 in a
-// fixed order.
+// fixed order.  perm (may be null) maps the triangles' order to the vectors' natural order:
+// the LOWER product gathers x[perm[i]], the upper one writes y[perm[row]].
 template <typename T, bool LOWER>
 __device__ __forceinline__ void trigemv_block(const T *__restrict__ A, int n, int ld,
-                                              const T *__restrict__ x, T *__restrict__ y, int blk)
+                                              const T *__restrict__ x, T *__restrict__ y, int blk,
+                                              const int *__restrict__ perm)
 {
   x += (int64_t)blockIdx.y * n;  // vector blockIdx.y of a block of right-hand sides
   y += (int64_t)blockIdx.y * n;
@@ -583,7 +589,8 @@ __device__ __forceinline__ void trigemv_block(const T *__restrict__ A, int n, in
     const int c = c_lo + lane + 64 * u;
     if (live && c < c_hi) va[u] = a4[c];
   }
-  for (int i = threadIdx.x; i < ld; i += NT) xs[i] = i < n ? x[i] : T(0);
+  for (int i = threadIdx.x; i < ld; i += NT)
+    xs[i] = i < n ? x[(LOWER && perm) ? perm[i] : i] : T(0);
   __syncthreads();
   if (!live) return;
   const V16 *x4 = reinterpret_cast<const V16 *>(xs);
@@ -610,14 +617,15 @@ __device__ __forceinline__ void trigemv_block(const T *__restrict__ A, int n, in
     }
   }
   s = wave_sum(s);
-  if (lane == 0) y[row] = (T)s;
+  if (lane == 0) y[(!LOWER && perm) ? perm[row] : row] = (T)s;
 }
 
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(NT) void k_trigemv(const T *__restrict__ A, int n, int ld,
-                                                const T *__restrict__ x, T *__restrict__ y)
+                                                const T *__restrict__ x, T *__restrict__ y,
+                                                const int *__restrict__ perm)
 {
-  trigemv_block<T, LOWER>(A, n, ld, x, y, blockIdx.x);
+  trigemv_block<T, LOWER>(A, n, ld, x, y, blockIdx.x, perm);
 }
 
 // Several instances' products in one grid (the dynamic LDS holds the largest ld).
@@ -627,7 +635,7 @@ __global__ __launch_bounds__(NT) void k_trigemv_batch(const TrigemvLaunch<T> *__
 {
   const int e = batch_entry(b0, n, blockIdx.x);
   const TrigemvLaunch<T> L = tab[e];
-  trigemv_block<T, LOWER>(L.A, L.n, L.ld, L.x, L.y, blockIdx.x - b0[e]);
+  trigemv_block<T, LOWER>(L.A, L.n, L.ld, L.x, L.y, blockIdx.x - b0[e], L.perm);
 }
 
 // ---- dense coarse solve: banded blocked Cholesky, X = L^-1, A^-1 = X^T X ------------
@@ -1176,11 +1184,13 @@ void stencil_diag(hipStream_t s, const double *S, int64_t nn, double *diag, doub
   TOFE_LAUNCH_CHECK();
 }
 
-void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A)
+void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A,
+                      const int *inv)
 {
   const int64_t nn = (int64_t)Nx * Ny * Nz;
   TOFE_HIP(hipMemsetAsync(A, 0, sizeof(double) * 9 * nn * nn, s));
-  k_stencil_dense<<<dim3((unsigned)((27 * nn + NT - 1) / NT)), dim3(NT), 0, s>>>(S, Nx, Ny, Nz, A);
+  k_stencil_dense<<<dim3((unsigned)((27 * nn + NT - 1) / NT)), dim3(NT), 0, s>>>(S, Nx, Ny, Nz, A,
+                                                                                 inv);
   TOFE_LAUNCH_CHECK();
 }
 
@@ -1194,15 +1204,15 @@ void tri_pack(hipStream_t s, const double *X, int n, int ld, T *Xr, T *XTr)
 
 template <typename T>
 void coarse_solve_tri(hipStream_t s, const T *Xr, const T *XTr, int n, int ld, const T *b, T *tmp, T *x,
-                      int nvec)
+                      int nvec, const int *perm)
 {
   if (ld % (16 / (int)sizeof(T)) != 0 || ld < n)
     throw Error(TOFE_ERR_ARG, "coarse solve: bad padded leading dimension");
   const dim3 grid((unsigned)((n + 3) / 4), nvec), block(NT);
   const size_t lds = sizeof(T) * (size_t)ld;
-  k_trigemv<T, true><<<grid, block, lds, s>>>(Xr, n, ld, b, tmp);
+  k_trigemv<T, true><<<grid, block, lds, s>>>(Xr, n, ld, b, tmp, perm);
   TOFE_LAUNCH_CHECK();
-  k_trigemv<T, false><<<grid, block, lds, s>>>(XTr, n, ld, tmp, x);
+  k_trigemv<T, false><<<grid, block, lds, s>>>(XTr, n, ld, tmp, x, perm);
   TOFE_LAUNCH_CHECK();
 }
 
@@ -1215,9 +1225,9 @@ template void stencil_op<float>(hipStream_t, const StencilOp<float> &, const flo
 template void tri_pack<double>(hipStream_t, const double *, int, int, double *, double *);
 template void tri_pack<float>(hipStream_t, const double *, int, int, float *, float *);
 template void coarse_solve_tri<double>(hipStream_t, const double *, const double *, int, int, const double *,
-                                       double *, double *, int);
+                                       double *, double *, int, const int *);
 template void coarse_solve_tri<float>(hipStream_t, const float *, const float *, int, int, const float *,
-                                      float *, float *, int);
+                                      float *, float *, int, const int *);
 
 template void stencil_sym_batch<float>(hipStream_t, Epi, const StencilLaunch<float> *, const int *,
                                        int, int, int);

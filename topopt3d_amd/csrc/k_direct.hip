@@ -162,13 +162,16 @@ __global__ __launch_bounds__(NT) void k_direct_cells(const double *__restrict__ 
 }
 
 // Dense coarsest matrix (column-major, n = 3 Ncx Ncy Ncz): entry (i, j) sums the cell matrices
-// of the cells holding both nodes, in cell order.
+// of the cells holding both nodes, in cell order.  perm (may be null): the natural DOF of each
+// row and column of the dense order.
 __global__ __launch_bounds__(NT) void k_direct_dense(const double *__restrict__ Cc, int Ncx, int Ncy,
-                                                     int Ncz, int n, double *__restrict__ A)
+                                                     int Ncz, int n, double *__restrict__ A,
+                                                     const int *__restrict__ perm)
 {
   const int64_t idx = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (idx >= (int64_t)n * n) return;
-  const int i = (int)(idx % n), j = (int)(idx / n);
+  const int p = (int)(idx % n), q = (int)(idx / n);
+  const int i = perm ? perm[p] : p, j = perm ? perm[q] : q;
   const int I = i / 3, a = i % 3, J = j / 3, b = j % 3;
   const int Ix = I % Ncx, Iy = (I / Ncx) % Ncy, Iz = I / (Ncx * Ncy);
   const int Jx = J % Ncx, Jy = (J / Ncx) % Ncy, Jz = J / (Ncx * Ncy);
@@ -251,7 +254,8 @@ void direct_coarse(hipStream_t s, const DirectCoarse &d, double *A)
   TOFE_LAUNCH_CHECK();
   const int Ncx = ncx + 1, Ncy = ncy + 1, Ncz = ncz + 1, n = 3 * Ncx * Ncy * Ncz;
   const int64_t nn = (int64_t)n * n;
-  k_direct_dense<<<dim3((unsigned)((nn + NT - 1) / NT)), dim3(NT), 0, s>>>(d.cells, Ncx, Ncy, Ncz, n, A);
+  k_direct_dense<<<dim3((unsigned)((nn + NT - 1) / NT)), dim3(NT), 0, s>>>(d.cells, Ncx, Ncy, Ncz, n, A,
+                                                                          d.perm);
   TOFE_LAUNCH_CHECK();
 }
 

@@ -136,6 +136,7 @@ struct TrigemvLaunch {
   const T *x = nullptr;
   T *y = nullptr;
   int nblocks = 0;
+  const int *perm = nullptr;  // triangles' order -> natural order of x and y (may be null)
 };
 // Batched launches (one grid, nblocks_total blocks, nvec rows of vectors): table `tab` and
 // block offsets `b0` (n + 1 entries) in device memory.
@@ -182,7 +183,9 @@ void galerkin(hipStream_t s, const double *Sf, int Nfx, int Nfy, int Nfz, double
               double *T2);
 int64_t galerkin_scratch(int Nfx, int Nfy, int Nfz, int64_t *n2);
 void stencil_diag(hipStream_t s, const double *S, int64_t nn, double *diag, double *dinv);
-void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A);
+// inv (may be null): the dense matrix's index of each natural DOF
+void stencil_to_dense(hipStream_t s, const double *S, int Nx, int Ny, int Nz, double *A,
+                      const int *inv = nullptr);
 // ---- coarsest operator straight from the fine level (k_direct.hip) ---------
 // Valid when levels >= 2 and every element count is a multiple of r = 2^(levels-1) <= 16.
 constexpr int kDirectMaxR = 16;
@@ -203,6 +206,7 @@ struct DirectCoarse {
   const double *cconst = nullptr;  // cells x 576: fixed DOFs' unit diagonal and springs
   double *part = nullptr, *cells = nullptr, *corr = nullptr;  // scratch (direct_coarse_chunks)
   int nx = 0, ny = 0, nz = 0, r = 0;
+  const int *perm = nullptr;  // dense order -> natural DOF of the coarsest grid (may be null)
 };
 // Host tables of the Dirichlet part (fixmask: per node, bit a = component a fixed; may be null).
 struct DirectStatic {
@@ -231,7 +235,7 @@ void tri_pack(hipStream_t s, const double *X, int n, int ld, T *Xr, T *XTr);
 // hold nvec vectors of n values one after another.
 template <typename T>
 void coarse_solve_tri(hipStream_t s, const T *Xr, const T *XTr, int n, int ld, const T *b, T *tmp, T *x,
-                      int nvec = 1);
+                      int nvec = 1, const int *perm = nullptr);
 
 // ---- level-1 Galerkin operator from E and KE (k_galerkin.hip) -------------
 struct MaskedElems {
