@@ -1222,8 +1222,8 @@ void chebyshev_coefficients(tofe_ctx *c)
 // A (dense SPD, column-major, in c->Ainv) -> X = L^-1 on the second stream: banded blocked
 // Cholesky (one launch per 64-column step, each also forming the previous row block of X),
 // then X and X^T packed as padded row-major triangles for the per-V-cycle solve
-// X^T (X b).  The coarse operator couples DOFs at most bw apart (27-point stencil,
-// x-fastest nodes) and the factor's fill stays inside that band.
+// X^T (X b).  The coarse operator couples DOFs at most bw apart in the dense order of
+// coarse_order (27-point stencil) and the factor's fill stays inside that band.
 // Tables of the direct coarsest operator (k_direct.hip) when TOFE_DIRECT_COARSE=1 and the mesh
 // allows it: M_p, the Dirichlet elements' table rows, the cells' constant part.  Off by default:
 // the set-up is bound by the bandwidth of the Galerkin chain and the Lanczos estimates, which the
