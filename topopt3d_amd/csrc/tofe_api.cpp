@@ -1470,8 +1470,8 @@ void check_ctx(const tofe_ctx *c)
 }
 
 // Runs an entry point: errors become return codes and the last error message.  The coarse
-// factorisation of the last tofe_assemble is settled first, except for the solves
-// (settle_first false), which settle it at their end.
+// factorisation of the last tofe_assemble is settled first, except for tofe_solve_dev
+// (settle_first false), which settles it at its end.
 template <typename F>
 int guarded(tofe_ctx *c, const char *name, F &&f, bool settle_first = true)
 {
