@@ -89,7 +89,7 @@ constexpr int kMaxEig = 64;
 // Right-hand sides per block pass: solves, V-cycles and operator applies with more vectors
 // run in chunks of this many (each chunk one launch per kernel).
 constexpr int kMaxBlockVec = 16;
-constexpr int kCoarseCUs = 64;  // compute units left to the coarse factorisation during set-up
+constexpr int kCoarseCUs = 32;  // compute units left to the coarse factorisation during set-up
 
 double now_s()
 {
