@@ -27,9 +27,11 @@ __global__ __launch_bounds__(NT) void k_dot_partial(const T *__restrict__ a,
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// mirror (may be null): device address of host memory that receives the vector's SL_COUNT
+// scalars once they are final, so the host reads them without a copy launch.
 __device__ __forceinline__ void reduce_final_block(const double *__restrict__ partial,
                                                    int nparts, double *out, int post,
-                                                   double *scal)
+                                                   double *scal, double *mirror = nullptr)
 {
   __shared__ double sred[NT / 64];
   // vector blockIdx.y: its partials, and its scalars (out included) SL_COUNT further on
@@ -69,14 +71,18 @@ __device__ __forceinline__ void reduce_final_block(const double *__restrict__ pa
         break;
       default: break;
     }
+    if (mirror && scal) {
+      double *mv = mirror + (int64_t)blockIdx.y * SL_COUNT;
+      for (int i = 0; i < SL_COUNT; ++i) mv[i] = scal[i];
+    }
   }
 }
 
 __global__ __launch_bounds__(NT) void k_reduce_final(const double *__restrict__ partial,
                                                      int nparts, double *out, int post,
-                                                     double *scal)
+                                                     double *scal, double *mirror)
 {
-  reduce_final_block(partial, nparts, out, post, scal);
+  reduce_final_block(partial, nparts, out, post, scal, mirror);
 }
 
 __global__ __launch_bounds__(NT) void k_reduce_final_batch(const ReduceLaunch *__restrict__ tab,
@@ -590,9 +596,9 @@ void dot_partial(hipStream_t s, const T *a, const T *b, int64_t n, double *parti
 }
 
 void reduce_final(hipStream_t s, const double *partial, int nparts, double *out, int post,
-                  double *scal, int nvec)
+                  double *scal, int nvec, double *mirror)
 {
-  k_reduce_final<<<dim3(1, nvec), dim3(NT), 0, s>>>(partial, nparts, out, post, scal);
+  k_reduce_final<<<dim3(1, nvec), dim3(NT), 0, s>>>(partial, nparts, out, post, scal, mirror);
   TOFE_LAUNCH_CHECK();
 }
 

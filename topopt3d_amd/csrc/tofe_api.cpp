@@ -186,7 +186,8 @@ struct tofe_ctx {
   bool shared_helpers = false;
   hipEvent_t ev_dense = nullptr, ev_inv = nullptr, ev_ops = nullptr, ev_lz = nullptr;
   hipEvent_t ev_scal = nullptr, ev_lz0 = nullptr, ev_diag = nullptr;
-  double *hscal = nullptr;  // pinned host copy of the device scalars
+  double *hscal = nullptr;      // pinned, mapped host copy of the device scalars
+  double *hscal_dev = nullptr;  // its device address (the solve's reductions write it directly)
   double KE[576];
   std::vector<int> tpl_off;
   std::vector<double> tpl_w;
@@ -570,9 +571,10 @@ void ensure_nvec(tofe_ctx *c, int nvec)
   c->scal.zero(c->stream);
   c->frozen_dirty = false;
   if (c->hscal) (void)hipHostFree(c->hscal);
-  c->hscal = nullptr;
+  c->hscal = c->hscal_dev = nullptr;
   TOFE_HIP(hipHostMalloc((void **)&c->hscal, (size_t)SL_COUNT * nvec * sizeof(double),
-                         hipHostMallocDefault));
+                         hipHostMallocMapped));
+  TOFE_HIP(hipHostGetDevicePointer((void **)&c->hscal_dev, c->hscal, 0));
   TOFE_HIP(hipStreamSynchronize(c->stream));
 }
 
@@ -994,15 +996,19 @@ void solve_block(tofe_ctx *c, const double *F, double *U, int nvec, double rtol,
       fcg_update_r(s, r, q, n, c->partial.p, c->partial2.p, nfb, c->scal.p, c->partial3.p, first,
                    nvec);
       x_pending = true;
-      reduce_final(s, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p, nvec);
+      reduce_final(s, c->partial3.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p, nvec,
+                   c->hscal_dev);
     } else {
       level_apply<double>(c, 0, p, EPI_APPLY_DOT, ep);
       reduce_final(s, c->partial.p, nfb, c->scal.p + SL_PQ, POST_ALPHA_CG, c->scal.p, nvec);
       cg_update_r(s, r, q, n, c->scal.p, SL_ALPHA, c->partial.p, nvec, first64);
       x_pending = true;
-      reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p, nvec);
+      reduce_final(s, c->partial.p, kRedBlocks, c->scal.p + SL_RR, POST_NONE, c->scal.p, nvec,
+                   c->hscal_dev);
     }
-    scal_to_host_async(c, nvec);
+    // the reduction has written this iteration's scalars to the mapped host copy: no copy
+    // launch between it and the next direction's kernels
+    TOFE_HIP(hipEventRecord(c->ev_scal, s));
     bool ahead = k + 1 < maxit;
     for (int v = 0; v < nvec && ahead; ++v)
       if (!reason[v] && !(rn_prev[v] * std::min(rate[v], 1.0) > 3 * ttol[v])) ahead = false;
@@ -1901,7 +1907,8 @@ int tofe_create(const tofe_desc *desc, tofe_ctx **out)
     for (hipEvent_t *e : {&c->ev_dense, &c->ev_inv, &c->ev_ops, &c->ev_lz, &c->ev_scal, &c->ev_lz0,
                           &c->ev_diag})
       TOFE_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
-    TOFE_HIP(hipHostMalloc((void **)&c->hscal, SL_COUNT * sizeof(double), hipHostMallocDefault));
+    TOFE_HIP(hipHostMalloc((void **)&c->hscal, SL_COUNT * sizeof(double), hipHostMallocMapped));
+    TOFE_HIP(hipHostGetDevicePointer((void **)&c->hscal_dev, c->hscal, 0));
 
     compute_ke(d.E0, d.nu, d.hx, d.hy, d.hz, c->KE);
     dmatrix(d.E0, d.nu, c->shape.D);

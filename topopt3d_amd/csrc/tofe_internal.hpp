@@ -261,8 +261,10 @@ void galerkin_springs(hipStream_t s, const double *spk, int Nfx, int Nfy, int Nf
 // dinv, the operator and the transfer weights are shared.
 template <typename T>
 void dot_partial(hipStream_t s, const T *a, const T *b, int64_t n, double *partial, int nvec = 1);
+// mirror (may be null): device address of mapped host memory that also receives each vector's
+// SL_COUNT scalars after the reduction
 void reduce_final(hipStream_t s, const double *partial, int nparts, double *out, int post,
-                  double *scal, int nvec = 1);
+                  double *scal, int nvec = 1, double *mirror = nullptr);
 // Right-hand sides of a block solve that have converged (bit v of mask): their alpha and
 // beta are forced to zero from the next reduction on.
 void set_frozen(hipStream_t s, double *scal, int nvec, unsigned long long mask);
